@@ -1,0 +1,172 @@
+/*
+ * ref_harness.c -- driver that runs the reference's own UPML solvers (compiled
+ * from the sources under /root/reference, see oracle/Makefile target `ref`) on a
+ * given configuration and dumps raw outputs.  It replaces the reference's
+ * main.c, whose model/solver choice is hard-coded (main.c:155-156).
+ *
+ * TEST INFRASTRUCTURE ONLY: used by tests/golden/make_golden.py to write the
+ * golden fixtures that pin oracle/fdtd_oracle.c.  Never shipped or called by
+ * the product.
+ *
+ * usage: ref_harness key=value ...
+ *   mode=run|eps   solver=<enum SOLVER>  model=<enum MODEL>  sweep=<models_isFinish calls>
+ *   width=<nm> height=<nm> h=<nm> pml=<cells> lambda=<nm> angle=<deg> steps=<n>
+ *   out=<directory>
+ * mode=run : simulator_init, `steps` x simulator_calc, then dumps
+ *            f0/f1/f2.bin (TM: Ez,Hx,Hy  TE: Ex,Ey,Hz; solver's own layout),
+ *            eps.bin (getEps), meta.txt, ff.bin (TM: ntffTM_Frequency on the
+ *            fields repacked to N_PX x N_PY, after ntffTM_init), then
+ *            plain solvers: simulator_reset() -> spectra file in `out`;
+ *            MPI TE: finish() -> MPI_TE_UPML/Eth_*.txt, Eph_*.txt in `out`.
+ * mode=eps : field_init + models_initModel, dumps eps_xy.bin (D_XY at (x,y)),
+ *            eps_dy.bin (D_Y at (x+.5,y)), eps_dx.bin (D_X at (x,y+.5)) for all
+ *            N_PX x N_PY cells.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <complex.h>
+#include <unistd.h>
+#include <sys/stat.h>
+#include <mpi.h>
+#include "simulator.h"
+#include "field.h"
+#include "models.h"
+#include "ntffTM.h"
+#include "mpiTM_UPML.h"
+#include "mpiTE_UPML.h"
+#include "fdtdTM_upml.h"
+#include "fdtdTE_upml.h"
+
+static const char *arg(int argc, char **argv, const char *key, const char *def)
+{
+  size_t n = strlen(key);
+  for (int i = 1; i < argc; i++)
+    if (strncmp(argv[i], key, n) == 0 && argv[i][n] == '=') return argv[i] + n + 1;
+  return def;
+}
+
+static void dump(const char *dir, const char *name, const void *p, size_t bytes)
+{
+  char path[1024];
+  snprintf(path, sizeof path, "%s/%s", dir, name);
+  FILE *fp = fopen(path, "wb");
+  if (!fp) { perror(path); exit(3); }
+  fwrite(p, 1, bytes, fp);
+  fclose(fp);
+}
+
+int main(int argc, char **argv)
+{
+  const char *mode = arg(argc, argv, "mode", "run");
+  int solver = atoi(arg(argc, argv, "solver", "4"));
+  int model = atoi(arg(argc, argv, "model", "1"));
+  int sweep = atoi(arg(argc, argv, "sweep", "0"));
+  FieldInfo fi;
+  fi.width_nm = atoi(arg(argc, argv, "width", "4000"));
+  fi.height_nm = atoi(arg(argc, argv, "height", "4000"));
+  fi.h_u_nm = atoi(arg(argc, argv, "h", "50"));
+  fi.pml = atoi(arg(argc, argv, "pml", "10"));
+  fi.lambda_nm = atoi(arg(argc, argv, "lambda", "500"));
+  fi.angle_deg = atoi(arg(argc, argv, "angle", "0"));
+  fi.stepNum = atoi(arg(argc, argv, "steps", "100"));
+  const char *out = arg(argc, argv, "out", ".");
+  mkdir(out, 0755);
+
+  MPI_Init(0, 0);
+  models_setModel((enum MODEL)model);
+  for (int k = 0; k < sweep; k++) models_isFinish();
+
+  if (strcmp(mode, "eps") == 0) {
+    field_init(fi);
+    models_initModel();
+    FieldInfo_S s = field_getFieldInfo_S();
+    size_t n = (size_t)s.N_PX * s.N_PY;
+    double *a = malloc(n * sizeof(double)), *b = malloc(n * sizeof(double)), *c = malloc(n * sizeof(double));
+    for (int x = 0; x < s.N_PX; x++)
+      for (int y = 0; y < s.N_PY; y++) {
+        size_t k = (size_t)x * s.N_PY + y;
+        a[k] = models_eps(x, y, D_XY);
+        b[k] = models_eps(x + 0.5, y, D_Y);
+        c[k] = models_eps(x, y + 0.5, D_X);
+      }
+    dump(out, "eps_xy.bin", a, n * sizeof(double));
+    dump(out, "eps_dy.bin", b, n * sizeof(double));
+    dump(out, "eps_dx.bin", c, n * sizeof(double));
+    char meta[256];
+    int len = snprintf(meta, sizeof meta, "%d %d\n", s.N_PX, s.N_PY);
+    dump(out, "meta.txt", meta, len);
+    MPI_Finalize();
+    return 0;
+  }
+
+  simulator_setSolver((enum SOLVER)solver);
+  simulator_init(fi);
+  for (int n = 0; n < fi.stepNum; n++) simulator_calc();
+
+  FieldInfo_S s = field_getFieldInfo_S();
+  int mpi = (solver == MPI_TM_UPML_2D || solver == MPI_TE_UPML_2D);
+  int tm = (solver == MPI_TM_UPML_2D || solver == TM_UPML_2D);
+  int npx = mpi ? s.N_PX + 2 : s.N_PX, npy = mpi ? s.N_PY + 2 : s.N_PY;
+  size_t n = (size_t)npx * npy;
+  double complex *f[3];
+  double *eps;
+  if (solver == MPI_TM_UPML_2D) {
+    f[0] = mpi_fdtdTM_upml_getEz(); f[1] = mpi_fdtdTM_upml_getHx(); f[2] = mpi_fdtdTM_upml_getHy();
+    eps = mpi_fdtdTM_upml_getEps();
+  } else if (solver == MPI_TE_UPML_2D) {
+    f[0] = mpi_fdtdTE_upml_getEx(); f[1] = mpi_fdtdTE_upml_getEy(); f[2] = mpi_fdtdTE_upml_getHz();
+    eps = mpi_fdtdTE_upml_getEps();
+  } else if (solver == TM_UPML_2D) {
+    f[0] = fdtdTM_upml_getEz(); f[1] = fdtdTM_upml_getHx(); f[2] = fdtdTM_upml_getHy();
+    eps = fdtdTM_upml_getEps();
+  } else {
+    f[0] = fdtdTE_upml_getEx(); f[1] = fdtdTE_upml_getEy(); f[2] = fdtdTE_upml_getHz();
+    eps = fdtdTE_upml_getEps();
+  }
+  dump(out, "f0.bin", f[0], n * sizeof(double complex));
+  dump(out, "f1.bin", f[1], n * sizeof(double complex));
+  dump(out, "f2.bin", f[2], n * sizeof(double complex));
+  /* the MPI ghost ring of eps is uninitialised in the reference: zero it */
+  double *e = malloc(n * sizeof(double));
+  memcpy(e, eps, n * sizeof(double));
+  if (mpi)
+    for (int i = 0; i < npx; i++)
+      for (int j = 0; j < npy; j++)
+        if (i == 0 || j == 0 || i == npx - 1 || j == npy - 1) e[i * npy + j] = 0;
+  dump(out, "eps.bin", e, n * sizeof(double));
+  NTFFInfo ni = field_getNTFFInfo();
+  char meta[512];
+  int len = snprintf(meta, sizeof meta, "%d %d %d %d %d %d %d %d %d %d %.17g %d\n", s.N_PX, s.N_PY, npx, npy,
+                     ni.top, ni.bottom, ni.left, ni.right, ni.cx, ni.cy, ni.RFperC, ni.arraySize);
+  dump(out, "meta.txt", meta, len);
+
+  if (tm) {
+    /* repack to N_PX x N_PY as mpiTM_UPML.c:542-544 does, then the plain-layout far field */
+    size_t np = (size_t)s.N_PX * s.N_PY;
+    double complex *P[3];
+    int order[3] = {1, 2, 0}; /* Hx, Hy, Ez */
+    for (int a = 0; a < 3; a++) {
+      P[a] = malloc(np * sizeof(double complex));
+      for (int x = 0; x < s.N_PX; x++)
+        for (int y = 0; y < s.N_PY; y++)
+          P[a][(size_t)x * s.N_PY + y] = mpi ? f[order[a]][(size_t)(x + 1) * npy + (y + 1)] : f[order[a]][(size_t)x * npy + y];
+    }
+    double complex res[360];
+    ntffTM_init();
+    ntffTM_Frequency(P[0], P[1], P[2], res);
+    dump(out, "ff.bin", res, sizeof res);
+  }
+
+  if (chdir(out) != 0) { perror(out); return 3; }
+  if (!mpi) {
+    simulator_reset(); /* plain solvers write the spectra files here */
+  } else if (!tm) {
+    mkdir("MPI_TE_UPML", 0755);
+    void (*fin)(void) = mpi_fdtdTE_upml_getFinish();
+    fin(); /* ntffOutput -> Eth/Eph text files; also calls MPI_Finalize */
+    return 0;
+  }
+  MPI_Finalize();
+  return 0;
+}
