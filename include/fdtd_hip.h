@@ -1,0 +1,142 @@
+/*
+ * fdtd_hip.h -- thin C ABI between the C host (field / models / simulator /
+ * solver entry points) and the gfx950 HIP kernels of libmpifdtd_amd.so.
+ *
+ * Plain pointers, sizes and PODs only.  All host pointers are ordinary host
+ * memory; device memory is owned by the grid / batch objects.  Every call
+ * returns 0 on success or a negative FDTD_E* code; fdtd_hip_last_error()
+ * returns a message for the last failure on the calling thread.
+ *
+ * This layer has no counterpart in the reference (which is CPU-only); the
+ * reference-facing entry points are declared in mpifdtd_solvers.h,
+ * mpifdtd_field.h, mpifdtd_models.h and mpifdtd_simulator.h.
+ */
+#ifndef FDTD_HIP_H
+#define FDTD_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { FDTD_OK = 0, FDTD_EINVAL = -1, FDTD_EHIP = -2, FDTD_ENOMEM = -3, FDTD_ESTATE = -4 };
+
+enum { FDTD_MODE_TM = 0, FDTD_MODE_TE = 1 };
+/* MPI flavour: padded (N_PX+2)x(N_PY+2) arrays, E half first, CW scattered wave
+ *              (mpiTM_UPML.c / mpiTE_UPML.c single rank).
+ * plain flavour: N_PX x N_PY arrays, H half first, Gaussian pulse
+ *              (fdtdTM_upml.c / fdtdTE_upml.c). */
+enum { FDTD_FLAVOUR_MPI = 0, FDTD_FLAVOUR_PLAIN = 1 };
+enum { FDTD_FP64 = 0, FDTD_FP32 = 1 };
+
+/* Model ids: enum MODEL (models.h:5-14) values */
+enum { FDTD_NO_MODEL = 0, FDTD_MIE_CYLINDER = 1, FDTD_LAYER = 2, FDTD_MORPHO_SCALE = 3, FDTD_ZIGZAG = 5 };
+
+#define FDTD_MAX_LAYERS 32
+
+/* Derived model state in cell units (what each model's *_init computes),
+ * shared by the host eps callback and the device eps-fill kernel. */
+typedef struct fdtd_model_state {
+  int model;
+  int N_PX, N_PY, N_X, N_Y, N_PML;
+  double eps0;
+  /* circle (circleModel.c:119-126) */
+  double ci_radius_s, ci_eps_s, ci_posx, ci_posy;
+  /* multilayer (multiLayerModel.c:331-348) */
+  int ml_layer_num, ml_uniaxial, ml_asymmetry, ml_use_gap;
+  double ml_width_s[2], ml_thick_s[2], ml_branch_s, ml_gap_s, ml_ep[2], ml_epx[2], ml_c0, ml_c1, ml_edge_rate;
+  /* zigzag (zigzagModel.c:146-160) */
+  int zz_layer_num;
+  double zz_rad, zz_width_s, zz_thick_s, zz_height_s, zz_ox, zz_oy, zz_exw, zz_exh, zz_ep;
+  /* morpho scale (morphoScaleModel.c:379-422) */
+  int mo_layer_num, mo_uniaxial;
+  double mo_width_s, mo_thick_s[2], mo_branch_s, mo_gap_s, mo_height_s, mo_ox, mo_oy, mo_ep, mo_epx;
+  double mo_edge_rate, mo_curve;
+  double mo_rand[2][FDTD_MAX_LAYERS];
+  double mo_lam_a[2][FDTD_MAX_LAYERS], mo_lam_b1[2][FDTD_MAX_LAYERS], mo_lam_b2[2][FDTD_MAX_LAYERS];
+} fdtd_model_state;
+
+/* Geometry and constants of one simulation instance (from field.c's state). */
+typedef struct fdtd_grid_desc {
+  int mode, flavour, precision;
+  int N_PX, N_PY, N_PML;
+  /* NTFF contour (field.c:133-142) */
+  int ntff_top, ntff_bottom, ntff_left, ntff_right, ntff_cx, ntff_cy;
+  double rf_per_c;
+  int array_size, max_time;
+  double c0, mu0, z0, eps0;
+  int ntff_time;     /* 1: per-step time-domain NTFF accumulation */
+  int point_source;  /* 1: soft point source at (N_PX/2, N_PY/2) in the MPI TM step */
+} fdtd_grid_desc;
+
+/* UPML coefficient tables, indexed by physical cell coordinate.  Built on the
+ * host from field_sigmaX/Y exactly as setCoefficient does
+ * (mpiTM_UPML.c:663-716, mpiTE_UPML.c:489-536, fdtdTM_upml.c:224-274,
+ * fdtdTE_upml.c:361-412); only the two coefficients that mix sigma_x and
+ * sigma_y are formed on the device, from the raw sigma tables sx / sy.
+ *   TM: x0 C_JZ  x1 C_JZHXHY  x2 C_BXMX1  x3 C_BXMX0  x4 C_BY      sx = sig_hy_x
+ *       y0 C_DZ  y1 C_DZJZ1(=C_DZJZ0)  y2 C_MX  y3 C_MXEZ         sy = sig_hy_y
+ *   TE: x0 C_DXJX1 x1 C_DXJX0 x2 C_DY  x3 C_MZ   x4 C_MZEXEY     sx = sig_ey_x
+ *       y0 C_JX  y1 C_JXHZ  y2 C_BZ  y3 C_BZMZ1(=C_BZMZ0)         sy = sig_ey_y
+ * Each x table has N_PX entries, each y table N_PY. */
+typedef struct fdtd_coef_tables {
+  const double *x[5];
+  const double *sx;
+  const double *y[4];
+  const double *sy;
+} fdtd_coef_tables;
+
+/* Per-step scalars, read from field.c's getters by the host each step. */
+typedef struct fdtd_step_params {
+  double time;       /* field_getTime() */
+  double ray_coef;   /* field_getRayCoef() */
+  double omega;      /* field_getOmega() */
+  double k;          /* field_getK() */
+  double angle_deg;  /* field_getWaveAngle() */
+  /* point source value ray_coef * cexp(i omega time) (field.c:145-152) */
+  double ps_re, ps_im;
+  /* plain TE: projection of the pulse onto Ex / Ey (fdtdTE_upml.c:182-189) */
+  double dot_x, dot_y;
+  int use_x, use_y;
+} fdtd_step_params;
+
+typedef struct fdtd_grid fdtd_grid;
+
+const char *fdtd_hip_last_error(void);
+int fdtd_hip_device_count(void);
+
+int fdtd_hip_grid_create(const fdtd_grid_desc *d, fdtd_grid **out);
+int fdtd_hip_grid_destroy(fdtd_grid *g);
+int fdtd_hip_grid_set_coefs(fdtd_grid *g, const fdtd_coef_tables *t);
+/* eps maps on the device from the model state (3 maps, see fdtd_hip_grid_get_eps) */
+int fdtd_hip_grid_fill_eps(fdtd_grid *g, const fdtd_model_state *m);
+/* which: TM 0 EPS_EZ 1 EPS_HX 2 EPS_HY ; TE 0 EPS_EX 1 EPS_EY 2 EPS_HZ; solver layout */
+int fdtd_hip_grid_set_eps(fdtd_grid *g, int which, const double *host);
+int fdtd_hip_grid_get_eps(fdtd_grid *g, int which, double *host);
+/* one reference update(): E/H halves, source, per-step NTFF */
+int fdtd_hip_grid_step(fdtd_grid *g, const fdtd_step_params *p);
+/* which 0..8: TM Ez Jz Dz Hx Mx Bx Hy My By / TE Ex Jx Dx Ey Jy Dy Hz Mz Bz;
+ * host buffers hold NPXa*NPYa complex doubles (re,im) in the solver layout */
+int fdtd_hip_grid_get_field(fdtd_grid *g, int which, double *host);
+int fdtd_hip_grid_set_field(fdtd_grid *g, int which, const double *host);
+/* time-domain NTFF accumulators (which 0..2), 360*array_size complex */
+int fdtd_hip_grid_get_ntff_time(fdtd_grid *g, int which, double *host);
+/* zero fields, aux fields and NTFF accumulators */
+int fdtd_hip_grid_reset(fdtd_grid *g);
+/* frequency-domain far field of the current fields (360 complex, re/im pairs):
+ * TM: ntffTM_Frequency semantics on the physical cells (ntffTM.c:72-158);
+ * TE: single-rank ntffTE_FrequencySplit semantics (ntffTE.c:341-470).
+ * coef = sqrt(i k/(8 pi R0)) exp(i k R0) is passed in as (re,im). */
+int fdtd_hip_grid_far_field(fdtd_grid *g, double k, double coef_re, double coef_im, double scale, double *out360);
+/* time-domain far field + spectra (ntffTM.c:161-233, ntffTE.c:20-55,160-196):
+ * translation coefficient (re,im), 4096 FFT twiddles e^{i pi k/4096} (re,im
+ * pairs), h_u_nm; out[321*360] lambda-major. */
+int fdtd_hip_grid_time_spectra(fdtd_grid *g, double coef_re, double coef_im, const double *twiddles,
+                               int h_u_nm, double *out);
+/* dimensions: out[0..3] = NPXa, NPYa, off, array_size */
+int fdtd_hip_grid_dims(const fdtd_grid *g, int out[4]);
+int fdtd_hip_grid_sync(fdtd_grid *g);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

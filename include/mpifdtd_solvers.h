@@ -1,0 +1,101 @@
+/*
+ * mpifdtd_solvers.h -- the four UPML solvers' entry points, backed by HIP.
+ *
+ * Same symbols and signatures as the reference's solver headers:
+ *   mpiTM_UPML.h:5-18   (mpiTM_UPML.c:88-136)   MPI TMz UPML, run as one rank on one GPU
+ *   mpiTE_UPML.h:5-19   (mpiTE_UPML.c:72-126)   MPI TEz UPML, run as one rank on one GPU
+ *   fdtdTM_upml.h:5-14  (fdtdTM_upml.c:117-152) plain TMz UPML
+ *   fdtdTE_upml.h:5-14                          plain TEz UPML
+ * get{Update,Init,Finish,Reset} return the solver's void(void) methods; the
+ * field getters return borrowed host pointers (valid from init to finish) that
+ * are refreshed from device memory on every getter call; getEps returns the
+ * same eps map the reference does (MPI TM: EPS_EZ, MPI TE: EPS_EY, plain TM:
+ * EPS_EZ, plain TE: EPS_EX).  The MPI variants keep the reference's padded
+ * (N_PX+2)x(N_PY+2) layout; none of them calls MPI_Finalize.
+ */
+#ifndef MPIFDTD_SOLVERS_H
+#define MPIFDTD_SOLVERS_H
+
+#include "mpifdtd_field.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void (*fdtd_method)(void);
+
+/* mpiTM_UPML.h:5-18 */
+fdtd_method mpi_fdtdTM_upml_getUpdate(void);
+fdtd_method mpi_fdtdTM_upml_getFinish(void);
+fdtd_method mpi_fdtdTM_upml_getReset(void);
+fdtd_method mpi_fdtdTM_upml_getInit(void);
+fdtd_dcomplex *mpi_fdtdTM_upml_getHx(void);
+fdtd_dcomplex *mpi_fdtdTM_upml_getHy(void);
+fdtd_dcomplex *mpi_fdtdTM_upml_getEz(void);
+double *mpi_fdtdTM_upml_getEps(void);
+int mpi_fdtdTM_upml_getSubNx(void);
+int mpi_fdtdTM_upml_getSubNy(void);
+int mpi_fdtdTM_upml_getSubNpx(void);
+int mpi_fdtdTM_upml_getSubNpy(void);
+int mpi_fdtdTM_upml_getSubNcell(void);
+
+/* mpiTE_UPML.h:5-19 */
+fdtd_method mpi_fdtdTE_upml_getUpdate(void);
+fdtd_method mpi_fdtdTE_upml_getFinish(void);
+fdtd_method mpi_fdtdTE_upml_getReset(void);
+fdtd_method mpi_fdtdTE_upml_getInit(void);
+fdtd_dcomplex *mpi_fdtdTE_upml_getEx(void);
+fdtd_dcomplex *mpi_fdtdTE_upml_getEy(void);
+fdtd_dcomplex *mpi_fdtdTE_upml_getHz(void);
+double *mpi_fdtdTE_upml_getEps(void);
+int mpi_fdtdTE_upml_getSubNx(void);
+int mpi_fdtdTE_upml_getSubNy(void);
+int mpi_fdtdTE_upml_getSubNpx(void);
+int mpi_fdtdTE_upml_getSubNpy(void);
+int mpi_fdtdTE_upml_getSubNcell(void);
+
+/* fdtdTM_upml.h:5-14 */
+fdtd_method fdtdTM_upml_getUpdate(void);
+fdtd_method fdtdTM_upml_getFinish(void);
+fdtd_method fdtdTM_upml_getReset(void);
+fdtd_method fdtdTM_upml_getInit(void);
+fdtd_dcomplex *fdtdTM_upml_getHx(void);
+fdtd_dcomplex *fdtdTM_upml_getHy(void);
+fdtd_dcomplex *fdtdTM_upml_getEz(void);
+double *fdtdTM_upml_getEps(void);
+
+/* fdtdTE_upml.h:5-14 */
+fdtd_method fdtdTE_upml_getUpdate(void);
+fdtd_method fdtdTE_upml_getFinish(void);
+fdtd_method fdtdTE_upml_getReset(void);
+fdtd_method fdtdTE_upml_getInit(void);
+fdtd_dcomplex *fdtdTE_upml_getEx(void);
+fdtd_dcomplex *fdtdTE_upml_getEy(void);
+fdtd_dcomplex *fdtdTE_upml_getHz(void);
+double *fdtdTE_upml_getEps(void);
+
+/* --- extensions on the active solver (no counterpart in the reference) --- */
+/* which 0..8 in the solver's slot order (TM Ez Jz Dz Hx Mx Bx Hy My By,
+ * TE Ex Jx Dx Ey Jy Dy Hz Mz Bz); NPXa*NPYa complex doubles */
+int fdtd_solver_getState(int which, double *out);
+int fdtd_solver_setState(int which, const double *in);
+/* time-domain NTFF accumulators (which 0..2: Ux Uy Wz / Wx Wy Uz), 360*arraySize complex */
+int fdtd_solver_getNTFFTime(int which, double *out);
+/* frequency-domain far field of the current fields, 360 complex (re,im):
+ * TM ntffTM_Frequency (ntffTM.c:72-158); TE single-rank ntffTE_FrequencySplit
+ * (ntffTE.c:341-470) with R0 of ntffTE_init */
+int fdtd_solver_farField(double *out360);
+/* time-domain far-field spectra 380..700 nm x 360 angles, as the plain
+ * solvers' ntff*_TimeOutput computes them (ntffTM.c:197-233, ntffTE.c:160-196) */
+int fdtd_solver_timeSpectra(double *out321x360);
+/* 0: skip per-step time-domain NTFF accumulation (default 1, as the reference); set before init */
+void fdtd_solver_setNTFFTime(int on);
+/* 0: reset()/finish() write no output files (default 1, as the reference) */
+void fdtd_solver_setFileOutput(int on);
+/* dims of the active solver: NPXa, NPYa, off, arraySize */
+int fdtd_solver_dims(int out[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

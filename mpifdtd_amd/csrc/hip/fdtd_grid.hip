@@ -1,0 +1,1086 @@
+// fdtd_grid.hip -- FP64 reference-order UPML path for gfx950.
+//
+// One fdtd_grid is one simulation instance of one of the reference's four UPML
+// solvers (MPI TM/TE single rank, plain TM/TE).  Per reference update():
+//   MPI   : E half (calcJD + calcE + scatteredWave [+ point source]) -> H half
+//           (calcMB + calcH) -> per-step NTFF          (mpiTM_UPML.c:196-217, mpiTE_UPML.c:290-304)
+//   plain : H half -> E half (+ Gaussian pulse) -> ntff*_TimeCalc
+//                                                       (fdtdTM_upml.c:54-66, fdtdTE_upml.c:168-192)
+// The arithmetic of every update keeps the reference's operation order and is
+// compiled without FMA contraction, so results agree with the CPU solver to
+// rounding of the device sin/cos/exp only.
+//
+// HBM layout: each complex field is an array of double2 (re, im), row-major
+// k = i*NPYa + j (j = y contiguous), exactly the reference's layout, so host
+// mirrors are plain memcpy targets.  UPML coefficients live in 1-D tables per
+// axis; only eps is a 2-D map.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#include <algorithm>
+
+#include "fdtd_hip.h"
+#include "fdtd_model_eps.h"
+#include "fdtd_internal.h"
+
+static thread_local std::string g_err;
+
+extern "C" const char *fdtd_hip_last_error(void) { return g_err.c_str(); }
+
+int fdtd_fail(int code, const char *fmt, ...)
+{
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+extern "C" int fdtd_hip_device_count(void)
+{
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+// ---------------------------------------------------------------- kernels
+
+struct GridPtrs {
+  double2 *F[9];
+  const double *eps[3];
+  int NPXa, NPYa, off;
+  double mu0;
+};
+
+struct Tabs {
+  const double *x[5];
+  const double *sx;
+  const double *y[4];
+  const double *sy;
+};
+
+struct StepArgs {
+  double w_t;             // omega * time, as the reference forms it
+  double ray_coef;
+  double ks_cos, ks_sin;  // MPI scattered wave
+  double ps_re, ps_im;    // point source value
+  int ps_i, ps_j, ps_on;
+  // plain Gaussian pulse (field.c:224-256), component 0 = Ez / Ex, 1 = Ey
+  double time, w_s, cpc, spc;
+  double t0[2], dot[2], gx[2], gy[2];
+  int on[2];
+};
+
+#define FOR_CELL()                                                   \
+  const int j = 1 + blockIdx.x * 64 + threadIdx.x;                   \
+  const int i = 1 + blockIdx.y * 4 + threadIdx.y;                    \
+  if (i > g.NPXa - 2 || j > g.NPYa - 2) return;                      \
+  const int NPY = g.NPYa;                                            \
+  const size_t k = (size_t)i * NPY + j;                              \
+  const int x = i - g.off, y = j - g.off;
+
+// Gaussian pulse term dot*g*(1/eps-1)*cexp(i r w) of field_scatteredPulse
+__device__ __forceinline__ void pulse_add(double2 &v, const StepArgs &s, int c, int i, int j, double e)
+{
+  const double r = (i + s.gx[c]) * s.cpc + (j + s.gy[c]) * s.spc - (s.time - s.t0[c]);
+  const double q = r / 50.0;
+  const double gauss = exp(-(q * q));
+  const double a = s.dot[c] * gauss * (1.0 / e - 1);
+  double sn, cs;
+  sincos(r * s.w_s, &sn, &cs);
+  v.x += a * cs;
+  v.y += a * sn;
+}
+
+// TM E half: calcJD + calcE (+ source).  x tables: 0 C_JZ 1 C_JZHXHY; y: 0 C_DZ 1 C_DZJZ
+template <int PLAIN>
+__global__ __launch_bounds__(256) void k_tm_e(GridPtrs g, Tabs t, StepArgs s)
+{
+  FOR_CELL();
+  double2 *Ez = g.F[0], *Jz = g.F[1], *Dz = g.F[2];
+  const double2 *Hx = g.F[3], *Hy = g.F[6];
+  const double2 jz = Jz[k], dz = Dz[k];
+  const double2 hy = Hy[k], hyl = Hy[k - NPY], hx = Hx[k], hxb = Hx[k - 1];
+  const double cr = ((hy.x - hyl.x) - hx.x) + hxb.x;
+  const double ci = ((hy.y - hyl.y) - hx.y) + hxb.y;
+  const double a = t.x[0][x], b = t.x[1][x];
+  double2 nj;
+  nj.x = a * jz.x + b * cr;
+  nj.y = a * jz.y + b * ci;
+  const double c = t.y[0][y], d = t.y[1][y];
+  double2 nd;
+  nd.x = (c * dz.x + d * nj.x) - d * jz.x;
+  nd.y = (c * dz.y + d * nj.y) - d * jz.y;
+  Jz[k] = nj;
+  Dz[k] = nd;
+  const double e = g.eps[0][k];
+  double2 ez;
+  ez.x = nd.x / e;
+  ez.y = nd.y / e;
+  if (!PLAIN) {
+    if (e != 1.0) {  // vacuum cells add an exact zero (mpiTM_UPML.c:370)
+      const double kr = x * s.ks_cos + y * s.ks_sin;
+      const double amp = s.ray_coef * (1.0 / e - 1.0);
+      double sn, cs;
+      sincos(kr - s.w_t, &sn, &cs);
+      ez.x += amp * cs;
+      ez.y += amp * sn;
+    }
+    if (s.ps_on && i == s.ps_i && j == s.ps_j) {
+      ez.x += s.ps_re;
+      ez.y += s.ps_im;
+    }
+  } else {
+    if (s.on[0] && e != 1.0 && i <= g.NPXa - 2 && j <= g.NPYa - 2) pulse_add(ez, s, 0, i, j, e);
+  }
+  Ez[k] = ez;
+}
+
+// TM H half: calcMB + calcH.  x: 2 C_BXMX1 3 C_BXMX0 4 C_BY, sx; y: 2 C_MX 3 C_MXEZ, sy
+__global__ __launch_bounds__(256) void k_tm_h(GridPtrs g, Tabs t)
+{
+  FOR_CELL();
+  const double2 *Ez = g.F[0];
+  double2 *Hx = g.F[3], *Mx = g.F[4], *Bx = g.F[5], *Hy = g.F[6], *My = g.F[7], *By = g.F[8];
+  const double2 ez = Ez[k], ezt = Ez[k + 1], ezr = Ez[k + NPY];
+  const double2 mx = Mx[k], bx = Bx[k], my = My[k], by = By[k];
+  const double cmx = t.y[2][y], cmxe = t.y[3][y];
+  double2 nmx;
+  nmx.x = cmx * mx.x - cmxe * (ezt.x - ez.x);
+  nmx.y = cmx * mx.y - cmxe * (ezt.y - ez.y);
+  const double b1 = t.x[2][x], b0 = t.x[3][x];
+  double2 nbx;  // C_BX == 1
+  nbx.x = (bx.x + b1 * nmx.x) - b0 * mx.x;
+  nbx.y = (bx.y + b1 * nmx.y) - b0 * mx.y;
+  double2 nmy;  // C_MY == C_MYEZ == 1
+  nmy.x = my.x - ((-ezr.x) + ez.x);
+  nmy.y = my.y - ((-ezr.y) + ez.y);
+  const double cby = t.x[4][x];
+  const double sxv = t.sx[x], syv = t.sy[y];
+  double c1 = 1.0, c0 = 1.0;
+  if (sxv != 0.0 || syv != 0.0) {
+    c1 = (2.0 + syv) / (2.0 + sxv);
+    c0 = (2.0 - syv) / (2.0 + sxv);
+  }
+  double2 nby;
+  nby.x = (cby * by.x + c1 * nmy.x) - c0 * my.x;
+  nby.y = (cby * by.y + c1 * nmy.y) - c0 * my.y;
+  Mx[k] = nmx;
+  Bx[k] = nbx;
+  My[k] = nmy;
+  By[k] = nby;
+  double2 hx, hy;
+  hx.x = nbx.x / g.mu0;
+  hx.y = nbx.y / g.mu0;
+  hy.x = nby.x / g.mu0;
+  hy.y = nby.y / g.mu0;
+  Hx[k] = hx;
+  Hy[k] = hy;
+}
+
+// TE E half.  x: 0 C_DXJX1 1 C_DXJX0 2 C_DY, sx; y: 0 C_JX 1 C_JXHZ, sy
+template <int PLAIN>
+__global__ __launch_bounds__(256) void k_te_e(GridPtrs g, Tabs t, StepArgs s)
+{
+  FOR_CELL();
+  double2 *Ex = g.F[0], *Jx = g.F[1], *Dx = g.F[2], *Ey = g.F[3], *Jy = g.F[4], *Dy = g.F[5];
+  const double2 *Hz = g.F[6];
+  const double2 hz = Hz[k], hzb = Hz[k - 1], hzl = Hz[k - NPY];
+  const double2 jx = Jx[k], dx = Dx[k], jy = Jy[k], dy = Dy[k];
+  const double cjx = t.y[0][y], cjxh = t.y[1][y];
+  double2 njx;
+  njx.x = cjx * jx.x + cjxh * (hz.x - hzb.x);
+  njx.y = cjx * jx.y + cjxh * (hz.y - hzb.y);
+  const double d1 = t.x[0][x], d0 = t.x[1][x];
+  double2 ndx;  // C_DX == 1
+  ndx.x = (dx.x + d1 * njx.x) - d0 * jx.x;
+  ndx.y = (dx.y + d1 * njx.y) - d0 * jx.y;
+  double2 njy;  // C_JY == C_JYHZ == 1
+  njy.x = jy.x + ((-hz.x) + hzl.x);
+  njy.y = jy.y + ((-hz.y) + hzl.y);
+  const double cdy = t.x[2][x];
+  const double sxv = t.sx[x], syv = t.sy[y];
+  double c1 = 1.0, c0 = 1.0;
+  if (sxv != 0.0 || syv != 0.0) {
+    c1 = (2.0 + syv) / (2.0 + sxv);
+    c0 = (2.0 - syv) / (2.0 + sxv);
+  }
+  double2 ndy;
+  ndy.x = (cdy * dy.x + c1 * njy.x) - c0 * jy.x;
+  ndy.y = (cdy * dy.y + c1 * njy.y) - c0 * jy.y;
+  Jx[k] = njx;
+  Dx[k] = ndx;
+  Jy[k] = njy;
+  Dy[k] = ndy;
+  const double ex_e = g.eps[0][k], ey_e = g.eps[1][k];
+  double2 ex, ey;
+  ex.x = ndx.x / ex_e;
+  ex.y = ndx.y / ex_e;
+  ey.x = ndy.x / ey_e;
+  ey.y = ndy.y / ey_e;
+  if (!PLAIN) {
+    if (ey_e != 1.0) {  // mpiTE_UPML.c:278, on Ey only
+      const double ikx = x * s.ks_cos + y * s.ks_sin;
+      const double amp = s.ray_coef * (1.0 / ey_e - 1);
+      double sn, cs;
+      sincos(ikx - s.w_t, &sn, &cs);
+      ey.x += amp * cs;
+      ey.y += amp * sn;
+    }
+  } else {
+    if (s.on[0] && ex_e != 1.0) pulse_add(ex, s, 0, i, j, ex_e);
+    if (s.on[1] && ey_e != 1.0) pulse_add(ey, s, 1, i, j, ey_e);
+  }
+  Ex[k] = ex;
+  Ey[k] = ey;
+}
+
+// TE H half.  x: 3 C_MZ 4 C_MZEXEY; y: 2 C_BZ 3 C_BZMZ
+__global__ __launch_bounds__(256) void k_te_h(GridPtrs g, Tabs t)
+{
+  FOR_CELL();
+  const double2 *Ex = g.F[0], *Ey = g.F[3];
+  double2 *Hz = g.F[6], *Mz = g.F[7], *Bz = g.F[8];
+  const double2 ey = Ey[k], eyr = Ey[k + NPY], ex = Ex[k], ext = Ex[k + 1];
+  const double2 mz = Mz[k], bz = Bz[k];
+  const double cmz = t.x[3][x], cmze = t.x[4][x];
+  double2 nmz;
+  nmz.x = cmz * mz.x - cmze * (((eyr.x - ey.x) - ext.x) + ex.x);
+  nmz.y = cmz * mz.y - cmze * (((eyr.y - ey.y) - ext.y) + ex.y);
+  const double cbz = t.y[2][y], cbm = t.y[3][y];
+  double2 nbz;
+  nbz.x = (cbz * bz.x + cbm * nmz.x) - cbm * mz.x;
+  nbz.y = (cbz * bz.y + cbm * nmz.y) - cbm * mz.y;
+  Mz[k] = nmz;
+  Bz[k] = nbz;
+  double2 hz;
+  hz.x = nbz.x / g.mu0;
+  hz.y = nbz.y / g.mu0;
+  Hz[k] = hz;
+}
+
+// ---------------------------------------------------------------- NTFF (time domain)
+
+// gather the contour samples of this step: E value (signed) and the averaged,
+// signed H value of every contour cell
+__global__ __launch_bounds__(256) void k_ntff_gather(const double2 *const *F, Contour ct, double2 *cE, double2 *cH)
+{
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= ct.total) return;
+  int s = 0;
+  while (s < 3 && p >= ct.s[s + 1].off) s++;
+  const Seg &sg = ct.s[s];
+  const int c = p - sg.off;
+  const long k = sg.k0 + (long)c * sg.dk;
+  const double2 e = F[sg.ef][k];
+  const double2 h0 = F[sg.hf][k], h1 = F[sg.hf][k + sg.hnb];
+  double2 ev, hv;
+  ev.x = sg.es * e.x;
+  ev.y = sg.es * e.y;
+  hv.x = sg.hs * (h0.x + h1.x);
+  hv.y = sg.hs * (h0.y + h1.y);
+  cE[p] = ev;
+  cH[p] = hv;
+}
+
+#define NTFF_CH 2048
+#define NT_GUARD_DEV 4  // guard bins allocated before and after the accumulators
+
+// one workgroup per angle: 3-tap retarded-time binning of every contour sample
+// into LDS, then one read-modify-write of the touched bins
+// (mpiTM_UPML.c:849-1037, mpiTE_UPML.c:611-794, ntffTM.c:279-371, ntffTE.c:57-157)
+__global__ __launch_bounds__(256) void k_ntff_bin(Contour ct, const double2 *cE, const double2 *cH,
+                                                  const double *tsTab, const double *tsMin, const double *tsMax,
+                                                  double2 *U0, double2 *U1, double2 *U2, int arraySize,
+                                                  double timeE, double timeH, double coef, int useCoef)
+{
+  __shared__ double lds[3][NTFF_CH][2];
+  const int ang = blockIdx.x;
+  const double *ts = tsTab + (size_t)ang * ct.total;
+  double2 *U[3] = {U0, U1, U2};
+  const long stp = (long)ang * arraySize;
+  const int lo = (int)floor(timeE + tsMin[ang] + 0.5) - 2;
+  const int hi = (int)floor(timeH + tsMax[ang] + 0.5) + 2;
+  for (int base = lo; base <= hi; base += NTFF_CH) {
+    for (int q = threadIdx.x; q < 3 * NTFF_CH; q += 256) {
+      lds[q / NTFF_CH][q % NTFF_CH][0] = 0.0;
+      lds[q / NTFF_CH][q % NTFF_CH][1] = 0.0;
+    }
+    __syncthreads();
+    for (int p = threadIdx.x; p < ct.total; p += 256) {
+      int s = 0;
+      while (s < 3 && p >= ct.s[s + 1].off) s++;
+      const double tsv = ts[p];
+      const double2 vals[2] = {cE[p], cH[p]};
+      const double tms[2] = {timeE, timeH};
+      const int arr[2] = {ct.s[s].ua, ct.s[s].wa};
+#pragma unroll
+      for (int w = 0; w < 2; w++) {
+        const double tt = tms[w] + tsv;
+        const int m = (int)floor(tt + 0.5);
+        const double a = (0.5 + tt) - m;
+        const double b = 1.0 - a;
+        const double ab = a - b;
+        double vb_r = vals[w].x * b, vb_i = vals[w].y * b;
+        double vab_r = vals[w].x * ab, vab_i = vals[w].y * ab;
+        double va_r = vals[w].x * a, va_i = vals[w].y * a;
+        if (useCoef) {
+          vb_r *= coef; vb_i *= coef; vab_r *= coef; vab_i *= coef; va_r *= coef; va_i *= coef;
+        }
+        const int r0 = m - 1 - base;
+        if (r0 >= 0 && r0 < NTFF_CH) { atomicAdd(&lds[arr[w]][r0][0], vb_r); atomicAdd(&lds[arr[w]][r0][1], vb_i); }
+        if (r0 + 1 >= 0 && r0 + 1 < NTFF_CH) { atomicAdd(&lds[arr[w]][r0 + 1][0], vab_r); atomicAdd(&lds[arr[w]][r0 + 1][1], vab_i); }
+        if (r0 + 2 >= 0 && r0 + 2 < NTFF_CH) { atomicAdd(&lds[arr[w]][r0 + 2][0], -va_r); atomicAdd(&lds[arr[w]][r0 + 2][1], -va_i); }
+      }
+    }
+    __syncthreads();
+    const int nb = min(NTFF_CH, hi - base + 1);
+    for (int q = threadIdx.x; q < 3 * nb; q += 256) {
+      const int a = q / nb, r = q % nb;
+      const double re = lds[a][r][0], im = lds[a][r][1];
+      if (re == 0.0 && im == 0.0) continue;
+      const int m = base + r;
+      // taps outside the [360][arraySize] allocation (+ guard) are dropped; the
+      // reference writes them out of bounds on strongly non-square grids
+      const long gidx = stp + m;
+      if (gidx < -NT_GUARD_DEV || gidx >= 360L * arraySize + NT_GUARD_DEV) continue;
+      double2 *dst = U[a] + gidx;
+      // bins near a row end are shared with the neighbouring angle (the flat
+      // [360][arraySize] layout lets m-1 / m+1 spill over, Q10): atomics there
+      if (m < 3 || m >= arraySize - 3) {
+        atomicAdd(&dst->x, re);
+        atomicAdd(&dst->y, im);
+      } else {
+        double2 v = *dst;
+        v.x += re;
+        v.y += im;
+        *dst = v;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------- far field
+
+__device__ __forceinline__ void cmul_acc(double2 &acc, const double2 a, double cs, double sn, double sgn)
+{
+  // acc += sgn * (a * (cs + i sn)), complex product (ac - bd, ad + bc)
+  const double re = a.x * cs - a.y * sn;
+  const double im = a.x * sn + a.y * cs;
+  acc.x += sgn * re;
+  acc.y += sgn * im;
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
+__device__ double block_sum(double v, double *red)
+{
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wv] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) r += red[w];
+  return r;
+}
+
+struct FFArgs {
+  int N_PX, N_PY, NPXa, NPYa, off;
+  int top, bottom, left, right;
+  double cx, cy, k, z0, coef_re, coef_im, scale;
+  const double *cosA, *sinA;
+};
+
+// ntffTM_Frequency (ntffTM.c:72-158) on the physical cells of either layout
+__global__ __launch_bounds__(256) void k_far_tm(const double2 *Ez, const double2 *Hx, const double2 *Hy, FFArgs a, double2 *out)
+{
+  __shared__ double red[8];
+  const int ang = blockIdx.x;
+  const double rx = a.cosA[ang], ry = a.sinA[ang];
+  const int NPY = a.NPYa;
+  auto idx = [&](int px, int py) { return (size_t)(px + a.off) * NPY + (py + a.off); };
+  double2 Nz = {0, 0}, Lx = {0, 0}, Ly = {0, 0};
+  const int nx = a.right - a.left, ny = a.top - a.bottom;
+  for (int c = threadIdx.x; c < 2 * nx + 2 * ny; c += blockDim.x) {
+    int side, px, py;
+    if (c < nx) { side = 0; px = a.left + c; py = a.bottom; }
+    else if (c < nx + ny) { side = 1; px = a.right; py = a.bottom + (c - nx); }
+    else if (c < 2 * nx + ny) { side = 2; px = a.left + (c - nx - ny); py = a.top; }
+    else { side = 3; px = a.left; py = a.bottom + (c - 2 * nx - ny); }
+    const double r2x = px - a.cx, r2y = py - a.cy;
+    const double ip = rx * r2x + ry * r2y;
+    double sn, cs;
+    sincos(a.k * ip, &sn, &cs);
+    const size_t kk = idx(px, py);
+    const double2 ez = Ez[kk];
+    double2 hv;
+    if (side == 0 || side == 2) {
+      const double2 h0 = Hx[kk], h1 = Hx[kk - 1];
+      hv.x = 0.5 * (h0.x + h1.x);
+      hv.y = 0.5 * (h0.y + h1.y);
+    } else {
+      const double2 h0 = Hy[kk], h1 = Hy[kk - NPY];
+      hv.x = 0.5 * (h0.x + h1.x);
+      hv.y = 0.5 * (h0.y + h1.y);
+    }
+    const double sg = (side < 2) ? 1.0 : -1.0;
+    cmul_acc(Nz, hv, cs, sn, sg);
+    if (side == 0 || side == 2) cmul_acc(Lx, ez, cs, sn, sg);
+    else cmul_acc(Ly, ez, cs, sn, sg);
+  }
+  double v[6] = {Nz.x, Nz.y, Lx.x, Lx.y, Ly.x, Ly.y};
+  for (int q = 0; q < 6; q++) v[q] = block_sum(v[q], red);
+  if (threadIdx.x == 0) {
+    const double sn = ry, cs = rx;
+    // Lphi = -Lx sin + Ly cos ; E = coef (Z0 Nz + Lphi) sqrt(h)
+    const double lpr = -v[2] * sn + v[4] * cs, lpi = -v[3] * sn + v[5] * cs;
+    const double tr = a.z0 * v[0] + lpr, ti = a.z0 * v[1] + lpi;
+    double2 r;
+    r.x = (a.coef_re * tr - a.coef_im * ti) * a.scale;
+    r.y = (a.coef_re * ti + a.coef_im * tr) * a.scale;
+    out[ang] = r;
+  }
+}
+
+// single-rank ntffTE_FrequencySplit (ntffTE.c:341-470) in padded coordinates
+__global__ __launch_bounds__(256) void k_far_te(const double2 *Ex, const double2 *Ey, const double2 *Hz, FFArgs a, double2 *out)
+{
+  __shared__ double red[8];
+  const int ang = blockIdx.x;
+  const double rx = a.cosA[ang], ry = a.sinA[ang];
+  const int SPX = a.N_PX + 2, SPY = a.N_PY + 2;
+  // padded (pi, pj) -> value in the solver layout; the ghost ring reads 0
+  auto val = [&](const double2 *f, int pi, int pj) -> double2 {
+    const int ai = pi - 1 + a.off, aj = pj - 1 + a.off;
+    if (ai < 0 || aj < 0 || ai >= a.NPXa || aj >= a.NPYa) return make_double2(0.0, 0.0);
+    return f[(size_t)ai * a.NPYa + aj];
+  };
+  const int L = max(1, a.left), Rx = min(SPX - 1, a.right);
+  const int B = max(1, a.bottom), T = min(SPY - 1, a.top);
+  const int nb = (0 < a.bottom && a.bottom < SPY - 1) ? max(0, Rx - L) : 0;
+  const int nr = (0 < a.right && a.right < SPX - 1) ? max(0, T - B) : 0;
+  const int nt = (0 < a.top && a.top < SPY - 1) ? max(0, Rx - L) : 0;
+  const int nl = (0 < a.left && a.left < SPX - 1) ? max(0, T - B) : 0;
+  double2 Nx = {0, 0}, Ny = {0, 0}, Lz = {0, 0};
+  for (int c = threadIdx.x; c < nb + nr + nt + nl; c += blockDim.x) {
+    int side, pi, pj;
+    double r2x, r2y;
+    if (c < nb) { side = 0; pi = L + c; pj = a.bottom; r2x = pi - a.cx + 0.5; r2y = pj - a.cy; }
+    else if (c < nb + nr) { side = 1; pi = a.right; pj = B + (c - nb); r2x = pi - a.cx; r2y = pj - a.cy + 0.5; }
+    else if (c < nb + nr + nt) { side = 2; pi = L + (c - nb - nr); pj = a.top; r2x = pi - a.cx + 0.5; r2y = pj - a.cy; }
+    else { side = 3; pi = a.left; pj = B + (c - nb - nr - nt); r2x = pi - a.cx; r2y = pj - a.cy + 0.5; }
+    const double ip = rx * r2x + ry * r2y;
+    double sn, cs;
+    sincos(a.k * ip, &sn, &cs);
+    const double2 h0 = val(Hz, pi, pj);
+    const double2 h1 = (side == 0 || side == 2) ? val(Hz, pi, pj - 1) : val(Hz, pi - 1, pj);
+    double2 hv;
+    hv.x = 0.5 * (h0.x + h1.x);
+    hv.y = 0.5 * (h0.y + h1.y);
+    const double2 ev = (side == 0 || side == 2) ? val(Ex, pi, pj) : val(Ey, pi, pj);
+    const double sg = (side < 2) ? -1.0 : 1.0;
+    if (side == 0 || side == 2) cmul_acc(Nx, hv, cs, sn, sg);
+    else cmul_acc(Ny, hv, cs, sn, sg);
+    cmul_acc(Lz, ev, cs, sn, sg);
+  }
+  double v[6] = {Nx.x, Nx.y, Ny.x, Ny.y, Lz.x, Lz.y};
+  for (int q = 0; q < 6; q++) v[q] = block_sum(v[q], red);
+  if (threadIdx.x == 0) {
+    const double sn = ry, cs = rx;
+    const double npr = -v[0] * sn + v[2] * cs, npi = -v[1] * sn + v[3] * cs;
+    const double tr = a.z0 * npr - v[4], ti = a.z0 * npi - v[5];
+    double2 r;
+    r.x = (a.coef_re * tr - a.coef_im * ti) * a.scale;
+    r.y = (a.coef_re * ti + a.coef_im * tr) * a.scale;
+    out[ang] = r;
+  }
+}
+
+// ---------------------------------------------------------------- time-domain far field + spectra
+
+#define NTFF_NUM 8192
+
+__device__ __forceinline__ int bitrev13(int v) { return (int)(__brev((unsigned)v) >> 19); }
+
+// TimeTranslate + 8192-point FFT (fft1x, flag 0) + |X|^2 interpolation, one
+// workgroup per angle, the transform held in LDS (128 KiB)
+__global__ __launch_bounds__(1024) void k_time_spectra(const double2 *U0, const double2 *U1, const double2 *U2, int arraySize,
+                                                       int maxTime, int tm, double z0, double coef_re, double coef_im,
+                                                       const double *cosA, const double *sinA, const double2 *tw,
+                                                       double c0h, double *out)
+{
+  extern __shared__ double2 buf[];
+  const int ang = blockIdx.x;
+  const long k0 = (long)ang * arraySize;
+  const double cphi = cosA[ang], sphi = sinA[ang];
+  const int ncopy = min(maxTime, NTFF_NUM);
+  for (int i = threadIdx.x; i < NTFF_NUM; i += blockDim.x) {
+    double2 v = make_double2(0.0, 0.0);
+    if (i < ncopy) {
+      double tr, ti;
+      if (tm) {
+        // Eth = coef (-Z0 WTH - UPH), WTH = -Wz, UPH = -Ux sin + Uy cos
+        const double2 ux = U0[k0 + i], uy = U1[k0 + i], wz = U2[k0 + i];
+        const double wthr = wz.x * -1.0, wthi = wz.y * -1.0;
+        const double uphr = ux.x * -sphi + uy.x * cphi, uphi = ux.y * -sphi + uy.y * cphi;
+        tr = -z0 * wthr - uphr;
+        ti = -z0 * wthi - uphi;
+      } else {
+        // Eph = coef (-Z0 WPH + UTH), WPH = -Wx sin + Wy cos, UTH = -Uz
+        const double2 wx = U0[k0 + i], wy = U1[k0 + i], uz = U2[k0 + i];
+        const double wphr = wx.x * -sphi + wy.x * cphi, wphi = wx.y * -sphi + wy.y * cphi;
+        const double uthr = uz.x * -1.0, uthi = uz.y * -1.0;
+        tr = -z0 * wphr + uthr;
+        ti = -z0 * wphi + uthi;
+      }
+      v.x = coef_re * tr - coef_im * ti;
+      v.y = coef_re * ti + coef_im * tr;
+    }
+    buf[i] = v;
+  }
+  __syncthreads();
+  // radix-2 decimation in frequency, twiddle e^{+i pi k / xp2} = tw[k * 4096/xp2]
+  for (int xp2 = NTFF_NUM / 2, sh = 0; xp2 >= 1; xp2 >>= 1, sh++) {
+    const int xp = xp2 * 2;
+    for (int bI = threadIdx.x; bI < NTFF_NUM / 2; bI += blockDim.x) {
+      const int kk = bI & (xp2 - 1);
+      const int base = (bI / xp2) * xp + kk;
+      const double2 p = buf[base], q = buf[base + xp2];
+      const double2 w = tw[kk << sh];
+      const double tr = p.x - q.x, ti = p.y - q.y;
+      buf[base] = make_double2(p.x + q.x, p.y + q.y);
+      buf[base + xp2] = make_double2(tr * w.x - ti * w.y, tr * w.y + ti * w.x);
+    }
+    __syncthreads();
+  }
+  for (int l = threadIdx.x; l <= 320; l += blockDim.x) {
+    const int lambda_nm = 380 + l;
+    double p = c0h * NTFF_NUM / lambda_nm;
+    const int index = (int)floor(p);
+    p = p - index;
+    if (index < 0 || index + 1 >= NTFF_NUM) {  // beyond the transform (the reference reads past its buffer)
+      out[(size_t)l * 360 + ang] = __builtin_nan("");
+      continue;
+    }
+    const double2 e0 = buf[bitrev13(index)], e1 = buf[bitrev13(index + 1)];
+    const double n0 = e0.x * e0.x + e0.y * e0.y, n1 = e1.x * e1.x + e1.y * e1.y;
+    out[(size_t)l * 360 + ang] = ((1 - p) * n0 + p * n1) / NTFF_NUM;
+  }
+}
+
+// ---------------------------------------------------------------- eps fill
+
+__global__ __launch_bounds__(256) void k_fill_eps(const fdtd_model_state *m, double *e0, double *e1, double *e2,
+                                                  int NPXa, int NPYa, int off, int tm, int plain)
+{
+  const int j = blockIdx.x * 64 + threadIdx.x;
+  const int i = blockIdx.y * 4 + threadIdx.y;
+  if (i >= NPXa || j >= NPYa) return;
+  const size_t k = (size_t)i * NPYa + j;
+  if (off && (i == 0 || j == 0 || i == NPXa - 1 || j == NPYa - 1)) {
+    e0[k] = 0.0; e1[k] = 0.0; e2[k] = 0.0;   // ghost ring (never read by the updates)
+    return;
+  }
+  const double x = i - off, y = j - off;
+  if (tm) {
+    e0[k] = fdtd_model_eps(m, x, y, 1, 1);
+    if (plain) {  // fdtdTM_upml.c:238-239 samples D_XY (Q19)
+      e1[k] = fdtd_model_eps(m, x, y + 0.5, 1, 1);
+      e2[k] = fdtd_model_eps(m, x + 0.5, y, 1, 1);
+    } else {
+      e1[k] = fdtd_model_eps(m, x, y + 0.5, 0, 1);
+      e2[k] = fdtd_model_eps(m, x + 0.5, y, 1, 0);
+    }
+  } else {
+    e0[k] = fdtd_model_eps(m, x + 0.5, y, 0, 1);
+    e1[k] = fdtd_model_eps(m, x, y + 0.5, 1, 0);
+    e2[k] = 0.5 * (fdtd_model_eps(m, x + 0.5, y + 0.5, 1, 0) + fdtd_model_eps(m, x + 0.5, y + 0.5, 0, 1));
+  }
+}
+
+// ---------------------------------------------------------------- host side
+
+#define HIPCHK(x)                                                                            \
+  do {                                                                                       \
+    hipError_t e_ = (x);                                                                     \
+    if (e_ != hipSuccess) return fdtd_fail(FDTD_EHIP, "%s failed: %s", #x, hipGetErrorString(e_)); \
+  } while (0)
+
+struct fdtd_grid {
+  fdtd_grid_desc d;
+  int NPXa, NPYa, off;
+  size_t ncell;
+  double2 *F[9];
+  double2 **Fdev;  // device copy of F[] for the gather kernel
+  double *eps[3];
+  double *tab;
+  Tabs tabs;
+  // NTFF
+  Contour ct;
+  double2 *ntb[3];
+  double2 *nt[3];
+  size_t ntn;
+  double2 *cE, *cH;
+  double *tsTab, *tsMin, *tsMax;
+  double *cosA, *sinA, *cosB, *sinB;
+  double coef;
+  double2 *ffout;
+  double *spec;
+  double2 *tw;
+  fdtd_model_state *mstate;
+};
+
+#define NT_GUARD NT_GUARD_DEV
+
+// contour segments of the per-step NTFF in array coordinates
+static void build_contour(fdtd_grid *g)
+{
+  const fdtd_grid_desc &d = g->d;
+  const int NPY = g->NPYa;
+  const bool tm = d.mode == FDTD_MODE_TM, mpi = d.flavour == FDTD_FLAVOUR_MPI;
+  Contour &ct = g->ct;
+  memset(&ct, 0, sizeof ct);
+  const int tp = d.ntff_top, bm = d.ntff_bottom, rt = d.ntff_right, lt = d.ntff_left;
+  // field slots: TM Ez 0, Hx 3, Hy 6; TE Ex 0, Ey 3, Hz 6
+  if (mpi) {
+    const int SPX = g->NPXa, SPY = g->NPYa;
+    const int cx = d.N_PX / 2, cy = d.N_PY / 2;
+    const int L = std::max(1, lt), R = std::min(SPX, rt), B = std::max(1, bm), T = std::min(SPY, tp);
+    const double hx0 = tm ? 0.0 : 0.5, hy0 = tm ? 0.0 : 0.5;
+    Seg s[4];
+    memset(s, 0, sizeof s);
+    // bottom (i = L..R-1, j = bm)
+    s[0].n = (0 < bm && bm < SPY - 1) ? std::max(0, R - L) : 0;
+    s[0].k0 = (long)L * NPY + bm; s[0].dk = NPY;
+    s[0].r2x0 = L - cx + hx0; s[0].r2y0 = bm - cy; s[0].dr2x = 1; s[0].dr2y = 0;
+    // right (i = rt, j = B..T-1)
+    s[1].n = (0 < rt && rt < SPX - 1) ? std::max(0, T - B) : 0;
+    s[1].k0 = (long)rt * NPY + B; s[1].dk = 1;
+    s[1].r2x0 = rt - cx; s[1].r2y0 = B - cy + hy0; s[1].dr2x = 0; s[1].dr2y = 1;
+    // top
+    s[2].n = (0 < tp && tp < SPY - 1) ? std::max(0, R - L) : 0;
+    s[2].k0 = (long)L * NPY + tp; s[2].dk = NPY;
+    s[2].r2x0 = L - cx + hx0; s[2].r2y0 = tp - cy; s[2].dr2x = 1; s[2].dr2y = 0;
+    // left
+    s[3].n = (0 < lt && lt < SPX) ? std::max(0, T - B) : 0;
+    s[3].k0 = (long)lt * NPY + B; s[3].dk = 1;
+    s[3].r2x0 = lt - cx; s[3].r2y0 = B - cy + hy0; s[3].dr2x = 0; s[3].dr2y = 1;
+    for (int q = 0; q < 4; q++) ct.s[q] = s[q];
+  } else {
+    const int cx = d.ntff_cx, cy = d.ntff_cy;
+    Seg s[4];
+    memset(s, 0, sizeof s);
+    const double hx0 = tm ? 0.0 : 0.5, hy0 = tm ? 0.0 : 0.5;
+    s[0].n = rt - lt; s[0].k0 = (long)lt * NPY + bm; s[0].dk = NPY;
+    s[0].r2x0 = lt - cx + hx0; s[0].r2y0 = bm - cy; s[0].dr2x = 1; s[0].dr2y = 0;
+    s[1].n = tp - bm; s[1].k0 = (long)rt * NPY + bm; s[1].dk = 1;
+    s[1].r2x0 = rt - cx; s[1].r2y0 = bm - cy + hy0; s[1].dr2x = 0; s[1].dr2y = 1;
+    s[2].n = rt - lt; s[2].k0 = (long)lt * NPY + tp; s[2].dk = NPY;
+    s[2].r2x0 = lt - cx + hx0; s[2].r2y0 = tp - cy; s[2].dr2x = 1; s[2].dr2y = 0;
+    s[3].n = tp - bm; s[3].k0 = (long)lt * NPY + bm; s[3].dk = 1;
+    s[3].r2x0 = lt - cx; s[3].r2y0 = bm - cy + hy0; s[3].dr2x = 0; s[3].dr2y = 1;
+    for (int q = 0; q < 4; q++) ct.s[q] = s[q];
+  }
+  // field roles and signs
+  for (int q = 0; q < 4; q++) {
+    Seg &s = ct.s[q];
+    const bool horiz = (q == 0 || q == 2);
+    const double sgn = tm ? (q < 2 ? 1.0 : -1.0) : (q < 2 ? -1.0 : 1.0);
+    if (tm) {
+      s.ef = 0;
+      s.hf = horiz ? 3 : 6;
+      s.ua = horiz ? 0 : 1;  // Ux / Uy
+      s.wa = 2;              // Wz
+    } else {
+      s.ef = horiz ? 0 : 3;
+      s.hf = 6;
+      s.ua = 2;              // Uz
+      s.wa = horiz ? 0 : 1;  // Wx / Wy
+    }
+    s.hnb = horiz ? -1 : -NPY;
+    s.es = sgn;
+    s.hs = 0.5 * sgn;
+  }
+  int off = 0;
+  for (int q = 0; q < 4; q++) {
+    ct.s[q].off = off;
+    off += ct.s[q].n;
+  }
+  ct.total = off;
+}
+
+// retarded-time shift of every (angle, contour cell), formed on the host with
+// the reference's own arithmetic: direct per cell for the MPI solvers
+// (mpiTM_UPML.c:894), accumulated along each side for the plain ones (ntffTM.c:329-334)
+// The reference forms the angle as ang*M_PI/180.0 in the MPI NTFF, the far
+// field and ntffTE_TimeTranslate (angle table A), but as ang*ToRad with
+// ToRad = M_PI/180.0 in ntffTM_TimeCalc / ntffTE_TimeCalc / ntffTM_TimeTranslate
+// (table B, ntffTM.c:314-318, ntffTE.c:89-92, ntffTM.c:169-173); the two differ in the last bit.
+static void build_time_shift(const fdtd_grid *g, std::vector<double> &tab, std::vector<double> &mn, std::vector<double> &mx,
+                             std::vector<double> &ca, std::vector<double> &sa, std::vector<double> &cb, std::vector<double> &sb)
+{
+  const fdtd_grid_desc &d = g->d;
+  const Contour &ct = g->ct;
+  const bool mpi = d.flavour == FDTD_FLAVOUR_MPI;
+  tab.assign((size_t)360 * ct.total, 0.0);
+  mn.assign(360, 0.0);
+  mx.assign(360, 0.0);
+  ca.resize(360);
+  sa.resize(360);
+  cb.resize(360);
+  sb.resize(360);
+  const double ToRad = M_PI / 180.0;
+  for (int ang = 0; ang < 360; ang++) {
+    const double radA = ang * M_PI / 180.0, radB = ang * ToRad;
+    ca[ang] = cos(radA);
+    sa[ang] = sin(radA);
+    cb[ang] = cos(radB);
+    sb[ang] = sin(radB);
+    const double r1x = mpi ? ca[ang] : cb[ang], r1y = mpi ? sa[ang] : sb[ang];
+    double *t = &tab[(size_t)ang * ct.total];
+    for (int q = 0; q < 4; q++) {
+      const Seg &s = ct.s[q];
+      if (mpi) {
+        for (int c = 0; c < s.n; c++) {
+          const double r2x = s.r2x0 + c * s.dr2x, r2y = s.r2y0 + c * s.dr2y;
+          t[s.off + c] = -(r1x * r2x + r1y * r2y) / d.c0 + d.rf_per_c;
+        }
+      } else {
+        const double r1x_per_c = r1x / d.c0, r1y_per_c = r1y / d.c0;
+        double ts = -(r1x_per_c * s.r2x0 + r1y_per_c * s.r2y0) + d.rf_per_c;
+        const double step = (q == 0 || q == 2) ? r1x_per_c : r1y_per_c;
+        for (int c = 0; c < s.n; c++) {
+          t[s.off + c] = ts;
+          ts -= step;
+        }
+      }
+    }
+    double lo = 1e300, hi = -1e300;
+    for (int p = 0; p < ct.total; p++) {
+      lo = std::min(lo, t[p]);
+      hi = std::max(hi, t[p]);
+    }
+    mn[ang] = ct.total ? lo : 0.0;
+    mx[ang] = ct.total ? hi : 0.0;
+  }
+}
+
+extern "C" int fdtd_hip_grid_create(const fdtd_grid_desc *d, fdtd_grid **out)
+{
+  if (!d || !out) return fdtd_fail(FDTD_EINVAL, "null argument");
+  if (d->N_PX < 4 || d->N_PY < 4) return fdtd_fail(FDTD_EINVAL, "grid %dx%d too small", d->N_PX, d->N_PY);
+  if (d->precision != FDTD_FP64) return fdtd_fail(FDTD_EINVAL, "fdtd_grid implements the FP64 reference-order path only");
+  if (d->mode != FDTD_MODE_TM && d->mode != FDTD_MODE_TE) return fdtd_fail(FDTD_EINVAL, "bad mode %d", d->mode);
+  if (d->flavour != FDTD_FLAVOUR_MPI && d->flavour != FDTD_FLAVOUR_PLAIN) return fdtd_fail(FDTD_EINVAL, "bad flavour");
+  if (d->ntff_left < 1 || d->ntff_bottom < 1 || d->ntff_right >= d->N_PX || d->ntff_top >= d->N_PY ||
+      d->ntff_left >= d->ntff_right || d->ntff_bottom >= d->ntff_top)
+    return fdtd_fail(FDTD_EINVAL, "NTFF contour (%d..%d x %d..%d) outside the %dx%d grid", d->ntff_left, d->ntff_right,
+                     d->ntff_bottom, d->ntff_top, d->N_PX, d->N_PY);
+  if (d->array_size < d->max_time || d->max_time < 0) return fdtd_fail(FDTD_EINVAL, "bad NTFF array size");
+  if (fdtd_hip_device_count() < 1) return fdtd_fail(FDTD_EHIP, "no HIP device");
+  fdtd_grid *g = new fdtd_grid();
+  memset((void *)g, 0, sizeof(*g));
+  g->d = *d;
+  g->off = d->flavour == FDTD_FLAVOUR_MPI ? 1 : 0;
+  g->NPXa = d->N_PX + 2 * g->off;
+  g->NPYa = d->N_PY + 2 * g->off;
+  g->ncell = (size_t)g->NPXa * g->NPYa;
+  *out = g;
+  for (int f = 0; f < 9; f++) {
+    HIPCHK(hipMalloc(&g->F[f], g->ncell * sizeof(double2)));
+    HIPCHK(hipMemset(g->F[f], 0, g->ncell * sizeof(double2)));
+  }
+  HIPCHK(hipMalloc(&g->Fdev, 9 * sizeof(double2 *)));
+  HIPCHK(hipMemcpy(g->Fdev, g->F, 9 * sizeof(double2 *), hipMemcpyHostToDevice));
+  for (int e = 0; e < 3; e++) {
+    HIPCHK(hipMalloc(&g->eps[e], g->ncell * sizeof(double)));
+    HIPCHK(hipMemset(g->eps[e], 0, g->ncell * sizeof(double)));
+  }
+  const size_t ntab = 6 * (size_t)d->N_PX + 5 * (size_t)d->N_PY;
+  HIPCHK(hipMalloc(&g->tab, ntab * sizeof(double)));
+  HIPCHK(hipMemset(g->tab, 0, ntab * sizeof(double)));
+  for (int q = 0; q < 5; q++) g->tabs.x[q] = g->tab + q * (size_t)d->N_PX;
+  g->tabs.sx = g->tab + 5 * (size_t)d->N_PX;
+  for (int q = 0; q < 4; q++) g->tabs.y[q] = g->tab + 6 * (size_t)d->N_PX + q * (size_t)d->N_PY;
+  g->tabs.sy = g->tab + 6 * (size_t)d->N_PX + 4 * (size_t)d->N_PY;
+  // NTFF
+  build_contour(g);
+  g->ntn = (size_t)360 * d->array_size + 2 * NT_GUARD;
+  for (int a = 0; a < 3; a++) {
+    HIPCHK(hipMalloc(&g->ntb[a], g->ntn * sizeof(double2)));
+    HIPCHK(hipMemset(g->ntb[a], 0, g->ntn * sizeof(double2)));
+    g->nt[a] = g->ntb[a] + NT_GUARD;
+  }
+  std::vector<double> tab, mn, mx, ca, sa, cb, sb;
+  build_time_shift(g, tab, mn, mx, ca, sa, cb, sb);
+  const int P = std::max(1, g->ct.total);
+  HIPCHK(hipMalloc(&g->cE, P * sizeof(double2)));
+  HIPCHK(hipMalloc(&g->cH, P * sizeof(double2)));
+  HIPCHK(hipMalloc(&g->tsTab, std::max<size_t>(1, tab.size()) * sizeof(double)));
+  if (!tab.empty()) HIPCHK(hipMemcpy(g->tsTab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc(&g->tsMin, 360 * sizeof(double)));
+  HIPCHK(hipMalloc(&g->tsMax, 360 * sizeof(double)));
+  HIPCHK(hipMalloc(&g->cosA, 360 * sizeof(double)));
+  HIPCHK(hipMalloc(&g->sinA, 360 * sizeof(double)));
+  HIPCHK(hipMemcpy(g->tsMin, mn.data(), 360 * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(g->tsMax, mx.data(), 360 * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(g->cosA, ca.data(), 360 * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(g->sinA, sa.data(), 360 * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc(&g->cosB, 360 * sizeof(double)));
+  HIPCHK(hipMalloc(&g->sinB, 360 * sizeof(double)));
+  HIPCHK(hipMemcpy(g->cosB, cb.data(), 360 * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(g->sinB, sb.data(), 360 * sizeof(double), hipMemcpyHostToDevice));
+  // 1/(4 pi c R), R = 1e6, applied per step by the MPI solvers (mpiTM_UPML.c:852-854)
+  g->coef = 1.0 / (4 * M_PI * d->c0 * 1.0e6);
+  HIPCHK(hipMalloc(&g->ffout, 360 * sizeof(double2)));
+  HIPCHK(hipMalloc(&g->spec, 321 * 360 * sizeof(double)));
+  HIPCHK(hipMalloc(&g->tw, 4096 * sizeof(double2)));
+  HIPCHK(hipMalloc(&g->mstate, sizeof(fdtd_model_state)));
+  return FDTD_OK;
+}
+
+extern "C" int fdtd_hip_grid_destroy(fdtd_grid *g)
+{
+  if (!g) return FDTD_OK;
+  for (int f = 0; f < 9; f++) (void)hipFree(g->F[f]);
+  (void)hipFree(g->Fdev);
+  for (int e = 0; e < 3; e++) (void)hipFree(g->eps[e]);
+  (void)hipFree(g->tab);
+  for (int a = 0; a < 3; a++) (void)hipFree(g->ntb[a]);
+  (void)hipFree(g->cE);
+  (void)hipFree(g->cH);
+  (void)hipFree(g->tsTab);
+  (void)hipFree(g->tsMin);
+  (void)hipFree(g->tsMax);
+  (void)hipFree(g->cosA);
+  (void)hipFree(g->sinA);
+  (void)hipFree(g->cosB);
+  (void)hipFree(g->sinB);
+  (void)hipFree(g->ffout);
+  (void)hipFree(g->spec);
+  (void)hipFree(g->tw);
+  (void)hipFree(g->mstate);
+  delete g;
+  return FDTD_OK;
+}
+
+extern "C" int fdtd_hip_grid_set_coefs(fdtd_grid *g, const fdtd_coef_tables *t)
+{
+  if (!g || !t) return fdtd_fail(FDTD_EINVAL, "null argument");
+  const size_t nx = g->d.N_PX, ny = g->d.N_PY;
+  for (int q = 0; q < 5; q++) {
+    if (!t->x[q]) return fdtd_fail(FDTD_EINVAL, "missing x table %d", q);
+    HIPCHK(hipMemcpy((void *)g->tabs.x[q], t->x[q], nx * sizeof(double), hipMemcpyHostToDevice));
+  }
+  for (int q = 0; q < 4; q++) {
+    if (!t->y[q]) return fdtd_fail(FDTD_EINVAL, "missing y table %d", q);
+    HIPCHK(hipMemcpy((void *)g->tabs.y[q], t->y[q], ny * sizeof(double), hipMemcpyHostToDevice));
+  }
+  if (!t->sx || !t->sy) return fdtd_fail(FDTD_EINVAL, "missing sigma tables");
+  HIPCHK(hipMemcpy((void *)g->tabs.sx, t->sx, nx * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy((void *)g->tabs.sy, t->sy, ny * sizeof(double), hipMemcpyHostToDevice));
+  return FDTD_OK;
+}
+
+extern "C" int fdtd_hip_grid_fill_eps(fdtd_grid *g, const fdtd_model_state *m)
+{
+  if (!g || !m) return fdtd_fail(FDTD_EINVAL, "null argument");
+  HIPCHK(hipMemcpy(g->mstate, m, sizeof(*m), hipMemcpyHostToDevice));
+  dim3 blk(64, 4), grd((g->NPYa + 63) / 64, (g->NPXa + 3) / 4);
+  hipLaunchKernelGGL(k_fill_eps, grd, blk, 0, 0, g->mstate, g->eps[0], g->eps[1], g->eps[2], g->NPXa, g->NPYa, g->off,
+                     g->d.mode == FDTD_MODE_TM ? 1 : 0, g->d.flavour == FDTD_FLAVOUR_PLAIN ? 1 : 0);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipDeviceSynchronize());
+  return FDTD_OK;
+}
+
+extern "C" int fdtd_hip_grid_set_eps(fdtd_grid *g, int which, const double *host)
+{
+  if (!g || !host || which < 0 || which > 2) return fdtd_fail(FDTD_EINVAL, "bad argument");
+  HIPCHK(hipMemcpy(g->eps[which], host, g->ncell * sizeof(double), hipMemcpyHostToDevice));
+  return FDTD_OK;
+}
+
+extern "C" int fdtd_hip_grid_get_eps(fdtd_grid *g, int which, double *host)
+{
+  if (!g || !host || which < 0 || which > 2) return fdtd_fail(FDTD_EINVAL, "bad argument");
+  HIPCHK(hipMemcpy(host, g->eps[which], g->ncell * sizeof(double), hipMemcpyDeviceToHost));
+  return FDTD_OK;
+}
+
+extern "C" int fdtd_hip_grid_step(fdtd_grid *g, const fdtd_step_params *p)
+{
+  if (!g || !p) return fdtd_fail(FDTD_EINVAL, "null argument");
+  const fdtd_grid_desc &d = g->d;
+  GridPtrs gp;
+  for (int f = 0; f < 9; f++) gp.F[f] = g->F[f];
+  for (int e = 0; e < 3; e++) gp.eps[e] = g->eps[e];
+  gp.NPXa = g->NPXa;
+  gp.NPYa = g->NPYa;
+  gp.off = g->off;
+  gp.mu0 = d.mu0;
+  StepArgs s;
+  memset(&s, 0, sizeof s);
+  const bool plain = d.flavour == FDTD_FLAVOUR_PLAIN;
+  const double rad = p->angle_deg * M_PI / 180.0;
+  s.w_t = p->omega * p->time;
+  s.ray_coef = p->ray_coef;
+  s.ks_cos = cos(rad) * p->k;
+  s.ks_sin = sin(rad) * p->k;
+  s.ps_on = d.point_source && d.mode == FDTD_MODE_TM && !plain;
+  s.ps_i = d.N_PX / 2 + g->off;
+  s.ps_j = d.N_PY / 2 + g->off;
+  s.ps_re = p->ps_re;
+  s.ps_im = p->ps_im;
+  if (plain) {
+    // field_scatteredPulse constants (field.c:226-241)
+    s.time = p->time;
+    s.w_s = p->omega;
+    s.cpc = cos(rad) / d.c0;
+    s.spc = sin(rad) / d.c0;
+    const double gx[2] = {d.mode == FDTD_MODE_TM ? 0.0 : 0.5, 0.0};
+    const double gy[2] = {0.0, d.mode == FDTD_MODE_TM ? 0.0 : 0.5};
+    for (int c = 0; c < 2; c++) {
+      s.gx[c] = gx[c];
+      s.gy[c] = gy[c];
+      const double center_peak = (d.N_PX / 2.0 + gx[c]) * s.cpc + (d.N_PY / 2 + gy[c]) * s.spc;
+      s.t0[c] = -center_peak + 500;
+    }
+    if (d.mode == FDTD_MODE_TM) {
+      s.dot[0] = 1.0;
+      s.on[0] = 1;
+    } else {
+      s.dot[0] = p->dot_x;
+      s.dot[1] = p->dot_y;
+      s.on[0] = p->use_x;
+      s.on[1] = p->use_y;
+    }
+  }
+  dim3 blk(64, 4), grd((g->NPYa - 2 + 63) / 64, (g->NPXa - 2 + 3) / 4);
+  if (d.mode == FDTD_MODE_TM) {
+    if (plain) {
+      hipLaunchKernelGGL(k_tm_h, grd, blk, 0, 0, gp, g->tabs);
+      hipLaunchKernelGGL(k_tm_e<1>, grd, blk, 0, 0, gp, g->tabs, s);
+    } else {
+      hipLaunchKernelGGL(k_tm_e<0>, grd, blk, 0, 0, gp, g->tabs, s);
+      hipLaunchKernelGGL(k_tm_h, grd, blk, 0, 0, gp, g->tabs);
+    }
+  } else {
+    if (plain) {
+      hipLaunchKernelGGL(k_te_h, grd, blk, 0, 0, gp, g->tabs);
+      hipLaunchKernelGGL(k_te_e<1>, grd, blk, 0, 0, gp, g->tabs, s);
+    } else {
+      hipLaunchKernelGGL(k_te_e<0>, grd, blk, 0, 0, gp, g->tabs, s);
+      hipLaunchKernelGGL(k_te_h, grd, blk, 0, 0, gp, g->tabs);
+    }
+  }
+  HIPCHK(hipGetLastError());
+  if (d.ntff_time && g->ct.total > 0) {
+    hipLaunchKernelGGL(k_ntff_gather, dim3((g->ct.total + 255) / 256), dim3(256), 0, 0, (const double2 *const *)g->Fdev,
+                       g->ct, g->cE, g->cH);
+    hipLaunchKernelGGL(k_ntff_bin, dim3(360), dim3(256), 0, 0, g->ct, g->cE, g->cH, g->tsTab, g->tsMin, g->tsMax, g->nt[0],
+                       g->nt[1], g->nt[2], d.array_size, p->time - 1, p->time - 0.5, g->coef, plain ? 0 : 1);
+    HIPCHK(hipGetLastError());
+  }
+  return FDTD_OK;
+}
+
+extern "C" int fdtd_hip_grid_get_field(fdtd_grid *g, int which, double *host)
+{
+  if (!g || !host || which < 0 || which > 8) return fdtd_fail(FDTD_EINVAL, "bad argument");
+  HIPCHK(hipMemcpy(host, g->F[which], g->ncell * sizeof(double2), hipMemcpyDeviceToHost));
+  return FDTD_OK;
+}
+
+extern "C" int fdtd_hip_grid_set_field(fdtd_grid *g, int which, const double *host)
+{
+  if (!g || !host || which < 0 || which > 8) return fdtd_fail(FDTD_EINVAL, "bad argument");
+  HIPCHK(hipMemcpy(g->F[which], host, g->ncell * sizeof(double2), hipMemcpyHostToDevice));
+  return FDTD_OK;
+}
+
+extern "C" int fdtd_hip_grid_get_ntff_time(fdtd_grid *g, int which, double *host)
+{
+  if (!g || !host || which < 0 || which > 2) return fdtd_fail(FDTD_EINVAL, "bad argument");
+  HIPCHK(hipMemcpy(host, g->nt[which], (size_t)360 * g->d.array_size * sizeof(double2), hipMemcpyDeviceToHost));
+  return FDTD_OK;
+}
+
+extern "C" int fdtd_hip_grid_reset(fdtd_grid *g)
+{
+  if (!g) return fdtd_fail(FDTD_EINVAL, "null argument");
+  for (int f = 0; f < 9; f++) HIPCHK(hipMemset(g->F[f], 0, g->ncell * sizeof(double2)));
+  for (int a = 0; a < 3; a++) HIPCHK(hipMemset(g->ntb[a], 0, g->ntn * sizeof(double2)));
+  HIPCHK(hipDeviceSynchronize());
+  return FDTD_OK;
+}
+
+extern "C" int fdtd_hip_grid_far_field(fdtd_grid *g, double k, double coef_re, double coef_im, double scale, double *out360)
+{
+  if (!g || !out360) return fdtd_fail(FDTD_EINVAL, "null argument");
+  const fdtd_grid_desc &d = g->d;
+  FFArgs a;
+  a.N_PX = d.N_PX; a.N_PY = d.N_PY; a.NPXa = g->NPXa; a.NPYa = g->NPYa; a.off = g->off;
+  a.top = d.ntff_top; a.bottom = d.ntff_bottom; a.left = d.ntff_left; a.right = d.ntff_right;
+  a.cx = d.ntff_cx; a.cy = d.ntff_cy; a.k = k; a.z0 = d.z0;
+  a.coef_re = coef_re; a.coef_im = coef_im; a.scale = scale;
+  a.cosA = g->cosA; a.sinA = g->sinA;
+  if (d.mode == FDTD_MODE_TM)
+    hipLaunchKernelGGL(k_far_tm, dim3(360), dim3(256), 0, 0, g->F[0], g->F[3], g->F[6], a, g->ffout);
+  else {
+    a.cx = d.N_PX / 2; a.cy = d.N_PY / 2;
+    hipLaunchKernelGGL(k_far_te, dim3(360), dim3(256), 0, 0, g->F[0], g->F[3], g->F[6], a, g->ffout);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpy(out360, g->ffout, 360 * sizeof(double2), hipMemcpyDeviceToHost));
+  return FDTD_OK;
+}
+
+extern "C" int fdtd_hip_grid_time_spectra(fdtd_grid *g, double coef_re, double coef_im, const double *twiddles, int h_u_nm,
+                                          double *out)
+{
+  if (!g || !twiddles || !out) return fdtd_fail(FDTD_EINVAL, "null argument");
+  const fdtd_grid_desc &d = g->d;
+  HIPCHK(hipMemcpy(g->tw, twiddles, 4096 * sizeof(double2), hipMemcpyHostToDevice));
+  const size_t lds = NTFF_NUM * sizeof(double2);
+  const bool tmPlain = d.mode == FDTD_MODE_TM && d.flavour == FDTD_FLAVOUR_PLAIN;
+  HIPCHK(hipFuncSetAttribute((const void *)k_time_spectra, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(k_time_spectra, dim3(360), dim3(1024), lds, 0, g->nt[0], g->nt[1], g->nt[2], d.array_size, d.max_time,
+                     d.mode == FDTD_MODE_TM ? 1 : 0, d.z0, coef_re, coef_im, tmPlain ? g->cosB : g->cosA,
+                     tmPlain ? g->sinB : g->sinA, g->tw, d.c0 * h_u_nm, g->spec);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpy(out, g->spec, 321 * 360 * sizeof(double), hipMemcpyDeviceToHost));
+  return FDTD_OK;
+}
+
+extern "C" int fdtd_hip_grid_dims(const fdtd_grid *g, int out[4])
+{
+  if (!g || !out) return fdtd_fail(FDTD_EINVAL, "null argument");
+  out[0] = g->NPXa;
+  out[1] = g->NPYa;
+  out[2] = g->off;
+  out[3] = g->d.array_size;
+  return FDTD_OK;
+}
+
+extern "C" int fdtd_hip_grid_sync(fdtd_grid *g)
+{
+  (void)g;
+  HIPCHK(hipDeviceSynchronize());
+  return FDTD_OK;
+}

@@ -1,0 +1,438 @@
+/*
+ * solvers.c -- the four UPML solvers' entry points on top of the HIP grid.
+ *
+ * Each solver keeps the reference's lifecycle (init -> update* -> reset /
+ * finish) and getter ABI:
+ *   MPI TM  mpiTM_UPML.c:88-250      MPI TE  mpiTE_UPML.c:72-335
+ *   TM      fdtdTM_upml.c:54-152     TE      fdtdTE_upml.c:101-192
+ * but the state lives in device memory.  setCoefficient (mpiTM_UPML.c:663-716,
+ * mpiTE_UPML.c:489-536, fdtdTM_upml.c:224-274, fdtdTE_upml.c:361-412) is
+ * restated as 1-D tables per axis, because every UPML coefficient depends on
+ * sigma_x(x) or sigma_y(y) alone except the two built on the device from both.
+ * Errors print the HIP message and exit(2), as the reference does on failure.
+ */
+#include <complex.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/stat.h>
+#include "fdtd_hip.h"
+#include "mpifdtd_field.h"
+#include "mpifdtd_models.h"
+#include "mpifdtd_simulator.h"
+#include "mpifdtd_solvers.h"
+
+#ifndef M_PI
+#define M_PI 3.1415926535897932384626433832795
+#endif
+
+enum { SLV_MPI_TM, SLV_MPI_TE, SLV_TM, SLV_TE, SLV_N };
+
+typedef struct {
+  fdtd_grid *g;
+  int NPXa, NPYa, off, arraySize;
+  double complex *mirror[3];  /* TM: Ez Hx Hy   TE: Ex Ey Hz */
+  double *eps_mirror;
+} solver_t;
+
+static solver_t slv_[SLV_N];
+static int active_ = -1;
+static int ntff_time_ = 1;
+static int file_output_ = 1;
+
+static const int slot_of_[3] = {0, 3, 6};
+
+static void die(const char *what)
+{
+  fprintf(stderr, "mpifdtd_amd: %s failed: %s\n", what, fdtd_hip_last_error());
+  exit(2);
+}
+#define CHECK(call) do { if ((call) != FDTD_OK) die(#call); } while (0)
+
+static int is_tm(int kind) { return kind == SLV_MPI_TM || kind == SLV_TM; }
+static int is_mpi(int kind) { return kind == SLV_MPI_TM || kind == SLV_MPI_TE; }
+
+void fdtd_solver_setNTFFTime(int on) { ntff_time_ = on ? 1 : 0; }
+void fdtd_solver_setFileOutput(int on) { file_output_ = on ? 1 : 0; }
+
+static void build_coefs(int kind, fdtd_grid *g)
+{
+  const int nx = N_PX, ny = N_PY;
+  const double R = 1.0e-8, M = 2.0, eps = 1.0, C = field_getLightSpeed();
+  double *buf = (double *)calloc((size_t)6 * nx + (size_t)5 * ny, sizeof(double));
+  double *X[5], *SX, *Y[4], *SY;
+  for (int q = 0; q < 5; q++) X[q] = buf + (size_t)q * nx;
+  SX = buf + (size_t)5 * nx;
+  for (int q = 0; q < 4; q++) Y[q] = buf + (size_t)6 * nx + (size_t)q * ny;
+  SY = buf + (size_t)6 * nx + (size_t)4 * ny;
+  const double sig_z = 0;
+  if (is_tm(kind)) {
+    const double sig_max = -(M + 1.0) * eps * C / 2.0 / N_PML / cos(M_PI / 3) * log(R);
+    for (int x = 0; x < nx; x++) {
+      double sig_ez_x = sig_max * field_sigmaX(x, 0);
+      double sig_hx_x = sig_max * field_sigmaX(x, 0.5);
+      double sig_hy_x = sig_max * field_sigmaX(x + 0.5, 0);
+      X[0][x] = (2 * eps - sig_ez_x) / (2 * eps + sig_ez_x);   /* C_JZ */
+      X[1][x] = (2 * eps) / (2 * eps + sig_ez_x);              /* C_JZHXHY */
+      X[2][x] = (2 * eps + sig_hx_x) / (2 * eps + sig_z);      /* C_BXMX1 */
+      X[3][x] = (2 * eps - sig_hx_x) / (2 * eps + sig_z);      /* C_BXMX0 */
+      X[4][x] = (2 * eps - sig_hy_x) / (2 * eps + sig_hy_x);   /* C_BY */
+      SX[x] = sig_hy_x;
+    }
+    for (int y = 0; y < ny; y++) {
+      double sig_ez_y = sig_max * field_sigmaY(0, y);
+      double sig_hx_y = sig_max * field_sigmaY(0, y + 0.5);
+      double sig_hy_y = sig_max * field_sigmaY(0.5, y);
+      Y[0][y] = (2 * eps - sig_ez_y) / (2 * eps + sig_ez_y);   /* C_DZ */
+      Y[1][y] = (2 * eps + sig_z) / (2 * eps + sig_ez_y);      /* C_DZJZ1 = C_DZJZ0 */
+      Y[2][y] = (2 * eps - sig_hx_y) / (2 * eps + sig_hx_y);   /* C_MX */
+      Y[3][y] = (2 * eps) / (2 * eps + sig_hx_y);              /* C_MXEZ */
+      SY[y] = sig_hy_y;
+    }
+  } else {
+    const double sig_max = -(M + 1.0) * eps * C / 2.0 / N_PML * log(R);
+    for (int x = 0; x < nx; x++) {
+      double sig_ex_x = sig_max * field_sigmaX(x + 0.5, 0);
+      double sig_ey_x = sig_max * field_sigmaX(x, 0.5);
+      double sig_hz_x = sig_max * field_sigmaX(x + 0.5, 0.5);
+      X[0][x] = (2 * eps + sig_ex_x) / (2 * eps + sig_z);      /* C_DXJX1 */
+      X[1][x] = (2 * eps - sig_ex_x) / (2 * eps + sig_z);      /* C_DXJX0 */
+      X[2][x] = (2 * eps - sig_ey_x) / (2 * eps + sig_ey_x);   /* C_DY */
+      X[3][x] = (2 * eps - sig_hz_x) / (2 * eps + sig_hz_x);   /* C_MZ */
+      X[4][x] = (2 * eps) / (2 * eps + sig_hz_x);              /* C_MZEXEY */
+      SX[x] = sig_ey_x;
+    }
+    for (int y = 0; y < ny; y++) {
+      double sig_ex_y = sig_max * field_sigmaY(0.5, y);
+      double sig_ey_y = sig_max * field_sigmaY(0, y + 0.5);
+      double sig_hz_y = sig_max * field_sigmaY(0.5, y + 0.5);
+      Y[0][y] = (2 * eps - sig_ex_y) / (2 * eps + sig_ex_y);   /* C_JX */
+      Y[1][y] = (2 * eps) / (2 * eps + sig_ex_y);              /* C_JXHZ */
+      Y[2][y] = (2 * eps - sig_hz_y) / (2 * eps + sig_hz_y);   /* C_BZ */
+      Y[3][y] = (2 * eps + sig_z) / (2 * eps + sig_hz_y);      /* C_BZMZ1 = C_BZMZ0 */
+      SY[y] = sig_ey_y;
+    }
+  }
+  fdtd_coef_tables t;
+  for (int q = 0; q < 5; q++) t.x[q] = X[q];
+  t.sx = SX;
+  for (int q = 0; q < 4; q++) t.y[q] = Y[q];
+  t.sy = SY;
+  CHECK(fdtd_hip_grid_set_coefs(g, &t));
+  free(buf);
+}
+
+static void solver_free(int kind)
+{
+  solver_t *s = &slv_[kind];
+  if (s->g) fdtd_hip_grid_destroy(s->g);
+  for (int q = 0; q < 3; q++) free(s->mirror[q]);
+  free(s->eps_mirror);
+  memset(s, 0, sizeof *s);
+  if (active_ == kind) active_ = -1;
+}
+
+static void solver_init(int kind)
+{
+  solver_t *s = &slv_[kind];
+  if (s->g) solver_free(kind);
+  FieldInfo_S fs = field_getFieldInfo_S();
+  NTFFInfo ni = field_getNTFFInfo();
+  fdtd_grid_desc d;
+  memset(&d, 0, sizeof d);
+  d.mode = is_tm(kind) ? FDTD_MODE_TM : FDTD_MODE_TE;
+  d.flavour = is_mpi(kind) ? FDTD_FLAVOUR_MPI : FDTD_FLAVOUR_PLAIN;
+  d.precision = FDTD_FP64;
+  d.N_PX = fs.N_PX;
+  d.N_PY = fs.N_PY;
+  d.N_PML = fs.N_PML;
+  d.ntff_top = ni.top;
+  d.ntff_bottom = ni.bottom;
+  d.ntff_left = ni.left;
+  d.ntff_right = ni.right;
+  d.ntff_cx = ni.cx;
+  d.ntff_cy = ni.cy;
+  d.rf_per_c = ni.RFperC;
+  d.array_size = ni.arraySize;
+  d.max_time = (int)field_getMaxTime();
+  d.c0 = field_getLightSpeed();
+  d.mu0 = field_getMu0();
+  d.z0 = field_getZ0();
+  d.eps0 = 1.0;
+  d.ntff_time = ntff_time_;
+  d.point_source = field_getPointSource();
+  CHECK(fdtd_hip_grid_create(&d, &s->g));
+  int dims[4];
+  CHECK(fdtd_hip_grid_dims(s->g, dims));
+  s->NPXa = dims[0];
+  s->NPYa = dims[1];
+  s->off = dims[2];
+  s->arraySize = dims[3];
+  build_coefs(kind, s->g);
+  CHECK(fdtd_hip_grid_fill_eps(s->g, models_getState()));
+  size_t n = (size_t)s->NPXa * s->NPYa;
+  for (int q = 0; q < 3; q++) s->mirror[q] = (double complex *)calloc(n, sizeof(double complex));
+  s->eps_mirror = (double *)calloc(n, sizeof(double));
+  active_ = kind;
+}
+
+static void solver_update(int kind)
+{
+  solver_t *s = &slv_[kind];
+  if (!s->g) { fprintf(stderr, "mpifdtd_amd: update before init\n"); exit(2); }
+  fdtd_step_params p;
+  memset(&p, 0, sizeof p);
+  p.time = field_getTime();
+  p.ray_coef = field_getRayCoef();
+  p.omega = field_getOmega();
+  p.k = field_getK();
+  p.angle_deg = field_getWaveAngle();
+  fdtd_dcomplex ps = field_pointLight();
+  p.ps_re = creal(ps);
+  p.ps_im = cimag(ps);
+  if (kind == SLV_TE) {
+    /* the pulse is projected on the direction 90 deg from the wave vector (fdtdTE_upml.c:182-189) */
+    WaveInfo_S w = field_getWaveInfo_S();
+    double co = cos((w.Angle_deg + 90) * M_PI / 180.0);
+    double si = sin((w.Angle_deg + 90) * M_PI / 180.0);
+    p.dot_x = co;
+    p.dot_y = si;
+    p.use_x = co != 0.0;
+    p.use_y = si != 0.0;
+  }
+  CHECK(fdtd_hip_grid_step(s->g, &p));
+}
+
+static double complex *solver_field(int kind, int q)
+{
+  solver_t *s = &slv_[kind];
+  if (!s->g) return NULL;
+  CHECK(fdtd_hip_grid_get_field(s->g, slot_of_[q], (double *)s->mirror[q]));
+  return s->mirror[q];
+}
+
+static double *solver_eps(int kind)
+{
+  solver_t *s = &slv_[kind];
+  if (!s->g) return NULL;
+  /* MPI TM: EPS_EZ, MPI TE: EPS_EY, plain TM: EPS_EZ, plain TE: EPS_EX */
+  int which = (kind == SLV_MPI_TE) ? 1 : 0;
+  CHECK(fdtd_hip_grid_get_eps(s->g, which, s->eps_mirror));
+  return s->eps_mirror;
+}
+
+/* ---------------------------------------------------------------- NTFF outputs */
+
+static double complex ff_coef(void)
+{
+  /* ntffTM_Frequency / ntffTE_FrequencySplit with R0 of ntffTM_init / ntffTE_init */
+  const double R0 = 1.0e6 * field_toCellUnit(500);
+  const double k_s = field_getK();
+  return csqrt(I * k_s / (8 * M_PI * R0)) * cexp(I * k_s * R0);
+}
+
+int fdtd_solver_farField(double *out360)
+{
+  if (active_ < 0) return FDTD_ESTATE;
+  double complex coef = ff_coef();
+  FieldInfo fi = field_getFieldInfo();
+  return fdtd_hip_grid_far_field(slv_[active_].g, field_getK(), creal(coef), cimag(coef), sqrt(fi.h_u_nm), out360);
+}
+
+static void twiddles(double *tw)
+{
+  /* e^{+i pi k/4096} with the argument formed as fft1x does (cfft.c:138-141) */
+  const double w = M_PI / 4096;
+  for (int k = 0; k < 4096; k++) {
+    double th = w * k;
+    tw[2 * k] = cos(th);
+    tw[2 * k + 1] = sin(th);
+  }
+}
+
+int fdtd_solver_timeSpectra(double *out)
+{
+  if (active_ < 0) return FDTD_ESTATE;
+  const double C = field_getLightSpeed(), w_s = field_getOmega();
+  /* ntffTM_TimeTranslate / ntffTE_TimeTranslate (ntffTM.c:165, ntffTE.c:25);
+     the MPI solvers' ntffOutput omits 1/(4 pi c) (mpiTM_UPML.c:802, mpiTE_UPML.c:843) */
+  double complex coef = is_mpi(active_) ? csqrt(2 * M_PI * C / (I * w_s))
+                                        : 1.0 / (4 * M_PI * C) * csqrt(2 * M_PI * C / (I * w_s));
+  double tw[8192];
+  twiddles(tw);
+  FieldInfo fi = field_getFieldInfo();
+  return fdtd_hip_grid_time_spectra(slv_[active_].g, creal(coef), cimag(coef), tw, fi.h_u_nm, out);
+}
+
+static void write_spectra(void)
+{
+  double *sp = (double *)malloc(321 * 360 * sizeof(double));
+  CHECK(fdtd_solver_timeSpectra(sp));
+  double *rows[321];
+  for (int l = 0; l < 321; l++) rows[l] = sp + (size_t)l * 360;
+  char buf[256];
+  snprintf(buf, sizeof buf, "%d[deg].txt", (int)field_getWaveAngle());
+  ntff_outputEnormTxt(rows, buf);
+  snprintf(buf, sizeof buf, "%d[deg]_%dnm_%dnm_b.dat", (int)field_getWaveAngle(), LAMBDA_ST_NM, LAMBDA_EN_NM);
+  ntff_outputEnormBin(rows, buf);
+  free(sp);
+}
+
+/* ntffOutput of the MPI TE solver (mpiTE_UPML.c:796-879): Eth / Eph time series per angle */
+static void mpi_te_ntff_output(void)
+{
+  solver_t *s = &slv_[SLV_MPI_TE];
+  const int maxTime = (int)field_getMaxTime();
+  const size_t n = (size_t)360 * s->arraySize;
+  double complex *W[3];
+  for (int a = 0; a < 3; a++) {
+    W[a] = (double complex *)malloc(n * sizeof(double complex));
+    CHECK(fdtd_hip_grid_get_ntff_time(s->g, a, (double *)W[a]));
+  }
+  const double w_s = field_getOmega(), Z0 = field_getZ0();
+  const double complex coef = csqrt(2 * M_PI * field_getLightSpeed() / (I * w_s));
+  mkdir("MPI_TE_UPML", 0775);
+  const char *names[4] = {"MPI_TE_UPML/Eph_r.txt", "MPI_TE_UPML/Eph_i.txt", "MPI_TE_UPML/Eth_r.txt", "MPI_TE_UPML/Eth_i.txt"};
+  FILE *fp[4];
+  for (int f = 0; f < 4; f++) {
+    fp[f] = fopen(names[f], "w");
+    if (!fp[f]) { printf("cannot open file %s \n", names[f]); exit(2); }
+  }
+  for (int ang = 0; ang < 360; ang++) {
+    double phi = ang * M_PI / 180.0;
+    double sx = cos(phi), sy = sin(phi), sz = -1.0, px = -sin(phi), py = cos(phi);
+    size_t k = (size_t)ang * s->arraySize;
+    for (int i = 0; i < maxTime; i++) {
+      double complex WTH = W[0][k + i] * sx + W[1][k + i] * sy + 0;
+      double complex WPH = W[0][k + i] * px + W[1][k + i] * py;
+      double complex UTH = 0 + 0 + W[2][k + i] * sz;
+      double complex ETH = coef * (-Z0 * WTH - 0);
+      double complex EPH = coef * (-Z0 * WPH + UTH);
+      fprintf(fp[0], "%.20lf ", creal(EPH));
+      fprintf(fp[1], "%.20lf ", cimag(EPH));
+      fprintf(fp[2], "%.20lf ", creal(ETH));
+      fprintf(fp[3], "%.20lf ", cimag(ETH));
+    }
+    for (int f = 0; f < 4; f++) fprintf(fp[f], "\n");
+  }
+  for (int f = 0; f < 4; f++) fclose(fp[f]);
+  for (int a = 0; a < 3; a++) free(W[a]);
+  /* ntffStrTE.txt: |E_phi|^2 per angle; the reference's split version prints NaN here (R0 unset) */
+  double ff[720];
+  CHECK(fdtd_solver_farField(ff));
+  FILE *f = fopen("ntffStrTE.txt", "w");
+  if (f) {
+    for (int ang = 0; ang < 360; ang++) fprintf(f, "%.18lf\n", ff[2 * ang] * ff[2 * ang] + ff[2 * ang + 1] * ff[2 * ang + 1]);
+    fclose(f);
+  }
+}
+
+static void solver_reset(int kind)
+{
+  solver_t *s = &slv_[kind];
+  if (!s->g) return;
+  if (!is_mpi(kind) && file_output_) write_spectra();  /* fdtdTM_upml.c:87, fdtdTE_upml.c:117 */
+  CHECK(fdtd_hip_grid_reset(s->g));
+}
+
+static void solver_finish(int kind)
+{
+  solver_t *s = &slv_[kind];
+  if (!s->g) return;
+  if (!is_mpi(kind)) solver_reset(kind);                               /* fdtdTM_upml.c:78-82 */
+  else if (kind == SLV_MPI_TE && file_output_) mpi_te_ntff_output();   /* mpiTE_UPML.c:306-318 */
+  solver_free(kind);
+}
+
+/* ---------------------------------------------------------------- extensions */
+int fdtd_solver_getState(int which, double *out)
+{
+  if (active_ < 0) return FDTD_ESTATE;
+  return fdtd_hip_grid_get_field(slv_[active_].g, which, out);
+}
+int fdtd_solver_setState(int which, const double *in)
+{
+  if (active_ < 0) return FDTD_ESTATE;
+  return fdtd_hip_grid_set_field(slv_[active_].g, which, in);
+}
+int fdtd_solver_getNTFFTime(int which, double *out)
+{
+  if (active_ < 0) return FDTD_ESTATE;
+  return fdtd_hip_grid_get_ntff_time(slv_[active_].g, which, out);
+}
+int fdtd_solver_dims(int out[4])
+{
+  if (active_ < 0) return FDTD_ESTATE;
+  out[0] = slv_[active_].NPXa;
+  out[1] = slv_[active_].NPYa;
+  out[2] = slv_[active_].off;
+  out[3] = slv_[active_].arraySize;
+  return FDTD_OK;
+}
+
+/* ---------------------------------------------------------------- MPI TM */
+static void mpi_tm_init(void) { solver_init(SLV_MPI_TM); }
+static void mpi_tm_update(void) { solver_update(SLV_MPI_TM); }
+static void mpi_tm_reset(void) { solver_reset(SLV_MPI_TM); }
+static void mpi_tm_finish(void) { solver_finish(SLV_MPI_TM); }
+fdtd_method mpi_fdtdTM_upml_getUpdate(void) { return mpi_tm_update; }
+fdtd_method mpi_fdtdTM_upml_getFinish(void) { return mpi_tm_finish; }
+fdtd_method mpi_fdtdTM_upml_getReset(void) { return mpi_tm_reset; }
+fdtd_method mpi_fdtdTM_upml_getInit(void) { return mpi_tm_init; }
+fdtd_dcomplex *mpi_fdtdTM_upml_getEz(void) { return (fdtd_dcomplex *)solver_field(SLV_MPI_TM, 0); }
+fdtd_dcomplex *mpi_fdtdTM_upml_getHx(void) { return (fdtd_dcomplex *)solver_field(SLV_MPI_TM, 1); }
+fdtd_dcomplex *mpi_fdtdTM_upml_getHy(void) { return (fdtd_dcomplex *)solver_field(SLV_MPI_TM, 2); }
+double *mpi_fdtdTM_upml_getEps(void) { return solver_eps(SLV_MPI_TM); }
+int mpi_fdtdTM_upml_getSubNx(void) { return N_PX; }
+int mpi_fdtdTM_upml_getSubNy(void) { return N_PY; }
+int mpi_fdtdTM_upml_getSubNpx(void) { return N_PX + 2; }
+int mpi_fdtdTM_upml_getSubNpy(void) { return N_PY + 2; }
+int mpi_fdtdTM_upml_getSubNcell(void) { return (N_PX + 2) * (N_PY + 2); }
+
+/* ---------------------------------------------------------------- MPI TE */
+static void mpi_te_init(void) { solver_init(SLV_MPI_TE); }
+static void mpi_te_update(void) { solver_update(SLV_MPI_TE); }
+static void mpi_te_reset(void) { solver_reset(SLV_MPI_TE); }
+static void mpi_te_finish(void) { solver_finish(SLV_MPI_TE); }
+fdtd_method mpi_fdtdTE_upml_getUpdate(void) { return mpi_te_update; }
+fdtd_method mpi_fdtdTE_upml_getFinish(void) { return mpi_te_finish; }
+fdtd_method mpi_fdtdTE_upml_getReset(void) { return mpi_te_reset; }
+fdtd_method mpi_fdtdTE_upml_getInit(void) { return mpi_te_init; }
+fdtd_dcomplex *mpi_fdtdTE_upml_getEx(void) { return (fdtd_dcomplex *)solver_field(SLV_MPI_TE, 0); }
+fdtd_dcomplex *mpi_fdtdTE_upml_getEy(void) { return (fdtd_dcomplex *)solver_field(SLV_MPI_TE, 1); }
+fdtd_dcomplex *mpi_fdtdTE_upml_getHz(void) { return (fdtd_dcomplex *)solver_field(SLV_MPI_TE, 2); }
+double *mpi_fdtdTE_upml_getEps(void) { return solver_eps(SLV_MPI_TE); }
+int mpi_fdtdTE_upml_getSubNx(void) { return N_PX; }
+int mpi_fdtdTE_upml_getSubNy(void) { return N_PY; }
+int mpi_fdtdTE_upml_getSubNpx(void) { return N_PX + 2; }
+int mpi_fdtdTE_upml_getSubNpy(void) { return N_PY + 2; }
+int mpi_fdtdTE_upml_getSubNcell(void) { return (N_PX + 2) * (N_PY + 2); }
+
+/* ---------------------------------------------------------------- plain TM */
+static void tm_init(void) { solver_init(SLV_TM); }
+static void tm_update(void) { solver_update(SLV_TM); }
+static void tm_reset(void) { solver_reset(SLV_TM); }
+static void tm_finish(void) { solver_finish(SLV_TM); }
+fdtd_method fdtdTM_upml_getUpdate(void) { return tm_update; }
+fdtd_method fdtdTM_upml_getFinish(void) { return tm_finish; }
+fdtd_method fdtdTM_upml_getReset(void) { return tm_reset; }
+fdtd_method fdtdTM_upml_getInit(void) { return tm_init; }
+fdtd_dcomplex *fdtdTM_upml_getEz(void) { return (fdtd_dcomplex *)solver_field(SLV_TM, 0); }
+fdtd_dcomplex *fdtdTM_upml_getHx(void) { return (fdtd_dcomplex *)solver_field(SLV_TM, 1); }
+fdtd_dcomplex *fdtdTM_upml_getHy(void) { return (fdtd_dcomplex *)solver_field(SLV_TM, 2); }
+double *fdtdTM_upml_getEps(void) { return solver_eps(SLV_TM); }
+
+/* ---------------------------------------------------------------- plain TE */
+static void te_init(void) { solver_init(SLV_TE); }
+static void te_update(void) { solver_update(SLV_TE); }
+static void te_reset(void) { solver_reset(SLV_TE); }
+static void te_finish(void) { solver_finish(SLV_TE); }
+fdtd_method fdtdTE_upml_getUpdate(void) { return te_update; }
+fdtd_method fdtdTE_upml_getFinish(void) { return te_finish; }
+fdtd_method fdtdTE_upml_getReset(void) { return te_reset; }
+fdtd_method fdtdTE_upml_getInit(void) { return te_init; }
+fdtd_dcomplex *fdtdTE_upml_getEx(void) { return (fdtd_dcomplex *)solver_field(SLV_TE, 0); }
+fdtd_dcomplex *fdtdTE_upml_getEy(void) { return (fdtd_dcomplex *)solver_field(SLV_TE, 1); }
+fdtd_dcomplex *fdtdTE_upml_getHz(void) { return (fdtd_dcomplex *)solver_field(SLV_TE, 2); }
+double *fdtdTE_upml_getEps(void) { return solver_eps(SLV_TE); }

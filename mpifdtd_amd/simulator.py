@@ -1,0 +1,118 @@
+"""Python view of the reference driver flow (main.c:150-213) over the C ABI.
+
+    sim = Simulator(MPI_TM_UPML_2D, MIE_CYLINDER, FieldInfo(...))
+    sim.calc(steps)             # simulator_calc() x steps (one native call)
+    ez, hx, hy = sim.fields()   # the solver's getters (borrowed pointers, copied)
+
+Everything runs through libmpifdtd_amd.so: models_setModel, simulator_setSolver,
+simulator_init (field_init -> models_initModel -> solver init), the solver's
+update on the GPU, its getters, and simulator_finish.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _native as N
+from ._native import FieldInfo, ModelParams, check
+
+_GETTERS = {
+    N.MPI_TM_UPML_2D: ("mpi_fdtdTM_upml_getEz", "mpi_fdtdTM_upml_getHx", "mpi_fdtdTM_upml_getHy", "mpi_fdtdTM_upml_getEps"),
+    N.MPI_TE_UPML_2D: ("mpi_fdtdTE_upml_getEx", "mpi_fdtdTE_upml_getEy", "mpi_fdtdTE_upml_getHz", "mpi_fdtdTE_upml_getEps"),
+    N.TM_UPML_2D: ("fdtdTM_upml_getEz", "fdtdTM_upml_getHx", "fdtdTM_upml_getHy", "fdtdTM_upml_getEps"),
+    N.TE_UPML_2D: ("fdtdTE_upml_getEx", "fdtdTE_upml_getEy", "fdtdTE_upml_getHz", "fdtdTE_upml_getEps"),
+}
+
+
+def field_info(width_nm, height_nm, h_u_nm, pml, lambda_nm, angle_deg, steps):
+    fi = FieldInfo()
+    fi.width_nm, fi.height_nm, fi.h_u_nm, fi.pml = width_nm, height_nm, h_u_nm, pml
+    fi.lambda_nm, fi.angle_deg, fi.stepNum = lambda_nm, angle_deg, steps
+    return fi
+
+
+def model_params(**kw):
+    p = ModelParams()
+    N.lib().models_defaultParams(ctypes.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+class Simulator:
+    """One simulation through the reference's API (one active solver per process)."""
+
+    def __init__(self, solver, model, fi, params=None, c0=0.7071, ntff_inset=5, point_source=False,
+                 ntff_time=True, file_output=False):
+        L = N.lib()
+        if L.fdtd_hip_device_count() < 1:
+            raise RuntimeError("no HIP device visible: the solvers run on the GPU only")
+        self.solver = solver
+        L.models_setModel(model)
+        L.models_setParams(ctypes.byref(params if params is not None else model_params()))
+        L.simulator_setSolver(solver)
+        L.field_setLightSpeed(c0)
+        L.field_setNTFFInset(ntff_inset)
+        L.field_setPointSource(1 if point_source else 0)
+        L.fdtd_solver_setNTFFTime(1 if ntff_time else 0)
+        L.fdtd_solver_setFileOutput(1 if file_output else 0)
+        L.simulator_init(fi)
+        d = (ctypes.c_int * 4)()
+        check(L.fdtd_solver_dims(d), "fdtd_solver_dims")
+        self.NPXa, self.NPYa, self.off, self.arraySize = list(d)
+        fs = L.field_getFieldInfo_S()
+        self.N_PX, self.N_PY = fs.N_PX, fs.N_PY
+        self._open = True
+
+    def calc(self, n=1):
+        N.lib().fdtd_simulator_run(int(n))
+
+    @property
+    def time(self):
+        return N.lib().field_getTime()
+
+    def fields(self):
+        """The three getters' arrays (TM: Ez, Hx, Hy; TE: Ex, Ey, Hz) in the solver layout."""
+        L = N.lib()
+        out = []
+        for name in _GETTERS[self.solver][:3]:
+            ptr = getattr(L, name)()
+            out.append(N.as_array(ptr, (self.NPXa, self.NPYa), np.complex128))
+        return out
+
+    def eps(self):
+        ptr = getattr(N.lib(), _GETTERS[self.solver][3])()
+        return N.as_array(ptr, (self.NPXa, self.NPYa), np.float64)
+
+    def state(self, which):
+        out = np.empty((self.NPXa, self.NPYa), np.complex128)
+        check(N.lib().fdtd_solver_getState(which, N.dptr(out)), "fdtd_solver_getState")
+        return out
+
+    def set_state(self, which, arr):
+        a = np.ascontiguousarray(arr, np.complex128).reshape(self.NPXa, self.NPYa)
+        check(N.lib().fdtd_solver_setState(which, N.dptr(a)), "fdtd_solver_setState")
+
+    def ntff_time(self, which):
+        out = np.empty((360, self.arraySize), np.complex128)
+        check(N.lib().fdtd_solver_getNTFFTime(which, N.dptr(out)), "fdtd_solver_getNTFFTime")
+        return out
+
+    def far_field(self):
+        out = np.empty(360, np.complex128)
+        check(N.lib().fdtd_solver_farField(N.dptr(out)), "fdtd_solver_farField")
+        return out
+
+    def time_spectra(self):
+        out = np.empty((321, 360), np.float64)
+        check(N.lib().fdtd_solver_timeSpectra(N.dptr(out)), "fdtd_solver_timeSpectra")
+        return out
+
+    def reset(self, angle_deg=None):
+        N.lib().simulator_reset()
+        if angle_deg is not None:
+            N.lib().field_setWaveAngle(int(angle_deg))
+
+    def finish(self):
+        if self._open:
+            N.lib().simulator_finish()
+            self._open = False

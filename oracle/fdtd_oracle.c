@@ -737,6 +737,17 @@ static void scattered_pulse(orc_sim *s, dcx *p, const double *eps, double gapX, 
 }
 
 /* ------------------------------------------------------------------ NTFF per step */
+/* Accumulate into the flat [360][arraySize] array.  Indices outside the
+   allocation (plus NT_GUARD bins each side) are dropped: the reference writes
+   them out of bounds on strongly non-square grids (RFperC comes from the y
+   extent only, field.c:140-142), which is undefined behaviour there.
+   `a -= v` is written `a += -v`, which rounds identically. */
+#define NT_ADD(slot, gidx, val)                                                   \
+  do {                                                                            \
+    long g_ = (long)(gidx);                                                       \
+    if (g_ >= -NT_GUARD && g_ < 360L * s->arraySize + NT_GUARD) s->NT[slot][g_] += (val); \
+  } while (0)
+
 /* ntffCoef (mpiTM_UPML.c:840-847, mpiTE_UPML.c:602-609) */
 static inline void ntff_coef(double time, double timeShift, int *m, double *a, double *b, double *ab)
 {
@@ -757,7 +768,6 @@ static void mpi_tm_ntff(orc_sim *s)
   double timeE = s->time - 1, timeH = s->time - 0.5;
   int tp = s->top, bm = s->bottom, rt = s->right, lt = s->left;
   dcx *Ez = s->F[TM_EZ], *Hx = s->F[TM_HX], *Hy = s->F[TM_HY];
-  dcx *Ux = s->NT[0], *Uy = s->NT[1], *Wz = s->NT[2];
   int m_e, m_h;
   double a_e, b_e, ab_e, a_h, b_h, ab_h;
   for (int ang = 0; ang < 360; ang++) {
@@ -774,12 +784,12 @@ static void mpi_tm_ntff(orc_sim *s)
         int k = IDX(i, bm);
         const dcx ez = Ez[k];
         const dcx hx = 0.5 * (Hx[k] + Hx[k - 1]);
-        Ux[stp + m_e - 1] += ez * b_e * coef;
-        Wz[stp + m_h - 1] += hx * b_h * coef;
-        Ux[stp + m_e] += ez * ab_e * coef;
-        Wz[stp + m_h] += hx * ab_h * coef;
-        Ux[stp + m_e + 1] -= ez * a_e * coef;
-        Wz[stp + m_h + 1] -= hx * a_h * coef;
+        NT_ADD(0, stp + m_e - 1, (ez * b_e * coef));
+        NT_ADD(2, stp + m_h - 1, (hx * b_h * coef));
+        NT_ADD(0, stp + m_e, (ez * ab_e * coef));
+        NT_ADD(2, stp + m_h, (hx * ab_h * coef));
+        NT_ADD(0, stp + m_e + 1, -(ez * a_e * coef));
+        NT_ADD(2, stp + m_h + 1, -(hx * a_h * coef));
       }
     }
     if (0 < rt && rt < SPX - 1) {
@@ -792,12 +802,12 @@ static void mpi_tm_ntff(orc_sim *s)
         int k = IDX(rt, j);
         const dcx ez = Ez[k];
         const dcx hy = 0.5 * (Hy[k] + Hy[k - SPY]);
-        Uy[stp + m_e - 1] += ez * b_e * coef;
-        Wz[stp + m_h - 1] += hy * b_h * coef;
-        Uy[stp + m_e] += ez * ab_e * coef;
-        Wz[stp + m_h] += hy * ab_h * coef;
-        Uy[stp + m_e + 1] -= ez * a_e * coef;
-        Wz[stp + m_h + 1] -= hy * a_h * coef;
+        NT_ADD(1, stp + m_e - 1, (ez * b_e * coef));
+        NT_ADD(2, stp + m_h - 1, (hy * b_h * coef));
+        NT_ADD(1, stp + m_e, (ez * ab_e * coef));
+        NT_ADD(2, stp + m_h, (hy * ab_h * coef));
+        NT_ADD(1, stp + m_e + 1, -(ez * a_e * coef));
+        NT_ADD(2, stp + m_h + 1, -(hy * a_h * coef));
       }
     }
     if (0 < tp && tp < SPY - 1) {
@@ -810,12 +820,12 @@ static void mpi_tm_ntff(orc_sim *s)
         int k = IDX(i, tp);
         const dcx ez = -Ez[k];
         const dcx hx = -0.5 * (Hx[k] + Hx[k - 1]);
-        Ux[stp + m_e - 1] += ez * b_e * coef;
-        Wz[stp + m_h - 1] += hx * b_h * coef;
-        Ux[stp + m_e] += ez * ab_e * coef;
-        Wz[stp + m_h] += hx * ab_h * coef;
-        Ux[stp + m_e + 1] -= ez * a_e * coef;
-        Wz[stp + m_h + 1] -= hx * a_h * coef;
+        NT_ADD(0, stp + m_e - 1, (ez * b_e * coef));
+        NT_ADD(2, stp + m_h - 1, (hx * b_h * coef));
+        NT_ADD(0, stp + m_e, (ez * ab_e * coef));
+        NT_ADD(2, stp + m_h, (hx * ab_h * coef));
+        NT_ADD(0, stp + m_e + 1, -(ez * a_e * coef));
+        NT_ADD(2, stp + m_h + 1, -(hx * a_h * coef));
       }
     }
     if (0 < lt && lt < SPX) {
@@ -828,12 +838,12 @@ static void mpi_tm_ntff(orc_sim *s)
         int k = IDX(lt, j);
         const dcx ez = -Ez[k];
         const dcx hy = -0.5 * (Hy[k] + Hy[k - SPY]);
-        Uy[stp + m_e - 1] += ez * b_e * coef;
-        Wz[stp + m_h - 1] += hy * b_h * coef;
-        Uy[stp + m_e] += ez * ab_e * coef;
-        Wz[stp + m_h] += hy * ab_h * coef;
-        Uy[stp + m_e + 1] -= ez * a_e * coef;
-        Wz[stp + m_h + 1] -= hy * a_h * coef;
+        NT_ADD(1, stp + m_e - 1, (ez * b_e * coef));
+        NT_ADD(2, stp + m_h - 1, (hy * b_h * coef));
+        NT_ADD(1, stp + m_e, (ez * ab_e * coef));
+        NT_ADD(2, stp + m_h, (hy * ab_h * coef));
+        NT_ADD(1, stp + m_e + 1, -(ez * a_e * coef));
+        NT_ADD(2, stp + m_h + 1, -(hy * a_h * coef));
       }
     }
   }
@@ -849,7 +859,6 @@ static void mpi_te_ntff(orc_sim *s)
   double timeE = s->time - 1, timeH = s->time - 0.5;
   int tp = s->top, bm = s->bottom, rt = s->right, lt = s->left;
   dcx *Ex = s->F[TE_EX], *Ey = s->F[TE_EY], *Hz = s->F[TE_HZ];
-  dcx *Wx = s->NT[0], *Wy = s->NT[1], *Uz = s->NT[2];
   int m_e, m_h;
   double a_e, b_e, ab_e, a_h, b_h, ab_h;
   int ang, stp;
@@ -866,12 +875,12 @@ static void mpi_te_ntff(orc_sim *s)
         int k = IDX(i, bm);
         dcx ex = -Ex[k];
         dcx hz = -0.5 * (Hz[k] + Hz[k - 1]);
-        Uz[stp + m_e - 1] += ex * b_e * coef;
-        Wx[stp + m_h - 1] += hz * b_h * coef;
-        Uz[stp + m_e] += ex * ab_e * coef;
-        Wx[stp + m_h] += hz * ab_h * coef;
-        Uz[stp + m_e + 1] -= ex * a_e * coef;
-        Wx[stp + m_h + 1] -= hz * a_h * coef;
+        NT_ADD(2, stp + m_e - 1, (ex * b_e * coef));
+        NT_ADD(0, stp + m_h - 1, (hz * b_h * coef));
+        NT_ADD(2, stp + m_e, (ex * ab_e * coef));
+        NT_ADD(0, stp + m_h, (hz * ab_h * coef));
+        NT_ADD(2, stp + m_e + 1, -(ex * a_e * coef));
+        NT_ADD(0, stp + m_h + 1, -(hz * a_h * coef));
       }
     }
     if (0 < rt && rt < SPX - 1) {
@@ -884,12 +893,12 @@ static void mpi_te_ntff(orc_sim *s)
         int k = IDX(rt, j);
         dcx ey = -Ey[k];
         dcx hz = -0.5 * (Hz[k] + Hz[k - SPY]);
-        Uz[stp + m_e - 1] += ey * b_e * coef;
-        Wy[stp + m_h - 1] += hz * b_h * coef;
-        Uz[stp + m_e] += ey * ab_e * coef;
-        Wy[stp + m_h] += hz * ab_h * coef;
-        Uz[stp + m_e + 1] -= ey * a_e * coef;
-        Wy[stp + m_h + 1] -= hz * a_h * coef;
+        NT_ADD(2, stp + m_e - 1, (ey * b_e * coef));
+        NT_ADD(1, stp + m_h - 1, (hz * b_h * coef));
+        NT_ADD(2, stp + m_e, (ey * ab_e * coef));
+        NT_ADD(1, stp + m_h, (hz * ab_h * coef));
+        NT_ADD(2, stp + m_e + 1, -(ey * a_e * coef));
+        NT_ADD(1, stp + m_h + 1, -(hz * a_h * coef));
       }
     }
     if (0 < tp && tp < SPY - 1) {
@@ -902,12 +911,12 @@ static void mpi_te_ntff(orc_sim *s)
         int k = IDX(i, tp);
         dcx ex = Ex[k];
         dcx hz = 0.5 * (Hz[k] + Hz[k - 1]);
-        Uz[stp + m_e - 1] += ex * b_e * coef;
-        Wx[stp + m_h - 1] += hz * b_h * coef;
-        Uz[stp + m_e] += ex * ab_e * coef;
-        Wx[stp + m_h] += hz * ab_h * coef;
-        Uz[stp + m_e + 1] -= ex * a_e * coef;
-        Wx[stp + m_h + 1] -= hz * a_h * coef;
+        NT_ADD(2, stp + m_e - 1, (ex * b_e * coef));
+        NT_ADD(0, stp + m_h - 1, (hz * b_h * coef));
+        NT_ADD(2, stp + m_e, (ex * ab_e * coef));
+        NT_ADD(0, stp + m_h, (hz * ab_h * coef));
+        NT_ADD(2, stp + m_e + 1, -(ex * a_e * coef));
+        NT_ADD(0, stp + m_h + 1, -(hz * a_h * coef));
       }
     }
     if (0 < lt && lt < SPX) {
@@ -920,27 +929,27 @@ static void mpi_te_ntff(orc_sim *s)
         int k = IDX(lt, j);
         dcx ey = Ey[k];
         dcx hz = 0.5 * (Hz[k] + Hz[k - SPY]);
-        Uz[stp + m_e - 1] += ey * b_e * coef;
-        Wy[stp + m_h - 1] += hz * b_h * coef;
-        Uz[stp + m_e] += ey * ab_e * coef;
-        Wy[stp + m_h] += hz * ab_h * coef;
-        Uz[stp + m_e + 1] -= ey * a_e * coef;
-        Wy[stp + m_h + 1] -= hz * a_h * coef;
+        NT_ADD(2, stp + m_e - 1, (ey * b_e * coef));
+        NT_ADD(1, stp + m_h - 1, (hz * b_h * coef));
+        NT_ADD(2, stp + m_e, (ey * ab_e * coef));
+        NT_ADD(1, stp + m_h, (hz * ab_h * coef));
+        NT_ADD(2, stp + m_e + 1, -(ey * a_e * coef));
+        NT_ADD(1, stp + m_h + 1, -(hz * a_h * coef));
       }
     }
   }
 }
 
 /* calc() of ntffTM.c:279-288 / ntffTE.c:57-66 (no coef) */
-static inline void ntff_calc(double tts, dcx eh, dcx *UW)
+static inline void ntff_calc_at(orc_sim *s, int slot, long row, double tts, dcx eh)
 {
   int m = floor(tts + 0.5);
   double a = (0.5 + tts - m);
   double b = 1.0 - a;
   double ab = a - b;
-  UW[m - 1] += eh * b;
-  UW[m] += eh * ab;
-  UW[m + 1] -= eh * a;
+  NT_ADD(slot, row + m - 1, eh * b);
+  NT_ADD(slot, row + m, eh * ab);
+  NT_ADD(slot, row + m + 1, -(eh * a));
 }
 
 /* ntffTM_TimeCalc (ntffTM.c:293-371): the along-side time shift is accumulated
@@ -959,13 +968,12 @@ static void plain_tm_ntff(orc_sim *s)
   for (int ang = 0; ang < 360; ang++, index_ang += s->arraySize) {
     double rad = ang * ToRad;
     double r1x_per_c = cos(rad) / s->C0, r1y_per_c = sin(rad) / s->C0;
-    dcx *Ux = &s->NT[0][index_ang], *Uy = &s->NT[1][index_ang], *Wz = &s->NT[2][index_ang];
     {
       const double r2x = lt_cx, r2y = bm_cy;
       double ts = -(r1x_per_c * r2x + r1y_per_c * r2y) + s->RFperC;
       for (int k = lb; k < rb; k += NPY) {
-        ntff_calc(timeE + ts, Ez[k], Ux);
-        ntff_calc(timeH + ts, 0.5 * (Hx[k] + Hx[k - 1]), Wz);
+        ntff_calc_at(s, 0, index_ang, timeE + ts, Ez[k]);
+        ntff_calc_at(s, 2, index_ang, timeH + ts, 0.5 * (Hx[k] + Hx[k - 1]));
         ts -= r1x_per_c;
       }
     }
@@ -973,8 +981,8 @@ static void plain_tm_ntff(orc_sim *s)
       const double r2x = rt_cx; double r2y = bm_cy;
       double ts = -(r1x_per_c * r2x + r1y_per_c * r2y) + s->RFperC;
       for (int k = rb; k < rtI; k++) {
-        ntff_calc(timeE + ts, Ez[k], Uy);
-        ntff_calc(timeH + ts, 0.5 * (Hy[k] + Hy[k - NPY]), Wz);
+        ntff_calc_at(s, 1, index_ang, timeE + ts, Ez[k]);
+        ntff_calc_at(s, 2, index_ang, timeH + ts, 0.5 * (Hy[k] + Hy[k - NPY]));
         ts -= r1y_per_c;
       }
     }
@@ -982,8 +990,8 @@ static void plain_tm_ntff(orc_sim *s)
       double r2x = lt_cx; const double r2y = tp_cy;
       double ts = -(r1x_per_c * r2x + r1y_per_c * r2y) + s->RFperC;
       for (int k = ltI; k < rtI; k += NPY) {
-        ntff_calc(timeE + ts, -Ez[k], Ux);
-        ntff_calc(timeH + ts, -0.5 * (Hx[k] + Hx[k - 1]), Wz);
+        ntff_calc_at(s, 0, index_ang, timeE + ts, -Ez[k]);
+        ntff_calc_at(s, 2, index_ang, timeH + ts, -0.5 * (Hx[k] + Hx[k - 1]));
         ts -= r1x_per_c;
       }
     }
@@ -991,8 +999,8 @@ static void plain_tm_ntff(orc_sim *s)
       const double r2x = lt_cx; double r2y = bm_cy;
       double ts = -(r1x_per_c * r2x + r1y_per_c * r2y) + s->RFperC;
       for (int k = lb; k < ltI; k++) {
-        ntff_calc(timeE + ts, -Ez[k], Uy);
-        ntff_calc(timeH + ts, -0.5 * (Hy[k] + Hy[k - NPY]), Wz);
+        ntff_calc_at(s, 1, index_ang, timeE + ts, -Ez[k]);
+        ntff_calc_at(s, 2, index_ang, timeH + ts, -0.5 * (Hy[k] + Hy[k - NPY]));
         ts -= r1y_per_c;
       }
     }
@@ -1013,15 +1021,14 @@ static void plain_te_ntff(orc_sim *s)
   for (int ang = 0, stp = 0; ang < 360; ang++, stp += s->arraySize) {
     double rad = ang * ToRad;
     double r1x_per_c = cos(rad) / s->C0, r1y_per_c = sin(rad) / s->C0;
-    dcx *Wx = &s->NT[0][stp], *Wy = &s->NT[1][stp], *Uz = &s->NT[2][stp];
     {
       const double r2x = lt_cx + 0.5, r2y = bm_cy;
       double ts = -(r1x_per_c * r2x + r1y_per_c * r2y) + s->RFperC;
       for (int k = lb; k < rb; k += NPY) {
         dcx ex = -Ex[k];
         dcx hz = -0.5 * (Hz[k] + Hz[k - 1]);
-        ntff_calc(timeE + ts, ex, Uz);
-        ntff_calc(timeH + ts, hz, Wx);
+        ntff_calc_at(s, 2, stp, timeE + ts, ex);
+        ntff_calc_at(s, 0, stp, timeH + ts, hz);
         ts -= r1x_per_c;
       }
     }
@@ -1031,8 +1038,8 @@ static void plain_te_ntff(orc_sim *s)
       for (int k = rb; k < rtI; k++) {
         dcx ey = -Ey[k];
         dcx hz = -0.5 * (Hz[k] + Hz[k - NPY]);
-        ntff_calc(timeE + ts, ey, Uz);
-        ntff_calc(timeH + ts, hz, Wy);
+        ntff_calc_at(s, 2, stp, timeE + ts, ey);
+        ntff_calc_at(s, 1, stp, timeH + ts, hz);
         ts -= r1y_per_c;
       }
     }
@@ -1042,8 +1049,8 @@ static void plain_te_ntff(orc_sim *s)
       for (int k = ltI; k < rtI; k += NPY) {
         dcx ex = Ex[k];
         dcx hz = 0.5 * (Hz[k] + Hz[k - 1]);
-        ntff_calc(timeE + ts, ex, Uz);
-        ntff_calc(timeH + ts, hz, Wx);
+        ntff_calc_at(s, 2, stp, timeE + ts, ex);
+        ntff_calc_at(s, 0, stp, timeH + ts, hz);
         ts -= r1x_per_c;
       }
     }
@@ -1053,8 +1060,8 @@ static void plain_te_ntff(orc_sim *s)
       for (int k = lb; k < ltI; k++) {
         dcx ey = Ey[k];
         dcx hz = 0.5 * (Hz[k] + Hz[k - NPY]);
-        ntff_calc(timeE + ts, ey, Uz);
-        ntff_calc(timeH + ts, hz, Wy);
+        ntff_calc_at(s, 2, stp, timeE + ts, ey);
+        ntff_calc_at(s, 1, stp, timeH + ts, hz);
         ts -= r1y_per_c;
       }
     }
@@ -1360,8 +1367,11 @@ void orc_time_translate(const orc_sim *s, double *eth_out, double *eph_out)
                           : 1.0 / (4 * M_PI * s->C0) * csqrt(2 * M_PI * s->C0 / (I * w_s));
   const int maxTime = (int)s->maxTime;
   double theta = 0;
+  const double ToRad = M_PI / 180.0;
   for (int ang = 0, k = 0; ang < 360; ang++, k += s->arraySize) {
-    double phi = ang * M_PI / 180.0;
+    /* ntffTM_TimeTranslate forms the angle as ang*ToRad (ntffTM.c:169-173);
+       ntffTE_TimeTranslate and the MPI ntffOutput as ang*M_PI/180.0 */
+    double phi = (s->tm && !s->mpi) ? ang * ToRad : ang * M_PI / 180.0;
     double sx = cos(theta) * cos(phi), sy = cos(theta) * sin(phi), sz = -cos(theta);
     double px = -sin(phi), py = cos(phi);
     for (int i = 0; i < maxTime; i++) {
