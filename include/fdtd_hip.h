@@ -136,6 +136,20 @@ int fdtd_hip_grid_time_spectra(fdtd_grid *g, double coef_re, double coef_im, con
 int fdtd_hip_grid_dims(const fdtd_grid *g, int out[4]);
 int fdtd_hip_grid_sync(fdtd_grid *g);
 
+/* FP32 grids: temporal block depth T (steps per fused launch, 1..15) and rows
+ * per wavefront R (4, 8, 16) of the interior tiles.  Steps are queued by
+ * fdtd_hip_grid_step and launched T at a time; every read of the state
+ * (get_field, far_field, ...) first launches the queued steps. */
+int fdtd_hip_grid_set_blocking(fdtd_grid *g, int T, int R);
+int fdtd_hip_grid_get_blocking(const fdtd_grid *g, int *T, int *R);
+int fdtd_hip_grid_flush(fdtd_grid *g);
+/* event timing of the FP32 fused launches since the last enable:
+ * out = {ms of the interior-tile launches, ms of the UPML-tile launches,
+ *        blocks launched, algorithmic bytes of the interior launches,
+ *        algorithmic bytes of the UPML launches, cell updates} */
+void fdtd_hip_profile_enable(int on);
+void fdtd_hip_profile_read(double out[6]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -94,6 +94,13 @@ void fdtd_solver_setNTFFTime(int on);
 void fdtd_solver_setFileOutput(int on);
 /* dims of the active solver: NPXa, NPYa, off, arraySize */
 int fdtd_solver_dims(int out[4]);
+/* FDTD_FP64 (default): reference-order FP64 kernels; FDTD_FP32: fused FP32
+ * kernels with temporal blocking (set before init).  Getters return double
+ * complex in both cases. */
+void fdtd_solver_setPrecision(int precision);
+int fdtd_solver_getPrecision(void);
+/* FP32: steps per fused launch T and rows per wavefront R of the active / next solver */
+int fdtd_solver_setBlocking(int T, int R);
 
 #ifdef __cplusplus
 }

@@ -134,6 +134,11 @@ def lib():
     sig("fdtd_solver_setNTFFTime", V, I)
     sig("fdtd_solver_setFileOutput", V, I)
     sig("fdtd_solver_dims", I, ip)
+    sig("fdtd_solver_setPrecision", V, I)
+    sig("fdtd_solver_getPrecision", I)
+    sig("fdtd_solver_setBlocking", I, I, I)
+    sig("fdtd_hip_profile_enable", V, I)
+    sig("fdtd_hip_profile_read", V, dp)
     sig("fdtd_hip_last_error", ctypes.c_char_p)
     sig("fdtd_hip_device_count", I)
     for n in ("mpi_fdtdTM_upml_getEz", "mpi_fdtdTM_upml_getHx", "mpi_fdtdTM_upml_getHy", "mpi_fdtdTM_upml_getEps",

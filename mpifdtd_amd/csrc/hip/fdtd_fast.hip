@@ -1,0 +1,933 @@
+// fdtd_fast.hip -- FP32 fused, temporally blocked UPML step for gfx950.
+//
+// One launch advances every tile of every instance by T time steps.  A
+// workgroup (4 wavefronts) owns a 64-column x 4R-row block of cells: lane l
+// holds column j0-(T+1)+l, wave w holds R consecutive rows, and each thread
+// keeps its R cells' E and H in registers for all T steps.  Neighbours along
+// j (the contiguous axis) come from the adjacent lane through cross-lane
+// shuffles; neighbours along i are in the same thread's registers except at
+// wave boundaries, which are exchanged through a 2 x 4 x 64-entry LDS row
+// buffer.  The block reads its cells once, runs T E/H half-step pairs, and
+// writes back the core cells [i0, i0+ni) x [j0, j0+nj) that stay valid after
+// T steps (halo T+1 below, T above).  Inputs and outputs are ping-pong
+// buffers, so the halos read by one tile are never written by another in the
+// same launch.  This replaces the reference's per-step MPI halo exchange
+// (mpiTM_UPML.c:252-296) with on-chip halos.
+//
+// Physics per reference update (mpiTM_UPML.c:196-217, mpiTE_UPML.c:290-304,
+// fdtdTM_upml.c:54-66, fdtdTE_upml.c:168-192).  Where every UPML coefficient
+// is 1 (the interior) the J/D and M/B recurrences collapse to
+//   E_n = E_{n-1} + curl(H)/eps + (s_n - s_{n-1}),   H += c0^2 * curl(E)
+// where E holds the total field the reference stores (D/eps plus the source
+// s_n of that step); UPML cells keep the full recurrence with their
+// auxiliaries.  All arithmetic is FP32.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "fdtd_hip.h"
+#include "fdtd_internal.h"
+
+__device__ __forceinline__ float2 c_add(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 c_sub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 c_scale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+__device__ __forceinline__ float2 c_mul(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ __forceinline__ float2 shfl_up1(float2 v)
+{
+  return make_float2(__shfl_up(v.x, 1, 64), __shfl_up(v.y, 1, 64));
+}
+__device__ __forceinline__ float2 shfl_dn1(float2 v)
+{
+  return make_float2(__shfl_down(v.x, 1, 64), __shfl_down(v.y, 1, 64));
+}
+
+// Gaussian pulse of field_scatteredPulse (field.c:224-256) for delay tn = n - t0
+__device__ __forceinline__ float2 pulse(const FastInst &I, int c, int i, int j, float ie, float tn)
+{
+  const float r = (i + I.gx[c]) * I.cpc + (j + I.gy[c]) * I.spc - tn;
+  const float q = r * (1.0f / 50.0f);
+  const float a = I.dot[c] * __expf(-q * q) * (ie - 1.0f);
+  float sn, cs;
+  __sincosf(r * I.w, &sn, &cs);
+  return make_float2(a * cs, a * sn);
+}
+
+// source term of component c at step s for a scatterer cell: (value now, value one step earlier)
+template <int FLAV>
+__device__ __forceinline__ void source(const FastInst &I, const FastStep &st, int c, int i, int j, float ie, float2 &now,
+                                       float2 &delta)
+{
+  if (FLAV == FDTD_FLAVOUR_MPI) {
+    const float2 g = c_scale(c_mul(I.ex[i], I.ey[j]), ie - 1.0f);
+    now = c_mul(g, st.c);
+    delta = c_mul(g, st.dc);
+  } else {
+    const float tn = c == 0 ? st.tn : st.tn1;
+    const float tp = c == 0 ? st.tp : st.tp1;
+    now = pulse(I, c, i, j, ie, tn);
+    const float2 prev = (st.hasprev != 0.0f) ? pulse(I, c, i, j, ie, tp) : make_float2(0.f, 0.f);
+    delta = c_sub(now, prev);
+  }
+}
+
+template <int MODE, int FLAV, int R, int PML>
+__global__ __launch_bounds__(256) void k_fast(FastArgs a)
+{
+  __shared__ float2 xE[4][64];
+  __shared__ float2 xH[4][64];
+  __shared__ int s_tile;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int T = a.T, H0 = T + 1;
+  const int cur = a.cur, nxt = cur ^ 1;
+  // XCD-aware queue: workgroups are dispatched round-robin over the 8 XCDs,
+  // and queue q holds a spatially contiguous eighth of the tiles
+  const int q0 = blockIdx.x & 7;
+  int q = q0;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      int got = -1;
+      for (int k = 0; k < 8 && got < 0; k++) {
+        const int qq = (q + k) & 7;
+        const int lo = a.qbeg[qq], hi = a.qbeg[qq + 1];
+        if (lo >= hi) continue;
+        const int v = atomicAdd(&a.counters[qq], 1);
+        if (v < hi - lo) {
+          got = lo + v;
+          q = qq;
+        }
+      }
+      s_tile = got;
+    }
+    __syncthreads();
+    const int tid = s_tile;
+    __syncthreads();
+    if (tid < 0) return;
+    const FastTile t = a.tiles[tid];
+    const FastInst &I = a.inst[t.inst];
+    const FastStep *steps = a.steps + (size_t)t.inst * FAST_MAX_T;
+    const int NPX = I.NPX, NPY = I.NPY, P = I.pitch;
+    const int jj = t.j0 - H0 + lane;
+    const int ib = t.i0 - H0 + w * R;
+    const bool jin = jj >= 0 && jj < NPY;
+    const float c2 = I.c2;
+
+    float2 F0[R], F1[R], F2[R];
+    float ie0[R], ie1[R];
+    unsigned upd = 0, pml = 0, scat0 = 0, scat1 = 0;
+    float2 A0[PML ? R : 1], A1[PML ? R : 1], A2[PML ? R : 1], A3[PML ? R : 1], A4[PML ? R : 1], A5[PML ? R : 1];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int i = ib + r;
+      const bool in = jin && i >= 0 && i < NPX;
+      F0[r] = F1[r] = F2[r] = make_float2(0.f, 0.f);
+      ie0[r] = ie1[r] = 1.0f;
+      if (in) {
+        const size_t k = (size_t)i * P + jj;
+        F0[r] = I.f0[cur][k];
+        F1[r] = I.f1[cur][k];
+        F2[r] = I.f2[cur][k];
+        ie0[r] = I.ie0[k];
+        if (MODE == FDTD_MODE_TE) ie1[r] = I.ie1[k];
+        const bool u = (FLAV == FDTD_FLAVOUR_MPI) || (i >= 1 && i <= NPX - 2 && jj >= 1 && jj <= NPY - 2);
+        if (u) upd |= 1u << r;
+        if (a.src) {
+          if (ie0[r] != 1.0f) scat0 |= 1u << r;
+          if (MODE == FDTD_MODE_TE && ie1[r] != 1.0f) scat1 |= 1u << r;
+        }
+        if (PML && (I.pmlx[i] | I.pmly[jj])) {
+          pml |= 1u << r;
+          A0[r] = I.aux[0][cur][k];
+          A1[r] = I.aux[1][cur][k];
+          A2[r] = I.aux[2][cur][k];
+          A3[r] = I.aux[3][cur][k];
+          A4[r] = I.aux[4][cur][k];
+          A5[r] = I.aux[5][cur][k];
+        }
+      }
+      if (PML && !(pml & (1u << r))) {
+        A0[r] = A1[r] = A2[r] = A3[r] = A4[r] = A5[r] = make_float2(0.f, 0.f);
+      }
+    }
+    // TM: the plain Gaussian pulse and the MPI scattered wave act on Ez (component 0)
+    // TE: MPI scattered wave on Ey only (mpiTE_UPML.c:296); plain pulse on Ex and Ey
+    if (MODE == FDTD_MODE_TE && FLAV == FDTD_FLAVOUR_MPI) scat0 = 0;
+    if (FLAV == FDTD_FLAVOUR_PLAIN) {
+      if (!I.on[0]) scat0 = 0;
+      if (!I.on[1]) scat1 = 0;
+    }
+    // per-lane (column) coefficient tables of the UPML cells
+    float cy0 = 1.f, cy1 = 1.f, cy2 = 1.f, cy3 = 1.f, sy = 0.f;
+    if (PML && jin) {
+      cy0 = I.ty[0][jj]; cy1 = I.ty[1][jj]; cy2 = I.ty[2][jj]; cy3 = I.ty[3][jj]; sy = I.tsy[jj];
+    }
+
+    // ------------------------------------------------------------ half steps
+    auto e_phase = [&](const FastStep &st) {
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const int i = ib + r;
+        if (MODE == FDTD_MODE_TM) {
+          const float2 hym = r > 0 ? F2[r - 1] : (w > 0 ? xH[w - 1][lane] : make_float2(0.f, 0.f));
+          const float2 hxm = shfl_up1(F1[r]);
+          const float2 curl = c_sub(c_sub(F2[r], hym), c_sub(F1[r], hxm));
+          if (upd & (1u << r)) {
+            float2 sn = make_float2(0.f, 0.f), sd = make_float2(0.f, 0.f);
+            if (scat0 & (1u << r)) source<FLAV>(I, st, 0, i, jj, ie0[r], sn, sd);
+            if (PML && (pml & (1u << r))) {
+              const float cjz = I.tx[0][i], cjzh = I.tx[1][i];
+              const float2 nj = c_add(c_scale(A0[r], cjz), c_scale(curl, cjzh));
+              const float2 nd = c_add(c_scale(A1[r], cy0), c_scale(c_sub(nj, A0[r]), cy1));
+              A0[r] = nj;
+              A1[r] = nd;
+              F0[r] = c_add(c_scale(nd, ie0[r]), sn);
+            } else {
+              F0[r] = c_add(c_add(F0[r], c_scale(curl, ie0[r])), sd);
+            }
+          }
+        } else {
+          const float2 hzj = shfl_up1(F2[r]);
+          const float2 hzi = r > 0 ? F2[r - 1] : (w > 0 ? xH[w - 1][lane] : make_float2(0.f, 0.f));
+          const float2 cx = c_sub(F2[r], hzj);   // Hz - Hz(j-1)
+          const float2 cyv = c_sub(hzi, F2[r]);  // Hz(i-1) - Hz
+          if (upd & (1u << r)) {
+            float2 sn0 = make_float2(0.f, 0.f), sd0 = sn0, sn1 = sn0, sd1 = sn0;
+            if (scat0 & (1u << r)) source<FLAV>(I, st, 0, i, jj, ie0[r], sn0, sd0);
+            if (scat1 & (1u << r)) source<FLAV>(I, st, 1, i, jj, ie1[r], sn1, sd1);
+            if (PML && (pml & (1u << r))) {
+              // Jx/Dx: x tables 0 C_DXJX1 1 C_DXJX0; y: 0 C_JX 1 C_JXHZ
+              const float2 njx = c_add(c_scale(A0[r], cy0), c_scale(cx, cy1));
+              const float2 ndx = c_sub(c_add(A1[r], c_scale(njx, I.tx[0][i])), c_scale(A0[r], I.tx[1][i]));
+              // Jy/Dy: x table 2 C_DY; C_DYJY1/0 from sigma_x, sigma_y
+              const float2 njy = c_add(A2[r], cyv);
+              const float sxv = I.tsx[i];
+              const float c1 = (2.0f + sy) / (2.0f + sxv), c0 = (2.0f - sy) / (2.0f + sxv);
+              const float2 ndy = c_sub(c_add(c_scale(A3[r], I.tx[2][i]), c_scale(njy, c1)), c_scale(A2[r], c0));
+              A0[r] = njx; A1[r] = ndx; A2[r] = njy; A3[r] = ndy;
+              F0[r] = c_add(c_scale(ndx, ie0[r]), sn0);
+              F1[r] = c_add(c_scale(ndy, ie1[r]), sn1);
+            } else {
+              F0[r] = c_add(c_add(F0[r], c_scale(cx, ie0[r])), sd0);
+              F1[r] = c_add(c_add(F1[r], c_scale(cyv, ie1[r])), sd1);
+            }
+          }
+        }
+      }
+      // first row of E for the wave above (TM Ez, TE Ey)
+      xE[w][lane] = MODE == FDTD_MODE_TM ? F0[0] : F1[0];
+    };
+
+    auto h_phase = [&]() {
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const int i = ib + r;
+        if (MODE == FDTD_MODE_TM) {
+          const float2 ejp = shfl_dn1(F0[r]);
+          const float2 eip = r < R - 1 ? F0[r + 1] : (w < 3 ? xE[w + 1][lane] : make_float2(0.f, 0.f));
+          const float2 dEy = c_sub(ejp, F0[r]);  // Ez(j+1) - Ez
+          const float2 dEx = c_sub(eip, F0[r]);  // Ez(i+1) - Ez
+          if (upd & (1u << r)) {
+            if (PML && (pml & (1u << r))) {
+              // Mx/Bx: y 2 C_MX 3 C_MXEZ, x 2 C_BXMX1 3 C_BXMX0 ; My/By: x 4 C_BY, C_BYMY1/0
+              const float2 nmx = c_sub(c_scale(A2[r], cy2), c_scale(dEy, cy3));
+              const float2 nbx = c_sub(c_add(A3[r], c_scale(nmx, I.tx[2][i])), c_scale(A2[r], I.tx[3][i]));
+              const float2 nmy = c_add(A4[r], dEx);
+              const float sxv = I.tsx[i];
+              const float c1 = (2.0f + sy) / (2.0f + sxv), c0 = (2.0f - sy) / (2.0f + sxv);
+              const float2 nby = c_sub(c_add(c_scale(A5[r], I.tx[4][i]), c_scale(nmy, c1)), c_scale(A4[r], c0));
+              A2[r] = nmx; A3[r] = nbx; A4[r] = nmy; A5[r] = nby;
+              F1[r] = c_scale(nbx, c2);
+              F2[r] = c_scale(nby, c2);
+            } else {
+              F1[r] = c_sub(F1[r], c_scale(dEy, c2));
+              F2[r] = c_add(F2[r], c_scale(dEx, c2));
+            }
+          }
+        } else {
+          const float2 exj = shfl_dn1(F0[r]);
+          const float2 eyi = r < R - 1 ? F1[r + 1] : (w < 3 ? xE[w + 1][lane] : make_float2(0.f, 0.f));
+          // ((Ey(i+1) - Ey) - Ex(j+1)) + Ex
+          const float2 ce = c_add(c_sub(c_sub(eyi, F1[r]), exj), F0[r]);
+          if (upd & (1u << r)) {
+            if (PML && (pml & (1u << r))) {
+              // Mz/Bz: x 3 C_MZ 4 C_MZEXEY ; y 2 C_BZ 3 C_BZMZ
+              const float2 nmz = c_sub(c_scale(A4[r], I.tx[3][i]), c_scale(ce, I.tx[4][i]));
+              const float2 nbz = c_sub(c_add(c_scale(A5[r], cy2), c_scale(nmz, cy3)), c_scale(A4[r], cy3));
+              A4[r] = nmz; A5[r] = nbz;
+              F2[r] = c_scale(nbz, c2);
+            } else {
+              F2[r] = c_sub(F2[r], c_scale(ce, c2));
+            }
+          }
+        }
+      }
+      xH[w][lane] = F2[R - 1];  // last row of Hy (TM) / Hz (TE) for the wave below
+    };
+
+    if (FLAV == FDTD_FLAVOUR_MPI) {
+      xH[w][lane] = F2[R - 1];
+      __syncthreads();
+      for (int s = 0; s < T; s++) {
+        e_phase(steps[s]);
+        __syncthreads();
+        h_phase();
+        __syncthreads();
+      }
+    } else {
+      xE[w][lane] = MODE == FDTD_MODE_TM ? F0[0] : F1[0];
+      __syncthreads();
+      for (int s = 0; s < T; s++) {
+        h_phase();
+        __syncthreads();
+        e_phase(steps[s]);
+        __syncthreads();
+      }
+    }
+
+    // ------------------------------------------------------------ write back the core
+    if (jj >= t.j0 && jj < t.j0 + t.nj) {
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const int i = ib + r;
+        if (i < t.i0 || i >= t.i0 + t.ni) continue;
+        const size_t k = (size_t)i * P + jj;
+        I.f0[nxt][k] = F0[r];
+        I.f1[nxt][k] = F1[r];
+        I.f2[nxt][k] = F2[r];
+        if (PML && (pml & (1u << r))) {
+          I.aux[0][nxt][k] = A0[r];
+          I.aux[1][nxt][k] = A1[r];
+          I.aux[2][nxt][k] = A2[r];
+          I.aux[3][nxt][k] = A3[r];
+          I.aux[4][nxt][k] = A4[r];
+          I.aux[5][nxt][k] = A5[r];
+        }
+      }
+    }
+  }
+}
+
+// reset of the 16 queue counters (8 interior + 8 UPML); a kernel, not a memset, so the
+// step loop can be graph-captured
+__global__ void k_fast_reset_counters(int *c)
+{
+  if (threadIdx.x < 16) c[threadIdx.x] = 0;
+}
+
+// FP32 physical layout -> FP64 solver layout (MPI: padded, ghost ring untouched)
+__global__ __launch_bounds__(256) void k_fast_export(const float2 *src, int pitch, int NPX, int NPY, double2 *dst, int NPYa,
+                                                     int off)
+{
+  const int j = blockIdx.x * 64 + threadIdx.x;
+  const int i = blockIdx.y * 4 + threadIdx.y;
+  if (i >= NPX || j >= NPY) return;
+  const float2 v = src[(size_t)i * pitch + j];
+  dst[(size_t)(i + off) * NPYa + (j + off)] = make_double2((double)v.x, (double)v.y);
+}
+
+__global__ __launch_bounds__(256) void k_fast_import(const double2 *src, int NPYa, int off, int NPX, int NPY, float2 *dst,
+                                                     int pitch)
+{
+  const int j = blockIdx.x * 64 + threadIdx.x;
+  const int i = blockIdx.y * 4 + threadIdx.y;
+  if (i >= NPX || j >= NPY) return;
+  const double2 v = src[(size_t)(i + off) * NPYa + (j + off)];
+  dst[(size_t)i * pitch + j] = make_float2((float)v.x, (float)v.y);
+}
+
+// 1/eps in FP32 from the FP64 eps map (solver layout)
+__global__ __launch_bounds__(256) void k_fast_inv_eps(const double *eps, int NPYa, int off, int NPX, int NPY, float *ie, int pitch)
+{
+  const int j = blockIdx.x * 64 + threadIdx.x;
+  const int i = blockIdx.y * 4 + threadIdx.y;
+  if (i >= NPX || j >= NPY) return;
+  const double e = eps[(size_t)(i + off) * NPYa + (j + off)];
+  ie[(size_t)i * pitch + j] = (e == 1.0) ? 1.0f : (float)(1.0 / e);
+}
+
+// contour samples of one step from the FP32 fields, in the packing of k_ntff_gather
+__global__ __launch_bounds__(256) void k_fast_gather(const FastInst *inst, int cur, Contour ct, int off, int mode,
+                                                     double2 *cE, double2 *cH)
+{
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= ct.total) return;
+  const FastInst &I = inst[0];
+  int s = 0;
+  while (s < 3 && p >= ct.s[s + 1].off) s++;
+  const Seg &sg = ct.s[s];
+  const int c = p - sg.off;
+  const long ka = sg.k0 + (long)c * sg.dk;  // solver-layout index
+  const int NPYa = I.NPY + 2 * off;
+  const int ia = (int)(ka / NPYa), ja = (int)(ka % NPYa);
+  const int i = ia - off, j = ja - off;
+  auto get = [&](int slot, int di, int dj) -> float2 {
+    const int ii = i + di, jv = j + dj;
+    if (ii < 0 || jv < 0 || ii >= I.NPX || jv >= I.NPY) return make_float2(0.f, 0.f);
+    const size_t k = (size_t)ii * I.pitch + jv;
+    return slot == 0 ? I.f0[cur][k] : (slot == 3 ? I.f1[cur][k] : I.f2[cur][k]);
+  };
+  // slots in the FP64 numbering: TM Ez 0 Hx 3 Hy 6 ; TE Ex 0 Ey 3 Hz 6
+  const float2 e = get(sg.ef, 0, 0);
+  const int di = sg.hnb == -1 ? 0 : -1, dj = sg.hnb == -1 ? -1 : 0;
+  const float2 h0 = get(sg.hf, 0, 0), h1 = get(sg.hf, di, dj);
+  cE[p] = make_double2(sg.es * (double)e.x, sg.es * (double)e.y);
+  cH[p] = make_double2(sg.hs * ((double)h0.x + (double)h1.x), sg.hs * ((double)h0.y + (double)h1.y));
+  (void)mode;
+}
+
+// ---------------------------------------------------------------- host side
+
+#define HIPCHK(x)                                                                                    \
+  do {                                                                                               \
+    hipError_t e_ = (x);                                                                             \
+    if (e_ != hipSuccess) return fdtd_fail(FDTD_EHIP, "%s failed: %s", #x, hipGetErrorString(e_)); \
+  } while (0)
+
+template <int MODE, int FLAV, int R, int PML>
+static void launch_one(const FastArgs &a, int nblocks, hipStream_t st)
+{
+  hipLaunchKernelGGL((k_fast<MODE, FLAV, R, PML>), dim3(nblocks), dim3(256), 0, st, a);
+}
+
+typedef void (*fast_launcher)(const FastArgs &, int, hipStream_t);
+
+template <int MODE, int FLAV>
+static fast_launcher pick(int R, int pmlv)
+{
+  if (pmlv) return launch_one<MODE, FLAV, 8, 1>;
+  if (R == 4) return launch_one<MODE, FLAV, 4, 0>;
+  if (R == 8) return launch_one<MODE, FLAV, 8, 0>;
+  return launch_one<MODE, FLAV, 16, 0>;
+}
+
+int fast_launch(int mode, int flavour, int R, int pmlv, const FastArgs &a, int nblocks, hipStream_t st)
+{
+  fast_launcher f;
+  if (mode == FDTD_MODE_TM) f = flavour == FDTD_FLAVOUR_MPI ? pick<FDTD_MODE_TM, FDTD_FLAVOUR_MPI>(R, pmlv)
+                                                              : pick<FDTD_MODE_TM, FDTD_FLAVOUR_PLAIN>(R, pmlv);
+  else f = flavour == FDTD_FLAVOUR_MPI ? pick<FDTD_MODE_TE, FDTD_FLAVOUR_MPI>(R, pmlv) : pick<FDTD_MODE_TE, FDTD_FLAVOUR_PLAIN>(R, pmlv);
+  f(a, nblocks, st);
+  return FDTD_OK;
+}
+
+int fast_reset_counters(int *c, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_fast_reset_counters, dim3(1), dim3(64), 0, st, c);
+  return FDTD_OK;
+}
+
+int fast_export(const float2 *src, int pitch, int NPX, int NPY, double2 *dst, int NPYa, int off, hipStream_t st)
+{
+  dim3 blk(64, 4), grd((NPY + 63) / 64, (NPX + 3) / 4);
+  hipLaunchKernelGGL(k_fast_export, grd, blk, 0, st, src, pitch, NPX, NPY, dst, NPYa, off);
+  return FDTD_OK;
+}
+
+int fast_import(const double2 *src, int NPYa, int off, int NPX, int NPY, float2 *dst, int pitch, hipStream_t st)
+{
+  dim3 blk(64, 4), grd((NPY + 63) / 64, (NPX + 3) / 4);
+  hipLaunchKernelGGL(k_fast_import, grd, blk, 0, st, src, NPYa, off, NPX, NPY, dst, pitch);
+  return FDTD_OK;
+}
+
+int fast_inv_eps(const double *eps, int NPYa, int off, int NPX, int NPY, float *ie, int pitch, hipStream_t st)
+{
+  dim3 blk(64, 4), grd((NPY + 63) / 64, (NPX + 3) / 4);
+  hipLaunchKernelGGL(k_fast_inv_eps, grd, blk, 0, st, eps, NPYa, off, NPX, NPY, ie, pitch);
+  return FDTD_OK;
+}
+
+int fast_gather(const FastInst *inst, int cur, const Contour &ct, int off, int mode, double2 *cE, double2 *cH, hipStream_t st)
+{
+  if (ct.total <= 0) return FDTD_OK;
+  hipLaunchKernelGGL(k_fast_gather, dim3((ct.total + 255) / 256), dim3(256), 0, st, inst, cur, ct, off, mode, cE, cH);
+  return FDTD_OK;
+}
+
+// ---------------------------------------------------------------- FP32 instance state (host)
+
+struct FastState {
+  fdtd_grid_desc d;
+  FastInst h;            // host copy of the device instance struct
+  FastInst *dinst = nullptr;
+  std::vector<void *> allocs;
+  float *ftab = nullptr;
+  unsigned char *pmlflags = nullptr;
+  std::vector<unsigned char> pmlx, pmly;
+  float2 *exy = nullptr;
+  // tiles: [0, n_int) interior (no UPML cell within the loaded block), [n_int, ntiles) UPML
+  FastTile *dtiles = nullptr;
+  int n_int = 0, n_pml = 0;
+  int qint[9], qpml[9];
+  int *counters = nullptr;  // 16: interior queues, UPML queues
+  FastStep *dsteps = nullptr;
+  FastStep hsteps[FAST_MAX_T];
+  int T = 4, R = 16, tilesT = -1, tilesR = -1;
+  int cur = 0, pending = 0;
+  int src_on = 0;
+  double prev_c_re = 0, prev_c_im = 0;
+  int steps_since_reset = 0;
+  double last_angle = 1e300, last_k = 0;
+  double t0[2] = {0, 0};
+  double bytes_per_step = 0, cells_per_step = 0;
+  double bytes_int_step = 0, bytes_pml_step = 0;  // algorithmic bytes of the two tile lists
+};
+
+// Event timing of the fused launches.  Events are recorded around the interior
+// and the UPML launch of every block and resolved when the totals are read, so
+// timing adds no host-device synchronisation to the step loop.
+struct ProfRec {
+  hipEvent_t e0, e1, e2;
+  double bytes_int, bytes_pml, cells;
+};
+static struct {
+  int on = 0;
+  std::vector<ProfRec> recs;
+  double ms_int = 0, ms_pml = 0, bytes_int = 0, bytes_pml = 0, cells = 0;
+  long launches = 0;
+} g_prof;
+
+static void prof_resolve()
+{
+  for (ProfRec &r : g_prof.recs) {
+    float a = 0, b = 0;
+    (void)hipEventSynchronize(r.e2);
+    (void)hipEventElapsedTime(&a, r.e0, r.e1);
+    (void)hipEventElapsedTime(&b, r.e1, r.e2);
+    g_prof.ms_int += a;
+    g_prof.ms_pml += b;
+    g_prof.bytes_int += r.bytes_int;
+    g_prof.bytes_pml += r.bytes_pml;
+    g_prof.cells += r.cells;
+    g_prof.launches += 1;
+    (void)hipEventDestroy(r.e0);
+    (void)hipEventDestroy(r.e1);
+    (void)hipEventDestroy(r.e2);
+  }
+  g_prof.recs.clear();
+}
+
+void fast_profile_enable(int on)
+{
+  prof_resolve();
+  g_prof.on = on;
+  g_prof.ms_int = g_prof.ms_pml = g_prof.bytes_int = g_prof.bytes_pml = g_prof.cells = 0;
+  g_prof.launches = 0;
+}
+
+// out: ms_int, ms_pml, launches, bytes_int, bytes_pml, cell updates
+void fast_profile_read6(double out[6])
+{
+  prof_resolve();
+  out[0] = g_prof.ms_int;
+  out[1] = g_prof.ms_pml;
+  out[2] = (double)g_prof.launches;
+  out[3] = g_prof.bytes_int;
+  out[4] = g_prof.bytes_pml;
+  out[5] = g_prof.cells;
+}
+
+void fast_profile_read(double *ms, long *launches, double *bytes, double *cells)
+{
+  double o[6];
+  fast_profile_read6(o);
+  if (ms) *ms = o[0] + o[1];
+  if (launches) *launches = (long)o[2];
+  if (bytes) *bytes = o[3] + o[4];
+  if (cells) *cells = o[5];
+}
+
+template <class T>
+static int dev_alloc(FastState *s, T **p, size_t n)
+{
+  HIPCHK(hipMalloc((void **)p, std::max<size_t>(1, n) * sizeof(T)));
+  HIPCHK(hipMemset(*p, 0, std::max<size_t>(1, n) * sizeof(T)));
+  s->allocs.push_back((void *)*p);
+  return FDTD_OK;
+}
+
+int fast_state_create(const fdtd_grid_desc &d, FastState **out)
+{
+  FastState *s = new FastState();
+  *out = s;
+  s->d = d;
+  FastInst &h = s->h;
+  memset(&h, 0, sizeof h);
+  h.NPX = d.N_PX;
+  h.NPY = d.N_PY;
+  h.pitch = (d.N_PY + 63) / 64 * 64;
+  h.flavour = d.flavour;
+  h.c2 = (float)(1.0 / d.mu0);
+  const size_t n = (size_t)h.NPX * h.pitch;
+  int rc;
+  for (int b = 0; b < 2; b++) {
+    if ((rc = dev_alloc(s, &h.f0[b], n))) return rc;
+    if ((rc = dev_alloc(s, &h.f1[b], n))) return rc;
+    if ((rc = dev_alloc(s, &h.f2[b], n))) return rc;
+    for (int q = 0; q < 6; q++)
+      if ((rc = dev_alloc(s, &h.aux[q][b], n))) return rc;
+  }
+  float *ie0, *ie1;
+  if ((rc = dev_alloc(s, &ie0, n))) return rc;
+  if ((rc = dev_alloc(s, &ie1, n))) return rc;
+  h.ie0 = ie0;
+  h.ie1 = ie1;
+  if ((rc = dev_alloc(s, &s->ftab, 6 * (size_t)h.NPX + 5 * (size_t)h.NPY))) return rc;
+  for (int q = 0; q < 5; q++) h.tx[q] = s->ftab + q * (size_t)h.NPX;
+  h.tsx = s->ftab + 5 * (size_t)h.NPX;
+  for (int q = 0; q < 4; q++) h.ty[q] = s->ftab + 6 * (size_t)h.NPX + q * (size_t)h.NPY;
+  h.tsy = s->ftab + 6 * (size_t)h.NPX + 4 * (size_t)h.NPY;
+  if ((rc = dev_alloc(s, &s->pmlflags, (size_t)h.NPX + h.NPY))) return rc;
+  h.pmlx = s->pmlflags;
+  h.pmly = s->pmlflags + h.NPX;
+  if ((rc = dev_alloc(s, &s->exy, (size_t)h.NPX + h.NPY))) return rc;
+  h.ex = s->exy;
+  h.ey = s->exy + h.NPX;
+  if ((rc = dev_alloc(s, &s->dinst, 1))) return rc;
+  if ((rc = dev_alloc(s, &s->counters, 16))) return rc;
+  if ((rc = dev_alloc(s, &s->dsteps, FAST_MAX_T))) return rc;
+  s->pmlx.assign(h.NPX, 1);
+  s->pmly.assign(h.NPY, 1);
+  HIPCHK(hipMemcpy(s->dinst, &h, sizeof h, hipMemcpyHostToDevice));
+  return FDTD_OK;
+}
+
+void fast_state_destroy(FastState *s)
+{
+  if (!s) return;
+  for (void *p : s->allocs) (void)hipFree(p);
+  if (s->dtiles) (void)hipFree(s->dtiles);
+  delete s;
+}
+
+int fast_state_set_coefs(FastState *s, const fdtd_coef_tables *t)
+{
+  const int nx = s->h.NPX, ny = s->h.NPY;
+  std::vector<float> f(6 * (size_t)nx + 5 * (size_t)ny);
+  for (int q = 0; q < 5; q++)
+    for (int x = 0; x < nx; x++) f[q * (size_t)nx + x] = (float)t->x[q][x];
+  for (int x = 0; x < nx; x++) f[5 * (size_t)nx + x] = (float)t->sx[x];
+  for (int q = 0; q < 4; q++)
+    for (int y = 0; y < ny; y++) f[6 * (size_t)nx + q * (size_t)ny + y] = (float)t->y[q][y];
+  for (int y = 0; y < ny; y++) f[6 * (size_t)nx + 4 * (size_t)ny + y] = (float)t->sy[y];
+  HIPCHK(hipMemcpy(s->ftab, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
+  // a cell runs the UPML recurrence when any coefficient of its row or column differs from
+  // the sigma = 0 value (x tables at sigma 0: 1, 1, 1, 1, 1 ; y tables: 1, 1, 1, 1)
+  for (int x = 0; x < nx; x++) {
+    bool p = t->sx[x] != 0.0;
+    for (int q = 0; q < 5; q++) p = p || t->x[q][x] != 1.0;
+    s->pmlx[x] = p;
+  }
+  for (int y = 0; y < ny; y++) {
+    bool p = t->sy[y] != 0.0;
+    for (int q = 0; q < 4; q++) p = p || t->y[q][y] != 1.0;
+    s->pmly[y] = p;
+  }
+  std::vector<unsigned char> fl(nx + ny);
+  std::copy(s->pmlx.begin(), s->pmlx.end(), fl.begin());
+  std::copy(s->pmly.begin(), s->pmly.end(), fl.begin() + nx);
+  HIPCHK(hipMemcpy(s->pmlflags, fl.data(), fl.size(), hipMemcpyHostToDevice));
+  s->tilesT = -1;  // rebuild tiles
+  return FDTD_OK;
+}
+
+int fast_state_set_inv_eps(FastState *s, double *const *eps_dev, int NPYa, int off)
+{
+  fast_inv_eps(eps_dev[0], NPYa, off, s->h.NPX, s->h.NPY, (float *)s->h.ie0, s->h.pitch, 0);
+  if (s->d.mode == FDTD_MODE_TE) fast_inv_eps(eps_dev[1], NPYa, off, s->h.NPX, s->h.NPY, (float *)s->h.ie1, s->h.pitch, 0);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipDeviceSynchronize());
+  return FDTD_OK;
+}
+
+static void tile_band(std::vector<FastTile> &v, int i_lo, int i_hi, int j_lo, int j_hi, int CH, int CW)
+{
+  for (int i0 = i_lo; i0 < i_hi; i0 += CH)
+    for (int j0 = j_lo; j0 < j_hi; j0 += CW) {
+      FastTile t;
+      t.inst = 0;
+      t.i0 = i0;
+      t.j0 = j0;
+      t.ni = std::min(CH, i_hi - i0);
+      t.nj = std::min(CW, j_hi - j0);
+      v.push_back(t);
+    }
+}
+
+static void split_queues(int *q, int base, int n)
+{
+  for (int k = 0; k <= 8; k++) q[k] = base + (int)((long)n * k / 8);
+}
+
+static int build_tiles(FastState *s)
+{
+  const int T = s->T, R = s->R, H0 = T + 1;
+  const int NPX = s->h.NPX, NPY = s->h.NPY;
+  const int CW = 64 - 2 * T - 1, CHi = 4 * R - 2 * T - 1, CHp = 32 - 2 * T - 1;
+  if (CW < 1 || CHi < 1 || CHp < 1) return fdtd_fail(FDTD_EINVAL, "temporal block depth %d too deep for R=%d", T, R);
+  // extent of the cells without UPML recurrence
+  int xlo = 0, xhi = NPX, ylo = 0, yhi = NPY;
+  while (xlo < NPX && s->pmlx[xlo]) xlo++;
+  while (xhi > xlo && s->pmlx[xhi - 1]) xhi--;
+  while (ylo < NPY && s->pmly[ylo]) ylo++;
+  while (yhi > ylo && s->pmly[yhi - 1]) yhi--;
+  for (int x = xlo; x < xhi; x++)
+    if (s->pmlx[x]) xhi = xlo;  // UPML cells inside: treat everything as UPML
+  for (int y = ylo; y < yhi; y++)
+    if (s->pmly[y]) yhi = ylo;
+  const int I_lo = xlo + H0, I_hi = xhi - T, J_lo = ylo + H0, J_hi = yhi - T;
+  std::vector<FastTile> ti, tp;
+  if (I_lo < I_hi && J_lo < J_hi) {
+    tile_band(ti, I_lo, I_hi, J_lo, J_hi, CHi, CW);
+    tile_band(tp, 0, I_lo, 0, NPY, CHp, CW);
+    tile_band(tp, I_hi, NPX, 0, NPY, CHp, CW);
+    tile_band(tp, I_lo, I_hi, 0, J_lo, CHp, CW);
+    tile_band(tp, I_lo, I_hi, J_hi, NPY, CHp, CW);
+  } else {
+    tile_band(tp, 0, NPX, 0, NPY, CHp, CW);
+  }
+  auto byrow = [](const FastTile &a, const FastTile &b) { return a.i0 != b.i0 ? a.i0 < b.i0 : a.j0 < b.j0; };
+  std::sort(ti.begin(), ti.end(), byrow);
+  std::sort(tp.begin(), tp.end(), byrow);
+  s->n_int = (int)ti.size();
+  s->n_pml = (int)tp.size();
+  std::vector<FastTile> all(ti);
+  all.insert(all.end(), tp.begin(), tp.end());
+  if (s->dtiles) (void)hipFree(s->dtiles);
+  s->dtiles = nullptr;
+  HIPCHK(hipMalloc(&s->dtiles, std::max<size_t>(1, all.size()) * sizeof(FastTile)));
+  if (!all.empty()) HIPCHK(hipMemcpy(s->dtiles, all.data(), all.size() * sizeof(FastTile), hipMemcpyHostToDevice));
+  split_queues(s->qint, 0, s->n_int);
+  split_queues(s->qpml, s->n_int, s->n_pml);
+  // algorithmic bytes per step (4-byte reals): complex E,H read+write everywhere, UPML
+  // auxiliaries read+write in UPML cells, 1/eps read (TM 1, TE 2)
+  double ncell = 0, npml = 0;
+  const bool plain = s->d.flavour == FDTD_FLAVOUR_PLAIN;
+  for (int x = 0; x < NPX; x++)
+    for (int y = 0; y < NPY; y++) {
+      if (plain && (x == 0 || y == 0 || x == NPX - 1 || y == NPY - 1)) continue;
+      ncell += 1;
+      if (s->pmlx[x] || s->pmly[y]) npml += 1;
+    }
+  const double neps = s->d.mode == FDTD_MODE_TM ? 1 : 2;
+  s->bytes_per_step = 4.0 * 2 * 2 * (3 * ncell + 6 * npml) + 4.0 * neps * ncell;
+  // split by tile list: the interior tiles hold no UPML cell
+  double cint = 0;
+  for (const FastTile &t : ti) cint += (double)t.ni * t.nj;
+  s->bytes_int_step = (4.0 * 2 * 2 * 3 + 4.0 * neps) * cint;
+  s->bytes_pml_step = s->bytes_per_step - s->bytes_int_step;
+  s->cells_per_step = ncell;
+  s->tilesT = T;
+  s->tilesR = R;
+  return FDTD_OK;
+}
+
+int fast_state_set_blocking(FastState *s, int T, int R)
+{
+  if (T < 1 || T >= FAST_MAX_T) return fdtd_fail(FDTD_EINVAL, "T=%d out of range", T);
+  if (R != 4 && R != 8 && R != 16) return fdtd_fail(FDTD_EINVAL, "R=%d not in {4,8,16}", R);
+  int rc;
+  if ((rc = fast_state_flush(s))) return rc;
+  s->T = T;
+  s->R = R;
+  s->tilesT = -1;
+  return FDTD_OK;
+}
+
+void fast_state_get_blocking(const FastState *s, int *T, int *R)
+{
+  if (T) *T = s->T;
+  if (R) *R = s->R;
+}
+
+// wave tables e^{i x k cos}, e^{i y k sin} and pulse constants for the current angle
+static int update_wave(FastState *s, const fdtd_step_params &p)
+{
+  if (p.angle_deg == s->last_angle && p.k == s->last_k) return FDTD_OK;
+  const fdtd_grid_desc &d = s->d;
+  const double rad = p.angle_deg * M_PI / 180.0;
+  const double ks_cos = cos(rad) * p.k, ks_sin = sin(rad) * p.k;
+  std::vector<float2> t(s->h.NPX + s->h.NPY);
+  for (int x = 0; x < s->h.NPX; x++) t[x] = make_float2((float)cos(x * ks_cos), (float)sin(x * ks_cos));
+  for (int y = 0; y < s->h.NPY; y++) t[s->h.NPX + y] = make_float2((float)cos(y * ks_sin), (float)sin(y * ks_sin));
+  HIPCHK(hipMemcpy(s->exy, t.data(), t.size() * sizeof(float2), hipMemcpyHostToDevice));
+  FastInst &h = s->h;
+  const double cpc = cos(rad) / d.c0, spc = sin(rad) / d.c0;
+  h.cpc = (float)cpc;
+  h.spc = (float)spc;
+  h.w = (float)p.omega;
+  const double gx[2] = {d.mode == FDTD_MODE_TM ? 0.0 : 0.5, 0.0};
+  const double gy[2] = {0.0, d.mode == FDTD_MODE_TM ? 0.0 : 0.5};
+  for (int c = 0; c < 2; c++) {
+    h.gx[c] = (float)gx[c];
+    h.gy[c] = (float)gy[c];
+    const double center_peak = (d.N_PX / 2.0 + gx[c]) * cpc + (d.N_PY / 2 + gy[c]) * spc;
+    s->t0[c] = -center_peak + 500;  // field.c:236-241
+  }
+  if (d.mode == FDTD_MODE_TM) {
+    h.dot[0] = 1.0f;
+    h.on[0] = 1;
+    h.on[1] = 0;
+  } else {
+    h.dot[0] = (float)p.dot_x;
+    h.dot[1] = (float)p.dot_y;
+    h.on[0] = p.use_x;
+    h.on[1] = p.use_y;
+  }
+  HIPCHK(hipMemcpy(s->dinst, &h, sizeof h, hipMemcpyHostToDevice));
+  s->last_angle = p.angle_deg;
+  s->last_k = p.k;
+  return FDTD_OK;
+}
+
+int fast_state_flush(FastState *s)
+{
+  if (s->pending == 0) return FDTD_OK;
+  int rc;
+  const int Tsave = s->T;
+  // a partial block runs with T = pending; the tiles of the full depth stay valid
+  // for any smaller depth, so only the block depth passed to the kernel changes
+  if (s->tilesT != s->T || s->tilesR != s->R)
+    if ((rc = build_tiles(s))) return rc;
+  HIPCHK(hipMemcpy(s->dsteps, s->hsteps, s->pending * sizeof(FastStep), hipMemcpyHostToDevice));
+  FastArgs a;
+  memset(&a, 0, sizeof a);
+  a.inst = s->dinst;
+  a.T = s->pending;
+  a.src = s->src_on;
+  a.cur = s->cur;
+  a.steps = s->dsteps;
+  ProfRec pr;
+  if (g_prof.on) {
+    HIPCHK(hipEventCreate(&pr.e0));
+    HIPCHK(hipEventCreate(&pr.e1));
+    HIPCHK(hipEventCreate(&pr.e2));
+    HIPCHK(hipEventRecord(pr.e0, 0));
+  }
+  fast_reset_counters(s->counters, 0);
+  if (s->n_int > 0) {
+    a.tiles = s->dtiles;
+    memcpy(a.qbeg, s->qint, sizeof a.qbeg);
+    a.counters = s->counters;
+    fast_launch(s->d.mode, s->d.flavour, s->R, 0, a, std::min(s->n_int, 2048) / 8 * 8 + 8, 0);
+  }
+  if (g_prof.on) HIPCHK(hipEventRecord(pr.e1, 0));
+  if (s->n_pml > 0) {
+    a.tiles = s->dtiles;
+    memcpy(a.qbeg, s->qpml, sizeof a.qbeg);
+    a.counters = s->counters + 8;
+    fast_launch(s->d.mode, s->d.flavour, 8, 1, a, std::min(s->n_pml, 2048) / 8 * 8 + 8, 0);
+  }
+  HIPCHK(hipGetLastError());
+  if (g_prof.on) {
+    HIPCHK(hipEventRecord(pr.e2, 0));
+    pr.bytes_int = s->bytes_int_step * s->pending;
+    pr.bytes_pml = s->bytes_pml_step * s->pending;
+    pr.cells = s->cells_per_step * s->pending;
+    g_prof.recs.push_back(pr);
+    if (g_prof.recs.size() > 4096) prof_resolve();
+  }
+  s->cur ^= 1;
+  s->pending = 0;
+  s->T = Tsave;
+  return FDTD_OK;
+}
+
+int fast_state_step(FastState *s, const fdtd_step_params *p, int *launched)
+{
+  int rc;
+  if ((rc = update_wave(s, *p))) return rc;
+  FastStep &st = s->hsteps[s->pending];
+  memset(&st, 0, sizeof st);
+  // MPI scattered wave: ray_coef * e^{-i w t}, w*t formed as the reference does
+  const double wt = p->omega * p->time;
+  const double cre = p->ray_coef * cos(wt), cim = -p->ray_coef * sin(wt);
+  st.c = make_float2((float)cre, (float)cim);
+  st.dc = make_float2((float)(cre - s->prev_c_re), (float)(cim - s->prev_c_im));
+  s->prev_c_re = cre;
+  s->prev_c_im = cim;
+  // plain Gaussian pulse delays
+  st.tn = (float)(p->time - s->t0[0]);
+  st.tn1 = (float)(p->time - s->t0[1]);
+  st.tp = (float)((p->time - 1) - s->t0[0]);
+  st.tp1 = (float)((p->time - 1) - s->t0[1]);
+  st.hasprev = s->steps_since_reset > 0 ? 1.0f : 0.0f;
+  s->src_on = 1;
+  s->steps_since_reset++;
+  s->pending++;
+  if (launched) *launched = 0;
+  if (s->pending >= s->T) {
+    if ((rc = fast_state_flush(s))) return rc;
+    if (launched) *launched = 1;
+  }
+  return FDTD_OK;
+}
+
+// slot numbering of the FP64 grid: TM Ez Jz Dz Hx Mx Bx Hy My By / TE Ex Jx Dx Ey Jy Dy Hz Mz Bz
+static float2 *slot_ptr(FastState *s, int slot, int buf)
+{
+  FastInst &h = s->h;
+  switch (slot) {
+  case 0: return h.f0[buf];
+  case 3: return s->d.mode == FDTD_MODE_TM ? h.f1[buf] : h.f1[buf];
+  case 6: return h.f2[buf];
+  case 1: return h.aux[0][buf];
+  case 2: return h.aux[1][buf];
+  case 4: return h.aux[2][buf];
+  case 5: return h.aux[3][buf];
+  case 7: return h.aux[4][buf];
+  case 8: return h.aux[5][buf];
+  default: return nullptr;
+  }
+}
+
+int fast_state_export(FastState *s, int slot, double2 *dst, int NPYa, int off)
+{
+  int rc;
+  if ((rc = fast_state_flush(s))) return rc;
+  float2 *src = slot_ptr(s, slot, s->cur);
+  if (!src) return fdtd_fail(FDTD_EINVAL, "bad slot %d", slot);
+  fast_export(src, s->h.pitch, s->h.NPX, s->h.NPY, dst, NPYa, off, 0);
+  HIPCHK(hipGetLastError());
+  return FDTD_OK;
+}
+
+int fast_state_import(FastState *s, int slot, const double2 *src, int NPYa, int off)
+{
+  int rc;
+  if ((rc = fast_state_flush(s))) return rc;
+  float2 *dst = slot_ptr(s, slot, s->cur);
+  if (!dst) return fdtd_fail(FDTD_EINVAL, "bad slot %d", slot);
+  fast_import(src, NPYa, off, s->h.NPX, s->h.NPY, dst, s->h.pitch, 0);
+  HIPCHK(hipGetLastError());
+  return FDTD_OK;
+}
+
+int fast_state_reset(FastState *s)
+{
+  s->pending = 0;
+  const size_t n = (size_t)s->h.NPX * s->h.pitch * sizeof(float2);
+  for (int b = 0; b < 2; b++) {
+    HIPCHK(hipMemset(s->h.f0[b], 0, n));
+    HIPCHK(hipMemset(s->h.f1[b], 0, n));
+    HIPCHK(hipMemset(s->h.f2[b], 0, n));
+    for (int q = 0; q < 6; q++) HIPCHK(hipMemset(s->h.aux[q][b], 0, n));
+  }
+  s->cur = 0;
+  s->prev_c_re = s->prev_c_im = 0;
+  s->steps_since_reset = 0;
+  HIPCHK(hipDeviceSynchronize());
+  return FDTD_OK;
+}
+
+int fast_state_gather(FastState *s, const Contour &ct, int off, double2 *cE, double2 *cH)
+{
+  int rc;
+  if ((rc = fast_state_flush(s))) return rc;
+  fast_gather(s->dinst, s->cur, ct, off, s->d.mode, cE, cH, 0);
+  HIPCHK(hipGetLastError());
+  return FDTD_OK;
+}

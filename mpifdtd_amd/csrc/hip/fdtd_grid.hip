@@ -643,6 +643,7 @@ struct fdtd_grid {
   double *spec;
   double2 *tw;
   fdtd_model_state *mstate;
+  FastState *fs;  // FP32 fused path (precision FDTD_FP32); F[] then serve as FP64 export buffers
 };
 
 #define NT_GUARD NT_GUARD_DEV
@@ -784,7 +785,7 @@ extern "C" int fdtd_hip_grid_create(const fdtd_grid_desc *d, fdtd_grid **out)
 {
   if (!d || !out) return fdtd_fail(FDTD_EINVAL, "null argument");
   if (d->N_PX < 4 || d->N_PY < 4) return fdtd_fail(FDTD_EINVAL, "grid %dx%d too small", d->N_PX, d->N_PY);
-  if (d->precision != FDTD_FP64) return fdtd_fail(FDTD_EINVAL, "fdtd_grid implements the FP64 reference-order path only");
+  if (d->precision != FDTD_FP64 && d->precision != FDTD_FP32) return fdtd_fail(FDTD_EINVAL, "bad precision %d", d->precision);
   if (d->mode != FDTD_MODE_TM && d->mode != FDTD_MODE_TE) return fdtd_fail(FDTD_EINVAL, "bad mode %d", d->mode);
   if (d->flavour != FDTD_FLAVOUR_MPI && d->flavour != FDTD_FLAVOUR_PLAIN) return fdtd_fail(FDTD_EINVAL, "bad flavour");
   if (d->ntff_left < 1 || d->ntff_bottom < 1 || d->ntff_right >= d->N_PX || d->ntff_top >= d->N_PY ||
@@ -851,8 +852,39 @@ extern "C" int fdtd_hip_grid_create(const fdtd_grid_desc *d, fdtd_grid **out)
   HIPCHK(hipMalloc(&g->spec, 321 * 360 * sizeof(double)));
   HIPCHK(hipMalloc(&g->tw, 4096 * sizeof(double2)));
   HIPCHK(hipMalloc(&g->mstate, sizeof(fdtd_model_state)));
+  if (d->precision == FDTD_FP32) {
+    int rc = fast_state_create(*d, &g->fs);
+    if (rc) return rc;
+    // the per-step time-domain NTFF needs every step's contour samples: one step per launch
+    if (d->ntff_time && (rc = fast_state_set_blocking(g->fs, 1, 16))) return rc;
+  }
   return FDTD_OK;
 }
+
+extern "C" int fdtd_hip_grid_set_blocking(fdtd_grid *g, int T, int R)
+{
+  if (!g || !g->fs) return fdtd_fail(FDTD_EINVAL, "temporal blocking applies to FP32 grids only");
+  if (g->d.ntff_time && T != 1) return fdtd_fail(FDTD_EINVAL, "the per-step time-domain NTFF needs T = 1");
+  return fast_state_set_blocking(g->fs, T, R);
+}
+
+extern "C" int fdtd_hip_grid_get_blocking(const fdtd_grid *g, int *T, int *R)
+{
+  if (!g || !g->fs) return fdtd_fail(FDTD_EINVAL, "temporal blocking applies to FP32 grids only");
+  fast_state_get_blocking(g->fs, T, R);
+  return FDTD_OK;
+}
+
+extern "C" int fdtd_hip_grid_flush(fdtd_grid *g)
+{
+  if (!g) return fdtd_fail(FDTD_EINVAL, "null argument");
+  if (g->fs) return fast_state_flush(g->fs);
+  return FDTD_OK;
+}
+
+extern "C" void fdtd_hip_profile_enable(int on) { fast_profile_enable(on); }
+
+extern "C" void fdtd_hip_profile_read(double out[6]) { fast_profile_read6(out); }
 
 extern "C" int fdtd_hip_grid_destroy(fdtd_grid *g)
 {
@@ -875,6 +907,7 @@ extern "C" int fdtd_hip_grid_destroy(fdtd_grid *g)
   (void)hipFree(g->spec);
   (void)hipFree(g->tw);
   (void)hipFree(g->mstate);
+  fast_state_destroy(g->fs);
   delete g;
   return FDTD_OK;
 }
@@ -894,6 +927,7 @@ extern "C" int fdtd_hip_grid_set_coefs(fdtd_grid *g, const fdtd_coef_tables *t)
   if (!t->sx || !t->sy) return fdtd_fail(FDTD_EINVAL, "missing sigma tables");
   HIPCHK(hipMemcpy((void *)g->tabs.sx, t->sx, nx * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy((void *)g->tabs.sy, t->sy, ny * sizeof(double), hipMemcpyHostToDevice));
+  if (g->fs) return fast_state_set_coefs(g->fs, t);
   return FDTD_OK;
 }
 
@@ -906,6 +940,7 @@ extern "C" int fdtd_hip_grid_fill_eps(fdtd_grid *g, const fdtd_model_state *m)
                      g->d.mode == FDTD_MODE_TM ? 1 : 0, g->d.flavour == FDTD_FLAVOUR_PLAIN ? 1 : 0);
   HIPCHK(hipGetLastError());
   HIPCHK(hipDeviceSynchronize());
+  if (g->fs) return fast_state_set_inv_eps(g->fs, g->eps, g->NPYa, g->off);
   return FDTD_OK;
 }
 
@@ -913,6 +948,7 @@ extern "C" int fdtd_hip_grid_set_eps(fdtd_grid *g, int which, const double *host
 {
   if (!g || !host || which < 0 || which > 2) return fdtd_fail(FDTD_EINVAL, "bad argument");
   HIPCHK(hipMemcpy(g->eps[which], host, g->ncell * sizeof(double), hipMemcpyHostToDevice));
+  if (g->fs) return fast_state_set_inv_eps(g->fs, g->eps, g->NPYa, g->off);
   return FDTD_OK;
 }
 
@@ -927,6 +963,19 @@ extern "C" int fdtd_hip_grid_step(fdtd_grid *g, const fdtd_step_params *p)
 {
   if (!g || !p) return fdtd_fail(FDTD_EINVAL, "null argument");
   const fdtd_grid_desc &d = g->d;
+  if (g->fs) {
+    int launched = 0, rc = fast_state_step(g->fs, p, &launched);
+    if (rc) return rc;
+    if (d.ntff_time && g->ct.total > 0) {
+      // T = 1 here: the step has run; bin its contour samples as the FP64 path does
+      if ((rc = fast_state_gather(g->fs, g->ct, g->off, g->cE, g->cH))) return rc;
+      hipLaunchKernelGGL(k_ntff_bin, dim3(360), dim3(256), 0, 0, g->ct, g->cE, g->cH, g->tsTab, g->tsMin, g->tsMax,
+                         g->nt[0], g->nt[1], g->nt[2], d.array_size, p->time - 1, p->time - 0.5, g->coef,
+                         d.flavour == FDTD_FLAVOUR_PLAIN ? 0 : 1);
+      HIPCHK(hipGetLastError());
+    }
+    return FDTD_OK;
+  }
   GridPtrs gp;
   for (int f = 0; f < 9; f++) gp.F[f] = g->F[f];
   for (int e = 0; e < 3; e++) gp.eps[e] = g->eps[e];
@@ -1003,6 +1052,10 @@ extern "C" int fdtd_hip_grid_step(fdtd_grid *g, const fdtd_step_params *p)
 extern "C" int fdtd_hip_grid_get_field(fdtd_grid *g, int which, double *host)
 {
   if (!g || !host || which < 0 || which > 8) return fdtd_fail(FDTD_EINVAL, "bad argument");
+  if (g->fs) {
+    int rc = fast_state_export(g->fs, which, g->F[which], g->NPYa, g->off);
+    if (rc) return rc;
+  }
   HIPCHK(hipMemcpy(host, g->F[which], g->ncell * sizeof(double2), hipMemcpyDeviceToHost));
   return FDTD_OK;
 }
@@ -1011,6 +1064,7 @@ extern "C" int fdtd_hip_grid_set_field(fdtd_grid *g, int which, const double *ho
 {
   if (!g || !host || which < 0 || which > 8) return fdtd_fail(FDTD_EINVAL, "bad argument");
   HIPCHK(hipMemcpy(g->F[which], host, g->ncell * sizeof(double2), hipMemcpyHostToDevice));
+  if (g->fs) return fast_state_import(g->fs, which, g->F[which], g->NPYa, g->off);
   return FDTD_OK;
 }
 
@@ -1027,6 +1081,7 @@ extern "C" int fdtd_hip_grid_reset(fdtd_grid *g)
   for (int f = 0; f < 9; f++) HIPCHK(hipMemset(g->F[f], 0, g->ncell * sizeof(double2)));
   for (int a = 0; a < 3; a++) HIPCHK(hipMemset(g->ntb[a], 0, g->ntn * sizeof(double2)));
   HIPCHK(hipDeviceSynchronize());
+  if (g->fs) return fast_state_reset(g->fs);
   return FDTD_OK;
 }
 
@@ -1034,6 +1089,12 @@ extern "C" int fdtd_hip_grid_far_field(fdtd_grid *g, double k, double coef_re, d
 {
   if (!g || !out360) return fdtd_fail(FDTD_EINVAL, "null argument");
   const fdtd_grid_desc &d = g->d;
+  if (g->fs) {  // the far field of the FP32 fields is formed in FP64
+    for (int slot : {0, 3, 6}) {
+      int rc = fast_state_export(g->fs, slot, g->F[slot], g->NPYa, g->off);
+      if (rc) return rc;
+    }
+  }
   FFArgs a;
   a.N_PX = d.N_PX; a.N_PY = d.N_PY; a.NPXa = g->NPXa; a.NPYa = g->NPYa; a.off = g->off;
   a.top = d.ntff_top; a.bottom = d.ntff_bottom; a.left = d.ntff_left; a.right = d.ntff_right;

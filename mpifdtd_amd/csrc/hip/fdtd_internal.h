@@ -23,4 +23,70 @@ struct Contour {
 
 int fdtd_fail(int code, const char *fmt, ...);
 
+// ---------------------------------------------------------------- FP32 fast path
+// One core tile of the fused kernel: cells [i0, i0+ni) x [j0, j0+nj) of instance `inst`.
+struct FastTile {
+  int inst, i0, j0, ni, nj;
+};
+
+// Per inner step scalars of the sources (host-computed in FP64, rounded to FP32).
+struct FastStep {
+  float2 c;        // MPI: ray_coef * e^{-i w n}
+  float2 dc;       // c_n - c_{n-1}
+  float tn, tn1;   // plain: n - t0 of pulse component 0 / 1
+  float tp, tp1;   // plain: (n - 1) - t0
+  float hasprev;   // 0 on the first step after a reset (no previous source)
+  float pad;
+};
+
+#define FAST_MAX_T 16
+
+// Device view of one FP32 instance (arrays in the physical N_PX x N_PY layout, row pitch `pitch`).
+struct FastInst {
+  int NPX, NPY, pitch, flavour;
+  float c2;                 // 1/mu0 = c0^2
+  // fields, ping-pong [2]: TM: f0 = Ez, f1 = Hx, f2 = Hy ; TE: f0 = Ex, f1 = Ey, f2 = Hz
+  float2 *f0[2], *f1[2], *f2[2];
+  // UPML auxiliaries (touched only in PML cells), ping-pong
+  //   TM: a0 Jz a1 Dz a2 Mx a3 Bx a4 My a5 By ; TE: a0 Jx a1 Dx a2 Jy a3 Dy a4 Mz a5 Bz
+  float2 *aux[6][2];
+  const float *ie0, *ie1;   // 1/eps of the E components (TM: Ez ; TE: Ex, Ey)
+  // 1-D coefficient tables (FP32), same slot meaning as fdtd_coef_tables
+  const float *tx[5], *tsx, *ty[4], *tsy;
+  const unsigned char *pmlx, *pmly;  // 1 where the cell uses the UPML recurrence
+  const float2 *ex, *ey;    // e^{i kx x}, e^{i ky y} (MPI scattered wave)
+  float cpc, spc, w;        // plain pulse: cos/c0, sin/c0, omega
+  float gx[2], gy[2], dot[2];
+  int on[2];
+};
+
+struct FastArgs {
+  const FastInst *inst;     // device array of instances
+  const FastTile *tiles;    // tile list of this launch
+  int qbeg[9];              // 8 XCD queues: tiles [qbeg[q], qbeg[q+1])
+  int *counters;            // 8 queue counters, zeroed before the launch
+  int T;                    // steps per launch
+  int src;                  // 1: sources active
+  int cur;                  // ping-pong index of the input buffers
+  const FastStep *steps;    // [n_inst][FAST_MAX_T]
+};
+
+struct FastState;
+int fast_state_create(const fdtd_grid_desc &d, FastState **out);
+void fast_state_destroy(FastState *s);
+int fast_state_set_coefs(FastState *s, const fdtd_coef_tables *t);
+int fast_state_set_inv_eps(FastState *s, double *const *eps_dev, int NPYa, int off);
+int fast_state_step(FastState *s, const fdtd_step_params *p, int *launched);
+int fast_state_flush(FastState *s);
+int fast_state_export(FastState *s, int slot, double2 *dst_dev, int NPYa, int off);
+int fast_state_import(FastState *s, int slot, const double2 *src_dev, int NPYa, int off);
+int fast_state_reset(FastState *s);
+int fast_state_gather(FastState *s, const Contour &ct, int off, double2 *cE, double2 *cH);
+int fast_state_set_blocking(FastState *s, int T, int R);
+void fast_state_get_blocking(const FastState *s, int *T, int *R);
+// profiling of the fused launches: total ms, launches, algorithmic bytes moved
+void fast_profile_enable(int on);
+void fast_profile_read(double *ms, long *launches, double *bytes, double *cell_updates);
+void fast_profile_read6(double out[6]);
+
 #endif

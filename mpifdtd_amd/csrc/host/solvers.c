@@ -40,6 +40,8 @@ static solver_t slv_[SLV_N];
 static int active_ = -1;
 static int ntff_time_ = 1;
 static int file_output_ = 1;
+static int precision_ = FDTD_FP64;
+static int block_T_ = 0, block_R_ = 16;  /* 0: default depth */
 
 static const int slot_of_[3] = {0, 3, 6};
 
@@ -55,6 +57,15 @@ static int is_mpi(int kind) { return kind == SLV_MPI_TM || kind == SLV_MPI_TE; }
 
 void fdtd_solver_setNTFFTime(int on) { ntff_time_ = on ? 1 : 0; }
 void fdtd_solver_setFileOutput(int on) { file_output_ = on ? 1 : 0; }
+void fdtd_solver_setPrecision(int precision) { precision_ = precision == FDTD_FP32 ? FDTD_FP32 : FDTD_FP64; }
+int fdtd_solver_getPrecision(void) { return precision_; }
+int fdtd_solver_setBlocking(int T, int R)
+{
+  block_T_ = T;
+  block_R_ = R;
+  if (active_ >= 0 && precision_ == FDTD_FP32 && !ntff_time_) return fdtd_hip_grid_set_blocking(slv_[active_].g, T, R);
+  return FDTD_OK;
+}
 
 static void build_coefs(int kind, fdtd_grid *g)
 {
@@ -143,7 +154,7 @@ static void solver_init(int kind)
   memset(&d, 0, sizeof d);
   d.mode = is_tm(kind) ? FDTD_MODE_TM : FDTD_MODE_TE;
   d.flavour = is_mpi(kind) ? FDTD_FLAVOUR_MPI : FDTD_FLAVOUR_PLAIN;
-  d.precision = FDTD_FP64;
+  d.precision = precision_;
   d.N_PX = fs.N_PX;
   d.N_PY = fs.N_PY;
   d.N_PML = fs.N_PML;
@@ -171,6 +182,7 @@ static void solver_init(int kind)
   s->arraySize = dims[3];
   build_coefs(kind, s->g);
   CHECK(fdtd_hip_grid_fill_eps(s->g, models_getState()));
+  if (precision_ == FDTD_FP32 && !ntff_time_) CHECK(fdtd_hip_grid_set_blocking(s->g, block_T_ > 0 ? block_T_ : 4, block_R_));
   size_t n = (size_t)s->NPXa * s->NPYa;
   for (int q = 0; q < 3; q++) s->mirror[q] = (double complex *)calloc(n, sizeof(double complex));
   s->eps_mirror = (double *)calloc(n, sizeof(double));

@@ -42,7 +42,9 @@ class Simulator:
     """One simulation through the reference's API (one active solver per process)."""
 
     def __init__(self, solver, model, fi, params=None, c0=0.7071, ntff_inset=5, point_source=False,
-                 ntff_time=True, file_output=False):
+                 ntff_time=True, file_output=False, precision="fp64", T=None, R=16):
+        """precision "fp64": reference-order FP64 kernels; "fp32": fused FP32 kernels
+        advancing T steps per launch (T forced to 1 when ntff_time is on)."""
         L = N.lib()
         if L.fdtd_hip_device_count() < 1:
             raise RuntimeError("no HIP device visible: the solvers run on the GPU only")
@@ -55,7 +57,14 @@ class Simulator:
         L.field_setPointSource(1 if point_source else 0)
         L.fdtd_solver_setNTFFTime(1 if ntff_time else 0)
         L.fdtd_solver_setFileOutput(1 if file_output else 0)
-        L.simulator_init(fi)
+        self.precision = precision
+        L.fdtd_solver_setPrecision(1 if precision == "fp32" else 0)
+        if precision == "fp32":
+            L.fdtd_solver_setBlocking(int(T or 4), int(R))
+        try:
+            L.simulator_init(fi)
+        finally:
+            L.fdtd_solver_setPrecision(0)
         d = (ctypes.c_int * 4)()
         check(L.fdtd_solver_dims(d), "fdtd_solver_dims")
         self.NPXa, self.NPYa, self.off, self.arraySize = list(d)
@@ -65,6 +74,10 @@ class Simulator:
 
     def calc(self, n=1):
         N.lib().fdtd_simulator_run(int(n))
+
+    def set_blocking(self, T, R=16):
+        """FP32: steps per fused launch and rows per wavefront of the interior tiles."""
+        check(N.lib().fdtd_solver_setBlocking(int(T), int(R)), "fdtd_solver_setBlocking")
 
     @property
     def time(self):

@@ -1,0 +1,117 @@
+"""FP32 fused, temporally blocked path against the FP64 oracle.
+
+The FP32 path stores fields and coefficients in single precision; its error
+against the FP64 oracle is bounded here (fields and far field), and its
+temporal blocking is checked to be exact: the per-cell arithmetic does not
+depend on how many steps one launch advances, so any block depth T gives
+bit-identical results for the same kernel variant.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+FIELD_TOL = 2e-4   # relative L2 of FP32 fields after a few hundred steps
+FF_TOL = 1e-4      # relative L2 of the far field
+
+
+def _mods():
+    import mpifdtd_amd as m
+    from mpifdtd_amd import _native as N
+    from oracle import pyoracle as po
+    return m, N, po
+
+
+def run_fp32(m, N, solver, model, w, h, hu, pml, lam, ang, steps, T=4, R=16, ntff_time=False, params=None):
+    L = N.lib()
+    L.fdtd_solver_setPrecision.argtypes = [ctypes.c_int]
+    L.fdtd_solver_setBlocking.argtypes = [ctypes.c_int, ctypes.c_int]
+    L.fdtd_solver_setPrecision(1)
+    L.fdtd_solver_setBlocking(T, R)
+    try:
+        sim = m.Simulator(solver, model, m.field_info(w, h, hu, pml, lam, ang, steps),
+                          params=m.model_params(**(params or {})), ntff_time=ntff_time)
+    finally:
+        L.fdtd_solver_setPrecision(0)
+    sim.calc(steps)
+    return sim
+
+
+CASES = [
+    (4, 1, 4000, 4000, 50, 10, 500, 0, 400),     # MPI TM Mie
+    (5, 1, 4000, 3600, 50, 10, 550, 30, 400),    # MPI TE Mie, oblique, non-square
+    (2, 1, 4000, 4000, 50, 10, 500, 0, 700),     # plain TM pulse
+    (3, 1, 4000, 4000, 50, 10, 500, 0, 700),     # plain TE pulse
+    (4, 1, 6000, 6000, 20, 16, 500, 0, 500),     # MPI TM, 300x300 with a real interior region
+]
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_fp32_against_fp64_oracle(case):
+    m, N, po = _mods()
+    solver, model, w, h, hu, pml, lam, ang, steps = case
+    print(f"fp32 case {case}", flush=True)
+    sim = run_fp32(m, N, solver, model, w, h, hu, pml, lam, ang, steps)
+    try:
+        f = sim.fields()
+        ff = sim.far_field()
+    finally:
+        sim.finish()
+    o = po.Sim(width_nm=w, height_nm=h, h_u_nm=hu, pml=pml, lambda_nm=lam, angle_deg=ang, stepNum=steps,
+               solver=solver, model=model, ntff_time=0)
+    o.step(steps)
+    for k, which in enumerate((0, 3, 6)):
+        err = rel_l2(f[k], o.field(which))
+        assert err <= FIELD_TOL, (case, k, err)
+    assert rel_l2(ff, o.far_field()) <= FF_TOL
+
+
+@pytest.mark.parametrize("solver", [4, 5, 2, 3])
+def test_temporal_blocking_is_exact(solver):
+    m, N, _ = _mods()
+    args = (solver, 1, 6000, 5000, 20, 16, 500, 0, 120)
+    ref = None
+    for T in (1, 3, 4, 7):
+        sim = run_fp32(m, N, *args, T=T, R=16)
+        try:
+            f = sim.fields()
+        finally:
+            sim.finish()
+        if ref is None:
+            ref = f
+        else:
+            for a, b in zip(f, ref):
+                assert np.array_equal(a, b), (solver, T)
+
+
+def test_rows_per_wave_variants_agree():
+    m, N, _ = _mods()
+    args = (4, 1, 6000, 6000, 20, 16, 500, 0, 150)
+    out = []
+    for R in (4, 8, 16):
+        sim = run_fp32(m, N, *args, T=4, R=R)
+        try:
+            out.append(sim.fields())
+        finally:
+            sim.finish()
+    for f in out[1:]:
+        for a, b in zip(f, out[0]):
+            assert rel_l2(a, b) <= 1e-6
+
+
+def test_fp32_time_domain_ntff_against_oracle():
+    """FP32 fields with the per-step time-domain NTFF binned in FP64 (T = 1)."""
+    m, N, po = _mods()
+    args = (2, 1, 4000, 4000, 50, 10, 500, 0, 700)
+    sim = run_fp32(m, N, *args, T=1, ntff_time=True)
+    try:
+        sp = sim.time_spectra()
+    finally:
+        sim.finish()
+    o = po.Sim(width_nm=4000, height_nm=4000, h_u_nm=50, pml=10, lambda_nm=500, stepNum=700, solver=2, model=1)
+    o.step(700)
+    assert rel_l2(sp, o.time_spectra()) <= 1e-4
