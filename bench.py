@@ -72,8 +72,18 @@ def cpu_baseline(mode):
                       f"({dt:.1f} s on one host core; gcc -O2 oracle)"}
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     args = parse()
+    # the C host prints progress messages like the reference does (printf on stdout);
+    # keep stdout for the one JSON line by sending fd 1 to stderr until the end
+    import ctypes
+    libc = ctypes.CDLL(None)
+    real_stdout = os.dup(1)
+    os.dup2(2, 1)
     if args.gpus != 1:
         sys.exit(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU sharding is "
                  "described in DESIGN.md and not built")
@@ -91,9 +101,11 @@ def main():
     T, R = args.T, args.R
     tuned = None
     if args.precision == "fp32" and (T == 0 or R == 0):
+        log("autotune")
         tuned = autotune.tune(m, lambda: make_sim(m, "fp32", args.mode, 2000), probe_steps=240)
         T = T or tuned["T"]
         R = R or tuned["R"]
+    log(f"timed run: {args.precision} T={T} R={R} warmup={W} steps={K}")
     sim = make_sim(m, args.precision, args.mode, total, T or 4, R or 16)
     cells = sim.N_PX * sim.N_PY
     sim.calc(W)
@@ -135,6 +147,7 @@ def main():
     extra["spectra_per_s"] = round(1.0 / (spec_steps * ms_step * 1e-3 + t_ff), 3)
     if not args.no_check and args.precision == "fp32":
         # FP32 far field vs the FP64 reference-order GPU path on the same run length
+        log("fp64 reference-order run for the far-field check")
         ref = make_sim(m, "fp64", args.mode, total)
         t2 = time.perf_counter()
         ref.calc(total)
@@ -145,6 +158,8 @@ def main():
         extra["fp64_mcups"] = round(cells * total / t64 / 1e6, 1)
         extra["fp64_spectra_per_s"] = round(1.0 / (spec_steps * t64 / total), 3)
         extra["speedup_vs_fp64_spectra"] = round(extra["spectra_per_s"] / extra["fp64_spectra_per_s"], 2)
+    if not args.no_cpu:
+        log("cpu baseline (oracle, one core)")
     cpu = None if args.no_cpu else cpu_baseline(args.mode)
     line = {
         "metric": "Mcell-updates/s + % HBM roofline; spectra/s at <=1e-4 far-field err",
@@ -158,7 +173,10 @@ def main():
         "cpu_baseline": cpu,
     }
     line.update(extra)
-    print(json.dumps(line))
+    libc.fflush(None)
+    sys.stdout.flush()
+    os.dup2(real_stdout, 1)
+    print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":

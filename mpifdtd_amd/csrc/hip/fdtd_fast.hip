@@ -79,36 +79,22 @@ __global__ __launch_bounds__(256) void k_fast(FastArgs a)
 {
   __shared__ float2 xE[4][64];
   __shared__ float2 xH[4][64];
-  __shared__ int s_tile;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int T = a.T, H0 = T + 1;
   const int cur = a.cur, nxt = cur ^ 1;
-  // XCD-aware queue: workgroups are dispatched round-robin over the 8 XCDs,
-  // and queue q holds a spatially contiguous eighth of the tiles
-  const int q0 = blockIdx.x & 7;
-  int q = q0;
-  for (;;) {
-    if (threadIdx.x == 0) {
-      int got = -1;
-      for (int k = 0; k < 8 && got < 0; k++) {
-        const int qq = (q + k) & 7;
-        const int lo = a.qbeg[qq], hi = a.qbeg[qq + 1];
-        if (lo >= hi) continue;
-        const int v = atomicAdd(&a.counters[qq], 1);
-        if (v < hi - lo) {
-          got = lo + v;
-          q = qq;
-        }
-      }
-      s_tile = got;
-    }
-    __syncthreads();
-    const int tid = s_tile;
-    __syncthreads();
-    if (tid < 0) return;
+  // XCD-aware static schedule: workgroups are dispatched round-robin over the
+  // 8 XCDs, so workgroup b runs on XCD b % 8 and walks that XCD's spatially
+  // contiguous partition of the tile list [qbeg[q], qbeg[q+1]) with stride
+  // gridDim/8.  Neighbouring tiles (which read each other's halos) share an
+  // L2, and no device-scope atomic is needed to hand out work.
+  const int q = blockIdx.x & 7;
+  const int nb = gridDim.x >> 3;
+  for (int tid = a.qbeg[q] + (int)(blockIdx.x >> 3); tid < a.qbeg[q + 1]; tid += nb) {
     const FastTile t = a.tiles[tid];
     const FastInst &I = a.inst[t.inst];
-    const FastStep *steps = a.steps + (size_t)t.inst * FAST_MAX_T;
+    const int T = a.tinst ? a.tinst[t.inst] : a.T;
+    const int H0 = a.T + 1;  // halo of the tiling depth; T <= a.T
+    const FastStep *steps = a.steps ? a.steps + (size_t)t.inst * FAST_MAX_T : a.st;
+    __syncthreads();  // the previous tile's last reads of xE / xH are done
     const int NPX = I.NPX, NPY = I.NPY, P = I.pitch;
     const int jj = t.j0 - H0 + lane;
     const int ib = t.i0 - H0 + w * R;
@@ -310,13 +296,6 @@ __global__ __launch_bounds__(256) void k_fast(FastArgs a)
   }
 }
 
-// reset of the 16 queue counters (8 interior + 8 UPML); a kernel, not a memset, so the
-// step loop can be graph-captured
-__global__ void k_fast_reset_counters(int *c)
-{
-  if (threadIdx.x < 16) c[threadIdx.x] = 0;
-}
-
 // FP32 physical layout -> FP64 solver layout (MPI: padded, ghost ring untouched)
 __global__ __launch_bounds__(256) void k_fast_export(const float2 *src, int pitch, int NPX, int NPY, double2 *dst, int NPYa,
                                                      int off)
@@ -413,12 +392,6 @@ int fast_launch(int mode, int flavour, int R, int pmlv, const FastArgs &a, int n
   return FDTD_OK;
 }
 
-int fast_reset_counters(int *c, hipStream_t st)
-{
-  hipLaunchKernelGGL(k_fast_reset_counters, dim3(1), dim3(64), 0, st, c);
-  return FDTD_OK;
-}
-
 int fast_export(const float2 *src, int pitch, int NPX, int NPY, double2 *dst, int NPYa, int off, hipStream_t st)
 {
   dim3 blk(64, 4), grd((NPY + 63) / 64, (NPX + 3) / 4);
@@ -462,8 +435,6 @@ struct FastState {
   FastTile *dtiles = nullptr;
   int n_int = 0, n_pml = 0;
   int qint[9], qpml[9];
-  int *counters = nullptr;  // 16: interior queues, UPML queues
-  FastStep *dsteps = nullptr;
   FastStep hsteps[FAST_MAX_T];
   int T = 4, R = 16, tilesT = -1, tilesR = -1;
   int cur = 0, pending = 0;
@@ -587,8 +558,6 @@ int fast_state_create(const fdtd_grid_desc &d, FastState **out)
   h.ex = s->exy;
   h.ey = s->exy + h.NPX;
   if ((rc = dev_alloc(s, &s->dinst, 1))) return rc;
-  if ((rc = dev_alloc(s, &s->counters, 16))) return rc;
-  if ((rc = dev_alloc(s, &s->dsteps, FAST_MAX_T))) return rc;
   s->pmlx.assign(h.NPX, 1);
   s->pmly.assign(h.NPY, 1);
   HIPCHK(hipMemcpy(s->dinst, &h, sizeof h, hipMemcpyHostToDevice));
@@ -792,14 +761,15 @@ int fast_state_flush(FastState *s)
   // for any smaller depth, so only the block depth passed to the kernel changes
   if (s->tilesT != s->T || s->tilesR != s->R)
     if ((rc = build_tiles(s))) return rc;
-  HIPCHK(hipMemcpy(s->dsteps, s->hsteps, s->pending * sizeof(FastStep), hipMemcpyHostToDevice));
   FastArgs a;
   memset(&a, 0, sizeof a);
   a.inst = s->dinst;
   a.T = s->pending;
   a.src = s->src_on;
   a.cur = s->cur;
-  a.steps = s->dsteps;
+  a.steps = nullptr;
+  a.tinst = nullptr;
+  memcpy(a.st, s->hsteps, sizeof a.st);
   ProfRec pr;
   if (g_prof.on) {
     HIPCHK(hipEventCreate(&pr.e0));
@@ -807,19 +777,17 @@ int fast_state_flush(FastState *s)
     HIPCHK(hipEventCreate(&pr.e2));
     HIPCHK(hipEventRecord(pr.e0, 0));
   }
-  fast_reset_counters(s->counters, 0);
+  // one workgroup per tile: 8 partitions of ceil(n/8) tiles
   if (s->n_int > 0) {
     a.tiles = s->dtiles;
     memcpy(a.qbeg, s->qint, sizeof a.qbeg);
-    a.counters = s->counters;
-    fast_launch(s->d.mode, s->d.flavour, s->R, 0, a, std::min(s->n_int, 2048) / 8 * 8 + 8, 0);
+    fast_launch(s->d.mode, s->d.flavour, s->R, 0, a, 8 * ((s->n_int + 7) / 8), 0);
   }
   if (g_prof.on) HIPCHK(hipEventRecord(pr.e1, 0));
   if (s->n_pml > 0) {
     a.tiles = s->dtiles;
     memcpy(a.qbeg, s->qpml, sizeof a.qbeg);
-    a.counters = s->counters + 8;
-    fast_launch(s->d.mode, s->d.flavour, 8, 1, a, std::min(s->n_pml, 2048) / 8 * 8 + 8, 0);
+    fast_launch(s->d.mode, s->d.flavour, 8, 1, a, 8 * ((s->n_pml + 7) / 8), 0);
   }
   HIPCHK(hipGetLastError());
   if (g_prof.on) {

@@ -63,12 +63,13 @@ struct FastInst {
 struct FastArgs {
   const FastInst *inst;     // device array of instances
   const FastTile *tiles;    // tile list of this launch
-  int qbeg[9];              // 8 XCD queues: tiles [qbeg[q], qbeg[q+1])
-  int *counters;            // 8 queue counters, zeroed before the launch
+  int qbeg[9];              // 8 XCD partitions: tiles [qbeg[q], qbeg[q+1]) run on XCD q
   int T;                    // steps per launch
   int src;                  // 1: sources active
   int cur;                  // ping-pong index of the input buffers
-  const FastStep *steps;    // [n_inst][FAST_MAX_T]
+  const FastStep *steps;    // batched runs: [n_inst][FAST_MAX_T] in device memory, else null
+  const int *tinst;         // batched runs: steps of this block per instance, else null
+  FastStep st[FAST_MAX_T];  // single-instance runs: the block's steps, passed by value
 };
 
 struct FastState;
