@@ -101,6 +101,9 @@ void fdtd_solver_setPrecision(int precision);
 int fdtd_solver_getPrecision(void);
 /* FP32: steps per fused launch T and rows per wavefront R of the active / next solver */
 int fdtd_solver_setBlocking(int T, int R);
+/* blocking of the active solver; FDTD_ESTATE when it does not queue steps (FP64, or
+ * FP32 with the per-step time-domain NTFF, which runs one step per launch) */
+int fdtd_solver_getBlocking(int *T, int *R);
 
 #ifdef __cplusplus
 }

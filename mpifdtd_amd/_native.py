@@ -137,6 +137,7 @@ def lib():
     sig("fdtd_solver_setPrecision", V, I)
     sig("fdtd_solver_getPrecision", I)
     sig("fdtd_solver_setBlocking", I, I, I)
+    sig("fdtd_solver_getBlocking", I, ip, ip)
     sig("fdtd_hip_profile_enable", V, I)
     sig("fdtd_hip_profile_read", V, dp)
     sig("fdtd_hip_last_error", ctypes.c_char_p)

@@ -25,6 +25,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -36,49 +37,77 @@ __device__ __forceinline__ float2 c_add(float2 a, float2 b) { return make_float2
 __device__ __forceinline__ float2 c_sub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ float2 c_scale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
 __device__ __forceinline__ float2 c_mul(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+// Neighbours along j (the lane axis) through DPP wave shifts, a VALU operand
+// modifier with no LDS round trip (the compiler folds the move into the
+// subtraction that consumes it): wave_shr:1 gives lane l the value of lane
+// l-1, wave_shl:1 that of lane l+1, and the lane without a source reads 0.
+// Lane 0 and lane 63 hold halo columns: a garbage value entering there moves
+// one lane per step, and the core starts T+1 lanes in, so after T steps no core
+// cell has seen it.
 __device__ __forceinline__ float2 shfl_up1(float2 v)
 {
-  return make_float2(__shfl_up(v.x, 1, 64), __shfl_up(v.y, 1, 64));
+  return make_float2(__builtin_amdgcn_mov_dpp(v.x, 0x138, 0xf, 0xf, true),
+                     __builtin_amdgcn_mov_dpp(v.y, 0x138, 0xf, 0xf, true));
 }
 __device__ __forceinline__ float2 shfl_dn1(float2 v)
 {
-  return make_float2(__shfl_down(v.x, 1, 64), __shfl_down(v.y, 1, 64));
+  return make_float2(__builtin_amdgcn_mov_dpp(v.x, 0x130, 0xf, 0xf, true),
+                     __builtin_amdgcn_mov_dpp(v.y, 0x130, 0xf, 0xf, true));
 }
 
-// Gaussian pulse of field_scatteredPulse (field.c:224-256) for delay tn = n - t0
-__device__ __forceinline__ float2 pulse(const FastInst &I, int c, int i, int j, float ie, float tn)
+// Wave-uniform values read once into scalar registers (every lane holds the same value).
+__device__ __forceinline__ int uni_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ float uni_f(float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); }
+// A device-memory pointer held in scalar registers and typed as global memory,
+// so that loads and stores use the global instructions rather than flat ones
+// (the host pass of hipcc has no address spaces and sees a plain pointer).
+#ifdef __HIP_DEVICE_COMPILE__
+#define GPTR(T) __attribute__((address_space(1))) T *
+#else
+#define GPTR(T) T *
+#endif
+template <class P>
+__device__ __forceinline__ GPTR(P) uni_p(P *p)
 {
-  const float r = (i + I.gx[c]) * I.cpc + (j + I.gy[c]) * I.spc - tn;
+  const unsigned long long v = (unsigned long long)p;
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)v);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v >> 32));
+  return (GPTR(P))(((unsigned long long)hi << 32) | lo);
+}
+
+// Constants of the plain flavour's Gaussian pulse, held in registers for a tile.
+struct PulseC {
+  float cpc, spc, w;
+  float gx[2], gy[2], dot[2];
+};
+
+// Gaussian pulse of field_scatteredPulse (field.c:224-256) for delay tn = n - t0
+__device__ __forceinline__ float2 pulse(const PulseC &pc, int c, int i, int j, float ie, float tn)
+{
+  const float r = (i + pc.gx[c]) * pc.cpc + (j + pc.gy[c]) * pc.spc - tn;
   const float q = r * (1.0f / 50.0f);
-  const float a = I.dot[c] * __expf(-q * q) * (ie - 1.0f);
+  const float a = pc.dot[c] * __expf(-q * q) * (ie - 1.0f);
   float sn, cs;
-  __sincosf(r * I.w, &sn, &cs);
+  __sincosf(r * pc.w, &sn, &cs);
   return make_float2(a * cs, a * sn);
 }
 
-// source term of component c at step s for a scatterer cell: (value now, value one step earlier)
-template <int FLAV>
-__device__ __forceinline__ void source(const FastInst &I, const FastStep &st, int c, int i, int j, float ie, float2 &now,
-                                       float2 &delta)
-{
-  if (FLAV == FDTD_FLAVOUR_MPI) {
-    const float2 g = c_scale(c_mul(I.ex[i], I.ey[j]), ie - 1.0f);
-    now = c_mul(g, st.c);
-    delta = c_mul(g, st.dc);
-  } else {
-    const float tn = c == 0 ? st.tn : st.tn1;
-    const float tp = c == 0 ? st.tp : st.tp1;
-    now = pulse(I, c, i, j, ie, tn);
-    const float2 prev = (st.hasprev != 0.0f) ? pulse(I, c, i, j, ie, tp) : make_float2(0.f, 0.f);
-    delta = c_sub(now, prev);
-  }
-}
-
+// Every value the step loop needs beyond the cell state is staged once per
+// tile: the step scalars and the per-row tables in LDS (rows are uniform across
+// a wavefront, so each read is a broadcast), the per-column tables and the
+// pulse constants in registers.  The loop then issues no global load: the
+// kernel writes global memory, so loads from it cannot go through the scalar
+// cache and would otherwise be dependent vector round trips on every step.
 template <int MODE, int FLAV, int R, int PML>
 __global__ __launch_bounds__(256) void k_fast(FastArgs a)
 {
+  constexpr int NR = 4 * R;  // rows of the loaded block
   __shared__ float2 xE[4][64];
   __shared__ float2 xH[4][64];
+  __shared__ FastStep sst[FAST_MAX_T];
+  __shared__ float2 sex[NR];  // e^{i kx x} of the block rows (MPI scattered wave)
+  // UPML x tables of the block rows: tx0..tx4, 1/(2 + sigma_x)
+  __shared__ float srow[6][PML ? NR : 1];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int cur = a.cur, nxt = cur ^ 1;
   // XCD-aware static schedule: workgroups are dispatched round-robin over the
@@ -94,77 +123,197 @@ __global__ __launch_bounds__(256) void k_fast(FastArgs a)
     const int T = a.tinst ? a.tinst[t.inst] : a.T;
     const int H0 = a.T + 1;  // halo of the tiling depth; T <= a.T
     const FastStep *steps = a.steps ? a.steps + (size_t)t.inst * FAST_MAX_T : a.st;
-    __syncthreads();  // the previous tile's last reads of xE / xH are done
-    const int NPX = I.NPX, NPY = I.NPY, P = I.pitch;
+    __syncthreads();  // the previous tile's last reads of the shared arrays are done
+    // Instance scalars and pointers, read once into scalar registers.  The
+    // readfirstlane is convergent, so the compiler cannot sink these loads into
+    // the per-row branches below (where each row would wait on its own round trip).
+    const int NPX = uni_i(I.NPX), NPY = uni_i(I.NPY), P = uni_i(I.pitch);
+    GPTR(const float2) g0 = uni_p(I.f0[cur]);
+    GPTR(const float2) g1 = uni_p(I.f1[cur]);
+    GPTR(const float2) g2 = uni_p(I.f2[cur]);
+    GPTR(const float) gie0 = uni_p(I.ie0);
+    GPTR(const float) gie1 = uni_p(I.ie1);
+    GPTR(const float2) ga[6];
+    GPTR(const float2) gex = nullptr;
+    GPTR(const float2) gey = nullptr;
+    GPTR(const float) gtx[5];
+    GPTR(const float) gty[4];
+    GPTR(const float) gtsx = nullptr;
+    GPTR(const float) gtsy = nullptr;
+    GPTR(const unsigned char) gpx = nullptr;
+    GPTR(const unsigned char) gpy = nullptr;
+    if (FLAV == FDTD_FLAVOUR_MPI) {
+      gex = uni_p(I.ex);
+      gey = uni_p(I.ey);
+    }
+    if (PML) {
+#pragma unroll
+      for (int qq = 0; qq < 6; qq++) ga[qq] = uni_p(I.aux[qq][cur]);
+#pragma unroll
+      for (int qq = 0; qq < 5; qq++) gtx[qq] = uni_p(I.tx[qq]);
+#pragma unroll
+      for (int qq = 0; qq < 4; qq++) gty[qq] = uni_p(I.ty[qq]);
+      gtsx = uni_p(I.tsx);
+      gtsy = uni_p(I.tsy);
+      gpx = uni_p(I.pmlx);
+      gpy = uni_p(I.pmly);
+    }
     const int jj = t.j0 - H0 + lane;
-    const int ib = t.i0 - H0 + w * R;
+    const int ib0 = t.i0 - H0;  // first row of the block
+    const int ib = ib0 + w * R;
     const bool jin = jj >= 0 && jj < NPY;
-    const float c2 = I.c2;
+    const int jc = min(max(jj, 0), NPY - 1);  // clamped column: every load below stays in bounds
+    const float c2 = uni_f(I.c2);
+    if ((int)threadIdx.x < T) sst[threadIdx.x] = steps[threadIdx.x];
+    for (int r = threadIdx.x; r < NR; r += 256) {
+      const int i = ib0 + r;
+      const bool ok = i >= 0 && i < NPX;
+      const int ic = min(max(i, 0), NPX - 1);
+      if (FLAV == FDTD_FLAVOUR_MPI) {
+        const float2 e = gex[ic];
+        sex[r] = ok ? e : make_float2(0.f, 0.f);
+      }
+      if (PML) {
+        float v[5];
+#pragma unroll
+        for (int qq = 0; qq < 5; qq++) v[qq] = gtx[qq][ic];
+        const float sx = gtsx[ic];
+#pragma unroll
+        for (int qq = 0; qq < 5; qq++) srow[qq][r] = ok ? v[qq] : 1.0f;
+        srow[5][r] = 1.0f / (2.0f + (ok ? sx : 0.0f));
+      }
+    }
+    float2 eyl = make_float2(0.f, 0.f);
+    if (FLAV == FDTD_FLAVOUR_MPI) {
+      const float2 e = gey[jc];
+      if (jin) eyl = e;
+    }
+    PulseC pc;
+    if (FLAV == FDTD_FLAVOUR_PLAIN) {
+      pc.cpc = uni_f(I.cpc); pc.spc = uni_f(I.spc); pc.w = uni_f(I.w);
+      pc.gx[0] = uni_f(I.gx[0]); pc.gx[1] = uni_f(I.gx[1]); pc.gy[0] = uni_f(I.gy[0]); pc.gy[1] = uni_f(I.gy[1]);
+      pc.dot[0] = uni_f(I.dot[0]); pc.dot[1] = uni_f(I.dot[1]);
+    }
+    // per-lane (column) coefficient tables of the UPML cells
+    float cy0 = 1.f, cy1 = 1.f, cy2 = 1.f, cy3 = 1.f, sy = 0.f;
+    unsigned char pmly = 0;
+    if (PML) {
+      const float v0 = gty[0][jc], v1 = gty[1][jc], v2 = gty[2][jc], v3 = gty[3][jc], vs = gtsy[jc];
+      const unsigned char fl = gpy[jc];
+      if (jin) { cy0 = v0; cy1 = v1; cy2 = v2; cy3 = v3; sy = vs; pmly = fl; }
+    }
 
+    // the block's cells: every load is issued before any value is used
     float2 F0[R], F1[R], F2[R];
     float ie0[R], ie1[R];
-    unsigned upd = 0, pml = 0, scat0 = 0, scat1 = 0;
+    unsigned inb = 0, pml = 0, scat0 = 0, scat1 = 0;
     float2 A0[PML ? R : 1], A1[PML ? R : 1], A2[PML ? R : 1], A3[PML ? R : 1], A4[PML ? R : 1], A5[PML ? R : 1];
+    unsigned char pmlx[PML ? R : 1];
 #pragma unroll
     for (int r = 0; r < R; r++) {
       const int i = ib + r;
-      const bool in = jin && i >= 0 && i < NPX;
-      F0[r] = F1[r] = F2[r] = make_float2(0.f, 0.f);
-      ie0[r] = ie1[r] = 1.0f;
-      if (in) {
-        const size_t k = (size_t)i * P + jj;
-        F0[r] = I.f0[cur][k];
-        F1[r] = I.f1[cur][k];
-        F2[r] = I.f2[cur][k];
-        ie0[r] = I.ie0[k];
-        if (MODE == FDTD_MODE_TE) ie1[r] = I.ie1[k];
-        const bool u = (FLAV == FDTD_FLAVOUR_MPI) || (i >= 1 && i <= NPX - 2 && jj >= 1 && jj <= NPY - 2);
-        if (u) upd |= 1u << r;
-        if (a.src) {
-          if (ie0[r] != 1.0f) scat0 |= 1u << r;
-          if (MODE == FDTD_MODE_TE && ie1[r] != 1.0f) scat1 |= 1u << r;
-        }
-        if (PML && (I.pmlx[i] | I.pmly[jj])) {
-          pml |= 1u << r;
-          A0[r] = I.aux[0][cur][k];
-          A1[r] = I.aux[1][cur][k];
-          A2[r] = I.aux[2][cur][k];
-          A3[r] = I.aux[3][cur][k];
-          A4[r] = I.aux[4][cur][k];
-          A5[r] = I.aux[5][cur][k];
-        }
+      const int ic = min(max(i, 0), NPX - 1);
+      const unsigned k = (unsigned)(ic * P + jc);  // < 2^31: checked at allocation
+      F0[r] = g0[k];
+      F1[r] = g1[k];
+      F2[r] = g2[k];
+      ie0[r] = gie0[k];
+      ie1[r] = MODE == FDTD_MODE_TE ? gie1[k] : 1.0f;
+      if (PML) {
+        pmlx[r] = gpx[ic];
+        A0[r] = ga[0][k];
+        A1[r] = ga[1][k];
+        A2[r] = ga[2][k];
+        A3[r] = ga[3][k];
+        A4[r] = ga[4][k];
+        A5[r] = ga[5][k];
       }
-      if (PML && !(pml & (1u << r))) {
-        A0[r] = A1[r] = A2[r] = A3[r] = A4[r] = A5[r] = make_float2(0.f, 0.f);
+      if (jin && i >= 0 && i < NPX) inb |= 1u << r;
+    }
+    unsigned upd = 0;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int i = ib + r;
+      const bool in = inb & (1u << r);
+      if (!in) {
+        // outside the grid: the reference's ghost ring, zero and never updated
+        F0[r] = F1[r] = F2[r] = make_float2(0.f, 0.f);
+        ie0[r] = ie1[r] = 1.0f;
+      }
+      const bool u = in && ((FLAV == FDTD_FLAVOUR_MPI) || (i >= 1 && i <= NPX - 2 && jj >= 1 && jj <= NPY - 2));
+      if (u) upd |= 1u << r;
+      if (a.src) {
+        if (ie0[r] != 1.0f) scat0 |= 1u << r;
+        if (MODE == FDTD_MODE_TE && ie1[r] != 1.0f) scat1 |= 1u << r;
+      }
+      if (PML) {
+        if (in && (pmlx[r] | pmly)) pml |= 1u << r;
+        if (!(pml & (1u << r))) A0[r] = A1[r] = A2[r] = A3[r] = A4[r] = A5[r] = make_float2(0.f, 0.f);
       }
     }
     // TM: the plain Gaussian pulse and the MPI scattered wave act on Ez (component 0)
     // TE: MPI scattered wave on Ey only (mpiTE_UPML.c:296); plain pulse on Ex and Ey
     if (MODE == FDTD_MODE_TE && FLAV == FDTD_FLAVOUR_MPI) scat0 = 0;
     if (FLAV == FDTD_FLAVOUR_PLAIN) {
-      if (!I.on[0]) scat0 = 0;
-      if (!I.on[1]) scat1 = 0;
-    }
-    // per-lane (column) coefficient tables of the UPML cells
-    float cy0 = 1.f, cy1 = 1.f, cy2 = 1.f, cy3 = 1.f, sy = 0.f;
-    if (PML && jin) {
-      cy0 = I.ty[0][jj]; cy1 = I.ty[1][jj]; cy2 = I.ty[2][jj]; cy3 = I.ty[3][jj]; sy = I.tsy[jj];
+      if (!uni_i(I.on[0])) scat0 = 0;
+      if (!uni_i(I.on[1])) scat1 = 0;
     }
 
+    // source term of component c in row r of a scatterer cell: (value now, change since the previous step)
+    auto source = [&](int c, int r, float ie, const FastStep &st, float2 &now, float2 &delta) {
+      if (FLAV == FDTD_FLAVOUR_MPI) {
+        const float2 g = c_scale(c_mul(sex[w * R + r], eyl), ie - 1.0f);
+        now = c_mul(g, st.c);
+        delta = c_mul(g, st.dc);
+      } else {
+        const int i = ib + r;
+        const float tn = c == 0 ? st.tn : st.tn1;
+        const float tp = c == 0 ? st.tp : st.tp1;
+        now = pulse(pc, c, i, jj, ie, tn);
+        const float2 prev = (st.hasprev != 0.0f) ? pulse(pc, c, i, jj, ie, tp) : make_float2(0.f, 0.f);
+        delta = c_sub(now, prev);
+      }
+    };
+
     // ------------------------------------------------------------ half steps
-    auto e_phase = [&](const FastStep &st) {
+    // Interior tiles (PML == 0) hold only regular cells within the dependency
+    // cone of their core (the tiling keeps UPML, ghost and fixed-edge cells
+    // outside it), so every row is updated without masks; UPML tiles keep the
+    // per-cell update / UPML bits.  In the MPI flavour the scattered-wave change
+    // (s_n - s_{n-1}) = g * dc with g = e^{i k.r} (1/eps - 1) is added to every
+    // row of a wavefront that holds a scatterer cell (g = 0 in vacuum).
+    const unsigned smask = MODE == FDTD_MODE_TM ? scat0 : (scat0 | scat1);
+    const bool wsrc = __ballot(smask != 0) != 0;  // wavefront-uniform
+    auto src_mpi = [&](int r, float ie, const FastStep &st) -> float2 {
+      const float2 g = c_scale(c_mul(sex[w * R + r], eyl), ie - 1.0f);
+      return c_mul(g, st.dc);
+    };
+
+    auto e_rows = [&](const FastStep &st, auto SRCT) {
+      constexpr bool SRC = decltype(SRCT)::value;
 #pragma unroll
       for (int r = 0; r < R; r++) {
-        const int i = ib + r;
+        const int rr = w * R + r;  // row within the block (LDS tables)
+        const unsigned bit = 1u << r;
         if (MODE == FDTD_MODE_TM) {
           const float2 hym = r > 0 ? F2[r - 1] : (w > 0 ? xH[w - 1][lane] : make_float2(0.f, 0.f));
           const float2 hxm = shfl_up1(F1[r]);
           const float2 curl = c_sub(c_sub(F2[r], hym), c_sub(F1[r], hxm));
-          if (upd & (1u << r)) {
+          if (!PML) {
+            float2 e = c_add(F0[r], c_scale(curl, ie0[r]));
+            if (FLAV == FDTD_FLAVOUR_MPI) {
+              if (SRC) e = c_add(e, src_mpi(r, ie0[r], st));
+            } else if (scat0 & bit) {
+              float2 sn, sd;
+              source(0, r, ie0[r], st, sn, sd);
+              e = c_add(e, sd);
+            }
+            F0[r] = e;
+          } else if (upd & bit) {
             float2 sn = make_float2(0.f, 0.f), sd = make_float2(0.f, 0.f);
-            if (scat0 & (1u << r)) source<FLAV>(I, st, 0, i, jj, ie0[r], sn, sd);
-            if (PML && (pml & (1u << r))) {
-              const float cjz = I.tx[0][i], cjzh = I.tx[1][i];
+            if (scat0 & bit) source(0, r, ie0[r], st, sn, sd);
+            if (pml & bit) {
+              const float cjz = srow[0][rr], cjzh = srow[1][rr];
               const float2 nj = c_add(c_scale(A0[r], cjz), c_scale(curl, cjzh));
               const float2 nd = c_add(c_scale(A1[r], cy0), c_scale(c_sub(nj, A0[r]), cy1));
               A0[r] = nj;
@@ -179,19 +328,38 @@ __global__ __launch_bounds__(256) void k_fast(FastArgs a)
           const float2 hzi = r > 0 ? F2[r - 1] : (w > 0 ? xH[w - 1][lane] : make_float2(0.f, 0.f));
           const float2 cx = c_sub(F2[r], hzj);   // Hz - Hz(j-1)
           const float2 cyv = c_sub(hzi, F2[r]);  // Hz(i-1) - Hz
-          if (upd & (1u << r)) {
+          if (!PML) {
+            float2 ex = c_add(F0[r], c_scale(cx, ie0[r]));
+            float2 ey = c_add(F1[r], c_scale(cyv, ie1[r]));
+            if (FLAV == FDTD_FLAVOUR_MPI) {
+              if (SRC) ey = c_add(ey, src_mpi(r, ie1[r], st));
+            } else {
+              if (scat0 & bit) {
+                float2 sn, sd;
+                source(0, r, ie0[r], st, sn, sd);
+                ex = c_add(ex, sd);
+              }
+              if (scat1 & bit) {
+                float2 sn, sd;
+                source(1, r, ie1[r], st, sn, sd);
+                ey = c_add(ey, sd);
+              }
+            }
+            F0[r] = ex;
+            F1[r] = ey;
+          } else if (upd & bit) {
             float2 sn0 = make_float2(0.f, 0.f), sd0 = sn0, sn1 = sn0, sd1 = sn0;
-            if (scat0 & (1u << r)) source<FLAV>(I, st, 0, i, jj, ie0[r], sn0, sd0);
-            if (scat1 & (1u << r)) source<FLAV>(I, st, 1, i, jj, ie1[r], sn1, sd1);
-            if (PML && (pml & (1u << r))) {
+            if (scat0 & bit) source(0, r, ie0[r], st, sn0, sd0);
+            if (scat1 & bit) source(1, r, ie1[r], st, sn1, sd1);
+            if (pml & bit) {
               // Jx/Dx: x tables 0 C_DXJX1 1 C_DXJX0; y: 0 C_JX 1 C_JXHZ
               const float2 njx = c_add(c_scale(A0[r], cy0), c_scale(cx, cy1));
-              const float2 ndx = c_sub(c_add(A1[r], c_scale(njx, I.tx[0][i])), c_scale(A0[r], I.tx[1][i]));
-              // Jy/Dy: x table 2 C_DY; C_DYJY1/0 from sigma_x, sigma_y
+              const float2 ndx = c_sub(c_add(A1[r], c_scale(njx, srow[0][rr])), c_scale(A0[r], srow[1][rr]));
+              // Jy/Dy: x table 2 C_DY; C_DYJY1/0 = (2 +- sigma_y) / (2 + sigma_x)
               const float2 njy = c_add(A2[r], cyv);
-              const float sxv = I.tsx[i];
-              const float c1 = (2.0f + sy) / (2.0f + sxv), c0 = (2.0f - sy) / (2.0f + sxv);
-              const float2 ndy = c_sub(c_add(c_scale(A3[r], I.tx[2][i]), c_scale(njy, c1)), c_scale(A2[r], c0));
+              const float rsx = srow[5][rr];
+              const float c1 = (2.0f + sy) * rsx, c0 = (2.0f - sy) * rsx;
+              const float2 ndy = c_sub(c_add(c_scale(A3[r], srow[2][rr]), c_scale(njy, c1)), c_scale(A2[r], c0));
               A0[r] = njx; A1[r] = ndx; A2[r] = njy; A3[r] = ndy;
               F0[r] = c_add(c_scale(ndx, ie0[r]), sn0);
               F1[r] = c_add(c_scale(ndy, ie1[r]), sn1);
@@ -205,25 +373,34 @@ __global__ __launch_bounds__(256) void k_fast(FastArgs a)
       // first row of E for the wave above (TM Ez, TE Ey)
       xE[w][lane] = MODE == FDTD_MODE_TM ? F0[0] : F1[0];
     };
+    auto e_phase = [&](int s) {
+      const FastStep st = sst[s];
+      if (FLAV == FDTD_FLAVOUR_MPI && wsrc) e_rows(st, std::true_type());
+      else e_rows(st, std::false_type());
+    };
 
     auto h_phase = [&]() {
 #pragma unroll
       for (int r = 0; r < R; r++) {
-        const int i = ib + r;
+        const int rr = w * R + r;
+        const unsigned bit = 1u << r;
         if (MODE == FDTD_MODE_TM) {
           const float2 ejp = shfl_dn1(F0[r]);
           const float2 eip = r < R - 1 ? F0[r + 1] : (w < 3 ? xE[w + 1][lane] : make_float2(0.f, 0.f));
           const float2 dEy = c_sub(ejp, F0[r]);  // Ez(j+1) - Ez
           const float2 dEx = c_sub(eip, F0[r]);  // Ez(i+1) - Ez
-          if (upd & (1u << r)) {
-            if (PML && (pml & (1u << r))) {
+          if (!PML) {
+            F1[r] = c_sub(F1[r], c_scale(dEy, c2));
+            F2[r] = c_add(F2[r], c_scale(dEx, c2));
+          } else if (upd & bit) {
+            if (pml & bit) {
               // Mx/Bx: y 2 C_MX 3 C_MXEZ, x 2 C_BXMX1 3 C_BXMX0 ; My/By: x 4 C_BY, C_BYMY1/0
               const float2 nmx = c_sub(c_scale(A2[r], cy2), c_scale(dEy, cy3));
-              const float2 nbx = c_sub(c_add(A3[r], c_scale(nmx, I.tx[2][i])), c_scale(A2[r], I.tx[3][i]));
+              const float2 nbx = c_sub(c_add(A3[r], c_scale(nmx, srow[2][rr])), c_scale(A2[r], srow[3][rr]));
               const float2 nmy = c_add(A4[r], dEx);
-              const float sxv = I.tsx[i];
-              const float c1 = (2.0f + sy) / (2.0f + sxv), c0 = (2.0f - sy) / (2.0f + sxv);
-              const float2 nby = c_sub(c_add(c_scale(A5[r], I.tx[4][i]), c_scale(nmy, c1)), c_scale(A4[r], c0));
+              const float rsx = srow[5][rr];
+              const float c1 = (2.0f + sy) * rsx, c0 = (2.0f - sy) * rsx;
+              const float2 nby = c_sub(c_add(c_scale(A5[r], srow[4][rr]), c_scale(nmy, c1)), c_scale(A4[r], c0));
               A2[r] = nmx; A3[r] = nbx; A4[r] = nmy; A5[r] = nby;
               F1[r] = c_scale(nbx, c2);
               F2[r] = c_scale(nby, c2);
@@ -237,10 +414,12 @@ __global__ __launch_bounds__(256) void k_fast(FastArgs a)
           const float2 eyi = r < R - 1 ? F1[r + 1] : (w < 3 ? xE[w + 1][lane] : make_float2(0.f, 0.f));
           // ((Ey(i+1) - Ey) - Ex(j+1)) + Ex
           const float2 ce = c_add(c_sub(c_sub(eyi, F1[r]), exj), F0[r]);
-          if (upd & (1u << r)) {
-            if (PML && (pml & (1u << r))) {
+          if (!PML) {
+            F2[r] = c_sub(F2[r], c_scale(ce, c2));
+          } else if (upd & bit) {
+            if (pml & bit) {
               // Mz/Bz: x 3 C_MZ 4 C_MZEXEY ; y 2 C_BZ 3 C_BZMZ
-              const float2 nmz = c_sub(c_scale(A4[r], I.tx[3][i]), c_scale(ce, I.tx[4][i]));
+              const float2 nmz = c_sub(c_scale(A4[r], srow[3][rr]), c_scale(ce, srow[4][rr]));
               const float2 nbz = c_sub(c_add(c_scale(A5[r], cy2), c_scale(nmz, cy3)), c_scale(A4[r], cy3));
               A4[r] = nmz; A5[r] = nbz;
               F2[r] = c_scale(nbz, c2);
@@ -257,7 +436,7 @@ __global__ __launch_bounds__(256) void k_fast(FastArgs a)
       xH[w][lane] = F2[R - 1];
       __syncthreads();
       for (int s = 0; s < T; s++) {
-        e_phase(steps[s]);
+        e_phase(s);
         __syncthreads();
         h_phase();
         __syncthreads();
@@ -268,28 +447,36 @@ __global__ __launch_bounds__(256) void k_fast(FastArgs a)
       for (int s = 0; s < T; s++) {
         h_phase();
         __syncthreads();
-        e_phase(steps[s]);
+        e_phase(s);
         __syncthreads();
       }
     }
 
     // ------------------------------------------------------------ write back the core
+    GPTR(float2) o0 = uni_p(I.f0[nxt]);
+    GPTR(float2) o1 = uni_p(I.f1[nxt]);
+    GPTR(float2) o2 = uni_p(I.f2[nxt]);
+    GPTR(float2) oa[6];
+    if (PML) {
+#pragma unroll
+      for (int qq = 0; qq < 6; qq++) oa[qq] = uni_p(I.aux[qq][nxt]);
+    }
     if (jj >= t.j0 && jj < t.j0 + t.nj) {
 #pragma unroll
       for (int r = 0; r < R; r++) {
         const int i = ib + r;
         if (i < t.i0 || i >= t.i0 + t.ni) continue;
-        const size_t k = (size_t)i * P + jj;
-        I.f0[nxt][k] = F0[r];
-        I.f1[nxt][k] = F1[r];
-        I.f2[nxt][k] = F2[r];
+        const unsigned k = (unsigned)(i * P + jj);
+        o0[k] = F0[r];
+        o1[k] = F1[r];
+        o2[k] = F2[r];
         if (PML && (pml & (1u << r))) {
-          I.aux[0][nxt][k] = A0[r];
-          I.aux[1][nxt][k] = A1[r];
-          I.aux[2][nxt][k] = A2[r];
-          I.aux[3][nxt][k] = A3[r];
-          I.aux[4][nxt][k] = A4[r];
-          I.aux[5][nxt][k] = A5[r];
+          oa[0][k] = A0[r];
+          oa[1][k] = A1[r];
+          oa[2][k] = A2[r];
+          oa[3][k] = A3[r];
+          oa[4][k] = A4[r];
+          oa[5][k] = A5[r];
         }
       }
     }
@@ -533,6 +720,8 @@ int fast_state_create(const fdtd_grid_desc &d, FastState **out)
   h.flavour = d.flavour;
   h.c2 = (float)(1.0 / d.mu0);
   const size_t n = (size_t)h.NPX * h.pitch;
+  // the kernels index a field with a 32-bit cell offset
+  if (n >= ((size_t)1 << 28)) return fdtd_fail(FDTD_EINVAL, "grid of %zu cells exceeds the FP32 path's 2^28 limit", n);
   int rc;
   for (int b = 0; b < 2; b++) {
     if ((rc = dev_alloc(s, &h.f0[b], n))) return rc;
@@ -647,6 +836,18 @@ static int build_tiles(FastState *s)
     if (s->pmlx[x]) xhi = xlo;  // UPML cells inside: treat everything as UPML
   for (int y = ylo; y < yhi; y++)
     if (s->pmly[y]) yhi = ylo;
+  if (s->d.flavour == FDTD_FLAVOUR_PLAIN) {
+    // the plain solvers never update their edge cells (fdtdTM_upml.c:158-159): keep
+    // them out of the interior tiles' dependency cones, which the interior kernel
+    // updates without masks
+    xlo = std::max(xlo, 1);
+    xhi = std::min(xhi, NPX - 1);
+    ylo = std::max(ylo, 1);
+    yhi = std::min(yhi, NPY - 1);
+  }
+  // Interior tile cores [I_lo, I_hi) x [J_lo, J_hi): with the block loaded from
+  // T+1 rows below to T rows above the core, every cell whose update a core
+  // cell depends on lies in [xlo+1, xhi-1] x [ylo+1, yhi-1].
   const int I_lo = xlo + H0, I_hi = xhi - T, J_lo = ylo + H0, J_hi = yhi - T;
   std::vector<FastTile> ti, tp;
   if (I_lo < I_hi && J_lo < J_hi) {

@@ -34,6 +34,7 @@ typedef struct {
   int NPXa, NPYa, off, arraySize;
   double complex *mirror[3];  /* TM: Ez Hx Hy   TE: Ex Ey Hz */
   double *eps_mirror;
+  int blocked;  /* FP32 grid without the per-step NTFF: steps are queued and run T per launch */
 } solver_t;
 
 static solver_t slv_[SLV_N];
@@ -63,8 +64,15 @@ int fdtd_solver_setBlocking(int T, int R)
 {
   block_T_ = T;
   block_R_ = R;
-  if (active_ >= 0 && precision_ == FDTD_FP32 && !ntff_time_) return fdtd_hip_grid_set_blocking(slv_[active_].g, T, R);
+  /* the active solver's own precision decides, not the setting for the next init */
+  if (active_ >= 0 && slv_[active_].blocked) return fdtd_hip_grid_set_blocking(slv_[active_].g, T, R);
   return FDTD_OK;
+}
+
+int fdtd_solver_getBlocking(int *T, int *R)
+{
+  if (active_ < 0 || !slv_[active_].blocked) return FDTD_ESTATE;
+  return fdtd_hip_grid_get_blocking(slv_[active_].g, T, R);
 }
 
 static void build_coefs(int kind, fdtd_grid *g)
@@ -182,7 +190,8 @@ static void solver_init(int kind)
   s->arraySize = dims[3];
   build_coefs(kind, s->g);
   CHECK(fdtd_hip_grid_fill_eps(s->g, models_getState()));
-  if (precision_ == FDTD_FP32 && !ntff_time_) CHECK(fdtd_hip_grid_set_blocking(s->g, block_T_ > 0 ? block_T_ : 4, block_R_));
+  s->blocked = precision_ == FDTD_FP32 && !ntff_time_;
+  if (s->blocked) CHECK(fdtd_hip_grid_set_blocking(s->g, block_T_ > 0 ? block_T_ : 4, block_R_));
   size_t n = (size_t)s->NPXa * s->NPYa;
   for (int q = 0; q < 3; q++) s->mirror[q] = (double complex *)calloc(n, sizeof(double complex));
   s->eps_mirror = (double *)calloc(n, sizeof(double));

@@ -79,6 +79,12 @@ class Simulator:
         """FP32: steps per fused launch and rows per wavefront of the interior tiles."""
         check(N.lib().fdtd_solver_setBlocking(int(T), int(R)), "fdtd_solver_setBlocking")
 
+    def blocking(self):
+        """(T, R) of this FP32 solver."""
+        t, r = ctypes.c_int(), ctypes.c_int()
+        check(N.lib().fdtd_solver_getBlocking(ctypes.byref(t), ctypes.byref(r)), "fdtd_solver_getBlocking")
+        return t.value, r.value
+
     @property
     def time(self):
         return N.lib().field_getTime()

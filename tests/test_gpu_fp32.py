@@ -6,8 +6,6 @@ temporal blocking is checked to be exact: the per-cell arithmetic does not
 depend on how many steps one launch advances, so any block depth T gives
 bit-identical results for the same kernel variant.
 """
-import ctypes
-
 import numpy as np
 import pytest
 
@@ -27,16 +25,11 @@ def _mods():
 
 
 def run_fp32(m, N, solver, model, w, h, hu, pml, lam, ang, steps, T=4, R=16, ntff_time=False, params=None):
-    L = N.lib()
-    L.fdtd_solver_setPrecision.argtypes = [ctypes.c_int]
-    L.fdtd_solver_setBlocking.argtypes = [ctypes.c_int, ctypes.c_int]
-    L.fdtd_solver_setPrecision(1)
-    L.fdtd_solver_setBlocking(T, R)
-    try:
-        sim = m.Simulator(solver, model, m.field_info(w, h, hu, pml, lam, ang, steps),
-                          params=m.model_params(**(params or {})), ntff_time=ntff_time)
-    finally:
-        L.fdtd_solver_setPrecision(0)
+    sim = m.Simulator(solver, model, m.field_info(w, h, hu, pml, lam, ang, steps),
+                      params=m.model_params(**(params or {})), ntff_time=ntff_time, precision="fp32", T=T, R=R)
+    assert N.lib().fdtd_solver_getPrecision() == 0  # the setting for the next init is restored
+    if not ntff_time:
+        assert sim.blocking() == (T, R)  # the solver runs the fused FP32 kernels
     sim.calc(steps)
     return sim
 
@@ -115,3 +108,45 @@ def test_fp32_time_domain_ntff_against_oracle():
     o = po.Sim(width_nm=4000, height_nm=4000, h_u_nm=50, pml=10, lambda_nm=500, stepNum=700, solver=2, model=1)
     o.step(700)
     assert rel_l2(sp, o.time_spectra()) <= 1e-4
+
+
+def test_blocking_change_mid_run():
+    """setBlocking on a live FP32 solver takes effect and keeps the trajectory
+    exact: the depth only changes how steps are grouped into launches."""
+    m, N, _ = _mods()
+    args = (4, 1, 6000, 5000, 20, 16, 500, 0, 130)
+    sim = run_fp32(m, N, *args[:-1], 0, T=4, R=16)
+    try:
+        assert sim.blocking() == (4, 16)
+        sim.calc(37)
+        sim.set_blocking(7, 16)
+        assert sim.blocking() == (7, 16)
+        sim.calc(93)
+        f = sim.fields()
+    finally:
+        sim.finish()
+    ref = run_fp32(m, N, *args, T=4, R=16)
+    try:
+        g = ref.fields()
+    finally:
+        ref.finish()
+    for a, b in zip(f, g):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("solver", [2, 3])
+def test_fp32_thin_upml_plain(solver):
+    """Plain solvers with a one-cell UPML: the interior tiles reach the fixed
+    edge cells, which the tiling keeps outside their dependency cones."""
+    m, N, po = _mods()
+    args = (solver, 1, 4000, 4000, 50, 1, 500, 0, 300)
+    sim = run_fp32(m, N, *args, T=4, R=8)
+    try:
+        f = sim.fields()
+    finally:
+        sim.finish()
+    o = po.Sim(width_nm=4000, height_nm=4000, h_u_nm=50, pml=1, lambda_nm=500, angle_deg=0, stepNum=300,
+               solver=solver, model=1, ntff_time=0)
+    o.step(300)
+    for k, which in enumerate((0, 3, 6)):
+        assert rel_l2(f[k], o.field(which)) <= FIELD_TOL, (solver, k)
