@@ -143,10 +143,10 @@ int fdtd_hip_grid_sync(fdtd_grid *g);
 int fdtd_hip_grid_set_blocking(fdtd_grid *g, int T, int R);
 int fdtd_hip_grid_get_blocking(const fdtd_grid *g, int *T, int *R);
 int fdtd_hip_grid_flush(fdtd_grid *g);
-/* event timing of the FP32 fused launches since the last enable:
- * out = {ms of the interior-tile launches, ms of the UPML-tile launches,
- *        blocks launched, algorithmic bytes of the interior launches,
- *        algorithmic bytes of the UPML launches, cell updates} */
+/* event timing of the FP32 fused launches since the last enable (one launch
+ * advances the UPML and the interior tiles together by T steps):
+ * out = {ms of the launches, 0 (reserved), launches, algorithmic bytes of the
+ *        interior tiles, algorithmic bytes of the UPML tiles, cell updates} */
 void fdtd_hip_profile_enable(int on);
 void fdtd_hip_profile_read(double out[6]);
 

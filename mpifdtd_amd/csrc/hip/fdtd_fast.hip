@@ -37,6 +37,7 @@ __device__ __forceinline__ float2 c_add(float2 a, float2 b) { return make_float2
 __device__ __forceinline__ float2 c_sub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ float2 c_scale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
 __device__ __forceinline__ float2 c_mul(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ __forceinline__ float2 sel(bool c, float2 a, float2 b) { return make_float2(c ? a.x : b.x, c ? a.y : b.y); }
 // Neighbours along j (the lane axis) through DPP wave shifts, a VALU operand
 // modifier with no LDS round trip (the compiler folds the move into the
 // subtraction that consumes it): wave_shr:1 gives lane l the value of lane
@@ -66,6 +67,19 @@ __device__ __forceinline__ float uni_f(float v) { return __int_as_float(__builti
 #else
 #define GPTR(T) T *
 #endif
+// Element access at a 32-bit byte offset from a scalar base: the global_load /
+// global_store "scalar base + vector offset" form, one offset register for all
+// arrays of the same shape instead of a 64-bit address per array and row.
+template <class T>
+__device__ __forceinline__ T ldo(GPTR(const T) base, unsigned boff)
+{
+  return *(GPTR(const T))((GPTR(const char))base + boff);
+}
+template <class T>
+__device__ __forceinline__ void sto(GPTR(T) base, unsigned boff, T v)
+{
+  *(GPTR(T))((GPTR(char))base + boff) = v;
+}
 template <class P>
 __device__ __forceinline__ GPTR(P) uni_p(P *p)
 {
@@ -99,7 +113,7 @@ __device__ __forceinline__ float2 pulse(const PulseC &pc, int c, int i, int j, f
 // kernel writes global memory, so loads from it cannot go through the scalar
 // cache and would otherwise be dependent vector round trips on every step.
 template <int MODE, int FLAV, int R, int PML>
-__global__ __launch_bounds__(256) void k_fast(FastArgs a)
+__device__ __forceinline__ void fast_tiles(const FastArgs &a, const int *qbeg, int b, int nblk)
 {
   constexpr int NR = 4 * R;  // rows of the loaded block
   __shared__ float2 xE[4][64];
@@ -111,13 +125,14 @@ __global__ __launch_bounds__(256) void k_fast(FastArgs a)
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int cur = a.cur, nxt = cur ^ 1;
   // XCD-aware static schedule: workgroups are dispatched round-robin over the
-  // 8 XCDs, so workgroup b runs on XCD b % 8 and walks that XCD's spatially
-  // contiguous partition of the tile list [qbeg[q], qbeg[q+1]) with stride
-  // gridDim/8.  Neighbouring tiles (which read each other's halos) share an
-  // L2, and no device-scope atomic is needed to hand out work.
-  const int q = blockIdx.x & 7;
-  const int nb = gridDim.x >> 3;
-  for (int tid = a.qbeg[q] + (int)(blockIdx.x >> 3); tid < a.qbeg[q + 1]; tid += nb) {
+  // 8 XCDs, so workgroup b (of nblk, both offsets that are multiples of 8) runs
+  // on XCD b % 8 and walks that XCD's spatially contiguous partition of the
+  // tile list [qbeg[q], qbeg[q+1]) with stride nblk/8.  Neighbouring tiles
+  // (which read each other's halos) share an L2, and no device-scope atomic is
+  // needed to hand out work.
+  const int q = b & 7;
+  const int nb = nblk >> 3;
+  for (int tid = qbeg[q] + (b >> 3); tid < qbeg[q + 1]; tid += nb) {
     const FastTile t = a.tiles[tid];
     const FastInst &I = a.inst[t.inst];
     const int T = a.tinst ? a.tinst[t.inst] : a.T;
@@ -213,20 +228,20 @@ __global__ __launch_bounds__(256) void k_fast(FastArgs a)
     for (int r = 0; r < R; r++) {
       const int i = ib + r;
       const int ic = min(max(i, 0), NPX - 1);
-      const unsigned k = (unsigned)(ic * P + jc);  // < 2^31: checked at allocation
-      F0[r] = g0[k];
-      F1[r] = g1[k];
-      F2[r] = g2[k];
-      ie0[r] = gie0[k];
-      ie1[r] = MODE == FDTD_MODE_TE ? gie1[k] : 1.0f;
+      const unsigned k = (unsigned)(ic * P + jc);  // cells < 2^28: checked at allocation
+      F0[r] = ldo(g0, 8u * k);
+      F1[r] = ldo(g1, 8u * k);
+      F2[r] = ldo(g2, 8u * k);
+      ie0[r] = ldo(gie0, 4u * k);
+      ie1[r] = MODE == FDTD_MODE_TE ? ldo(gie1, 4u * k) : 1.0f;
       if (PML) {
         pmlx[r] = gpx[ic];
-        A0[r] = ga[0][k];
-        A1[r] = ga[1][k];
-        A2[r] = ga[2][k];
-        A3[r] = ga[3][k];
-        A4[r] = ga[4][k];
-        A5[r] = ga[5][k];
+        A0[r] = ldo(ga[0], 8u * k);
+        A1[r] = ldo(ga[1], 8u * k);
+        A2[r] = ldo(ga[2], 8u * k);
+        A3[r] = ldo(ga[3], 8u * k);
+        A4[r] = ldo(ga[4], 8u * k);
+        A5[r] = ldo(ga[5], 8u * k);
       }
       if (jin && i >= 0 && i < NPX) inb |= 1u << r;
     }
@@ -284,10 +299,26 @@ __global__ __launch_bounds__(256) void k_fast(FastArgs a)
     // row of a wavefront that holds a scatterer cell (g = 0 in vacuum).
     const unsigned smask = MODE == FDTD_MODE_TM ? scat0 : (scat0 | scat1);
     const bool wsrc = __ballot(smask != 0) != 0;  // wavefront-uniform
-    auto src_mpi = [&](int r, float ie, const FastStep &st) -> float2 {
-      const float2 g = c_scale(c_mul(sex[w * R + r], eyl), ie - 1.0f);
-      return c_mul(g, st.dc);
+    auto src_g = [&](int r, float ie) -> float2 { return c_scale(c_mul(sex[w * R + r], eyl), ie - 1.0f); };
+    auto src_mpi = [&](int r, float ie, const FastStep &st) -> float2 { return c_mul(src_g(r, ie), st.dc); };
+    // the source of component c in row r: value now (sn) and change (sd); zero in vacuum
+    auto src_any = [&](int c, int r, float ie, const FastStep &st, auto SRCT, float2 &sn, float2 &sd) {
+      constexpr bool SRC = decltype(SRCT)::value;
+      if (FLAV == FDTD_FLAVOUR_MPI) {
+        if (SRC) {
+          const float2 g = src_g(r, ie);
+          sn = c_mul(g, st.c);
+          sd = c_mul(g, st.dc);
+        }
+      } else if ((c == 0 ? scat0 : scat1) & (1u << r)) {
+        source(c, r, ie, st, sn, sd);
+      }
     };
+    // UPML tiles run both recurrences on every cell and keep the right one with
+    // per-cell selects (no divergent branches); an auxiliary is written back
+    // only where the cell is updated, so the registers of the other cells may
+    // hold anything.
+    const float s2p = 2.0f + sy, s2m = 2.0f - sy;  // C_BYMY1/0, C_DYJY1/0 numerators of this column
 
     auto e_rows = [&](const FastStep &st, auto SRCT) {
       constexpr bool SRC = decltype(SRCT)::value;
@@ -309,19 +340,16 @@ __global__ __launch_bounds__(256) void k_fast(FastArgs a)
               e = c_add(e, sd);
             }
             F0[r] = e;
-          } else if (upd & bit) {
-            float2 sn = make_float2(0.f, 0.f), sd = make_float2(0.f, 0.f);
-            if (scat0 & bit) source(0, r, ie0[r], st, sn, sd);
-            if (pml & bit) {
-              const float cjz = srow[0][rr], cjzh = srow[1][rr];
-              const float2 nj = c_add(c_scale(A0[r], cjz), c_scale(curl, cjzh));
-              const float2 nd = c_add(c_scale(A1[r], cy0), c_scale(c_sub(nj, A0[r]), cy1));
-              A0[r] = nj;
-              A1[r] = nd;
-              F0[r] = c_add(c_scale(nd, ie0[r]), sn);
-            } else {
-              F0[r] = c_add(c_add(F0[r], c_scale(curl, ie0[r])), sd);
-            }
+          } else {
+            float2 sn = make_float2(0.f, 0.f), sd = sn;
+            src_any(0, r, ie0[r], st, SRCT, sn, sd);
+            const float cjz = srow[0][rr], cjzh = srow[1][rr];
+            const float2 nj = c_add(c_scale(A0[r], cjz), c_scale(curl, cjzh));
+            const float2 nd = c_add(c_scale(A1[r], cy0), c_scale(c_sub(nj, A0[r]), cy1));
+            A0[r] = nj;
+            A1[r] = nd;
+            const float2 e = sel(pml & bit, c_add(c_scale(nd, ie0[r]), sn), c_add(c_add(F0[r], c_scale(curl, ie0[r])), sd));
+            F0[r] = sel(upd & bit, e, F0[r]);
           }
         } else {
           const float2 hzj = shfl_up1(F2[r]);
@@ -347,25 +375,26 @@ __global__ __launch_bounds__(256) void k_fast(FastArgs a)
             }
             F0[r] = ex;
             F1[r] = ey;
-          } else if (upd & bit) {
+          } else {
             float2 sn0 = make_float2(0.f, 0.f), sd0 = sn0, sn1 = sn0, sd1 = sn0;
-            if (scat0 & bit) source(0, r, ie0[r], st, sn0, sd0);
-            if (scat1 & bit) source(1, r, ie1[r], st, sn1, sd1);
-            if (pml & bit) {
+            // TE MPI: the scattered wave acts on Ey only
+            if (FLAV == FDTD_FLAVOUR_PLAIN) src_any(0, r, ie0[r], st, SRCT, sn0, sd0);
+            src_any(1, r, ie1[r], st, SRCT, sn1, sd1);
+            {
               // Jx/Dx: x tables 0 C_DXJX1 1 C_DXJX0; y: 0 C_JX 1 C_JXHZ
               const float2 njx = c_add(c_scale(A0[r], cy0), c_scale(cx, cy1));
               const float2 ndx = c_sub(c_add(A1[r], c_scale(njx, srow[0][rr])), c_scale(A0[r], srow[1][rr]));
               // Jy/Dy: x table 2 C_DY; C_DYJY1/0 = (2 +- sigma_y) / (2 + sigma_x)
               const float2 njy = c_add(A2[r], cyv);
               const float rsx = srow[5][rr];
-              const float c1 = (2.0f + sy) * rsx, c0 = (2.0f - sy) * rsx;
+              const float c1 = s2p * rsx, c0 = s2m * rsx;
               const float2 ndy = c_sub(c_add(c_scale(A3[r], srow[2][rr]), c_scale(njy, c1)), c_scale(A2[r], c0));
               A0[r] = njx; A1[r] = ndx; A2[r] = njy; A3[r] = ndy;
-              F0[r] = c_add(c_scale(ndx, ie0[r]), sn0);
-              F1[r] = c_add(c_scale(ndy, ie1[r]), sn1);
-            } else {
-              F0[r] = c_add(c_add(F0[r], c_scale(cx, ie0[r])), sd0);
-              F1[r] = c_add(c_add(F1[r], c_scale(cyv, ie1[r])), sd1);
+              const bool p = pml & bit, u = upd & bit;
+              const float2 ex = sel(p, c_add(c_scale(ndx, ie0[r]), sn0), c_add(c_add(F0[r], c_scale(cx, ie0[r])), sd0));
+              const float2 ey = sel(p, c_add(c_scale(ndy, ie1[r]), sn1), c_add(c_add(F1[r], c_scale(cyv, ie1[r])), sd1));
+              F0[r] = sel(u, ex, F0[r]);
+              F1[r] = sel(u, ey, F1[r]);
             }
           }
         }
@@ -392,21 +421,21 @@ __global__ __launch_bounds__(256) void k_fast(FastArgs a)
           if (!PML) {
             F1[r] = c_sub(F1[r], c_scale(dEy, c2));
             F2[r] = c_add(F2[r], c_scale(dEx, c2));
-          } else if (upd & bit) {
-            if (pml & bit) {
+          } else {
+            {
               // Mx/Bx: y 2 C_MX 3 C_MXEZ, x 2 C_BXMX1 3 C_BXMX0 ; My/By: x 4 C_BY, C_BYMY1/0
               const float2 nmx = c_sub(c_scale(A2[r], cy2), c_scale(dEy, cy3));
               const float2 nbx = c_sub(c_add(A3[r], c_scale(nmx, srow[2][rr])), c_scale(A2[r], srow[3][rr]));
               const float2 nmy = c_add(A4[r], dEx);
               const float rsx = srow[5][rr];
-              const float c1 = (2.0f + sy) * rsx, c0 = (2.0f - sy) * rsx;
+              const float c1 = s2p * rsx, c0 = s2m * rsx;
               const float2 nby = c_sub(c_add(c_scale(A5[r], srow[4][rr]), c_scale(nmy, c1)), c_scale(A4[r], c0));
               A2[r] = nmx; A3[r] = nbx; A4[r] = nmy; A5[r] = nby;
-              F1[r] = c_scale(nbx, c2);
-              F2[r] = c_scale(nby, c2);
-            } else {
-              F1[r] = c_sub(F1[r], c_scale(dEy, c2));
-              F2[r] = c_add(F2[r], c_scale(dEx, c2));
+              const bool p = pml & bit, u = upd & bit;
+              const float2 hx = sel(p, c_scale(nbx, c2), c_sub(F1[r], c_scale(dEy, c2)));
+              const float2 hy = sel(p, c_scale(nby, c2), c_add(F2[r], c_scale(dEx, c2)));
+              F1[r] = sel(u, hx, F1[r]);
+              F2[r] = sel(u, hy, F2[r]);
             }
           }
         } else {
@@ -416,16 +445,13 @@ __global__ __launch_bounds__(256) void k_fast(FastArgs a)
           const float2 ce = c_add(c_sub(c_sub(eyi, F1[r]), exj), F0[r]);
           if (!PML) {
             F2[r] = c_sub(F2[r], c_scale(ce, c2));
-          } else if (upd & bit) {
-            if (pml & bit) {
-              // Mz/Bz: x 3 C_MZ 4 C_MZEXEY ; y 2 C_BZ 3 C_BZMZ
-              const float2 nmz = c_sub(c_scale(A4[r], srow[3][rr]), c_scale(ce, srow[4][rr]));
-              const float2 nbz = c_sub(c_add(c_scale(A5[r], cy2), c_scale(nmz, cy3)), c_scale(A4[r], cy3));
-              A4[r] = nmz; A5[r] = nbz;
-              F2[r] = c_scale(nbz, c2);
-            } else {
-              F2[r] = c_sub(F2[r], c_scale(ce, c2));
-            }
+          } else {
+            // Mz/Bz: x 3 C_MZ 4 C_MZEXEY ; y 2 C_BZ 3 C_BZMZ
+            const float2 nmz = c_sub(c_scale(A4[r], srow[3][rr]), c_scale(ce, srow[4][rr]));
+            const float2 nbz = c_sub(c_add(c_scale(A5[r], cy2), c_scale(nmz, cy3)), c_scale(A4[r], cy3));
+            A4[r] = nmz; A5[r] = nbz;
+            const float2 hz = sel(pml & bit, c_scale(nbz, c2), c_sub(F2[r], c_scale(ce, c2)));
+            F2[r] = sel(upd & bit, hz, F2[r]);
           }
         }
       }
@@ -466,21 +492,31 @@ __global__ __launch_bounds__(256) void k_fast(FastArgs a)
       for (int r = 0; r < R; r++) {
         const int i = ib + r;
         if (i < t.i0 || i >= t.i0 + t.ni) continue;
-        const unsigned k = (unsigned)(i * P + jj);
-        o0[k] = F0[r];
-        o1[k] = F1[r];
-        o2[k] = F2[r];
-        if (PML && (pml & (1u << r))) {
-          oa[0][k] = A0[r];
-          oa[1][k] = A1[r];
-          oa[2][k] = A2[r];
-          oa[3][k] = A3[r];
-          oa[4][k] = A4[r];
-          oa[5][k] = A5[r];
+        const unsigned bo = 8u * (unsigned)(i * P + jj);
+        sto(o0, bo, F0[r]);
+        sto(o1, bo, F1[r]);
+        sto(o2, bo, F2[r]);
+        if (PML && (pml & upd & (1u << r))) {
+          sto(oa[0], bo, A0[r]);
+          sto(oa[1], bo, A1[r]);
+          sto(oa[2], bo, A2[r]);
+          sto(oa[3], bo, A3[r]);
+          sto(oa[4], bo, A4[r]);
+          sto(oa[5], bo, A5[r]);
         }
       }
     }
   }
+}
+
+// One launch per block of T steps: workgroups [0, nb_pml) take the UPML tiles
+// (R = 8, full recurrence), the rest the interior tiles, so the two kinds run
+// side by side instead of one launch after the other.
+template <int MODE, int FLAV, int R>
+__global__ __launch_bounds__(256) void k_fast(FastArgs a)
+{
+  if ((int)blockIdx.x < a.nb_pml) fast_tiles<MODE, FLAV, 8, 1>(a, a.qb[0], blockIdx.x, a.nb_pml);
+  else fast_tiles<MODE, FLAV, R, 0>(a, a.qb[1], (int)blockIdx.x - a.nb_pml, (int)gridDim.x - a.nb_pml);
 }
 
 // FP32 physical layout -> FP64 solver layout (MPI: padded, ghost ring untouched)
@@ -552,29 +588,29 @@ __global__ __launch_bounds__(256) void k_fast_gather(const FastInst *inst, int c
     if (e_ != hipSuccess) return fdtd_fail(FDTD_EHIP, "%s failed: %s", #x, hipGetErrorString(e_)); \
   } while (0)
 
-template <int MODE, int FLAV, int R, int PML>
+template <int MODE, int FLAV, int R>
 static void launch_one(const FastArgs &a, int nblocks, hipStream_t st)
 {
-  hipLaunchKernelGGL((k_fast<MODE, FLAV, R, PML>), dim3(nblocks), dim3(256), 0, st, a);
+  hipLaunchKernelGGL((k_fast<MODE, FLAV, R>), dim3(nblocks), dim3(256), 0, st, a);
 }
 
 typedef void (*fast_launcher)(const FastArgs &, int, hipStream_t);
 
 template <int MODE, int FLAV>
-static fast_launcher pick(int R, int pmlv)
+static fast_launcher pick(int R)
 {
-  if (pmlv) return launch_one<MODE, FLAV, 8, 1>;
-  if (R == 4) return launch_one<MODE, FLAV, 4, 0>;
-  if (R == 8) return launch_one<MODE, FLAV, 8, 0>;
-  return launch_one<MODE, FLAV, 16, 0>;
+  if (R == 4) return launch_one<MODE, FLAV, 4>;
+  if (R == 8) return launch_one<MODE, FLAV, 8>;
+  return launch_one<MODE, FLAV, 16>;
 }
 
-int fast_launch(int mode, int flavour, int R, int pmlv, const FastArgs &a, int nblocks, hipStream_t st)
+// nblocks = a.nb_pml UPML workgroups + the interior workgroups
+int fast_launch(int mode, int flavour, int R, const FastArgs &a, int nblocks, hipStream_t st)
 {
   fast_launcher f;
-  if (mode == FDTD_MODE_TM) f = flavour == FDTD_FLAVOUR_MPI ? pick<FDTD_MODE_TM, FDTD_FLAVOUR_MPI>(R, pmlv)
-                                                              : pick<FDTD_MODE_TM, FDTD_FLAVOUR_PLAIN>(R, pmlv);
-  else f = flavour == FDTD_FLAVOUR_MPI ? pick<FDTD_MODE_TE, FDTD_FLAVOUR_MPI>(R, pmlv) : pick<FDTD_MODE_TE, FDTD_FLAVOUR_PLAIN>(R, pmlv);
+  if (mode == FDTD_MODE_TM) f = flavour == FDTD_FLAVOUR_MPI ? pick<FDTD_MODE_TM, FDTD_FLAVOUR_MPI>(R)
+                                                              : pick<FDTD_MODE_TM, FDTD_FLAVOUR_PLAIN>(R);
+  else f = flavour == FDTD_FLAVOUR_MPI ? pick<FDTD_MODE_TE, FDTD_FLAVOUR_MPI>(R) : pick<FDTD_MODE_TE, FDTD_FLAVOUR_PLAIN>(R);
   f(a, nblocks, st);
   return FDTD_OK;
 }
@@ -634,36 +670,33 @@ struct FastState {
   double bytes_int_step = 0, bytes_pml_step = 0;  // algorithmic bytes of the two tile lists
 };
 
-// Event timing of the fused launches.  Events are recorded around the interior
-// and the UPML launch of every block and resolved when the totals are read, so
-// timing adds no host-device synchronisation to the step loop.
+// Event timing of the fused launches.  Events are recorded around every launch
+// and resolved when the totals are read, so timing adds no host-device
+// synchronisation to the step loop.
 struct ProfRec {
-  hipEvent_t e0, e1, e2;
+  hipEvent_t e0, e1;
   double bytes_int, bytes_pml, cells;
 };
 static struct {
   int on = 0;
   std::vector<ProfRec> recs;
-  double ms_int = 0, ms_pml = 0, bytes_int = 0, bytes_pml = 0, cells = 0;
+  double ms = 0, bytes_int = 0, bytes_pml = 0, cells = 0;
   long launches = 0;
 } g_prof;
 
 static void prof_resolve()
 {
   for (ProfRec &r : g_prof.recs) {
-    float a = 0, b = 0;
-    (void)hipEventSynchronize(r.e2);
+    float a = 0;
+    (void)hipEventSynchronize(r.e1);
     (void)hipEventElapsedTime(&a, r.e0, r.e1);
-    (void)hipEventElapsedTime(&b, r.e1, r.e2);
-    g_prof.ms_int += a;
-    g_prof.ms_pml += b;
+    g_prof.ms += a;
     g_prof.bytes_int += r.bytes_int;
     g_prof.bytes_pml += r.bytes_pml;
     g_prof.cells += r.cells;
     g_prof.launches += 1;
     (void)hipEventDestroy(r.e0);
     (void)hipEventDestroy(r.e1);
-    (void)hipEventDestroy(r.e2);
   }
   g_prof.recs.clear();
 }
@@ -672,16 +705,17 @@ void fast_profile_enable(int on)
 {
   prof_resolve();
   g_prof.on = on;
-  g_prof.ms_int = g_prof.ms_pml = g_prof.bytes_int = g_prof.bytes_pml = g_prof.cells = 0;
+  g_prof.ms = g_prof.bytes_int = g_prof.bytes_pml = g_prof.cells = 0;
   g_prof.launches = 0;
 }
 
-// out: ms_int, ms_pml, launches, bytes_int, bytes_pml, cell updates
+// out: ms of the fused launches, 0 (reserved), launches, algorithmic bytes of the
+// interior and of the UPML tiles, cell updates
 void fast_profile_read6(double out[6])
 {
   prof_resolve();
-  out[0] = g_prof.ms_int;
-  out[1] = g_prof.ms_pml;
+  out[0] = g_prof.ms;
+  out[1] = 0.0;
   out[2] = (double)g_prof.launches;
   out[3] = g_prof.bytes_int;
   out[4] = g_prof.bytes_pml;
@@ -692,7 +726,7 @@ void fast_profile_read(double *ms, long *launches, double *bytes, double *cells)
 {
   double o[6];
   fast_profile_read6(o);
-  if (ms) *ms = o[0] + o[1];
+  if (ms) *ms = o[0];
   if (launches) *launches = (long)o[2];
   if (bytes) *bytes = o[3] + o[4];
   if (cells) *cells = o[5];
@@ -975,24 +1009,19 @@ int fast_state_flush(FastState *s)
   if (g_prof.on) {
     HIPCHK(hipEventCreate(&pr.e0));
     HIPCHK(hipEventCreate(&pr.e1));
-    HIPCHK(hipEventCreate(&pr.e2));
     HIPCHK(hipEventRecord(pr.e0, 0));
   }
-  // one workgroup per tile: 8 partitions of ceil(n/8) tiles
-  if (s->n_int > 0) {
-    a.tiles = s->dtiles;
-    memcpy(a.qbeg, s->qint, sizeof a.qbeg);
-    fast_launch(s->d.mode, s->d.flavour, s->R, 0, a, 8 * ((s->n_int + 7) / 8), 0);
-  }
-  if (g_prof.on) HIPCHK(hipEventRecord(pr.e1, 0));
-  if (s->n_pml > 0) {
-    a.tiles = s->dtiles;
-    memcpy(a.qbeg, s->qpml, sizeof a.qbeg);
-    fast_launch(s->d.mode, s->d.flavour, 8, 1, a, 8 * ((s->n_pml + 7) / 8), 0);
-  }
+  // one launch: the UPML tiles' workgroups first, then the interior tiles', each
+  // split into 8 XCD partitions of ceil(n/8) workgroups
+  a.tiles = s->dtiles;
+  memcpy(a.qb[0], s->qpml, sizeof a.qb[0]);
+  memcpy(a.qb[1], s->qint, sizeof a.qb[1]);
+  a.nb_pml = 8 * ((s->n_pml + 7) / 8);
+  const int nblk = a.nb_pml + 8 * ((s->n_int + 7) / 8);
+  if (nblk > 0) fast_launch(s->d.mode, s->d.flavour, s->R, a, nblk, 0);
   HIPCHK(hipGetLastError());
   if (g_prof.on) {
-    HIPCHK(hipEventRecord(pr.e2, 0));
+    HIPCHK(hipEventRecord(pr.e1, 0));
     pr.bytes_int = s->bytes_int_step * s->pending;
     pr.bytes_pml = s->bytes_pml_step * s->pending;
     pr.cells = s->cells_per_step * s->pending;

@@ -63,7 +63,9 @@ struct FastInst {
 struct FastArgs {
   const FastInst *inst;     // device array of instances
   const FastTile *tiles;    // tile list of this launch
-  int qbeg[9];              // 8 XCD partitions: tiles [qbeg[q], qbeg[q+1]) run on XCD q
+  int qb[2][9];             // 8 XCD partitions of the UPML (0) and interior (1) tiles:
+                            // tiles [qb[k][q], qb[k][q+1]) run on XCD q
+  int nb_pml;               // workgroups [0, nb_pml) run the UPML tiles (a multiple of 8)
   int T;                    // steps per launch
   int src;                  // 1: sources active
   int cur;                  // ping-pong index of the input buffers
