@@ -1,14 +1,19 @@
-"""Benchmark: Mie-cylinder scattering (TM), 1024x1024 cells + 16-cell UPML at h = 5 nm.
+"""Benchmark of the FP32 fused UPML step through the reference's driver API.
 
-One step = one FDTD time step of the whole 1056 x 1056 grid (E and H halves,
-UPML, scattered-wave source).  The timed region is K steps of the FP32 fused,
-temporally blocked kernels through the reference's own driver API
-(simulator_calc via libmpifdtd_amd.so); inputs are resident in HBM.  The far
-field (360 angles) of the final fields is formed in FP64 and its error is
-reported against the FP64 reference-order GPU path, which the test suite pins
-to the CPU oracle at 1e-10.
+Default workload: Mie-cylinder scattering (circleModel.c, r = 500 nm, n = 1.6),
+1024 x 1024 cells + 16-cell UPML (1056 x 1056) at h = 5 nm, lambda = 550 nm,
+MPI TM solver, 8000 steps.  One step = one FDTD time step of the whole grid (E
+and H halves, UPML, scattered-wave source).  The timed region is K steps of
+simulator_calc through libmpifdtd_amd.so, whose FP32 path advances T steps per
+launch; inputs are resident in HBM.  The far field (360 angles) of the final
+fields is formed in FP64 and compared with the FP64 reference-order GPU path,
+which the test suite pins to the CPU oracle at 1e-10.
 
-    python bench.py [--steps K] [--warmup W] [--precision fp32|fp64] [--mode tm|te]
+--workload morpho8256 runs the large HBM-bound case instead: the Morpho-scale
+model on 8256 x 4160 cells (h = 5 nm, 32-cell UPML), 34.3 M cells, whose FP32
+state (1.6 GB) is far beyond the 256 MB Infinity Cache.
+
+    python bench.py [--steps K] [--warmup W] [--workload mie1056|morpho8256] [--mode tm|te]
 prints one JSON line.
 """
 import argparse
@@ -20,16 +25,24 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# Mie cylinder (circleModel.c): r = 500 nm, n = 1.6, lambda = 550 nm, +x incidence
-W_NM, H_NM, HU, PML, LAMBDA, ANGLE = 5280, 5280, 5, 16, 550, 0
 HBM_PEAK_GBS = 8000.0
+
+# name -> (model, width_nm, height_nm, h_nm, pml, lambda_nm, angle_deg, model params, description)
+WORKLOADS = {
+    "mie1056": ("MIE_CYLINDER", 5280, 5280, 5, 16, 550, 0, {"circle_radius_nm": 500, "circle_n": 1.6},
+                "Mie cylinder {mode}, 1024x1024 + 16-cell UPML (1056x1056), h=5 nm, r=500 nm, n=1.6, lambda=550 nm"),
+    "morpho8256": ("MORPHO_SCALE", 41280, 20800, 5, 32, 550, 90, {},
+                   "Morpho scale {mode}, 8192x4096 + 32-cell UPML (8256x4160), h=5 nm, lambda=550 nm, "
+                   "normal incidence (+y)"),
+}
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=8000)
-    ap.add_argument("--warmup", type=int, default=400)
+    ap.add_argument("--steps", type=int, default=0, help="timed steps (default 8000; 400 for morpho8256)")
+    ap.add_argument("--warmup", type=int, default=-1, help="untimed steps (default 400; 40 for morpho8256)")
+    ap.add_argument("--workload", default="mie1056", choices=sorted(WORKLOADS))
     ap.add_argument("--precision", default="fp32", choices=("fp32", "fp64"))
     ap.add_argument("--mode", default="tm", choices=("tm", "te"))
     ap.add_argument("--T", type=int, default=0, help="temporal block depth (0: autotune)")
@@ -39,7 +52,13 @@ def parse():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (5 times): the reference kernel "
                          "that scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters to bytes")
-    return ap.parse_args()
+    a = ap.parse_args()
+    big = a.workload != "mie1056"
+    if a.steps <= 0:
+        a.steps = 400 if big else 8000
+    if a.warmup < 0:
+        a.warmup = 40 if big else 400
+    return a
 
 
 def copy_calibration(mib):
@@ -54,11 +73,12 @@ def copy_calibration(mib):
     print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 5}), flush=True)
 
 
-def make_sim(m, precision, mode, steps, T=4, R=16):
+def make_sim(m, precision, mode, steps, T=4, R=16, workload="mie1056"):
+    model, w, h, hu, pml, lam, ang, params, _ = WORKLOADS[workload]
     solver = m.MPI_TM_UPML_2D if mode == "tm" else m.MPI_TE_UPML_2D
-    fi = m.field_info(W_NM, H_NM, HU, PML, LAMBDA, ANGLE, steps)
-    params = m.model_params(circle_radius_nm=500, circle_n=1.6)
-    return m.Simulator(solver, m.MIE_CYLINDER, fi, params=params, ntff_time=False, precision=precision, T=T, R=R)
+    fi = m.field_info(w, h, hu, pml, lam, ang, steps)
+    return m.Simulator(solver, getattr(m, model), fi, params=m.model_params(**params), ntff_time=False,
+                       precision=precision, T=T, R=R)
 
 
 def prof_read(L):
@@ -68,23 +88,24 @@ def prof_read(L):
     return list(out)
 
 
-def cpu_baseline(mode):
+def cpu_baseline(mode, workload):
     """The FP64 CPU oracle (the reference's MPI-TM/TE step restated in C, gcc -O2,
     one core) on a bounded sample of the same workload, including the per-step
     time-domain NTFF that every reference step performs."""
     from oracle import pyoracle as po
+    model, w, h, hu, pml, lam, ang, params, _ = WORKLOADS[workload]
     solver = po.MPI_TM_UPML if mode == "tm" else po.MPI_TE_UPML
-    o = po.Sim(width_nm=W_NM, height_nm=H_NM, h_u_nm=HU, pml=PML, lambda_nm=LAMBDA, angle_deg=ANGLE, stepNum=8000,
-               solver=solver, model=po.MIE_CYLINDER, ntff_time=1)
+    o = po.Sim(width_nm=w, height_nm=h, h_u_nm=hu, pml=pml, lambda_nm=lam, angle_deg=ang, stepNum=8000,
+               solver=solver, model=getattr(po, model), ntff_time=1, **params)
     o.step(2)
-    n = 120
+    cells = o.N_PX * o.N_PY
+    n = max(4, int(130e6 // cells))  # about 5 s of one core
     t0 = time.perf_counter()
     o.step(n)
     dt = time.perf_counter() - t0
-    cells = o.N_PX * o.N_PY
     return {"value": round(cells * n / dt / 1e6, 3), "unit": "Mcell-updates/s", "cores": 1, "kind": "port",
-            "sample": f"{n} steps of the same 1056x1056 Mie {mode.upper()} grid, FP64, with the per-step NTFF "
-                      f"({dt:.1f} s on one host core; gcc -O2 oracle)"}
+            "sample": f"{n} steps of the same {o.N_PX}x{o.N_PY} {workload} {mode.upper()} grid, FP64, with the "
+                      f"per-step NTFF ({dt:.1f} s on one host core; gcc -O2 oracle)"}
 
 
 def measured_traffic(mode, T, R, cells):
@@ -131,19 +152,21 @@ def main():
     torch.cuda.set_device(0)
     K, W = args.steps, args.warmup
     total = K + W
+    wl = args.workload
 
     T, R = args.T, args.R
     tuned = None
     if args.precision == "fp32" and (T == 0 or R == 0):
         log("autotune")
-        tuned = autotune.tune(m, lambda: make_sim(m, "fp32", args.mode, 2000), probe_steps=240)
+        probe = 240 if wl == "mie1056" else 48
+        tuned = autotune.tune(m, lambda: make_sim(m, "fp32", args.mode, 2000, workload=wl), probe_steps=probe)
         T = T or tuned["T"]
         R = R or tuned["R"]
-    log(f"timed run: {args.precision} T={T} R={R} warmup={W} steps={K}")
-    sim = make_sim(m, args.precision, args.mode, total, T or 4, R or 16)
+    log(f"timed run: {wl} {args.precision} T={T} R={R} warmup={W} steps={K}")
+    sim = make_sim(m, args.precision, args.mode, total, T or 4, R or 16, workload=wl)
     cells = sim.N_PX * sim.N_PY
     sim.calc(W)
-    ez_w = sim.fields()[0]  # flushes queued steps
+    sim.fields()  # launches the queued steps
     torch.cuda.synchronize()
     L.fdtd_hip_profile_enable(1)
     torch.cuda.synchronize()
@@ -158,6 +181,7 @@ def main():
     t1 = time.perf_counter()
     ff = sim.far_field()
     t_ff = time.perf_counter() - t1
+    f32_0 = sim.fields()[0]  # TM Ez / TE Ex (a copy)
     sim.finish()
     ms_step = dt * 1e3 / K
     mcups = cells * K / dt / 1e6
@@ -177,33 +201,39 @@ def main():
         roof = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None, "traffic": None}
 
     extra = {}
-    # spectra/s: one spectrum = a full 8000-step run plus its far field
-    spec_steps = 8000
-    extra["spectra_per_s"] = round(1.0 / (spec_steps * ms_step * 1e-3 + t_ff), 3)
+    # spectra/s: one spectrum = a full run of the workload's step count plus its far field
+    spec_steps = 8000 if wl == "mie1056" else 20000
+    extra["spectra_per_s"] = round(1.0 / (spec_steps * ms_step * 1e-3 + t_ff), 4)
     if not args.no_check and args.precision == "fp32":
         # FP32 far field vs the FP64 reference-order GPU path on the same run length
         log("fp64 reference-order run for the far-field check")
-        ref = make_sim(m, "fp64", args.mode, total)
+        ref = make_sim(m, "fp64", args.mode, total, workload=wl)
         t2 = time.perf_counter()
         ref.calc(total)
         ff64 = ref.far_field()
         t64 = time.perf_counter() - t2
+        f64 = ref.fields()[0]
         ref.finish()
-        extra["ff_rel_err_vs_fp64"] = float(np.linalg.norm(ff - ff64) / np.linalg.norm(ff64))
+        nref = float(np.linalg.norm(ff64))
+        # a short run of a large grid has no scattered field on the NTFF contour yet
+        extra["ff_rel_err_vs_fp64"] = float(np.linalg.norm(ff - ff64) / nref) if nref > 0 else None
+        extra["field0_rel_err_vs_fp64"] = float(np.linalg.norm(f32_0 - f64) / max(np.linalg.norm(f64), 1e-300))
         extra["fp64_mcups"] = round(cells * total / t64 / 1e6, 1)
-        extra["fp64_spectra_per_s"] = round(1.0 / (spec_steps * t64 / total), 3)
+        extra["fp64_spectra_per_s"] = round(1.0 / (spec_steps * t64 / total), 4)
         extra["speedup_vs_fp64_spectra"] = round(extra["spectra_per_s"] / extra["fp64_spectra_per_s"], 2)
-    if not args.no_cpu:
+    # the CPU leg runs on the default workload only: the one-core oracle needs minutes
+    # just to sample the Morpho model's eps on 34 M cells
+    run_cpu = not args.no_cpu and wl == "mie1056"
+    if run_cpu:
         log("cpu baseline (oracle, one core)")
-    cpu = None if args.no_cpu else cpu_baseline(args.mode)
+    cpu = cpu_baseline(args.mode, wl) if run_cpu else None
     line = {
         "metric": "Mcell-updates/s + % HBM roofline; spectra/s at <=1e-4 far-field err",
         "value": round(mcups, 1), "unit": "Mcell-updates/s", "n_gpus": 1, "steps": K, "warmup": W,
         "ms_per_step": round(ms_step, 5), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32" if args.precision == "fp32" else "f64", "data": "synthetic",
-        "config": {"workload": f"Mie cylinder {args.mode.upper()}, 1024x1024 + 16-cell UPML (1056x1056), h=5 nm, "
-                               f"r=500 nm, n=1.6, lambda=550 nm", "cells": cells, "precision": args.precision,
-                   "T": T, "R": R, "autotune": tuned},
+        "config": {"workload": WORKLOADS[wl][8].format(mode=args.mode.upper()), "cells": cells,
+                   "precision": args.precision, "T": T, "R": R, "autotune": tuned},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -509,14 +509,24 @@ __device__ __forceinline__ void fast_tiles(const FastArgs &a, const int *qbeg, i
   }
 }
 
-// One launch per block of T steps: workgroups [0, nb_pml) take the UPML tiles
-// (R = 8, full recurrence), the rest the interior tiles, so the two kinds run
-// side by side instead of one launch after the other.
-template <int MODE, int FLAV, int R>
+// KIND 0: one launch per block of T steps, workgroups [0, nb_pml) take the UPML
+// tiles (R = 8, full recurrence) and the rest the interior tiles, so the two
+// kinds run side by side.  Its register budget is that of the UPML body, which
+// limits a compute unit to two workgroups; on large grids, where UPML tiles are
+// a small share and the interior runs in many dispatch rounds, the interior
+// (KIND 1) and UPML (KIND 2) tiles are launched separately instead, so the
+// interior kernel keeps the occupancy of its own, smaller register budget.
+template <int MODE, int FLAV, int R, int KIND>
 __global__ __launch_bounds__(256) void k_fast(FastArgs a)
 {
-  if ((int)blockIdx.x < a.nb_pml) fast_tiles<MODE, FLAV, 8, 1>(a, a.qb[0], blockIdx.x, a.nb_pml);
-  else fast_tiles<MODE, FLAV, R, 0>(a, a.qb[1], (int)blockIdx.x - a.nb_pml, (int)gridDim.x - a.nb_pml);
+  if (KIND == 1) {
+    fast_tiles<MODE, FLAV, R, 0>(a, a.qb[1], blockIdx.x, gridDim.x);
+  } else if (KIND == 2) {
+    fast_tiles<MODE, FLAV, 8, 1>(a, a.qb[0], blockIdx.x, gridDim.x);
+  } else {
+    if ((int)blockIdx.x < a.nb_pml) fast_tiles<MODE, FLAV, 8, 1>(a, a.qb[0], blockIdx.x, a.nb_pml);
+    else fast_tiles<MODE, FLAV, R, 0>(a, a.qb[1], (int)blockIdx.x - a.nb_pml, (int)gridDim.x - a.nb_pml);
+  }
 }
 
 // FP32 physical layout -> FP64 solver layout (MPI: padded, ghost ring untouched)
@@ -588,29 +598,37 @@ __global__ __launch_bounds__(256) void k_fast_gather(const FastInst *inst, int c
     if (e_ != hipSuccess) return fdtd_fail(FDTD_EHIP, "%s failed: %s", #x, hipGetErrorString(e_)); \
   } while (0)
 
-template <int MODE, int FLAV, int R>
+template <int MODE, int FLAV, int R, int KIND>
 static void launch_one(const FastArgs &a, int nblocks, hipStream_t st)
 {
-  hipLaunchKernelGGL((k_fast<MODE, FLAV, R>), dim3(nblocks), dim3(256), 0, st, a);
+  hipLaunchKernelGGL((k_fast<MODE, FLAV, R, KIND>), dim3(nblocks), dim3(256), 0, st, a);
 }
 
 typedef void (*fast_launcher)(const FastArgs &, int, hipStream_t);
 
 template <int MODE, int FLAV>
-static fast_launcher pick(int R)
+static fast_launcher pick(int R, int kind)
 {
-  if (R == 4) return launch_one<MODE, FLAV, 4>;
-  if (R == 8) return launch_one<MODE, FLAV, 8>;
-  return launch_one<MODE, FLAV, 16>;
+  if (kind == 2) return launch_one<MODE, FLAV, 8, 2>;
+  if (kind == 1) {
+    if (R == 4) return launch_one<MODE, FLAV, 4, 1>;
+    if (R == 8) return launch_one<MODE, FLAV, 8, 1>;
+    return launch_one<MODE, FLAV, 16, 1>;
+  }
+  if (R == 4) return launch_one<MODE, FLAV, 4, 0>;
+  if (R == 8) return launch_one<MODE, FLAV, 8, 0>;
+  return launch_one<MODE, FLAV, 16, 0>;
 }
 
-// nblocks = a.nb_pml UPML workgroups + the interior workgroups
-int fast_launch(int mode, int flavour, int R, const FastArgs &a, int nblocks, hipStream_t st)
+// kind 0: nblocks = a.nb_pml UPML workgroups + the interior workgroups;
+// kind 1: interior workgroups only; kind 2: UPML workgroups only
+int fast_launch(int mode, int flavour, int R, int kind, const FastArgs &a, int nblocks, hipStream_t st)
 {
   fast_launcher f;
-  if (mode == FDTD_MODE_TM) f = flavour == FDTD_FLAVOUR_MPI ? pick<FDTD_MODE_TM, FDTD_FLAVOUR_MPI>(R)
-                                                              : pick<FDTD_MODE_TM, FDTD_FLAVOUR_PLAIN>(R);
-  else f = flavour == FDTD_FLAVOUR_MPI ? pick<FDTD_MODE_TE, FDTD_FLAVOUR_MPI>(R) : pick<FDTD_MODE_TE, FDTD_FLAVOUR_PLAIN>(R);
+  if (mode == FDTD_MODE_TM) f = flavour == FDTD_FLAVOUR_MPI ? pick<FDTD_MODE_TM, FDTD_FLAVOUR_MPI>(R, kind)
+                                                              : pick<FDTD_MODE_TM, FDTD_FLAVOUR_PLAIN>(R, kind);
+  else f = flavour == FDTD_FLAVOUR_MPI ? pick<FDTD_MODE_TE, FDTD_FLAVOUR_MPI>(R, kind)
+                                       : pick<FDTD_MODE_TE, FDTD_FLAVOUR_PLAIN>(R, kind);
   f(a, nblocks, st);
   return FDTD_OK;
 }
@@ -680,9 +698,21 @@ struct ProfRec {
 static struct {
   int on = 0;
   std::vector<ProfRec> recs;
+  std::vector<hipEvent_t> pool;  // created once and reused: creating events per launch stalls the host
   double ms = 0, bytes_int = 0, bytes_pml = 0, cells = 0;
   long launches = 0;
 } g_prof;
+
+static int prof_event(hipEvent_t *e)
+{
+  if (!g_prof.pool.empty()) {
+    *e = g_prof.pool.back();
+    g_prof.pool.pop_back();
+    return FDTD_OK;
+  }
+  HIPCHK(hipEventCreate(e));
+  return FDTD_OK;
+}
 
 static void prof_resolve()
 {
@@ -695,8 +725,8 @@ static void prof_resolve()
     g_prof.bytes_pml += r.bytes_pml;
     g_prof.cells += r.cells;
     g_prof.launches += 1;
-    (void)hipEventDestroy(r.e0);
-    (void)hipEventDestroy(r.e1);
+    g_prof.pool.push_back(r.e0);
+    g_prof.pool.push_back(r.e1);
   }
   g_prof.recs.clear();
 }
@@ -908,14 +938,15 @@ static int build_tiles(FastState *s)
   split_queues(s->qpml, s->n_int, s->n_pml);
   // algorithmic bytes per step (4-byte reals): complex E,H read+write everywhere, UPML
   // auxiliaries read+write in UPML cells, 1/eps read (TM 1, TE 2)
-  double ncell = 0, npml = 0;
+  // updated cells (the plain solvers skip their edge cells) and how many of them
+  // are UPML cells: a cell is UPML when its row or its column is
   const bool plain = s->d.flavour == FDTD_FLAVOUR_PLAIN;
-  for (int x = 0; x < NPX; x++)
-    for (int y = 0; y < NPY; y++) {
-      if (plain && (x == 0 || y == 0 || x == NPX - 1 || y == NPY - 1)) continue;
-      ncell += 1;
-      if (s->pmlx[x] || s->pmly[y]) npml += 1;
-    }
+  const int x0 = plain ? 1 : 0, x1 = plain ? NPX - 1 : NPX, y0 = plain ? 1 : 0, y1 = plain ? NPY - 1 : NPY;
+  double nx_in = 0, ny_in = 0;
+  for (int x = x0; x < x1; x++) nx_in += s->pmlx[x] ? 0 : 1;
+  for (int y = y0; y < y1; y++) ny_in += s->pmly[y] ? 0 : 1;
+  const double ncell = (double)std::max(0, x1 - x0) * std::max(0, y1 - y0);
+  const double npml = ncell - nx_in * ny_in;
   const double neps = s->d.mode == FDTD_MODE_TM ? 1 : 2;
   s->bytes_per_step = 4.0 * 2 * 2 * (3 * ncell + 6 * npml) + 4.0 * neps * ncell;
   // split by tile list: the interior tiles hold no UPML cell
@@ -936,8 +967,7 @@ int fast_state_set_blocking(FastState *s, int T, int R)
   int rc;
   if ((rc = fast_state_flush(s))) return rc;
   s->T = T;
-  s->R = R;
-  s->tilesT = -1;
+  s->R = R;  // the tiles are rebuilt at the next launch when T or R changed
   return FDTD_OK;
 }
 
@@ -1007,8 +1037,7 @@ int fast_state_flush(FastState *s)
   memcpy(a.st, s->hsteps, sizeof a.st);
   ProfRec pr;
   if (g_prof.on) {
-    HIPCHK(hipEventCreate(&pr.e0));
-    HIPCHK(hipEventCreate(&pr.e1));
+    if ((rc = prof_event(&pr.e0)) || (rc = prof_event(&pr.e1))) return rc;
     HIPCHK(hipEventRecord(pr.e0, 0));
   }
   // one launch: the UPML tiles' workgroups first, then the interior tiles', each
@@ -1016,9 +1045,16 @@ int fast_state_flush(FastState *s)
   a.tiles = s->dtiles;
   memcpy(a.qb[0], s->qpml, sizeof a.qb[0]);
   memcpy(a.qb[1], s->qint, sizeof a.qb[1]);
-  a.nb_pml = 8 * ((s->n_pml + 7) / 8);
-  const int nblk = a.nb_pml + 8 * ((s->n_int + 7) / 8);
-  if (nblk > 0) fast_launch(s->d.mode, s->d.flavour, s->R, a, nblk, 0);
+  const int nbp = 8 * ((s->n_pml + 7) / 8), nbi = 8 * ((s->n_int + 7) / 8);
+  if (nbi >= 8 * nbp && nbi > 4096) {
+    // large grid: interior and UPML tiles as two launches with their own occupancy
+    a.nb_pml = 0;
+    if (nbi > 0) fast_launch(s->d.mode, s->d.flavour, s->R, 1, a, nbi, 0);
+    if (nbp > 0) fast_launch(s->d.mode, s->d.flavour, s->R, 2, a, nbp, 0);
+  } else {
+    a.nb_pml = nbp;
+    if (nbp + nbi > 0) fast_launch(s->d.mode, s->d.flavour, s->R, 0, a, nbp + nbi, 0);
+  }
   HIPCHK(hipGetLastError());
   if (g_prof.on) {
     HIPCHK(hipEventRecord(pr.e1, 0));

@@ -150,3 +150,24 @@ def test_fp32_thin_upml_plain(solver):
     o.step(300)
     for k, which in enumerate((0, 3, 6)):
         assert rel_l2(f[k], o.field(which)) <= FIELD_TOL, (solver, k)
+
+
+def test_fp32_large_grid_split_launch_against_fp64():
+    """A grid with thousands of interior tiles takes the two-launch path (interior
+    tiles, then UPML tiles); its fields agree with the FP64 reference-order path."""
+    m, N, _ = _mods()
+    args = (4, 1, 8000, 8000, 5, 16, 550, 0, 200)   # 1600 x 1600 cells, MPI TM Mie
+    sim = run_fp32(m, N, *args, T=4, R=4, params={"circle_radius_nm": 500, "circle_n": 1.6})
+    try:
+        f = sim.fields()
+    finally:
+        sim.finish()
+    ref = m.Simulator(4, 1, m.field_info(8000, 8000, 5, 16, 550, 0, 200),
+                      params=m.model_params(circle_radius_nm=500, circle_n=1.6), ntff_time=False)
+    try:
+        ref.calc(200)
+        g = ref.fields()
+    finally:
+        ref.finish()
+    for k in range(3):
+        assert rel_l2(f[k], g[k]) <= FIELD_TOL, k
