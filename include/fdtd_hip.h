@@ -150,6 +150,26 @@ int fdtd_hip_grid_flush(fdtd_grid *g);
 void fdtd_hip_profile_enable(int on);
 void fdtd_hip_profile_read(double out[6]);
 
+/* Batched instances (no counterpart in the reference, whose solvers hold one
+ * instance per process; its sweeps run one instance per MPI rank, main.c:114-138).
+ * Several FP32, MPI-flavour grids without the per-step NTFF advance together:
+ * each block of T steps is one launch per mode whose workgroups pull the tiles
+ * of every unfinished instance from device queues, and the loop of blocks is a
+ * replayed hipGraph.  Instance i stops after steps[i] steps in total; p0[i]
+ * gives its wave (omega, k, angle_deg; the time and ray_coef fields are not
+ * used: the device forms every step's source from omega and the step count).
+ * The grids stay owned by the caller and must not be used between
+ * fdtd_hip_batch_run and fdtd_hip_batch_sync, which hands each grid its
+ * progress back. */
+typedef struct fdtd_hip_batch fdtd_hip_batch;
+int fdtd_hip_batch_create(fdtd_grid *const *grids, const fdtd_step_params *p0, const int *steps, int n, int T, int R,
+                          fdtd_hip_batch **out);
+/* blocks >= 0: queue that many blocks (asynchronous); blocks < 0: run every
+ * instance to its step count and synchronise */
+int fdtd_hip_batch_run(fdtd_hip_batch *b, int blocks);
+int fdtd_hip_batch_sync(fdtd_hip_batch *b, int *steps_done);
+int fdtd_hip_batch_destroy(fdtd_hip_batch *b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -59,6 +59,11 @@ class ModelParams(ctypes.Structure):
     ]
 
 
+class BatchItem(ctypes.Structure):
+    """fdtd_batch_item of mpifdtd_batch.h: one instance of a batch."""
+    _fields_ = [("field", FieldInfo), ("solver", ctypes.c_int), ("model", ctypes.c_int), ("params", ModelParams)]
+
+
 _lib = None
 
 
@@ -138,6 +143,16 @@ def lib():
     sig("fdtd_solver_getPrecision", I)
     sig("fdtd_solver_setBlocking", I, I, I)
     sig("fdtd_solver_getBlocking", I, ip, ip)
+    # mpifdtd_batch.h
+    sig("fdtd_batch_create", I, ctypes.POINTER(BatchItem), I, I, I, ctypes.POINTER(P))
+    sig("fdtd_batch_run", I, P)
+    sig("fdtd_batch_run_blocks", I, P, I)
+    sig("fdtd_batch_sync", I, P, ip)
+    sig("fdtd_batch_size", I, P)
+    sig("fdtd_batch_dims", I, P, I, ip)
+    sig("fdtd_batch_field", I, P, I, I, dp)
+    sig("fdtd_batch_far_field", I, P, I, dp)
+    sig("fdtd_batch_destroy", V, P)
     sig("fdtd_hip_profile_enable", V, I)
     sig("fdtd_hip_profile_read", V, dp)
     sig("fdtd_hip_last_error", ctypes.c_char_p)

@@ -112,8 +112,38 @@ __device__ __forceinline__ float2 pulse(const PulseC &pc, int c, int i, int j, f
 // pulse constants in registers.  The loop then issues no global load: the
 // kernel writes global memory, so loads from it cannot go through the scalar
 // cache and would otherwise be dependent vector round trips on every step.
-template <int MODE, int FLAV, int R, int PML>
-__device__ __forceinline__ void fast_tiles(const FastArgs &a, const int *qbeg, int b, int nblk)
+// step scalars of a batched instance at step t, formed on the device in FP64 as
+// the single-instance host path forms them (field.c ray_coef ramp, e^{-i w t}):
+// c_t = ray_coef(t) e^{-i w t}, dc = c_t - c_{t-1} with c_{-1} = 0
+__device__ __forceinline__ void batch_c(double omega, int t, double &re, double &im)
+{
+  const double x = 0.01 * t;
+  const double rc = 1.0 - exp(-(x * x));
+  double sn, cs;
+  sincos(omega * t, &sn, &cs);
+  re = rc * cs;
+  im = -rc * sn;
+}
+__device__ __forceinline__ FastStep batch_step(const FastBatchInst &B, int s)
+{
+  const int t = B.n + s;
+  double re, im, pre = 0.0, pim = 0.0;
+  batch_c(B.omega, t, re, im);
+  if (t >= 1) batch_c(B.omega, t - 1, pre, pim);
+  FastStep st;
+  st.c = make_float2((float)re, (float)im);
+  st.dc = make_float2((float)(re - pre), (float)(im - pim));
+  st.tn = st.tn1 = st.tp = st.tp1 = 0.0f;
+  st.hasprev = t >= 1 ? 1.0f : 0.0f;
+  st.pad = 0.0f;
+  return st;
+}
+
+// One tile: load the block, run its T steps, write back the core.  BATCH: the
+// instance's depth, ping-pong index and step scalars come from its device
+// progress record instead of the launch arguments.
+template <int MODE, int FLAV, int R, int PML, int BATCH>
+__device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
 {
   constexpr int NR = 4 * R;  // rows of the loaded block
   __shared__ float2 xE[4][64];
@@ -123,22 +153,18 @@ __device__ __forceinline__ void fast_tiles(const FastArgs &a, const int *qbeg, i
   // UPML x tables of the block rows: tx0..tx4, 1/(2 + sigma_x)
   __shared__ float srow[6][PML ? NR : 1];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int cur = a.cur, nxt = cur ^ 1;
-  // XCD-aware static schedule: workgroups are dispatched round-robin over the
-  // 8 XCDs, so workgroup b (of nblk, both offsets that are multiples of 8) runs
-  // on XCD b % 8 and walks that XCD's spatially contiguous partition of the
-  // tile list [qbeg[q], qbeg[q+1]) with stride nblk/8.  Neighbouring tiles
-  // (which read each other's halos) share an L2, and no device-scope atomic is
-  // needed to hand out work.
-  const int q = b & 7;
-  const int nb = nblk >> 3;
-  for (int tid = qbeg[q] + (b >> 3); tid < qbeg[q + 1]; tid += nb) {
+  {
     const FastTile t = a.tiles[tid];
     const FastInst &I = a.inst[t.inst];
-    const int T = a.tinst ? a.tinst[t.inst] : a.T;
+    int T = a.T, cur = a.cur;
+    if (BATCH) {
+      T = uni_i(a.binst[t.inst].Ti);
+      cur = uni_i(a.binst[t.inst].cur);
+    }
+    const int nxt = cur ^ 1;
     const int H0 = a.T + 1;  // halo of the tiling depth; T <= a.T
-    const FastStep *steps = a.steps ? a.steps + (size_t)t.inst * FAST_MAX_T : a.st;
     __syncthreads();  // the previous tile's last reads of the shared arrays are done
+    if (BATCH && T == 0) return;  // instance finished (uniform)
     // Instance scalars and pointers, read once into scalar registers.  The
     // readfirstlane is convergent, so the compiler cannot sink these loads into
     // the per-row branches below (where each row would wait on its own round trip).
@@ -179,7 +205,7 @@ __device__ __forceinline__ void fast_tiles(const FastArgs &a, const int *qbeg, i
     const bool jin = jj >= 0 && jj < NPY;
     const int jc = min(max(jj, 0), NPY - 1);  // clamped column: every load below stays in bounds
     const float c2 = uni_f(I.c2);
-    if ((int)threadIdx.x < T) sst[threadIdx.x] = steps[threadIdx.x];
+    if ((int)threadIdx.x < T) sst[threadIdx.x] = BATCH ? batch_step(a.binst[t.inst], threadIdx.x) : a.st[threadIdx.x];
     for (int r = threadIdx.x; r < NR; r += 256) {
       const int i = ib0 + r;
       const bool ok = i >= 0 && i < NPX;
@@ -257,7 +283,7 @@ __device__ __forceinline__ void fast_tiles(const FastArgs &a, const int *qbeg, i
       }
       const bool u = in && ((FLAV == FDTD_FLAVOUR_MPI) || (i >= 1 && i <= NPX - 2 && jj >= 1 && jj <= NPY - 2));
       if (u) upd |= 1u << r;
-      if (a.src) {
+      if (BATCH || a.src) {
         if (ie0[r] != 1.0f) scat0 |= 1u << r;
         if (MODE == FDTD_MODE_TE && ie1[r] != 1.0f) scat1 |= 1u << r;
       }
@@ -509,6 +535,19 @@ __device__ __forceinline__ void fast_tiles(const FastArgs &a, const int *qbeg, i
   }
 }
 
+// XCD-aware static schedule: workgroups are dispatched round-robin over the 8
+// XCDs, so workgroup b (of nblk, both offsets that are multiples of 8) runs on
+// XCD b % 8 and walks that XCD's spatially contiguous partition of the tile
+// list [qbeg[q], qbeg[q+1]) with stride nblk/8.  Neighbouring tiles (which read
+// each other's halos) share an L2, and no device-scope atomic is needed.
+template <int MODE, int FLAV, int R, int PML>
+__device__ __forceinline__ void fast_tiles(const FastArgs &a, const int *qbeg, int b, int nblk)
+{
+  const int q = b & 7;
+  const int nb = nblk >> 3;
+  for (int tid = qbeg[q] + (b >> 3); tid < qbeg[q + 1]; tid += nb) fast_tile<MODE, FLAV, R, PML, 0>(a, tid);
+}
+
 // KIND 0: one launch per block of T steps, workgroups [0, nb_pml) take the UPML
 // tiles (R = 8, full recurrence) and the rest the interior tiles, so the two
 // kinds run side by side.  Its register budget is that of the UPML body, which
@@ -526,6 +565,47 @@ __global__ __launch_bounds__(256) void k_fast(FastArgs a)
   } else {
     if ((int)blockIdx.x < a.nb_pml) fast_tiles<MODE, FLAV, 8, 1>(a, a.qb[0], blockIdx.x, a.nb_pml);
     else fast_tiles<MODE, FLAV, R, 0>(a, a.qb[1], (int)blockIdx.x - a.nb_pml, (int)gridDim.x - a.nb_pml);
+  }
+}
+
+// Batched instances: every workgroup pulls tiles from the launch's two device
+// queues, first the UPML tiles (the longest per tile), then the interior tiles,
+// each queue ordered by decreasing instance size, until both are empty; every
+// workgroup then exits.  One launch advances every unfinished instance by its
+// own depth; k_batch_advance between launches moves the progress records.
+template <int MODE, int R>
+__global__ __launch_bounds__(256) void k_fast_batch(FastArgs a)
+{
+  __shared__ int s_tid;
+  for (int k = 0; k < 2; k++) {
+    for (;;) {
+      __syncthreads();  // every thread has read s_tid of the previous pull
+      if (threadIdx.x == 0) s_tid = atomicAdd(&a.qctr[k], 1);
+      __syncthreads();
+      const int tid = s_tid;
+      if (tid >= a.qn[k]) break;
+      if (k == 0) fast_tile<MODE, FDTD_FLAVOUR_MPI, 8, 1, 1>(a, a.qoff[0] + tid);
+      else fast_tile<MODE, FDTD_FLAVOUR_MPI, R, 0, 1>(a, a.qoff[1] + tid);
+    }
+  }
+}
+
+// After a batched launch: count its steps into every instance, flip its
+// ping-pong index, set the depth of the next launch (0 once finished) and clear
+// the queue counters.  With n0 = 1 it only sets the depths (first launch).
+__global__ __launch_bounds__(256) void k_batch_advance(FastBatchInst *b, int n, int T, int *qctr, int nq, int first)
+{
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < nq) qctr[i] = 0;
+  if (i < n) {
+    FastBatchInst x = b[i];
+    if (!first && x.Ti > 0) {
+      x.n += x.Ti;
+      x.cur ^= 1;
+    }
+    const int left = x.steps - x.n;
+    x.Ti = left <= 0 ? 0 : (left < T ? left : T);
+    b[i] = x;
   }
 }
 
@@ -686,6 +766,7 @@ struct FastState {
   double t0[2] = {0, 0};
   double bytes_per_step = 0, cells_per_step = 0;
   double bytes_int_step = 0, bytes_pml_step = 0;  // algorithmic bytes of the two tile lists
+  std::vector<FastTile> htp, hti;  // host copies of the UPML and interior tiles (batch builder)
 };
 
 // Event timing of the fused launches.  Events are recorded around every launch
@@ -926,6 +1007,8 @@ static int build_tiles(FastState *s)
   auto byrow = [](const FastTile &a, const FastTile &b) { return a.i0 != b.i0 ? a.i0 < b.i0 : a.j0 < b.j0; };
   std::sort(ti.begin(), ti.end(), byrow);
   std::sort(tp.begin(), tp.end(), byrow);
+  s->htp = tp;
+  s->hti = ti;
   s->n_int = (int)ti.size();
   s->n_pml = (int)tp.size();
   std::vector<FastTile> all(ti);
@@ -1032,8 +1115,6 @@ int fast_state_flush(FastState *s)
   a.T = s->pending;
   a.src = s->src_on;
   a.cur = s->cur;
-  a.steps = nullptr;
-  a.tinst = nullptr;
   memcpy(a.st, s->hsteps, sizeof a.st);
   ProfRec pr;
   if (g_prof.on) {
@@ -1163,5 +1244,236 @@ int fast_state_gather(FastState *s, const Contour &ct, int off, double2 *cE, dou
   if ((rc = fast_state_flush(s))) return rc;
   fast_gather(s->dinst, s->cur, ct, off, s->d.mode, cE, cH, 0);
   HIPCHK(hipGetLastError());
+  return FDTD_OK;
+}
+
+// ---------------------------------------------------------------- batched instances
+//
+// Several FP32 MPI-flavour instances of uneven size advance together: one
+// k_fast_batch launch per T-step block and mode pulls the UPML and then the
+// interior tiles of every unfinished instance from device queues, and
+// k_batch_advance moves the per-instance progress records on the device.  No
+// host value changes between blocks, so the loop of blocks is captured once as
+// a hipGraph and replayed.
+
+struct FastBatch {
+  int n = 0, T = 4, R = 16;
+  std::vector<FastState *> st;
+  std::vector<double> omega;
+  std::vector<int> steps;
+  FastInst *dinst = nullptr;
+  FastBatchInst *dbinst = nullptr;
+  FastTile *dtiles = nullptr;
+  int *dqctr = nullptr;             // [2 modes][2 queues]
+  int nt[2][2] = {{0, 0}, {0, 0}};  // tiles per mode and queue
+  int off[2][2] = {{0, 0}, {0, 0}};
+  int nblk[2] = {0, 0};
+  hipStream_t stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int glevels = 0;
+};
+
+template <int MODE>
+static void batch_launch_mode(const FastArgs &a, int R, int nblk, hipStream_t st)
+{
+  if (R == 4) hipLaunchKernelGGL((k_fast_batch<MODE, 4>), dim3(nblk), dim3(256), 0, st, a);
+  else if (R == 8) hipLaunchKernelGGL((k_fast_batch<MODE, 8>), dim3(nblk), dim3(256), 0, st, a);
+  else hipLaunchKernelGGL((k_fast_batch<MODE, 16>), dim3(nblk), dim3(256), 0, st, a);
+}
+
+static FastArgs batch_args(const FastBatch *b, int m)
+{
+  FastArgs a;
+  memset(&a, 0, sizeof a);
+  a.inst = b->dinst;
+  a.tiles = b->dtiles;
+  a.T = b->T;
+  a.src = 1;
+  a.binst = b->dbinst;
+  a.qctr = b->dqctr + 2 * m;
+  a.qn[0] = b->nt[m][0];
+  a.qn[1] = b->nt[m][1];
+  a.qoff[0] = b->off[m][0];
+  a.qoff[1] = b->off[m][1];
+  return a;
+}
+
+// one block of T steps for every unfinished instance, then the progress update
+static int batch_level(FastBatch *b)
+{
+  for (int m = 0; m < 2; m++) {
+    if (b->nblk[m] == 0) continue;
+    const FastArgs a = batch_args(b, m);
+    if (m == FDTD_MODE_TM) batch_launch_mode<FDTD_MODE_TM>(a, b->R, b->nblk[m], b->stream);
+    else batch_launch_mode<FDTD_MODE_TE>(a, b->R, b->nblk[m], b->stream);
+  }
+  hipLaunchKernelGGL(k_batch_advance, dim3((std::max(b->n, 4) + 255) / 256), dim3(256), 0, b->stream, b->dbinst, b->n,
+                     b->T, b->dqctr, 4, 0);
+  HIPCHK(hipGetLastError());
+  return FDTD_OK;
+}
+
+void fast_batch_destroy(FastBatch *b)
+{
+  if (!b) return;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->dinst) (void)hipFree(b->dinst);
+  if (b->dbinst) (void)hipFree(b->dbinst);
+  if (b->dtiles) (void)hipFree(b->dtiles);
+  if (b->dqctr) (void)hipFree(b->dqctr);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+int fast_batch_create(FastState *const *st, const fdtd_step_params *p0, const int *steps, int n, int T, int R,
+                      FastBatch **out)
+{
+  *out = nullptr;
+  if (n < 1) return fdtd_fail(FDTD_EINVAL, "empty batch");
+  if (T < 1 || T >= FAST_MAX_T) return fdtd_fail(FDTD_EINVAL, "T=%d out of range", T);
+  if (R != 4 && R != 8 && R != 16) return fdtd_fail(FDTD_EINVAL, "R=%d not in {4,8,16}", R);
+  FastBatch *b = new FastBatch();
+  b->n = n;
+  b->T = T;
+  b->R = R;
+  int rc;
+  std::vector<FastInst> hinst(n);
+  std::vector<FastBatchInst> hb(n);
+  // instance order of the queues: largest first (longest-processing-time order)
+  std::vector<int> order(n);
+  for (int i = 0; i < n; i++) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+    return (long)st[x]->h.NPX * st[x]->h.NPY > (long)st[y]->h.NPX * st[y]->h.NPY;
+  });
+  for (int i = 0; i < n; i++) {
+    FastState *s = st[i];
+    if (s->d.flavour != FDTD_FLAVOUR_MPI) {
+      fast_batch_destroy(b);
+      return fdtd_fail(FDTD_EINVAL, "batched instances use the MPI (CW source) solvers; instance %d does not", i);
+    }
+    if (steps[i] < 0) {
+      fast_batch_destroy(b);
+      return fdtd_fail(FDTD_EINVAL, "instance %d: negative step count", i);
+    }
+    // the source tables of the instance's wave (angle, k), as its first step would set them
+    if ((rc = fast_state_flush(s)) || (rc = fast_state_set_blocking(s, T, R)) || (rc = update_wave(s, p0[i]))) {
+      fast_batch_destroy(b);
+      return rc;
+    }
+    if (s->tilesT != s->T || s->tilesR != s->R) {
+      if ((rc = build_tiles(s))) {
+        fast_batch_destroy(b);
+        return rc;
+      }
+    }
+    b->st.push_back(s);
+    b->omega.push_back(p0[i].omega);
+    b->steps.push_back(steps[i]);
+    hinst[i] = s->h;
+    hb[i].omega = p0[i].omega;
+    hb[i].n = s->steps_since_reset;
+    hb[i].steps = steps[i];
+    hb[i].Ti = 0;
+    hb[i].cur = s->cur;
+  }
+  // tiles: per mode, the UPML queue then the interior queue, instances largest first
+  std::vector<FastTile> all;
+  for (int m = 0; m < 2; m++) {
+    for (int k = 0; k < 2; k++) {
+      b->off[m][k] = (int)all.size();
+      for (int i : order) {
+        FastState *s = st[i];
+        if (s->d.mode != m) continue;
+        for (FastTile t : (k == 0 ? s->htp : s->hti)) {
+          t.inst = i;
+          all.push_back(t);
+        }
+      }
+      b->nt[m][k] = (int)all.size() - b->off[m][k];
+    }
+    // resident workgroups: 2 per compute unit at the UPML body's register budget
+    const int ntile = b->nt[m][0] + b->nt[m][1];
+    b->nblk[m] = std::min(ntile, 512);
+  }
+  HIPCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  HIPCHK(hipMalloc(&b->dinst, n * sizeof(FastInst)));
+  HIPCHK(hipMalloc(&b->dbinst, n * sizeof(FastBatchInst)));
+  HIPCHK(hipMalloc(&b->dtiles, std::max<size_t>(1, all.size()) * sizeof(FastTile)));
+  HIPCHK(hipMalloc(&b->dqctr, 4 * sizeof(int)));
+  HIPCHK(hipMemcpy(b->dinst, hinst.data(), n * sizeof(FastInst), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(b->dbinst, hb.data(), n * sizeof(FastBatchInst), hipMemcpyHostToDevice));
+  if (!all.empty()) HIPCHK(hipMemcpy(b->dtiles, all.data(), all.size() * sizeof(FastTile), hipMemcpyHostToDevice));
+  HIPCHK(hipDeviceSynchronize());
+  // depths of the first block, queue counters cleared
+  hipLaunchKernelGGL(k_batch_advance, dim3((std::max(n, 4) + 255) / 256), dim3(256), 0, b->stream, b->dbinst, n, T,
+                     b->dqctr, 4, 1);
+  HIPCHK(hipGetLastError());
+  *out = b;
+  return FDTD_OK;
+}
+
+// advance every instance by `blocks` blocks of T steps (instances stop at their own step count)
+int fast_batch_run(FastBatch *b, int blocks, int use_graph)
+{
+  int rc;
+  const int G = 16;
+  if (use_graph && blocks >= G && !b->gexec) {
+    HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
+    for (int l = 0; l < G; l++)
+      if ((rc = batch_level(b))) {
+        hipGraph_t g;
+        (void)hipStreamEndCapture(b->stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+    HIPCHK(hipStreamEndCapture(b->stream, &b->graph));
+    HIPCHK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+    b->glevels = G;
+  }
+  int done = 0;
+  if (use_graph && b->gexec)
+    for (; done + b->glevels <= blocks; done += b->glevels) HIPCHK(hipGraphLaunch(b->gexec, b->stream));
+  for (; done < blocks; done++)
+    if ((rc = batch_level(b))) return rc;
+  return FDTD_OK;
+}
+
+// blocks still needed by the slowest instance
+int fast_batch_blocks_left(const FastBatch *b)
+{
+  int most = 0;
+  for (int i = 0; i < b->n; i++) {
+    const int left = b->steps[i] - b->st[i]->steps_since_reset;
+    most = std::max(most, (left + b->T - 1) / b->T);
+  }
+  return most;
+}
+
+// wait for the queued blocks and hand every instance's progress back to its state
+int fast_batch_sync(FastBatch *b, int *steps_done)
+{
+  HIPCHK(hipStreamSynchronize(b->stream));
+  std::vector<FastBatchInst> hb(b->n);
+  HIPCHK(hipMemcpy(hb.data(), b->dbinst, b->n * sizeof(FastBatchInst), hipMemcpyDeviceToHost));
+  for (int i = 0; i < b->n; i++) {
+    FastState *s = b->st[i];
+    s->cur = hb[i].cur;
+    s->pending = 0;
+    s->steps_since_reset = hb[i].n;
+    s->src_on = 1;
+    // the single-instance path continues from c_{n-1}
+    const int t = hb[i].n - 1;
+    if (t >= 1) {
+      const double x = 0.01 * t, rc = 1.0 - exp(-(x * x));
+      s->prev_c_re = rc * cos(b->omega[i] * t);
+      s->prev_c_im = -rc * sin(b->omega[i] * t);
+    } else {
+      s->prev_c_re = s->prev_c_im = 0.0;
+    }
+    if (steps_done) steps_done[i] = hb[i].n;
+  }
   return FDTD_OK;
 }

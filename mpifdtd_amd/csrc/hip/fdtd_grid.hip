@@ -821,8 +821,10 @@ extern "C" int fdtd_hip_grid_create(const fdtd_grid_desc *d, fdtd_grid **out)
   g->tabs.sy = g->tab + 6 * (size_t)d->N_PX + 4 * (size_t)d->N_PY;
   // NTFF
   build_contour(g);
-  g->ntn = (size_t)360 * d->array_size + 2 * NT_GUARD;
-  for (int a = 0; a < 3; a++) {
+  // time-domain accumulators only when the per-step NTFF runs (3 x 360 x arraySize
+  // complex: 240 MB for a 10000-step run of a 2080-cell grid)
+  g->ntn = d->ntff_time ? (size_t)360 * d->array_size + 2 * NT_GUARD : 0;
+  for (int a = 0; a < 3 && g->ntn; a++) {
     HIPCHK(hipMalloc(&g->ntb[a], g->ntn * sizeof(double2)));
     HIPCHK(hipMemset(g->ntb[a], 0, g->ntn * sizeof(double2)));
     g->nt[a] = g->ntb[a] + NT_GUARD;
@@ -1071,6 +1073,7 @@ extern "C" int fdtd_hip_grid_set_field(fdtd_grid *g, int which, const double *ho
 extern "C" int fdtd_hip_grid_get_ntff_time(fdtd_grid *g, int which, double *host)
 {
   if (!g || !host || which < 0 || which > 2) return fdtd_fail(FDTD_EINVAL, "bad argument");
+  if (!g->ntn) return fdtd_fail(FDTD_ESTATE, "the grid was created without the per-step time-domain NTFF");
   HIPCHK(hipMemcpy(host, g->nt[which], (size_t)360 * g->d.array_size * sizeof(double2), hipMemcpyDeviceToHost));
   return FDTD_OK;
 }
@@ -1079,7 +1082,7 @@ extern "C" int fdtd_hip_grid_reset(fdtd_grid *g)
 {
   if (!g) return fdtd_fail(FDTD_EINVAL, "null argument");
   for (int f = 0; f < 9; f++) HIPCHK(hipMemset(g->F[f], 0, g->ncell * sizeof(double2)));
-  for (int a = 0; a < 3; a++) HIPCHK(hipMemset(g->ntb[a], 0, g->ntn * sizeof(double2)));
+  for (int a = 0; a < 3 && g->ntn; a++) HIPCHK(hipMemset(g->ntb[a], 0, g->ntn * sizeof(double2)));
   HIPCHK(hipDeviceSynchronize());
   if (g->fs) return fast_state_reset(g->fs);
   return FDTD_OK;
@@ -1116,6 +1119,7 @@ extern "C" int fdtd_hip_grid_time_spectra(fdtd_grid *g, double coef_re, double c
                                           double *out)
 {
   if (!g || !twiddles || !out) return fdtd_fail(FDTD_EINVAL, "null argument");
+  if (!g->ntn) return fdtd_fail(FDTD_ESTATE, "the grid was created without the per-step time-domain NTFF");
   const fdtd_grid_desc &d = g->d;
   HIPCHK(hipMemcpy(g->tw, twiddles, 4096 * sizeof(double2), hipMemcpyHostToDevice));
   const size_t lds = NTFF_NUM * sizeof(double2);
@@ -1144,4 +1148,59 @@ extern "C" int fdtd_hip_grid_sync(fdtd_grid *g)
   (void)g;
   HIPCHK(hipDeviceSynchronize());
   return FDTD_OK;
+}
+
+// ---------------------------------------------------------------- batched instances
+
+struct fdtd_hip_batch {
+  FastBatch *fb;
+  std::vector<fdtd_grid *> grids;
+};
+
+extern "C" int fdtd_hip_batch_create(fdtd_grid *const *grids, const fdtd_step_params *p0, const int *steps, int n,
+                                     int T, int R, fdtd_hip_batch **out)
+{
+  if (!grids || !p0 || !steps || !out || n < 1) return fdtd_fail(FDTD_EINVAL, "bad argument");
+  *out = nullptr;
+  std::vector<FastState *> st(n);
+  for (int i = 0; i < n; i++) {
+    if (!grids[i] || !grids[i]->fs) return fdtd_fail(FDTD_EINVAL, "instance %d is not an FP32 grid", i);
+    if (grids[i]->d.ntff_time) return fdtd_fail(FDTD_EINVAL, "instance %d runs the per-step NTFF (one step per launch)", i);
+    st[i] = grids[i]->fs;
+  }
+  FastBatch *fb = nullptr;
+  int rc = fast_batch_create(st.data(), p0, steps, n, T, R, &fb);
+  if (rc) return rc;
+  fdtd_hip_batch *b = new fdtd_hip_batch();
+  b->fb = fb;
+  b->grids.assign(grids, grids + n);
+  *out = b;
+  return FDTD_OK;
+}
+
+extern "C" int fdtd_hip_batch_run(fdtd_hip_batch *b, int blocks)
+{
+  if (!b) return fdtd_fail(FDTD_EINVAL, "null batch");
+  int rc;
+  if (blocks < 0) {
+    if ((rc = fast_batch_sync(b->fb, nullptr))) return rc;
+    if ((rc = fast_batch_run(b->fb, fast_batch_blocks_left(b->fb), 1))) return rc;
+    return fast_batch_sync(b->fb, nullptr);
+  }
+  return fast_batch_run(b->fb, blocks, 1);
+}
+
+extern "C" int fdtd_hip_batch_sync(fdtd_hip_batch *b, int *steps_done)
+{
+  if (!b) return fdtd_fail(FDTD_EINVAL, "null batch");
+  return fast_batch_sync(b->fb, steps_done);
+}
+
+extern "C" int fdtd_hip_batch_destroy(fdtd_hip_batch *b)
+{
+  if (!b) return FDTD_OK;
+  int rc = fast_batch_sync(b->fb, nullptr);
+  fast_batch_destroy(b->fb);
+  delete b;
+  return rc;
 }

@@ -60,6 +60,15 @@ struct FastInst {
   int on[2];
 };
 
+// Progress of one batched instance (device memory, advanced between launches).
+struct FastBatchInst {
+  double omega;  // angular frequency of the scattered wave (MPI flavour)
+  int n;         // steps done
+  int steps;     // steps to run in total
+  int Ti;        // steps of the current launch (0: finished)
+  int cur;       // ping-pong index of the input buffers
+};
+
 struct FastArgs {
   const FastInst *inst;     // device array of instances
   const FastTile *tiles;    // tile list of this launch
@@ -69,8 +78,11 @@ struct FastArgs {
   int T;                    // steps per launch
   int src;                  // 1: sources active
   int cur;                  // ping-pong index of the input buffers
-  const FastStep *steps;    // batched runs: [n_inst][FAST_MAX_T] in device memory, else null
-  const int *tinst;         // batched runs: steps of this block per instance, else null
+  // batched runs (k_fast_batch): per-instance progress and the launch's two tile
+  // queues, UPML (0) and interior (1), at tiles[qoff[k], qoff[k] + qn[k])
+  FastBatchInst *binst;
+  int *qctr;
+  int qn[2], qoff[2];
   FastStep st[FAST_MAX_T];  // single-instance runs: the block's steps, passed by value
 };
 
@@ -86,6 +98,14 @@ int fast_state_import(FastState *s, int slot, const double2 *src_dev, int NPYa, 
 int fast_state_reset(FastState *s);
 int fast_state_gather(FastState *s, const Contour &ct, int off, double2 *cE, double2 *cH);
 int fast_state_set_blocking(FastState *s, int T, int R);
+
+struct FastBatch;
+int fast_batch_create(FastState *const *st, const fdtd_step_params *p0, const int *steps, int n, int T, int R,
+                      FastBatch **out);
+int fast_batch_run(FastBatch *b, int blocks, int use_graph);
+int fast_batch_blocks_left(const FastBatch *b);
+int fast_batch_sync(FastBatch *b, int *steps_done);
+void fast_batch_destroy(FastBatch *b);
 void fast_state_get_blocking(const FastState *s, int *T, int *R);
 // profiling of the fused launches: total ms, launches, algorithmic bytes moved
 void fast_profile_enable(int on);

@@ -21,20 +21,21 @@
 #include "mpifdtd_field.h"
 #include "mpifdtd_models.h"
 #include "mpifdtd_simulator.h"
+#include "host_internal.h"
 #include "mpifdtd_solvers.h"
 
 #ifndef M_PI
 #define M_PI 3.1415926535897932384626433832795
 #endif
 
-enum { SLV_MPI_TM, SLV_MPI_TE, SLV_TM, SLV_TE, SLV_N };
 
 typedef struct {
   fdtd_grid *g;
   int NPXa, NPYa, off, arraySize;
   double complex *mirror[3];  /* TM: Ez Hx Hy   TE: Ex Ey Hz */
   double *eps_mirror;
-  int blocked;  /* FP32 grid without the per-step NTFF: steps are queued and run T per launch */
+  int blocked;    /* FP32 grid without the per-step NTFF: steps are queued and run T per launch */
+  int ntff_time;  /* the per-step time-domain NTFF runs (its outputs exist) */
 } solver_t;
 
 static solver_t slv_[SLV_N];
@@ -152,17 +153,18 @@ static void solver_free(int kind)
   if (active_ == kind) active_ = -1;
 }
 
-static void solver_init(int kind)
+/* The grid of solver `kind` for the current field/model globals: descriptor,
+ * UPML coefficient tables, eps fill and, for FP32 without the per-step NTFF,
+ * the temporal blocking.  Shared by solver init and the batch builder. */
+int solver_make_grid(int kind, int precision, int ntff_time, int T, int R, fdtd_grid **out)
 {
-  solver_t *s = &slv_[kind];
-  if (s->g) solver_free(kind);
   FieldInfo_S fs = field_getFieldInfo_S();
   NTFFInfo ni = field_getNTFFInfo();
   fdtd_grid_desc d;
   memset(&d, 0, sizeof d);
   d.mode = is_tm(kind) ? FDTD_MODE_TM : FDTD_MODE_TE;
   d.flavour = is_mpi(kind) ? FDTD_FLAVOUR_MPI : FDTD_FLAVOUR_PLAIN;
-  d.precision = precision_;
+  d.precision = precision;
   d.N_PX = fs.N_PX;
   d.N_PY = fs.N_PY;
   d.N_PML = fs.N_PML;
@@ -179,19 +181,35 @@ static void solver_init(int kind)
   d.mu0 = field_getMu0();
   d.z0 = field_getZ0();
   d.eps0 = 1.0;
-  d.ntff_time = ntff_time_;
+  d.ntff_time = ntff_time;
   d.point_source = field_getPointSource();
-  CHECK(fdtd_hip_grid_create(&d, &s->g));
+  fdtd_grid *g = NULL;
+  int rc = fdtd_hip_grid_create(&d, &g);
+  if (rc != FDTD_OK) return rc;
+  build_coefs(kind, g);
+  if ((rc = fdtd_hip_grid_fill_eps(g, models_getState())) == FDTD_OK && precision == FDTD_FP32 && !ntff_time)
+    rc = fdtd_hip_grid_set_blocking(g, T > 0 ? T : 4, R);
+  if (rc != FDTD_OK) {
+    fdtd_hip_grid_destroy(g);
+    return rc;
+  }
+  *out = g;
+  return FDTD_OK;
+}
+
+static void solver_init(int kind)
+{
+  solver_t *s = &slv_[kind];
+  if (s->g) solver_free(kind);
+  CHECK(solver_make_grid(kind, precision_, ntff_time_, block_T_, block_R_, &s->g));
   int dims[4];
   CHECK(fdtd_hip_grid_dims(s->g, dims));
   s->NPXa = dims[0];
   s->NPYa = dims[1];
   s->off = dims[2];
   s->arraySize = dims[3];
-  build_coefs(kind, s->g);
-  CHECK(fdtd_hip_grid_fill_eps(s->g, models_getState()));
   s->blocked = precision_ == FDTD_FP32 && !ntff_time_;
-  if (s->blocked) CHECK(fdtd_hip_grid_set_blocking(s->g, block_T_ > 0 ? block_T_ : 4, block_R_));
+  s->ntff_time = ntff_time_;
   size_t n = (size_t)s->NPXa * s->NPYa;
   for (int q = 0; q < 3; q++) s->mirror[q] = (double complex *)calloc(n, sizeof(double complex));
   s->eps_mirror = (double *)calloc(n, sizeof(double));
@@ -245,7 +263,7 @@ static double *solver_eps(int kind)
 
 /* ---------------------------------------------------------------- NTFF outputs */
 
-static double complex ff_coef(void)
+double complex solver_ff_coef(void)
 {
   /* ntffTM_Frequency / ntffTE_FrequencySplit with R0 of ntffTM_init / ntffTE_init */
   const double R0 = 1.0e6 * field_toCellUnit(500);
@@ -256,7 +274,7 @@ static double complex ff_coef(void)
 int fdtd_solver_farField(double *out360)
 {
   if (active_ < 0) return FDTD_ESTATE;
-  double complex coef = ff_coef();
+  double complex coef = solver_ff_coef();
   FieldInfo fi = field_getFieldInfo();
   return fdtd_hip_grid_far_field(slv_[active_].g, field_getK(), creal(coef), cimag(coef), sqrt(fi.h_u_nm), out360);
 }
@@ -353,7 +371,7 @@ static void solver_reset(int kind)
 {
   solver_t *s = &slv_[kind];
   if (!s->g) return;
-  if (!is_mpi(kind) && file_output_) write_spectra();  /* fdtdTM_upml.c:87, fdtdTE_upml.c:117 */
+  if (!is_mpi(kind) && file_output_ && s->ntff_time) write_spectra();  /* fdtdTM_upml.c:87, fdtdTE_upml.c:117 */
   CHECK(fdtd_hip_grid_reset(s->g));
 }
 
@@ -362,7 +380,7 @@ static void solver_finish(int kind)
   solver_t *s = &slv_[kind];
   if (!s->g) return;
   if (!is_mpi(kind)) solver_reset(kind);                               /* fdtdTM_upml.c:78-82 */
-  else if (kind == SLV_MPI_TE && file_output_) mpi_te_ntff_output();   /* mpiTE_UPML.c:306-318 */
+  else if (kind == SLV_MPI_TE && file_output_ && s->ntff_time) mpi_te_ntff_output();  /* mpiTE_UPML.c:306-318 */
   solver_free(kind);
 }
 
