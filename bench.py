@@ -13,6 +13,11 @@ which the test suite pins to the CPU oracle at 1e-10.
 model on 8256 x 4160 cells (h = 5 nm, 32-cell UPML), 34.3 M cells, whose FP32
 state (1.6 GB) is far beyond the 256 MB Infinity Cache.
 
+--workload sweep64 runs a batched sweep: 64 seeded instances (multilayer or
+zigzag, sides 512 ... 2048 cells + 16-cell UPML, TM or TE, lambda 380-780 nm,
+6000-10000 steps each) advanced together to completion; value is the aggregate
+cell-update rate and spectra_per_s the completed instances per second.
+
     python bench.py [--steps K] [--warmup W] [--workload mie1056|morpho8256] [--mode tm|te]
 prints one JSON line.
 """
@@ -42,7 +47,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=0, help="timed steps (default 8000; 400 for morpho8256)")
     ap.add_argument("--warmup", type=int, default=-1, help="untimed steps (default 400; 40 for morpho8256)")
-    ap.add_argument("--workload", default="mie1056", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="mie1056", choices=sorted(WORKLOADS) + ["sweep64"])
+    ap.add_argument("--seed", type=int, default=1, help="sweep64: generator seed")
     ap.add_argument("--precision", default="fp32", choices=("fp32", "fp64"))
     ap.add_argument("--mode", default="tm", choices=("tm", "te"))
     ap.add_argument("--T", type=int, default=0, help="temporal block depth (0: autotune)")
@@ -128,6 +134,65 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def run_sweep(args, m):
+    """The batched sweep: every instance to completion, timed as one job."""
+    import numpy as np
+    import torch
+    from mpifdtd_amd import batch as B
+    T, R = args.T or 8, args.R or 16
+    spec = B.sweep_spec(args.seed, 64)
+    items = B.sweep_instances(spec)
+    cells = [(s["side"] + 2 * s["pml"]) ** 2 for s in spec]
+    upd = sum(c * s["steps"] for c, s in zip(cells, spec))
+    log(f"sweep64: {len(spec)} instances, {sum(cells) / 1e6:.1f} M cells, {upd / 1e9:.1f} G cell-updates, T={T} R={R}")
+    t0 = time.perf_counter()
+    b = B.Batch(items, T=T, R=R)
+    torch.cuda.synchronize()
+    t_init = time.perf_counter() - t0
+    log(f"created in {t_init:.1f} s")
+    t0 = time.perf_counter()
+    b.run()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    done = b.sync()
+    assert done == [s["steps"] for s in spec], "an instance did not finish"
+    ff = [b.far_field(i) for i in (0, 1)]
+    check = {}
+    if not args.no_check:
+        # two instances again through the FP64 reference-order single-instance path
+        for i in (0, 1):
+            s = spec[i]
+            solver = m.MPI_TM_UPML_2D if s["mode"] == "tm" else m.MPI_TE_UPML_2D
+            model = m.LAYER if s["kind"] == "multilayer" else m.ZIGZAG
+            ref = m.Simulator(solver, model, m.field_info(s["width_nm"], s["height_nm"], s["h_nm"], s["pml"],
+                                                         s["lambda_nm"], 0, s["steps"]),
+                              params=m.model_params(**s["params"]), ntff_time=False)
+            ref.calc(s["steps"])
+            ff64 = ref.far_field()
+            ref.finish()
+            check[f"ff_rel_err_vs_fp64_inst{i}"] = float(np.linalg.norm(ff[i] - ff64) / np.linalg.norm(ff64))
+    b.close()
+    alg = sum(c * s["steps"] * (52 if s["mode"] == "tm" else 56) for c, s in zip(cells, spec))
+    achieved = alg / dt / 1e9
+    line = {
+        "metric": "Mcell-updates/s + % HBM roofline; spectra/s at <=1e-4 far-field err",
+        "value": round(upd / dt / 1e6, 1), "unit": "Mcell-updates/s", "n_gpus": 1, "steps": max(s["steps"] for s in spec),
+        "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"batched sweep of 64 seeded instances (seed {args.seed}): multilayer/zigzag, "
+                               "sides 512-2048 + 16-cell UPML, TM/TE, lambda 380-780 nm, 6000-10000 steps",
+                   "cells": sum(cells), "cell_updates": upd, "T": T, "R": R, "create_s": round(t_init, 2)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "note": "algorithmic bytes (interior-cell 52/56 B) of the whole sweep over its wall time"},
+        "cpu_baseline": None,
+        "spectra_per_s": round(len(spec) / dt, 3),
+        "wall_s": round(dt, 3),
+    }
+    line.update(check)
+    return line
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -150,6 +215,13 @@ def main():
     L = m.lib()
     assert torch.cuda.is_available(), "no GPU"
     torch.cuda.set_device(0)
+    if args.workload == "sweep64":
+        line = run_sweep(args, m)
+        libc.fflush(None)
+        sys.stdout.flush()
+        os.dup2(real_stdout, 1)
+        print(json.dumps(line), flush=True)
+        return
     K, W = args.steps, args.warmup
     total = K + W
     wl = args.workload
