@@ -238,7 +238,8 @@ def main():
     sim = make_sim(m, args.precision, args.mode, total, T or 4, R or 16, workload=wl)
     cells = sim.N_PX * sim.N_PY
     sim.calc(W)
-    sim.fields()  # launches the queued steps
+    if args.precision == "fp32":
+        L.fdtd_solver_setBlocking(T, R)  # launches the queued warm-up steps
     torch.cuda.synchronize()
     L.fdtd_hip_profile_enable(1)
     torch.cuda.synchronize()

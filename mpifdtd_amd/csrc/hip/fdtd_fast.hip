@@ -1127,8 +1127,11 @@ int fast_state_flush(FastState *s)
   memcpy(a.qb[0], s->qpml, sizeof a.qb[0]);
   memcpy(a.qb[1], s->qint, sizeof a.qb[1]);
   const int nbp = 8 * ((s->n_pml + 7) / 8), nbi = 8 * ((s->n_int + 7) / 8);
-  if (nbi >= 8 * nbp && nbi > 4096) {
-    // large grid: interior and UPML tiles as two launches with their own occupancy
+  if (s->R <= 8 && nbi >= 8 * nbp && nbi > 4096) {
+    // large grid with a small interior register budget (R <= 8: four workgroups per
+    // compute unit, against two at the UPML body's budget): interior and UPML
+    // tiles as two launches, each with its own occupancy.  At R = 16 the interior
+    // body already allows only two, and one launch lets the UPML tiles overlap it.
     a.nb_pml = 0;
     if (nbi > 0) fast_launch(s->d.mode, s->d.flavour, s->R, 1, a, nbi, 0);
     if (nbp > 0) fast_launch(s->d.mode, s->d.flavour, s->R, 2, a, nbp, 0);

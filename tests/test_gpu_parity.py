@@ -161,3 +161,27 @@ def test_angle_change_after_reset_matches_oracle():
         assert rel_l2(sim.fields()[0], o.field(0)) <= TOL
     finally:
         sim.finish()
+
+
+@pytest.mark.parametrize("solver,pol", [(4, "tm"), (5, "te")])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_far_field_follows_mie_series_at_100_cells_per_wavelength(solver, pol, precision):
+    """The far fields against the Mie series of the dielectric cylinder at h = 5 nm
+    (100 cells per wavelength): the pattern error falls to a few per cent, as the
+    staircasing of the cylinder shrinks (about 10 % at 25 cells per wavelength)."""
+    import math
+    from mie import compare
+    m, _ = _mods()
+    print(f"mie {pol} {precision}", flush=True)
+    kw = {"ntff_time": False}
+    if precision == "fp32":
+        kw.update(precision="fp32", T=8, R=16)
+    sim = run_gpu(m, solver, 1, 4000, 4000, 5, 10, 500, 0, 5000, {"circle_radius_nm": 500, "circle_n": 1.6}, **kw)
+    try:
+        ff = sim.far_field()
+    finally:
+        sim.finish()
+    err, other, asym = compare(ff, 2 * math.pi, 1.6, pol)
+    assert err < 0.05, err
+    assert other > 0.3, other
+    assert asym < 0.01, asym
