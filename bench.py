@@ -55,6 +55,9 @@ def parse():
     ap.add_argument("--R", type=int, default=0, help="rows per wavefront (0: autotune)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 far-field check")
+    ap.add_argument("--ntff", action="store_true",
+                    help="run the reference's per-step time-domain NTFF inside the timed steps (the MPI solvers "
+                         "do it every step; default off, the far field then comes from the final fields)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (5 times): the reference kernel "
                          "that scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters to bytes")
@@ -79,11 +82,11 @@ def copy_calibration(mib):
     print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 5}), flush=True)
 
 
-def make_sim(m, precision, mode, steps, T=4, R=16, workload="mie1056"):
+def make_sim(m, precision, mode, steps, T=4, R=16, workload="mie1056", ntff=False):
     model, w, h, hu, pml, lam, ang, params, _ = WORKLOADS[workload]
     solver = m.MPI_TM_UPML_2D if mode == "tm" else m.MPI_TE_UPML_2D
     fi = m.field_info(w, h, hu, pml, lam, ang, steps)
-    return m.Simulator(solver, getattr(m, model), fi, params=m.model_params(**params), ntff_time=False,
+    return m.Simulator(solver, getattr(m, model), fi, params=m.model_params(**params), ntff_time=ntff,
                        precision=precision, T=T, R=R)
 
 
@@ -231,11 +234,12 @@ def main():
     if args.precision == "fp32" and (T == 0 or R == 0):
         log("autotune")
         probe = 240 if wl == "mie1056" else 48
-        tuned = autotune.tune(m, lambda: make_sim(m, "fp32", args.mode, 2000, workload=wl), probe_steps=probe)
+        tuned = autotune.tune(m, lambda: make_sim(m, "fp32", args.mode, 2000, workload=wl, ntff=args.ntff),
+                              probe_steps=probe)
         T = T or tuned["T"]
         R = R or tuned["R"]
     log(f"timed run: {wl} {args.precision} T={T} R={R} warmup={W} steps={K}")
-    sim = make_sim(m, args.precision, args.mode, total, T or 4, R or 16, workload=wl)
+    sim = make_sim(m, args.precision, args.mode, total, T or 4, R or 16, workload=wl, ntff=args.ntff)
     cells = sim.N_PX * sim.N_PY
     sim.calc(W)
     if args.precision == "fp32":
@@ -280,7 +284,7 @@ def main():
     if not args.no_check and args.precision == "fp32":
         # FP32 far field vs the FP64 reference-order GPU path on the same run length
         log("fp64 reference-order run for the far-field check")
-        ref = make_sim(m, "fp64", args.mode, total, workload=wl)
+        ref = make_sim(m, "fp64", args.mode, total, workload=wl, ntff=args.ntff)
         t2 = time.perf_counter()
         ref.calc(total)
         ff64 = ref.far_field()
@@ -306,7 +310,7 @@ def main():
         "ms_per_step": round(ms_step, 5), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32" if args.precision == "fp32" else "f64", "data": "synthetic",
         "config": {"workload": WORKLOADS[wl][8].format(mode=args.mode.upper()), "cells": cells,
-                   "precision": args.precision, "T": T, "R": R, "autotune": tuned},
+                   "precision": args.precision, "T": T, "R": R, "per_step_ntff": args.ntff, "autotune": tuned},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -142,7 +142,7 @@ __device__ __forceinline__ FastStep batch_step(const FastBatchInst &B, int s)
 // One tile: load the block, run its T steps, write back the core.  BATCH: the
 // instance's depth, ping-pong index and step scalars come from its device
 // progress record instead of the launch arguments.
-template <int MODE, int FLAV, int R, int PML, int BATCH>
+template <int MODE, int FLAV, int R, int PML, int BATCH, int SMP>
 __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
 {
   constexpr int NR = 4 * R;  // rows of the loaded block
@@ -484,6 +484,78 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
       xH[w][lane] = F2[R - 1];  // last row of Hy (TM) / Hz (TE) for the wave below
     };
 
+    // per-step NTFF samples of the contour cells in this tile's core (the state
+    // after each full step, as the reference's per-step NTFF reads it).  The
+    // sampling plan is fixed for the tile: a side with j fixed lies in one lane
+    // (rows `srows`, packed index sc + r), a side with i fixed in one row of the
+    // block (sr1; its neighbour line sr2), read by the lanes in its range.
+    bool csmp = false;
+    unsigned srows[4];
+    int srole[4], sr1[4], sr2[4], sc[4];
+    if (SMP) {
+      const bool jcore = jj >= t.j0 && jj < t.j0 + t.nj;
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const FastCtrSide &g = a.ct.s[q];
+        srows[q] = 0;
+        srole[q] = 0;
+        sr1[q] = sr2[q] = -1;
+        sc[q] = 0;
+        if (g.n <= 0) continue;
+        if (g.horiz) {
+          for (int r = 0; r < R; r++) {
+            const int i = ib + r;
+            if (i >= t.i0 && i < t.i0 + t.ni && i >= g.lo && i < g.lo + g.n) srows[q] |= 1u << r;
+          }
+          srole[q] = !jcore ? 0 : (jj == g.fi ? 1 : (jj == g.fi - 1 ? 2 : 0));
+          sc[q] = g.off + (ib - g.lo);
+          if (__ballot(srole[q] != 0 && srows[q] != 0)) csmp = true;
+        } else {
+          const bool core1 = g.fi >= t.i0 && g.fi < t.i0 + t.ni, core2 = g.fi - 1 >= t.i0 && g.fi - 1 < t.i0 + t.ni;
+          sr1[q] = core1 ? g.fi - ib : -1;
+          sr2[q] = core2 ? g.fi - 1 - ib : -1;
+          srole[q] = jcore && jj >= g.lo && jj < g.lo + g.n;
+          sc[q] = g.off + (jj - g.lo);
+          if (__ballot(srole[q] != 0) && ((sr1[q] >= 0 && sr1[q] < R) || (sr2[q] >= 0 && sr2[q] < R))) csmp = true;
+        }
+      }
+    }
+    auto sample = [&](int s) {
+      float2 *base = a.csamp + (size_t)s * 3 * a.ct.total;
+      const int tot = a.ct.total;
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const FastCtrSide &g = a.ct.s[q];
+        if (g.horiz) {
+          if (srole[q]) {
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+              if (!(srows[q] & (1u << r))) continue;
+              const int c = sc[q] + r;
+              const float2 h = g.hf == 1 ? F1[r] : F2[r];
+              if (srole[q] == 1) {
+                base[c] = c_scale(g.ef == 0 ? F0[r] : F1[r], g.es);
+                base[tot + c] = h;
+              } else {
+                base[2 * tot + c] = h;
+              }
+            }
+          }
+        } else if (srole[q]) {
+#pragma unroll
+          for (int r = 0; r < R; r++) {
+            const float2 h = g.hf == 1 ? F1[r] : F2[r];
+            if (r == sr1[q]) {
+              base[sc[q]] = c_scale(g.ef == 0 ? F0[r] : F1[r], g.es);
+              base[tot + sc[q]] = h;
+            } else if (r == sr2[q]) {
+              base[2 * tot + sc[q]] = h;
+            }
+          }
+        }
+      }
+    };
+
     if (FLAV == FDTD_FLAVOUR_MPI) {
       xH[w][lane] = F2[R - 1];
       __syncthreads();
@@ -492,6 +564,7 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
         __syncthreads();
         h_phase();
         __syncthreads();
+        if (SMP && csmp) sample(s);
       }
     } else {
       xE[w][lane] = MODE == FDTD_MODE_TM ? F0[0] : F1[0];
@@ -501,6 +574,7 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
         __syncthreads();
         e_phase(s);
         __syncthreads();
+        if (SMP && csmp) sample(s);
       }
     }
 
@@ -540,12 +614,12 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
 // XCD b % 8 and walks that XCD's spatially contiguous partition of the tile
 // list [qbeg[q], qbeg[q+1]) with stride nblk/8.  Neighbouring tiles (which read
 // each other's halos) share an L2, and no device-scope atomic is needed.
-template <int MODE, int FLAV, int R, int PML>
+template <int MODE, int FLAV, int R, int PML, int SMP = 0>
 __device__ __forceinline__ void fast_tiles(const FastArgs &a, const int *qbeg, int b, int nblk)
 {
   const int q = b & 7;
   const int nb = nblk >> 3;
-  for (int tid = qbeg[q] + (b >> 3); tid < qbeg[q + 1]; tid += nb) fast_tile<MODE, FLAV, R, PML, 0>(a, tid);
+  for (int tid = qbeg[q] + (b >> 3); tid < qbeg[q + 1]; tid += nb) fast_tile<MODE, FLAV, R, PML, 0, SMP>(a, tid);
 }
 
 // KIND 0: one launch per block of T steps, workgroups [0, nb_pml) take the UPML
@@ -555,10 +629,15 @@ __device__ __forceinline__ void fast_tiles(const FastArgs &a, const int *qbeg, i
 // a small share and the interior runs in many dispatch rounds, the interior
 // (KIND 1) and UPML (KIND 2) tiles are launched separately instead, so the
 // interior kernel keeps the occupancy of its own, smaller register budget.
+// KIND 3: KIND 0 with the per-step NTFF samples (a separate instantiation, so
+// that the other kinds do not carry the sampling code's registers).
 template <int MODE, int FLAV, int R, int KIND>
 __global__ __launch_bounds__(256) void k_fast(FastArgs a)
 {
-  if (KIND == 1) {
+  if (KIND == 3) {
+    if ((int)blockIdx.x < a.nb_pml) fast_tiles<MODE, FLAV, 8, 1, 1>(a, a.qb[0], blockIdx.x, a.nb_pml);
+    else fast_tiles<MODE, FLAV, R, 0, 1>(a, a.qb[1], (int)blockIdx.x - a.nb_pml, (int)gridDim.x - a.nb_pml);
+  } else if (KIND == 1) {
     fast_tiles<MODE, FLAV, R, 0>(a, a.qb[1], blockIdx.x, gridDim.x);
   } else if (KIND == 2) {
     fast_tiles<MODE, FLAV, 8, 1>(a, a.qb[0], blockIdx.x, gridDim.x);
@@ -584,8 +663,8 @@ __global__ __launch_bounds__(256) void k_fast_batch(FastArgs a)
       __syncthreads();
       const int tid = s_tid;
       if (tid >= a.qn[k]) break;
-      if (k == 0) fast_tile<MODE, FDTD_FLAVOUR_MPI, 8, 1, 1>(a, a.qoff[0] + tid);
-      else fast_tile<MODE, FDTD_FLAVOUR_MPI, R, 0, 1>(a, a.qoff[1] + tid);
+      if (k == 0) fast_tile<MODE, FDTD_FLAVOUR_MPI, 8, 1, 1, 0>(a, a.qoff[0] + tid);
+      else fast_tile<MODE, FDTD_FLAVOUR_MPI, R, 0, 1, 0>(a, a.qoff[1] + tid);
     }
   }
 }
@@ -690,6 +769,11 @@ template <int MODE, int FLAV>
 static fast_launcher pick(int R, int kind)
 {
   if (kind == 2) return launch_one<MODE, FLAV, 8, 2>;
+  if (kind == 3) {
+    if (R == 4) return launch_one<MODE, FLAV, 4, 3>;
+    if (R == 8) return launch_one<MODE, FLAV, 8, 3>;
+    return launch_one<MODE, FLAV, 16, 3>;
+  }
   if (kind == 1) {
     if (R == 4) return launch_one<MODE, FLAV, 4, 1>;
     if (R == 8) return launch_one<MODE, FLAV, 8, 1>;
@@ -700,7 +784,7 @@ static fast_launcher pick(int R, int kind)
   return launch_one<MODE, FLAV, 16, 0>;
 }
 
-// kind 0: nblocks = a.nb_pml UPML workgroups + the interior workgroups;
+// kind 0 (3 with NTFF samples): nblocks = a.nb_pml UPML workgroups + the interior workgroups;
 // kind 1: interior workgroups only; kind 2: UPML workgroups only
 int fast_launch(int mode, int flavour, int R, int kind, const FastArgs &a, int nblocks, hipStream_t st)
 {
@@ -767,6 +851,12 @@ struct FastState {
   double bytes_per_step = 0, cells_per_step = 0;
   double bytes_int_step = 0, bytes_pml_step = 0;  // algorithmic bytes of the two tile lists
   std::vector<FastTile> htp, hti;  // host copies of the UPML and interior tiles (batch builder)
+  // per-step NTFF samples (null when the contour is not sampled)
+  float2 *csamp = nullptr;
+  FastContour hct;
+  fast_post_fn post = nullptr;
+  void *post_ctx = nullptr;
+  double times[FAST_MAX_T];
 };
 
 // Event timing of the fused launches.  Events are recorded around every launch
@@ -903,6 +993,7 @@ void fast_state_destroy(FastState *s)
   if (!s) return;
   for (void *p : s->allocs) (void)hipFree(p);
   if (s->dtiles) (void)hipFree(s->dtiles);
+  if (s->csamp) (void)hipFree(s->csamp);
   delete s;
 }
 
@@ -1124,10 +1215,14 @@ int fast_state_flush(FastState *s)
   // one launch: the UPML tiles' workgroups first, then the interior tiles', each
   // split into 8 XCD partitions of ceil(n/8) workgroups
   a.tiles = s->dtiles;
+  if (s->csamp) {
+    a.csamp = s->csamp;
+    a.ct = s->hct;
+  }
   memcpy(a.qb[0], s->qpml, sizeof a.qb[0]);
   memcpy(a.qb[1], s->qint, sizeof a.qb[1]);
   const int nbp = 8 * ((s->n_pml + 7) / 8), nbi = 8 * ((s->n_int + 7) / 8);
-  if (s->R <= 8 && nbi >= 8 * nbp && nbi > 4096) {
+  if (!s->csamp && s->R <= 8 && nbi >= 8 * nbp && nbi > 4096) {
     // large grid with a small interior register budget (R <= 8: four workgroups per
     // compute unit, against two at the UPML body's budget): interior and UPML
     // tiles as two launches, each with its own occupancy.  At R = 16 the interior
@@ -1137,7 +1232,7 @@ int fast_state_flush(FastState *s)
     if (nbp > 0) fast_launch(s->d.mode, s->d.flavour, s->R, 2, a, nbp, 0);
   } else {
     a.nb_pml = nbp;
-    if (nbp + nbi > 0) fast_launch(s->d.mode, s->d.flavour, s->R, 0, a, nbp + nbi, 0);
+    if (nbp + nbi > 0) fast_launch(s->d.mode, s->d.flavour, s->R, s->csamp ? 3 : 0, a, nbp + nbi, 0);
   }
   HIPCHK(hipGetLastError());
   if (g_prof.on) {
@@ -1149,8 +1244,10 @@ int fast_state_flush(FastState *s)
     if (g_prof.recs.size() > 4096) prof_resolve();
   }
   s->cur ^= 1;
+  const int n = s->pending;
   s->pending = 0;
   s->T = Tsave;
+  if (s->csamp && s->post) return s->post(s->post_ctx, s->csamp, n, s->times);
   return FDTD_OK;
 }
 
@@ -1160,6 +1257,7 @@ int fast_state_step(FastState *s, const fdtd_step_params *p, int *launched)
   if ((rc = update_wave(s, *p))) return rc;
   FastStep &st = s->hsteps[s->pending];
   memset(&st, 0, sizeof st);
+  s->times[s->pending] = p->time;
   // MPI scattered wave: ray_coef * e^{-i w t}, w*t formed as the reference does
   const double wt = p->omega * p->time;
   const double cre = p->ray_coef * cos(wt), cim = -p->ray_coef * sin(wt);
@@ -1478,5 +1576,37 @@ int fast_batch_sync(FastBatch *b, int *steps_done)
     }
     if (steps_done) steps_done[i] = hb[i].n;
   }
+  return FDTD_OK;
+}
+
+// Contour of the per-step NTFF in the physical layout: the sides of `ct`
+// (solver-layout indices, `off` = 1 for the padded MPI layout) converted for
+// the fused kernel, and a sample buffer for FAST_MAX_T steps.
+int fast_state_set_contour(FastState *s, const Contour &ct, int off, fast_post_fn post, void *ctx)
+{
+  int rc;
+  if ((rc = fast_state_flush(s))) return rc;
+  const long NPYa = s->h.NPY + 2L * off;
+  FastContour &c = s->hct;
+  memset(&c, 0, sizeof c);
+  for (int q = 0; q < 4; q++) {
+    const Seg &g = ct.s[q];
+    FastCtrSide &f = c.s[q];
+    const int ia = (int)(g.k0 / NPYa) - off, ja = (int)(g.k0 % NPYa) - off;
+    f.horiz = g.dk == 1 ? 0 : 1;
+    f.fi = f.horiz ? ja : ia;
+    f.lo = f.horiz ? ia : ja;
+    f.n = g.n;
+    f.off = g.off;
+    f.ef = g.ef == 0 ? 0 : 1;
+    f.hf = g.hf == 3 ? 1 : 2;
+    f.es = (float)g.es;
+  }
+  c.total = ct.total;
+  if (s->csamp) (void)hipFree(s->csamp);
+  s->csamp = nullptr;
+  if (ct.total > 0) HIPCHK(hipMalloc(&s->csamp, (size_t)FAST_MAX_T * 3 * ct.total * sizeof(float2)));
+  s->post = post;
+  s->post_ctx = ctx;
   return FDTD_OK;
 }

@@ -22,6 +22,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <climits>
 #include <vector>
 #include <algorithm>
 
@@ -294,60 +295,163 @@ __global__ __launch_bounds__(256) void k_ntff_gather(const double2 *const *F, Co
 #define NTFF_CH 2048
 #define NT_GUARD_DEV 4  // guard bins allocated before and after the accumulators
 
-// one workgroup per angle: 3-tap retarded-time binning of every contour sample
-// into LDS, then one read-modify-write of the touched bins
-// (mpiTM_UPML.c:849-1037, mpiTE_UPML.c:611-794, ntffTM.c:279-371, ntffTE.c:57-157)
-__global__ __launch_bounds__(256) void k_ntff_bin(Contour ct, const double2 *cE, const double2 *cH,
-                                                  const double *tsTab, const double *tsMin, const double *tsMax,
-                                                  double2 *U0, double2 *U1, double2 *U2, int arraySize,
-                                                  double timeE, double timeH, double coef, int useCoef)
+// Binning of n consecutive steps per launch, one workgroup per angle.
+//
+// For one angle and contour cell the time shift ts is fixed, so the E sample of
+// the step with base time tb = time - 1 lands on bins tb + dE + (-1, 0, +1) and
+// the H sample on tb + dH + (-1, 0, +1), with dE = floor(ts + 0.5), dH =
+// floor(ts) + 1 and 3-tap weights (b, a - b, -a) that do not depend on the step
+// (a = 0.5 + ts - dE for E, ts - (dH - 1) for H): the bins and weights of the
+// reference's calc() (ntffTM.c:279-288, mpiTM_UPML.c:994-1036).  The host sorts
+// each angle's 2 x total samples by offset d.  Each thread walks a contiguous
+// run of that order, so its offsets only grow: it keeps the three bins its
+// current sample touches in registers (per target array) and adds a bin to the
+// LDS window only when its offset moves past it.  Cells of one side that share
+// an offset (the shift barely changes along a side seen edge-on) therefore cost
+// one LDS atomic per bin instead of one per sample and tap.
+struct BinAng {
+  int dmin, nd, start;  // smallest offset, offsets, first entry of dstart for this angle
+};
+
+__global__ __launch_bounds__(256) void k_ntff_bin_n(Contour ct, const double2 *cE, const double2 *cH, int n,
+                                                    const double *tsTab, const int *ent, const int *dstart,
+                                                    const BinAng *ai, double2 *U0, double2 *U1, double2 *U2,
+                                                    int arraySize, long tb0, double coef, int useCoef)
 {
   __shared__ double lds[3][NTFF_CH][2];
   const int ang = blockIdx.x;
+  const BinAng A = ai[ang];
   const double *ts = tsTab + (size_t)ang * ct.total;
+  const int *e = ent + (size_t)ang * 2 * ct.total;
+  const int *ds = dstart + A.start;
   double2 *U[3] = {U0, U1, U2};
   const long stp = (long)ang * arraySize;
-  const int lo = (int)floor(timeE + tsMin[ang] + 0.5) - 2;
-  const int hi = (int)floor(timeH + tsMax[ang] + 0.5) + 2;
-  for (int base = lo; base <= hi; base += NTFF_CH) {
+  const int span = NTFF_CH - n - 1;  // offsets per window (the window's bins: span + n + 1)
+  for (int j0 = 0; j0 < A.nd; j0 += span) {
+    const int nj = min(span, A.nd - j0);
+    const int nb = nj + n + 1;
     for (int q = threadIdx.x; q < 3 * NTFF_CH; q += 256) {
       lds[q / NTFF_CH][q % NTFF_CH][0] = 0.0;
       lds[q / NTFF_CH][q % NTFF_CH][1] = 0.0;
     }
     __syncthreads();
-    for (int p = threadIdx.x; p < ct.total; p += 256) {
-      int s = 0;
-      while (s < 3 && p >= ct.s[s + 1].off) s++;
-      const double tsv = ts[p];
-      const double2 vals[2] = {cE[p], cH[p]};
-      const double tms[2] = {timeE, timeH};
-      const int arr[2] = {ct.s[s].ua, ct.s[s].wa};
+    // the samples of this window's offsets, split into one contiguous run per thread
+    const int x0 = ds[j0], x1 = ds[j0 + nj];
+    const int per = (x1 - x0 + 255) / 256;
+    const int xa = x0 + (int)threadIdx.x * per, xb = min(x1, xa + per);
+    for (int s = 0; s < n; s++) {
+      const double2 *vE = cE + (size_t)s * ct.total, *vH = cH + (size_t)s * ct.total;
+      double acc[3][3][2];  // [array][bin - rb][re, im]
 #pragma unroll
-      for (int w = 0; w < 2; w++) {
-        const double tt = tms[w] + tsv;
-        const int m = (int)floor(tt + 0.5);
-        const double a = (0.5 + tt) - m;
-        const double b = 1.0 - a;
-        const double ab = a - b;
-        double vb_r = vals[w].x * b, vb_i = vals[w].y * b;
-        double vab_r = vals[w].x * ab, vab_i = vals[w].y * ab;
-        double va_r = vals[w].x * a, va_i = vals[w].y * a;
-        if (useCoef) {
-          vb_r *= coef; vb_i *= coef; vab_r *= coef; vab_i *= coef; va_r *= coef; va_i *= coef;
+      for (int aa = 0; aa < 3; aa++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) acc[aa][k][0] = acc[aa][k][1] = 0.0;
+      int rb = INT_MIN;  // window bin of acc[.][0]
+      auto flush = [&](int k) {
+        const int r = rb + k;
+#pragma unroll
+        for (int aa = 0; aa < 3; aa++) {
+          if (acc[aa][k][0] != 0.0 || acc[aa][k][1] != 0.0) {
+            atomicAdd(&lds[aa][r][0], acc[aa][k][0]);
+            atomicAdd(&lds[aa][r][1], acc[aa][k][1]);
+          }
+          acc[aa][k][0] = acc[aa][k][1] = 0.0;
         }
-        const int r0 = m - 1 - base;
-        if (r0 >= 0 && r0 < NTFF_CH) { atomicAdd(&lds[arr[w]][r0][0], vb_r); atomicAdd(&lds[arr[w]][r0][1], vb_i); }
-        if (r0 + 1 >= 0 && r0 + 1 < NTFF_CH) { atomicAdd(&lds[arr[w]][r0 + 1][0], vab_r); atomicAdd(&lds[arr[w]][r0 + 1][1], vab_i); }
-        if (r0 + 2 >= 0 && r0 + 2 < NTFF_CH) { atomicAdd(&lds[arr[w]][r0 + 2][0], -va_r); atomicAdd(&lds[arr[w]][r0 + 2][1], -va_i); }
+      };
+      // samples in batches of 8: their loads are issued together instead of
+      // one dependent round trip per sample
+      constexpr int NB = 8;
+      int pv[NB];
+      double tv[NB];
+      double2 vv[NB];
+      for (int x = xa; x < xb; x++) {
+        const int kx = (x - xa) % NB;
+        if (kx == 0) {
+#pragma unroll
+          for (int u = 0; u < NB; u++) pv[u] = x + u < xb ? e[x + u] : 0;
+#pragma unroll
+          for (int u = 0; u < NB; u++) {
+            const int pp = pv[u] & 0x0fffffff;
+            tv[u] = ts[pp];
+            vv[u] = (pv[u] >> 30) & 1 ? vH[pp] : vE[pp];
+          }
+        }
+        int v = pv[0];
+        double t = tv[0];
+        double2 val = vv[0];
+#pragma unroll
+        for (int u = 1; u < NB; u++)
+          if (kx == u) {
+            v = pv[u];
+            t = tv[u];
+            val = vv[u];
+          }
+        const int p = v & 0x0fffffff, arr = (v >> 28) & 3, isH = (v >> 30) & 1;
+        (void)p;
+        const int d = isH ? (int)floor(t) + 1 : (int)floor(t + 0.5);
+        const int r0 = d - A.dmin - j0 + s;  // window bin of this sample's first tap
+        if (r0 != rb) {
+          // offsets grow along the run: move the register bins forward
+          // (register indices stay compile-time constants: no private-memory arrays)
+          if (rb != INT_MIN) {
+            const int shift = r0 - rb;
+            if (shift >= 3) {
+              flush(0);
+              flush(1);
+              flush(2);
+            } else if (shift == 2) {
+              flush(0);
+              flush(1);
+#pragma unroll
+              for (int aa = 0; aa < 3; aa++) {
+                acc[aa][0][0] = acc[aa][2][0];
+                acc[aa][0][1] = acc[aa][2][1];
+                acc[aa][2][0] = acc[aa][2][1] = 0.0;
+              }
+            } else {
+              flush(0);
+#pragma unroll
+              for (int aa = 0; aa < 3; aa++) {
+                acc[aa][0][0] = acc[aa][1][0];
+                acc[aa][0][1] = acc[aa][1][1];
+                acc[aa][1][0] = acc[aa][2][0];
+                acc[aa][1][1] = acc[aa][2][1];
+                acc[aa][2][0] = acc[aa][2][1] = 0.0;
+              }
+            }
+          }
+          rb = r0;
+        }
+        const double a = isH ? t - (double)(d - 1) : (0.5 + t) - (double)d;
+        const double b = 1.0 - a;
+        if (useCoef) {
+          val.x *= coef;
+          val.y *= coef;
+        }
+        const double w[3] = {b, a - b, -a};
+#pragma unroll
+        for (int aa = 0; aa < 3; aa++)
+          if (aa == arr) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+              acc[aa][k][0] += val.x * w[k];
+              acc[aa][k][1] += val.y * w[k];
+            }
+          }
+      }
+      if (rb != INT_MIN) {
+        flush(0);
+        flush(1);
+        flush(2);
       }
     }
     __syncthreads();
-    const int nb = min(NTFF_CH, hi - base + 1);
+    // window bins [tb0 + dmin + j0 - 1, + nb) into the accumulators
     for (int q = threadIdx.x; q < 3 * nb; q += 256) {
       const int a = q / nb, r = q % nb;
       const double re = lds[a][r][0], im = lds[a][r][1];
       if (re == 0.0 && im == 0.0) continue;
-      const int m = base + r;
+      const long m = tb0 + A.dmin + j0 - 1 + r;
       // taps outside the [360][arraySize] allocation (+ guard) are dropped; the
       // reference writes them out of bounds on strongly non-square grids
       const long gidx = stp + m;
@@ -636,7 +740,10 @@ struct fdtd_grid {
   double2 *nt[3];
   size_t ntn;
   double2 *cE, *cH;
-  double *tsTab, *tsMin, *tsMax;
+  double *tsTab;
+  int *binEnt, *binStart;  // samples of every angle grouped by bin offset (k_ntff_bin_n)
+  BinAng *binAng;
+  int nbuf;                // steps the sample buffers cE / cH hold
   double *cosA, *sinA, *cosB, *sinB;
   double coef;
   double2 *ffout;
@@ -781,6 +888,92 @@ static void build_time_shift(const fdtd_grid *g, std::vector<double> &tab, std::
   }
 }
 
+// Samples of every angle grouped by their bin offset d (see k_ntff_bin_n): for
+// angle a, entries [2 total a, 2 total (a+1)) hold cell | target array << 28 |
+// H << 30 in order of d, and dstart[ai[a].start + j] is the first entry of
+// offset ai[a].dmin + j.
+static int build_bin_groups(fdtd_grid *g, const std::vector<double> &tab)
+{
+  const Contour &ct = g->ct;
+  const int P = ct.total;
+  std::vector<int> ent((size_t)360 * 2 * P), dst;
+  std::vector<BinAng> ai(360);
+  std::vector<int> side(P), dv(2 * P);
+  for (int q = 0; q < 4; q++)
+    for (int c = 0; c < ct.s[q].n; c++) side[ct.s[q].off + c] = q;
+  for (int ang = 0; ang < 360; ang++) {
+    const double *ts = &tab[(size_t)ang * P];
+    int dmin = INT_MAX, dmax = INT_MIN;
+    for (int p = 0; p < P; p++) {
+      dv[2 * p] = (int)floor(ts[p] + 0.5);   // E
+      dv[2 * p + 1] = (int)floor(ts[p]) + 1;  // H
+      dmin = std::min(dmin, std::min(dv[2 * p], dv[2 * p + 1]));
+      dmax = std::max(dmax, std::max(dv[2 * p], dv[2 * p + 1]));
+    }
+    const int nd = dmax - dmin + 1;
+    ai[ang].dmin = dmin;
+    ai[ang].nd = nd;
+    ai[ang].start = (int)dst.size();
+    std::vector<int> cnt(nd + 1, 0);
+    for (int x = 0; x < 2 * P; x++) cnt[dv[x] - dmin + 1]++;
+    for (int j = 0; j < nd; j++) cnt[j + 1] += cnt[j];
+    dst.insert(dst.end(), cnt.begin(), cnt.end());
+    std::vector<int> pos(cnt.begin(), cnt.end() - 1);
+    int *e = &ent[(size_t)ang * 2 * P];
+    for (int x = 0; x < 2 * P; x++) {
+      const int p = x >> 1, isH = x & 1;
+      const Seg &sg = ct.s[side[p]];
+      const int arr = isH ? sg.wa : sg.ua;
+      e[pos[dv[x] - dmin]++] = p | (arr << 28) | (isH << 30);
+    }
+  }
+  HIPCHK(hipMalloc(&g->binEnt, ent.size() * sizeof(int)));
+  HIPCHK(hipMalloc(&g->binStart, dst.size() * sizeof(int)));
+  HIPCHK(hipMalloc(&g->binAng, ai.size() * sizeof(BinAng)));
+  HIPCHK(hipMemcpy(g->binEnt, ent.data(), ent.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(g->binStart, dst.data(), dst.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(g->binAng, ai.data(), ai.size() * sizeof(BinAng), hipMemcpyHostToDevice));
+  return FDTD_OK;
+}
+
+static void launch_bin(fdtd_grid *g, int n, double time0)
+{
+  const fdtd_grid_desc &d = g->d;
+  hipLaunchKernelGGL(k_ntff_bin_n, dim3(360), dim3(256), 0, 0, g->ct, g->cE, g->cH, n, g->tsTab, g->binEnt,
+                     g->binStart, g->binAng, g->nt[0], g->nt[1], g->nt[2], d.array_size, (long)llround(time0) - 1,
+                     g->coef, d.flavour == FDTD_FLAVOUR_PLAIN ? 0 : 1);
+}
+
+// contour samples of one inner step of the FP32 kernel ([3][total] float2: signed
+// E, H on the side, H one cell back) -> the packing of k_ntff_gather
+__global__ __launch_bounds__(256) void k_ctr_combine(const float2 *smp, Contour ct, double2 *cE, double2 *cH)
+{
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= ct.total) return;
+  int q = 0;
+  while (q < 3 && p >= ct.s[q + 1].off) q++;
+  const double hs = ct.s[q].hs;
+  const float2 e = smp[p], h0 = smp[ct.total + p], h1 = smp[2 * ct.total + p];
+  cE[p] = make_double2((double)e.x, (double)e.y);
+  cH[p] = make_double2(hs * ((double)h0.x + (double)h1.x), hs * ((double)h0.y + (double)h1.y));
+}
+
+// after a launch of n steps of an FP32 grid: bin every step's samples at its time
+static int grid_bin_samples(void *ctx, const float2 *samples, int n, const double *times)
+{
+  fdtd_grid *g = (fdtd_grid *)ctx;
+  if (n > g->nbuf) return fdtd_fail(FDTD_EINVAL, "%d steps exceed the %d-step sample buffers", n, g->nbuf);
+  for (int s = 0; s < n; s++) {
+    if (s > 0 && times[s] != times[0] + s) return fdtd_fail(FDTD_ESTATE, "steps of a block are not consecutive");
+    hipLaunchKernelGGL(k_ctr_combine, dim3((g->ct.total + 255) / 256), dim3(256), 0, 0,
+                       samples + (size_t)s * 3 * g->ct.total, g->ct, g->cE + (size_t)s * g->ct.total,
+                       g->cH + (size_t)s * g->ct.total);
+  }
+  launch_bin(g, n, times[0]);
+  HIPCHK(hipGetLastError());
+  return FDTD_OK;
+}
+
 extern "C" int fdtd_hip_grid_create(const fdtd_grid_desc *d, fdtd_grid **out)
 {
   if (!d || !out) return fdtd_fail(FDTD_EINVAL, "null argument");
@@ -832,16 +1025,18 @@ extern "C" int fdtd_hip_grid_create(const fdtd_grid_desc *d, fdtd_grid **out)
   std::vector<double> tab, mn, mx, ca, sa, cb, sb;
   build_time_shift(g, tab, mn, mx, ca, sa, cb, sb);
   const int P = std::max(1, g->ct.total);
-  HIPCHK(hipMalloc(&g->cE, P * sizeof(double2)));
-  HIPCHK(hipMalloc(&g->cH, P * sizeof(double2)));
+  // FP32 grids bin a whole block of steps at once: sample buffers for FAST_MAX_T steps
+  g->nbuf = (d->ntff_time && d->precision == FDTD_FP32) ? FAST_MAX_T : 1;
+  HIPCHK(hipMalloc(&g->cE, (size_t)g->nbuf * P * sizeof(double2)));
+  HIPCHK(hipMalloc(&g->cH, (size_t)g->nbuf * P * sizeof(double2)));
+  if (d->ntff_time && g->ct.total > 0) {
+    int rc = build_bin_groups(g, tab);
+    if (rc) return rc;
+  }
   HIPCHK(hipMalloc(&g->tsTab, std::max<size_t>(1, tab.size()) * sizeof(double)));
   if (!tab.empty()) HIPCHK(hipMemcpy(g->tsTab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
-  HIPCHK(hipMalloc(&g->tsMin, 360 * sizeof(double)));
-  HIPCHK(hipMalloc(&g->tsMax, 360 * sizeof(double)));
   HIPCHK(hipMalloc(&g->cosA, 360 * sizeof(double)));
   HIPCHK(hipMalloc(&g->sinA, 360 * sizeof(double)));
-  HIPCHK(hipMemcpy(g->tsMin, mn.data(), 360 * sizeof(double), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(g->tsMax, mx.data(), 360 * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(g->cosA, ca.data(), 360 * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(g->sinA, sa.data(), 360 * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMalloc(&g->cosB, 360 * sizeof(double)));
@@ -857,8 +1052,10 @@ extern "C" int fdtd_hip_grid_create(const fdtd_grid_desc *d, fdtd_grid **out)
   if (d->precision == FDTD_FP32) {
     int rc = fast_state_create(*d, &g->fs);
     if (rc) return rc;
-    // the per-step time-domain NTFF needs every step's contour samples: one step per launch
-    if (d->ntff_time && (rc = fast_state_set_blocking(g->fs, 1, 16))) return rc;
+    // the per-step time-domain NTFF: the fused kernel samples the contour after
+    // every inner step, and each block's samples are binned step by step
+    if (d->ntff_time && g->ct.total > 0 && (rc = fast_state_set_contour(g->fs, g->ct, g->off, grid_bin_samples, g)))
+      return rc;
   }
   return FDTD_OK;
 }
@@ -866,7 +1063,6 @@ extern "C" int fdtd_hip_grid_create(const fdtd_grid_desc *d, fdtd_grid **out)
 extern "C" int fdtd_hip_grid_set_blocking(fdtd_grid *g, int T, int R)
 {
   if (!g || !g->fs) return fdtd_fail(FDTD_EINVAL, "temporal blocking applies to FP32 grids only");
-  if (g->d.ntff_time && T != 1) return fdtd_fail(FDTD_EINVAL, "the per-step time-domain NTFF needs T = 1");
   return fast_state_set_blocking(g->fs, T, R);
 }
 
@@ -898,9 +1094,10 @@ extern "C" int fdtd_hip_grid_destroy(fdtd_grid *g)
   for (int a = 0; a < 3; a++) (void)hipFree(g->ntb[a]);
   (void)hipFree(g->cE);
   (void)hipFree(g->cH);
+  (void)hipFree(g->binEnt);
+  (void)hipFree(g->binStart);
+  (void)hipFree(g->binAng);
   (void)hipFree(g->tsTab);
-  (void)hipFree(g->tsMin);
-  (void)hipFree(g->tsMax);
   (void)hipFree(g->cosA);
   (void)hipFree(g->sinA);
   (void)hipFree(g->cosB);
@@ -966,17 +1163,8 @@ extern "C" int fdtd_hip_grid_step(fdtd_grid *g, const fdtd_step_params *p)
   if (!g || !p) return fdtd_fail(FDTD_EINVAL, "null argument");
   const fdtd_grid_desc &d = g->d;
   if (g->fs) {
-    int launched = 0, rc = fast_state_step(g->fs, p, &launched);
-    if (rc) return rc;
-    if (d.ntff_time && g->ct.total > 0) {
-      // T = 1 here: the step has run; bin its contour samples as the FP64 path does
-      if ((rc = fast_state_gather(g->fs, g->ct, g->off, g->cE, g->cH))) return rc;
-      hipLaunchKernelGGL(k_ntff_bin, dim3(360), dim3(256), 0, 0, g->ct, g->cE, g->cH, g->tsTab, g->tsMin, g->tsMax,
-                         g->nt[0], g->nt[1], g->nt[2], d.array_size, p->time - 1, p->time - 0.5, g->coef,
-                         d.flavour == FDTD_FLAVOUR_PLAIN ? 0 : 1);
-      HIPCHK(hipGetLastError());
-    }
-    return FDTD_OK;
+    int launched = 0;
+    return fast_state_step(g->fs, p, &launched);
   }
   GridPtrs gp;
   for (int f = 0; f < 9; f++) gp.F[f] = g->F[f];
@@ -1044,8 +1232,7 @@ extern "C" int fdtd_hip_grid_step(fdtd_grid *g, const fdtd_step_params *p)
   if (d.ntff_time && g->ct.total > 0) {
     hipLaunchKernelGGL(k_ntff_gather, dim3((g->ct.total + 255) / 256), dim3(256), 0, 0, (const double2 *const *)g->Fdev,
                        g->ct, g->cE, g->cH);
-    hipLaunchKernelGGL(k_ntff_bin, dim3(360), dim3(256), 0, 0, g->ct, g->cE, g->cH, g->tsTab, g->tsMin, g->tsMax, g->nt[0],
-                       g->nt[1], g->nt[2], d.array_size, p->time - 1, p->time - 0.5, g->coef, plain ? 0 : 1);
+    launch_bin(g, 1, p->time);
     HIPCHK(hipGetLastError());
   }
   return FDTD_OK;
@@ -1074,6 +1261,8 @@ extern "C" int fdtd_hip_grid_get_ntff_time(fdtd_grid *g, int which, double *host
 {
   if (!g || !host || which < 0 || which > 2) return fdtd_fail(FDTD_EINVAL, "bad argument");
   if (!g->ntn) return fdtd_fail(FDTD_ESTATE, "the grid was created without the per-step time-domain NTFF");
+  int rc;
+  if (g->fs && (rc = fast_state_flush(g->fs))) return rc;  // bins the queued steps
   HIPCHK(hipMemcpy(host, g->nt[which], (size_t)360 * g->d.array_size * sizeof(double2), hipMemcpyDeviceToHost));
   return FDTD_OK;
 }
@@ -1120,6 +1309,8 @@ extern "C" int fdtd_hip_grid_time_spectra(fdtd_grid *g, double coef_re, double c
 {
   if (!g || !twiddles || !out) return fdtd_fail(FDTD_EINVAL, "null argument");
   if (!g->ntn) return fdtd_fail(FDTD_ESTATE, "the grid was created without the per-step time-domain NTFF");
+  int rc;
+  if (g->fs && (rc = fast_state_flush(g->fs))) return rc;  // bins the queued steps
   const fdtd_grid_desc &d = g->d;
   HIPCHK(hipMemcpy(g->tw, twiddles, 4096 * sizeof(double2), hipMemcpyHostToDevice));
   const size_t lds = NTFF_NUM * sizeof(double2);

@@ -60,6 +60,23 @@ struct FastInst {
   int on[2];
 };
 
+// One side of the NTFF contour in the FP32 physical layout, for sampling inside
+// the fused kernel: cells (fi, lo..lo+n-1) of a vertical side (i fixed) or
+// (lo..lo+n-1, fi) of a horizontal one (j fixed), packed from `off`; the H
+// average of a contour cell takes its neighbour one cell back across the side.
+struct FastCtrSide {
+  int horiz;      // 1: j fixed (bottom, top), 0: i fixed (right, left)
+  int fi, lo, n;  // fixed coordinate, first varying coordinate, cells
+  int off;        // offset in the packed contour
+  int ef;         // E component: 0 -> f0, 1 -> f1
+  int hf;         // H component: 1 -> f1, 2 -> f2
+  float es;       // sign of the E sample
+};
+struct FastContour {
+  FastCtrSide s[4];
+  int total;
+};
+
 // Progress of one batched instance (device memory, advanced between launches).
 struct FastBatchInst {
   double omega;  // angular frequency of the scattered wave (MPI flavour)
@@ -83,6 +100,11 @@ struct FastArgs {
   FastBatchInst *binst;
   int *qctr;
   int qn[2], qoff[2];
+  // per-step NTFF samples: after every inner step s, each contour cell of a tile's
+  // core writes csamp[s][0][c] = es * E, csamp[s][1][c] = H, and each cell one
+  // back across a side writes csamp[s][2][c] = H of that neighbour (null: off)
+  float2 *csamp;
+  FastContour ct;
   FastStep st[FAST_MAX_T];  // single-instance runs: the block's steps, passed by value
 };
 
@@ -97,6 +119,11 @@ int fast_state_export(FastState *s, int slot, double2 *dst_dev, int NPYa, int of
 int fast_state_import(FastState *s, int slot, const double2 *src_dev, int NPYa, int off);
 int fast_state_reset(FastState *s);
 int fast_state_gather(FastState *s, const Contour &ct, int off, double2 *cE, double2 *cH);
+// Per-step contour sampling inside the fused kernel: after every launch of n
+// steps, post(ctx, samples, n, times) receives the device buffer
+// [n][3][total] float2 and the time of each step.
+typedef int (*fast_post_fn)(void *ctx, const float2 *samples, int n, const double *times);
+int fast_state_set_contour(FastState *s, const Contour &ct, int off, fast_post_fn post, void *ctx);
 int fast_state_set_blocking(FastState *s, int T, int R);
 
 struct FastBatch;

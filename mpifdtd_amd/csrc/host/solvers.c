@@ -34,7 +34,7 @@ typedef struct {
   int NPXa, NPYa, off, arraySize;
   double complex *mirror[3];  /* TM: Ez Hx Hy   TE: Ex Ey Hz */
   double *eps_mirror;
-  int blocked;    /* FP32 grid without the per-step NTFF: steps are queued and run T per launch */
+  int blocked;    /* FP32 grid: steps are queued and run T per launch */
   int ntff_time;  /* the per-step time-domain NTFF runs (its outputs exist) */
 } solver_t;
 
@@ -187,7 +187,7 @@ int solver_make_grid(int kind, int precision, int ntff_time, int T, int R, fdtd_
   int rc = fdtd_hip_grid_create(&d, &g);
   if (rc != FDTD_OK) return rc;
   build_coefs(kind, g);
-  if ((rc = fdtd_hip_grid_fill_eps(g, models_getState())) == FDTD_OK && precision == FDTD_FP32 && !ntff_time)
+  if ((rc = fdtd_hip_grid_fill_eps(g, models_getState())) == FDTD_OK && precision == FDTD_FP32)
     rc = fdtd_hip_grid_set_blocking(g, T > 0 ? T : 4, R);
   if (rc != FDTD_OK) {
     fdtd_hip_grid_destroy(g);
@@ -208,7 +208,7 @@ static void solver_init(int kind)
   s->NPYa = dims[1];
   s->off = dims[2];
   s->arraySize = dims[3];
-  s->blocked = precision_ == FDTD_FP32 && !ntff_time_;
+  s->blocked = precision_ == FDTD_FP32;
   s->ntff_time = ntff_time_;
   size_t n = (size_t)s->NPXa * s->NPYa;
   for (int q = 0; q < 3; q++) s->mirror[q] = (double complex *)calloc(n, sizeof(double complex));

@@ -44,7 +44,8 @@ class Simulator:
     def __init__(self, solver, model, fi, params=None, c0=0.7071, ntff_inset=5, point_source=False,
                  ntff_time=True, file_output=False, precision="fp64", T=None, R=16):
         """precision "fp64": reference-order FP64 kernels; "fp32": fused FP32 kernels
-        advancing T steps per launch (T forced to 1 when ntff_time is on)."""
+        advancing T steps per launch (with ntff_time on they sample the NTFF
+        contour after every inner step)."""
         L = N.lib()
         if L.fdtd_hip_device_count() < 1:
             raise RuntimeError("no HIP device visible: the solvers run on the GPU only")
@@ -58,6 +59,7 @@ class Simulator:
         L.fdtd_solver_setNTFFTime(1 if ntff_time else 0)
         L.fdtd_solver_setFileOutput(1 if file_output else 0)
         self.precision = precision
+        self.per_step_ntff = bool(ntff_time)
         L.fdtd_solver_setPrecision(1 if precision == "fp32" else 0)
         if precision == "fp32":
             L.fdtd_solver_setBlocking(int(T or 4), int(R))
