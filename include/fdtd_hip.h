@@ -132,6 +132,14 @@ int fdtd_hip_grid_far_field(fdtd_grid *g, double k, double coef_re, double coef_
  * pairs), h_u_nm; out[321*360] lambda-major. */
 int fdtd_hip_grid_time_spectra(fdtd_grid *g, double coef_re, double coef_im, const double *twiddles,
                                int h_u_nm, double *out);
+/* running DFT of the NTFF contour at nf angular frequencies (rad per step):
+ * every later step adds its contour samples times e^{i w tb} (tb = time - 1,
+ * the binning's base time) into FP64 per-cell accumulators; nf = 0 turns it off.
+ * No counterpart in the reference (ntffTM.c:279-371 bins in the time domain). */
+int fdtd_hip_grid_set_dft(fdtd_grid *g, const double *omega, int nf);
+/* out[nf][3][360] complex: sum_m U_a[m] e^{i w m} over the rows of the three
+ * time-domain accumulators as the binning would fill them */
+int fdtd_hip_grid_get_dft(fdtd_grid *g, double *out);
 /* dimensions: out[0..3] = NPXa, NPYa, off, array_size */
 int fdtd_hip_grid_dims(const fdtd_grid *g, int out[4]);
 int fdtd_hip_grid_sync(fdtd_grid *g);

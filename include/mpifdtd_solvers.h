@@ -88,6 +88,18 @@ int fdtd_solver_farField(double *out360);
 /* time-domain far-field spectra 380..700 nm x 360 angles, as the plain
  * solvers' ntff*_TimeOutput computes them (ntffTM.c:197-233, ntffTE.c:160-196) */
 int fdtd_solver_timeSpectra(double *out321x360);
+/* running DFT of the NTFF contour at n wavelengths (nm) of the active solver,
+ * from the next step on (n = 0 turns it off).  No counterpart in the
+ * reference: the spectra of its pulse solvers come from the time-domain
+ * accumulators and an 8192-point FFT (ntffTM.c:197-233); the running DFT gives
+ * the same transform at exact frequencies without the [360][arraySize] bins. */
+int fdtd_solver_setDFT(const double *lambda_nm, int n);
+/* out[n][3][360] complex: sum_m U_a[m] e^{i w m} of the three time-domain
+ * accumulator rows (Ux Uy Wz / Wx Wy Uz) of every angle at each frequency */
+int fdtd_solver_getDFT(double *out);
+/* out[n][360]: |E|^2 / 8192 with E the time translation of fdtd_solver_timeSpectra
+ * applied to fdtd_solver_getDFT at each wavelength */
+int fdtd_solver_dftSpectra(double *out);
 /* 0: skip per-step time-domain NTFF accumulation (default 1, as the reference); set before init */
 void fdtd_solver_setNTFFTime(int on);
 /* 0: reset()/finish() write no output files (default 1, as the reference) */
@@ -101,8 +113,7 @@ void fdtd_solver_setPrecision(int precision);
 int fdtd_solver_getPrecision(void);
 /* FP32: steps per fused launch T and rows per wavefront R of the active / next solver */
 int fdtd_solver_setBlocking(int T, int R);
-/* blocking of the active solver; FDTD_ESTATE when it does not queue steps (FP64, or
- * FP32 with the per-step time-domain NTFF, which runs one step per launch) */
+/* blocking of the active solver; FDTD_ESTATE when it does not queue steps (FP64) */
 int fdtd_solver_getBlocking(int *T, int *R);
 
 #ifdef __cplusplus

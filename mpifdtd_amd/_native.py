@@ -136,6 +136,9 @@ def lib():
     sig("fdtd_solver_getNTFFTime", I, I, dp)
     sig("fdtd_solver_farField", I, dp)
     sig("fdtd_solver_timeSpectra", I, dp)
+    sig("fdtd_solver_setDFT", I, dp, I)
+    sig("fdtd_solver_getDFT", I, dp)
+    sig("fdtd_solver_dftSpectra", I, dp)
     sig("fdtd_solver_setNTFFTime", V, I)
     sig("fdtd_solver_setFileOutput", V, I)
     sig("fdtd_solver_dims", I, ip)
@@ -153,6 +156,18 @@ def lib():
     sig("fdtd_batch_field", I, P, I, I, dp)
     sig("fdtd_batch_far_field", I, P, I, dp)
     sig("fdtd_batch_destroy", V, P)
+    # mpifdtd_surrogate.h
+    vp = ctypes.c_void_p
+    fp = ctypes.POINTER(ctypes.c_float)
+    sig("mlp_create", I, I, ip, I, ctypes.c_ulonglong, ctypes.POINTER(P))
+    sig("mlp_destroy", I, P)
+    sig("mlp_forward", I, P, vp, I, vp)
+    sig("mlp_grads", I, P, vp, vp, I, fp)
+    sig("mlp_train_step", I, P, vp, vp, I, ctypes.c_float, fp)
+    sig("mlp_get_params", I, P, I, fp, fp)
+    sig("mlp_set_params", I, P, I, fp, fp)
+    sig("mlp_get_grads", I, P, I, fp, fp)
+    sig("mlp_gemm_bf16", I, vp, vp, vp, I, I, I, I, I)
     sig("fdtd_hip_profile_enable", V, I)
     sig("fdtd_hip_profile_read", V, dp)
     sig("fdtd_hip_last_error", ctypes.c_char_p)

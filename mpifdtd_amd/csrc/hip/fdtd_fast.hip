@@ -486,70 +486,86 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
 
     // per-step NTFF samples of the contour cells in this tile's core (the state
     // after each full step, as the reference's per-step NTFF reads it).  The
-    // sampling plan is fixed for the tile: a side with j fixed lies in one lane
-    // (rows `srows`, packed index sc + r), a side with i fixed in one row of the
-    // block (sr1; its neighbour line sr2), read by the lanes in its range.
-    bool csmp = false;
-    unsigned srows[4];
-    int srole[4], sr1[4], sr2[4], sc[4];
+    // plan is fixed for the tile.  A side with j fixed lies in one lane: that
+    // lane stores its rows `lmask` (packed index lc + r) with a store per row
+    // under its own mask; the host keeps the two such sides' columns apart, so
+    // a lane has one role at most (1: E and H of the side, 2: H one column
+    // back).  A side with i fixed lies in one row of one wave: the wave keeps
+    // the rows holding such a side (evr), with the side (2 bits per row, evq)
+    // and whether the row is the side itself (E and H) or its H neighbour (evk);
+    // the host keeps the two sides' rows apart, so a row has one event at most.
+    bool smpL = false;
+    unsigned lmask = 0, evr = 0, evq = 0, evk = 0;
+    int lrole = 0, lc = 0;
+    bool lef = false, lhf = false;
+    float les = 1.f;
+    const bool jcore = jj >= t.j0 && jj < t.j0 + t.nj;
     if (SMP) {
-      const bool jcore = jj >= t.j0 && jj < t.j0 + t.nj;
 #pragma unroll
       for (int q = 0; q < 4; q++) {
         const FastCtrSide &g = a.ct.s[q];
-        srows[q] = 0;
-        srole[q] = 0;
-        sr1[q] = sr2[q] = -1;
-        sc[q] = 0;
         if (g.n <= 0) continue;
         if (g.horiz) {
-          for (int r = 0; r < R; r++) {
-            const int i = ib + r;
-            if (i >= t.i0 && i < t.i0 + t.ni && i >= g.lo && i < g.lo + g.n) srows[q] |= 1u << r;
+          const int role = !jcore ? 0 : (jj == g.fi ? 1 : (jj == g.fi - 1 ? 2 : 0));
+          if (role) {
+            unsigned m = 0;
+            for (int r = 0; r < R; r++) {
+              const int i = ib + r;
+              if (i >= t.i0 && i < t.i0 + t.ni && i >= g.lo && i < g.lo + g.n) m |= 1u << r;
+            }
+            lmask = m;
+            lrole = role;
+            lc = g.off + (ib - g.lo);
+            lef = g.ef != 0;
+            lhf = g.hf == 1;
+            les = g.es;
           }
-          srole[q] = !jcore ? 0 : (jj == g.fi ? 1 : (jj == g.fi - 1 ? 2 : 0));
-          sc[q] = g.off + (ib - g.lo);
-          if (__ballot(srole[q] != 0 && srows[q] != 0)) csmp = true;
         } else {
-          const bool core1 = g.fi >= t.i0 && g.fi < t.i0 + t.ni, core2 = g.fi - 1 >= t.i0 && g.fi - 1 < t.i0 + t.ni;
-          sr1[q] = core1 ? g.fi - ib : -1;
-          sr2[q] = core2 ? g.fi - 1 - ib : -1;
-          srole[q] = jcore && jj >= g.lo && jj < g.lo + g.n;
-          sc[q] = g.off + (jj - g.lo);
-          if (__ballot(srole[q] != 0) && ((sr1[q] >= 0 && sr1[q] < R) || (sr2[q] >= 0 && sr2[q] < R))) csmp = true;
+          const int r1 = g.fi - ib, r2 = g.fi - 1 - ib;
+          if (g.fi >= t.i0 && g.fi < t.i0 + t.ni && r1 >= 0 && r1 < R) {
+            evr |= 1u << r1;
+            evq |= (unsigned)q << (2 * r1);
+            evk |= 1u << r1;
+          }
+          if (g.fi - 1 >= t.i0 && g.fi - 1 < t.i0 + t.ni && r2 >= 0 && r2 < R) {
+            evr |= 1u << r2;
+            evq |= (unsigned)q << (2 * r2);
+          }
         }
       }
+      smpL = __ballot(lmask != 0) != 0;
+      evr = (unsigned)uni_i((int)evr);
+      evq = (unsigned)uni_i((int)evq);
+      evk = (unsigned)uni_i((int)evk);
     }
     auto sample = [&](int s) {
       float2 *base = a.csamp + (size_t)s * 3 * a.ct.total;
       const int tot = a.ct.total;
+      if (smpL) {
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const FastCtrSide &g = a.ct.s[q];
-        if (g.horiz) {
-          if (srole[q]) {
-#pragma unroll
-            for (int r = 0; r < R; r++) {
-              if (!(srows[q] & (1u << r))) continue;
-              const int c = sc[q] + r;
-              const float2 h = g.hf == 1 ? F1[r] : F2[r];
-              if (srole[q] == 1) {
-                base[c] = c_scale(g.ef == 0 ? F0[r] : F1[r], g.es);
-                base[tot + c] = h;
-              } else {
-                base[2 * tot + c] = h;
-              }
-            }
+        for (int r = 0; r < R; r++) {
+          if (lmask & (1u << r)) {
+            const float2 h = lhf ? F1[r] : F2[r];
+            const float2 e = c_scale(lef ? F1[r] : F0[r], les);
+            const int c = lc + r;
+            base[lrole == 1 ? c : 2 * tot + c] = lrole == 1 ? e : h;
+            if (lrole == 1) base[tot + c] = h;
           }
-        } else if (srole[q]) {
+        }
+      }
+      if (evr) {
 #pragma unroll
-          for (int r = 0; r < R; r++) {
+        for (int r = 0; r < R; r++) {
+          if (!(evr & (1u << r))) continue;
+          const FastCtrSide &g = a.ct.s[(evq >> (2 * r)) & 3];
+          if (jcore && jj >= g.lo && jj < g.lo + g.n) {
+            const int c = g.off + (jj - g.lo);
             const float2 h = g.hf == 1 ? F1[r] : F2[r];
-            if (r == sr1[q]) {
-              base[sc[q]] = c_scale(g.ef == 0 ? F0[r] : F1[r], g.es);
-              base[tot + sc[q]] = h;
-            } else if (r == sr2[q]) {
-              base[2 * tot + sc[q]] = h;
+            if (evk & (1u << r)) {
+              base[c] = c_scale(g.ef == 0 ? F0[r] : F1[r], g.es);
+              base[tot + c] = h;
+            } else {
+              base[2 * tot + c] = h;
             }
           }
         }
@@ -564,7 +580,7 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
         __syncthreads();
         h_phase();
         __syncthreads();
-        if (SMP && csmp) sample(s);
+        if (SMP) sample(s);
       }
     } else {
       xE[w][lane] = MODE == FDTD_MODE_TM ? F0[0] : F1[0];
@@ -574,7 +590,7 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
         __syncthreads();
         e_phase(s);
         __syncthreads();
-        if (SMP && csmp) sample(s);
+        if (SMP) sample(s);
       }
     }
 
@@ -1603,6 +1619,16 @@ int fast_state_set_contour(FastState *s, const Contour &ct, int off, fast_post_f
     f.es = (float)g.es;
   }
   c.total = ct.total;
+  // the kernel gives a lane one role at most on the sides with j fixed, and a
+  // row one event at most on the sides with i fixed: the lines (fi and fi - 1)
+  // of the two sides of each kind must not meet
+  for (int hz = 0; hz < 2; hz++) {
+    int fl[2], nc = 0;
+    for (int q = 0; q < 4; q++)
+      if (c.s[q].horiz == hz && c.s[q].n > 0 && nc < 2) fl[nc++] = c.s[q].fi;
+    if (nc == 2 && abs(fl[0] - fl[1]) < 2)
+      return fdtd_fail(FDTD_EINVAL, "NTFF contour %d cells across: too small for the FP32 path", abs(fl[0] - fl[1]));
+  }
   if (s->csamp) (void)hipFree(s->csamp);
   s->csamp = nullptr;
   if (ct.total > 0) HIPCHK(hipMalloc(&s->csamp, (size_t)FAST_MAX_T * 3 * ct.total * sizeof(float2)));

@@ -292,184 +292,241 @@ __global__ __launch_bounds__(256) void k_ntff_gather(const double2 *const *F, Co
   cH[p] = hv;
 }
 
-#define NTFF_CH 2048
 #define NT_GUARD_DEV 4  // guard bins allocated before and after the accumulators
+#define NTFF_WIN 1024  // bin offsets per LDS window of k_ntff_bin
+#define NTFF_GRP 32    // sample groups per workgroup of k_ntff_bin
+#define NTFF_RUN 16    // samples per group and part
 
-// Binning of n consecutive steps per launch, one workgroup per angle.
+// Binning of n consecutive steps per launch.
 //
 // For one angle and contour cell the time shift ts is fixed, so the E sample of
 // the step with base time tb = time - 1 lands on bins tb + dE + (-1, 0, +1) and
 // the H sample on tb + dH + (-1, 0, +1), with dE = floor(ts + 0.5), dH =
-// floor(ts) + 1 and 3-tap weights (b, a - b, -a) that do not depend on the step
-// (a = 0.5 + ts - dE for E, ts - (dH - 1) for H): the bins and weights of the
-// reference's calc() (ntffTM.c:279-288, mpiTM_UPML.c:994-1036).  The host sorts
-// each angle's 2 x total samples by offset d.  Each thread walks a contiguous
-// run of that order, so its offsets only grow: it keeps the three bins its
-// current sample touches in registers (per target array) and adds a bin to the
-// LDS window only when its offset moves past it.  Cells of one side that share
-// an offset (the shift barely changes along a side seen edge-on) therefore cost
-// one LDS atomic per bin instead of one per sample and tap.
-struct BinAng {
-  int dmin, nd, start;  // smallest offset, offsets, first entry of dstart for this angle
+// floor(ts) + 1 and 3-tap weights (b, a - b, -a), b = 1 - a, that do not depend
+// on the step (a = 0.5 + ts - dE for E, ts - (dH - 1) for H): the bins and
+// weights of the reference's calc() (ntffTM.c:279-288, mpiTM_UPML.c:994-1036).
+//
+// The host sorts the samples of each (angle, target array) by offset d and cuts
+// the sorted list into parts of at most NTFF_GRP x NTFF_RUN samples spanning at
+// most NTFF_WIN offsets, one workgroup each.  A workgroup is NTFF_GRP groups of
+// NS >= n lanes; group g takes a contiguous run of the part's sorted samples
+// (records stored [iteration][group], so that their loads coalesce) and lane s
+// of the group handles step s: the n steps of a sample are contiguous in the
+// step-minor sample buffers ([cell][n]), so one load instruction reads whole
+// cache lines.  Lane s keeps the three bins d + s - 1 + k of its step in
+// registers; as the run's offsets grow the window shifts, and only the bins it
+// leaves go to the LDS window with an atomic.  The LDS window goes to the
+// accumulators once per launch: its first and last n + 2 bins with atomics (the
+// neighbouring parts of the same row reach them), the others with a plain
+// read-modify-write.
+struct BinRec {
+  int p;     // cell | H << 30, or -1 (padding at the end of a group's run)
+  int r;     // d minus the part's smallest offset
+  double a;  // tap fraction
+};
+struct BinPart {
+  int rec0, per;  // first record, iterations (records per group)
+  int dlo, nd;    // smallest offset, offsets spanned
+  int row, arr;   // angle, target array
 };
 
-__global__ __launch_bounds__(256) void k_ntff_bin_n(Contour ct, const double2 *cE, const double2 *cH, int n,
-                                                    const double *tsTab, const int *ent, const int *dstart,
-                                                    const BinAng *ai, double2 *U0, double2 *U1, double2 *U2,
-                                                    int arraySize, long tb0, double coef, int useCoef)
+template <int NS>
+__global__ __launch_bounds__(NTFF_GRP * NS) void k_ntff_bin(const double2 *cE, const double2 *cH, int n,
+                                                             const BinPart *parts, const BinRec *rec, double2 *U0,
+                                                             double2 *U1, double2 *U2, int arraySize, long tb0,
+                                                             double coef, int useCoef)
 {
-  __shared__ double lds[3][NTFF_CH][2];
-  const int ang = blockIdx.x;
-  const BinAng A = ai[ang];
-  const double *ts = tsTab + (size_t)ang * ct.total;
-  const int *e = ent + (size_t)ang * 2 * ct.total;
-  const int *ds = dstart + A.start;
-  double2 *U[3] = {U0, U1, U2};
-  const long stp = (long)ang * arraySize;
-  const int span = NTFF_CH - n - 1;  // offsets per window (the window's bins: span + n + 1)
-  for (int j0 = 0; j0 < A.nd; j0 += span) {
-    const int nj = min(span, A.nd - j0);
-    const int nb = nj + n + 1;
-    for (int q = threadIdx.x; q < 3 * NTFF_CH; q += 256) {
-      lds[q / NTFF_CH][q % NTFF_CH][0] = 0.0;
-      lds[q / NTFF_CH][q % NTFF_CH][1] = 0.0;
+  constexpr int NT = NTFF_GRP * NS;
+  __shared__ double lds[NTFF_WIN + NS + 1][2];
+  const BinPart W = parts[blockIdx.x];
+  double2 *U = W.arr == 0 ? U0 : (W.arr == 1 ? U1 : U2);
+  const long stp = (long)W.row * arraySize;
+  const double cf = useCoef ? coef : 1.0;
+  const int nb = W.nd + n + 1;
+  for (int q = threadIdx.x; q < nb; q += NT) lds[q][0] = lds[q][1] = 0.0;
+  __syncthreads();
+  const int grp = threadIdx.x / NS, s = threadIdx.x % NS;
+  const bool act = s < n;
+  double xr0 = 0.0, xr1 = 0.0, xr2 = 0.0, xi0 = 0.0, xi1 = 0.0, xi2 = 0.0;
+  int rc = -1;  // offset (window-relative) of the run's current sample; xr0 is bin rc + s
+  const BinRec *rr = rec + W.rec0 + grp;
+  const BinRec pad = {-1, 0, 0.0};
+  auto value = [&](int p) {
+    // padding (p < 0) and lanes past n read nothing; the address stays in cell 0
+    const int pc = p < 0 ? 0 : p;
+    const double2 *src = ((pc >> 30) ? cH : cE) + (size_t)(pc & 0x3fffffff) * n;
+    return (p >= 0 && act) ? src[s] : make_double2(0.0, 0.0);
+  };
+  BinRec R = W.per > 0 ? rr[0] : pad;
+  double2 v = value(R.p);
+  for (int i = 0; i < W.per && R.p >= 0; i++) {
+    // the next record and its value in flight while this one is added
+    const BinRec Rn = i + 1 < W.per ? rr[(size_t)(i + 1) * NTFF_GRP] : pad;
+    const double2 vn = value(Rn.p);
+    if (R.r != rc) {
+      if (rc >= 0) {
+        const int delta = R.r - rc;
+        if (act) {
+          if (delta >= 3) {
+            atomicAdd(&lds[rc + s][0], xr0);
+            atomicAdd(&lds[rc + s][1], xi0);
+            atomicAdd(&lds[rc + s + 1][0], xr1);
+            atomicAdd(&lds[rc + s + 1][1], xi1);
+            atomicAdd(&lds[rc + s + 2][0], xr2);
+            atomicAdd(&lds[rc + s + 2][1], xi2);
+            xr0 = xr1 = xr2 = xi0 = xi1 = xi2 = 0.0;
+          } else if (delta == 2) {
+            atomicAdd(&lds[rc + s][0], xr0);
+            atomicAdd(&lds[rc + s][1], xi0);
+            atomicAdd(&lds[rc + s + 1][0], xr1);
+            atomicAdd(&lds[rc + s + 1][1], xi1);
+            xr0 = xr2;
+            xi0 = xi2;
+            xr1 = xr2 = xi1 = xi2 = 0.0;
+          } else {
+            atomicAdd(&lds[rc + s][0], xr0);
+            atomicAdd(&lds[rc + s][1], xi0);
+            xr0 = xr1;
+            xi0 = xi1;
+            xr1 = xr2;
+            xi1 = xi2;
+            xr2 = xi2 = 0.0;
+          }
+        }
+      }
+      rc = R.r;
     }
-    __syncthreads();
-    // the samples of this window's offsets, split into one contiguous run per thread
-    const int x0 = ds[j0], x1 = ds[j0 + nj];
-    const int per = (x1 - x0 + 255) / 256;
-    const int xa = x0 + (int)threadIdx.x * per, xb = min(x1, xa + per);
-    for (int s = 0; s < n; s++) {
-      const double2 *vE = cE + (size_t)s * ct.total, *vH = cH + (size_t)s * ct.total;
-      double acc[3][3][2];  // [array][bin - rb][re, im]
+    const double a = R.a, b = 1.0 - a;
+    const double vr = v.x * cf, vi = v.y * cf;
+    xr0 += vr * b;
+    xi0 += vi * b;
+    xr1 += vr * (a - b);
+    xi1 += vi * (a - b);
+    xr2 -= vr * a;
+    xi2 -= vi * a;
+    R = Rn;
+    v = vn;
+  }
+  if (rc >= 0 && act) {
+    atomicAdd(&lds[rc + s][0], xr0);
+    atomicAdd(&lds[rc + s][1], xi0);
+    atomicAdd(&lds[rc + s + 1][0], xr1);
+    atomicAdd(&lds[rc + s + 1][1], xi1);
+    atomicAdd(&lds[rc + s + 2][0], xr2);
+    atomicAdd(&lds[rc + s + 2][1], xi2);
+  }
+  __syncthreads();
+  // window bins [tb0 + dlo - 1, + nb) into the accumulators
+  for (int q = threadIdx.x; q < nb; q += NT) {
+    const double re = lds[q][0], im = lds[q][1];
+    if (re == 0.0 && im == 0.0) continue;
+    const long m = tb0 + W.dlo - 1 + q;
+    // taps outside the [360][arraySize] allocation (+ guard) are dropped; the
+    // reference writes them out of bounds on strongly non-square grids
+    const long gidx = stp + m;
+    if (gidx < -NT_GUARD_DEV || gidx >= 360L * arraySize + NT_GUARD_DEV) continue;
+    double2 *dst = U + gidx;
+    // bins near a row end are shared with the neighbouring angle (the flat
+    // [360][arraySize] layout lets m-1 / m+1 spill over, Q10), the window's
+    // edge bins with the neighbouring parts: atomics there
+    if (m < 3 || m >= arraySize - 3 || q < n + 2 || q >= nb - (n + 2)) {
+      atomicAdd(&dst->x, re);
+      atomicAdd(&dst->y, im);
+    } else {
+      double2 u = *dst;
+      u.x += re;
+      u.y += im;
+      *dst = u;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- running DFT of the contour
+
+// V_f[c] += sum_s val_s[c] e^{i w_f tb_s}, tb_s = tb0 + s the binning's base time
+// of step s: n steps of the [cell][n] sample buffers per launch, one workgroup
+// per (256 cells, frequency); the n phase factors come from FP64 sincos.
+__global__ __launch_bounds__(256) void k_ctr_dft(const double2 *cE, const double2 *cH, int n, int P, const double *omega,
+                                                 long tb0, double2 *VE, double2 *VH)
+{
+  __shared__ double2 ph[FAST_MAX_T];
+  const int f = blockIdx.y;
+  if ((int)threadIdx.x < n) {
+    double sn, cs;
+    sincos(omega[f] * (double)(tb0 + (long)threadIdx.x), &sn, &cs);
+    ph[threadIdx.x] = make_double2(cs, sn);
+  }
+  __syncthreads();
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= P) return;
+  double2 ae = VE[(size_t)f * P + p], ah = VH[(size_t)f * P + p];
+  for (int s = 0; s < n; s++) {
+    const double2 e = cE[(size_t)p * n + s], h = cH[(size_t)p * n + s], w = ph[s];
+    ae.x += e.x * w.x - e.y * w.y;
+    ae.y += e.x * w.y + e.y * w.x;
+    ah.x += h.x * w.x - h.y * w.y;
+    ah.y += h.x * w.y + h.y * w.x;
+  }
+  VE[(size_t)f * P + p] = ae;
+  VH[(size_t)f * P + p] = ah;
+}
+
+// The DFT at w_f of the three time-domain accumulator rows of every angle,
+// sum_m U[m] e^{i w m}, formed from the running DFT of the contour: a sample
+// binned at tb + d + (-1, 0, 1) with weights (b, a - b, -a) contributes
+// V e^{i w (d - 1)} (b + (a - b) e^{i w} - a e^{2 i w}), with d and a from the
+// same time shifts as the binning.  One workgroup per (angle, frequency);
+// out[f][array][angle].
+__global__ __launch_bounds__(256) void k_dft_far(const double2 *VE, const double2 *VH, Contour ct, const double *ts,
+                                                 const double *omega, double cf, double2 *out)
+{
+  __shared__ double red[6][4];
+  const int ang = blockIdx.x, f = blockIdx.y, P = ct.total;
+  const double w = omega[f];
+  double sn1, cs1, sn2, cs2;
+  sincos(w, &sn1, &cs1);
+  sincos(2.0 * w, &sn2, &cs2);
+  double acc[3][2] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+  for (int p = threadIdx.x; p < P; p += 256) {
+    int q = 0;
+    while (q < 3 && p >= ct.s[q + 1].off) q++;
+    const double t = ts[(size_t)ang * P + p];
+#pragma unroll
+    for (int hh = 0; hh < 2; hh++) {
+      const int d = hh ? (int)floor(t) + 1 : (int)floor(t + 0.5);
+      const double a = hh ? t - (double)(d - 1) : (0.5 + t) - (double)d, b = 1.0 - a;
+      // (b + (a - b) e^{iw} - a e^{2iw}) e^{i w (d - 1)}
+      const double kr = b + (a - b) * cs1 - a * cs2, ki = (a - b) * sn1 - a * sn2;
+      double sd, cd;
+      sincos(w * (double)(d - 1), &sd, &cd);
+      const double Kr = kr * cd - ki * sd, Ki = kr * sd + ki * cd;
+      const double2 v = hh ? VH[(size_t)f * P + p] : VE[(size_t)f * P + p];
+      const int arr = hh ? ct.s[q].wa : ct.s[q].ua;
+      const double re = v.x * Kr - v.y * Ki, im = v.x * Ki + v.y * Kr;
 #pragma unroll
       for (int aa = 0; aa < 3; aa++)
-#pragma unroll
-        for (int k = 0; k < 3; k++) acc[aa][k][0] = acc[aa][k][1] = 0.0;
-      int rb = INT_MIN;  // window bin of acc[.][0]
-      auto flush = [&](int k) {
-        const int r = rb + k;
-#pragma unroll
-        for (int aa = 0; aa < 3; aa++) {
-          if (acc[aa][k][0] != 0.0 || acc[aa][k][1] != 0.0) {
-            atomicAdd(&lds[aa][r][0], acc[aa][k][0]);
-            atomicAdd(&lds[aa][r][1], acc[aa][k][1]);
-          }
-          acc[aa][k][0] = acc[aa][k][1] = 0.0;
+        if (aa == arr) {
+          acc[aa][0] += re;
+          acc[aa][1] += im;
         }
-      };
-      // samples in batches of 8: their loads are issued together instead of
-      // one dependent round trip per sample
-      constexpr int NB = 8;
-      int pv[NB];
-      double tv[NB];
-      double2 vv[NB];
-      for (int x = xa; x < xb; x++) {
-        const int kx = (x - xa) % NB;
-        if (kx == 0) {
-#pragma unroll
-          for (int u = 0; u < NB; u++) pv[u] = x + u < xb ? e[x + u] : 0;
-#pragma unroll
-          for (int u = 0; u < NB; u++) {
-            const int pp = pv[u] & 0x0fffffff;
-            tv[u] = ts[pp];
-            vv[u] = (pv[u] >> 30) & 1 ? vH[pp] : vE[pp];
-          }
-        }
-        int v = pv[0];
-        double t = tv[0];
-        double2 val = vv[0];
-#pragma unroll
-        for (int u = 1; u < NB; u++)
-          if (kx == u) {
-            v = pv[u];
-            t = tv[u];
-            val = vv[u];
-          }
-        const int p = v & 0x0fffffff, arr = (v >> 28) & 3, isH = (v >> 30) & 1;
-        (void)p;
-        const int d = isH ? (int)floor(t) + 1 : (int)floor(t + 0.5);
-        const int r0 = d - A.dmin - j0 + s;  // window bin of this sample's first tap
-        if (r0 != rb) {
-          // offsets grow along the run: move the register bins forward
-          // (register indices stay compile-time constants: no private-memory arrays)
-          if (rb != INT_MIN) {
-            const int shift = r0 - rb;
-            if (shift >= 3) {
-              flush(0);
-              flush(1);
-              flush(2);
-            } else if (shift == 2) {
-              flush(0);
-              flush(1);
-#pragma unroll
-              for (int aa = 0; aa < 3; aa++) {
-                acc[aa][0][0] = acc[aa][2][0];
-                acc[aa][0][1] = acc[aa][2][1];
-                acc[aa][2][0] = acc[aa][2][1] = 0.0;
-              }
-            } else {
-              flush(0);
-#pragma unroll
-              for (int aa = 0; aa < 3; aa++) {
-                acc[aa][0][0] = acc[aa][1][0];
-                acc[aa][0][1] = acc[aa][1][1];
-                acc[aa][1][0] = acc[aa][2][0];
-                acc[aa][1][1] = acc[aa][2][1];
-                acc[aa][2][0] = acc[aa][2][1] = 0.0;
-              }
-            }
-          }
-          rb = r0;
-        }
-        const double a = isH ? t - (double)(d - 1) : (0.5 + t) - (double)d;
-        const double b = 1.0 - a;
-        if (useCoef) {
-          val.x *= coef;
-          val.y *= coef;
-        }
-        const double w[3] = {b, a - b, -a};
-#pragma unroll
-        for (int aa = 0; aa < 3; aa++)
-          if (aa == arr) {
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-              acc[aa][k][0] += val.x * w[k];
-              acc[aa][k][1] += val.y * w[k];
-            }
-          }
-      }
-      if (rb != INT_MIN) {
-        flush(0);
-        flush(1);
-        flush(2);
-      }
     }
-    __syncthreads();
-    // window bins [tb0 + dmin + j0 - 1, + nb) into the accumulators
-    for (int q = threadIdx.x; q < 3 * nb; q += 256) {
-      const int a = q / nb, r = q % nb;
-      const double re = lds[a][r][0], im = lds[a][r][1];
-      if (re == 0.0 && im == 0.0) continue;
-      const long m = tb0 + A.dmin + j0 - 1 + r;
-      // taps outside the [360][arraySize] allocation (+ guard) are dropped; the
-      // reference writes them out of bounds on strongly non-square grids
-      const long gidx = stp + m;
-      if (gidx < -NT_GUARD_DEV || gidx >= 360L * arraySize + NT_GUARD_DEV) continue;
-      double2 *dst = U[a] + gidx;
-      // bins near a row end are shared with the neighbouring angle (the flat
-      // [360][arraySize] layout lets m-1 / m+1 spill over, Q10): atomics there
-      if (m < 3 || m >= arraySize - 3) {
-        atomicAdd(&dst->x, re);
-        atomicAdd(&dst->y, im);
-      } else {
-        double2 v = *dst;
-        v.x += re;
-        v.y += im;
-        *dst = v;
-      }
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    double v = acc[k >> 1][k & 1];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    if (lane == 0) red[k][wv] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const int aa = threadIdx.x;
+    double re = 0.0, im = 0.0;
+    for (int k = 0; k < 4; k++) {
+      re += red[2 * aa][k];
+      im += red[2 * aa + 1][k];
     }
-    __syncthreads();
+    out[((size_t)f * 3 + aa) * 360 + ang] = make_double2(cf * re, cf * im);
   }
 }
 
@@ -740,10 +797,17 @@ struct fdtd_grid {
   double2 *nt[3];
   size_t ntn;
   double2 *cE, *cH;
-  double *tsTab;
-  int *binEnt, *binStart;  // samples of every angle grouped by bin offset (k_ntff_bin_n)
-  BinAng *binAng;
-  int nbuf;                // steps the sample buffers cE / cH hold
+  BinPart *binPart;  // records of k_ntff_bin: parts, samples
+  BinRec *binRec;
+  int nparts;
+  int nbuf;  // steps the sample buffers cE / cH hold
+  // running DFT of the contour samples (fdtd_hip_grid_set_dft)
+  int nf;
+  double *dftOmega;      // [nf] angular frequencies (rad per step)
+  double2 *dftE, *dftH;  // [nf][total] FP64 accumulators
+  double *dftTs;         // [360][total] the binning's time shifts
+  double2 *dftOut;       // [nf][3][360]
+  int ctrSet;            // FP32: the fused kernel samples the contour
   double *cosA, *sinA, *cosB, *sinB;
   double coef;
   double2 *ffout;
@@ -888,74 +952,112 @@ static void build_time_shift(const fdtd_grid *g, std::vector<double> &tab, std::
   }
 }
 
-// Samples of every angle grouped by their bin offset d (see k_ntff_bin_n): for
-// angle a, entries [2 total a, 2 total (a+1)) hold cell | target array << 28 |
-// H << 30 in order of d, and dstart[ai[a].start + j] is the first entry of
-// offset ai[a].dmin + j.
+// The records of k_ntff_bin: the samples of each (angle, target array) sorted by
+// offset d, cut into parts of at most NTFF_GRP x NTFF_RUN samples and NTFF_WIN
+// offsets, each part split into NTFF_GRP contiguous runs stored [iteration][group].
 static int build_bin_groups(fdtd_grid *g, const std::vector<double> &tab)
 {
   const Contour &ct = g->ct;
   const int P = ct.total;
-  std::vector<int> ent((size_t)360 * 2 * P), dst;
-  std::vector<BinAng> ai(360);
-  std::vector<int> side(P), dv(2 * P);
+  std::vector<BinPart> parts;
+  std::vector<BinRec> rec;
+  std::vector<int> side(P);
   for (int q = 0; q < 4; q++)
     for (int c = 0; c < ct.s[q].n; c++) side[ct.s[q].off + c] = q;
+  struct Smp {
+    int d, p;
+    double a;
+  };
+  std::vector<Smp> by[3];
   for (int ang = 0; ang < 360; ang++) {
     const double *ts = &tab[(size_t)ang * P];
-    int dmin = INT_MAX, dmax = INT_MIN;
+    for (int arr = 0; arr < 3; arr++) by[arr].clear();
     for (int p = 0; p < P; p++) {
-      dv[2 * p] = (int)floor(ts[p] + 0.5);   // E
-      dv[2 * p + 1] = (int)floor(ts[p]) + 1;  // H
-      dmin = std::min(dmin, std::min(dv[2 * p], dv[2 * p + 1]));
-      dmax = std::max(dmax, std::max(dv[2 * p], dv[2 * p + 1]));
-    }
-    const int nd = dmax - dmin + 1;
-    ai[ang].dmin = dmin;
-    ai[ang].nd = nd;
-    ai[ang].start = (int)dst.size();
-    std::vector<int> cnt(nd + 1, 0);
-    for (int x = 0; x < 2 * P; x++) cnt[dv[x] - dmin + 1]++;
-    for (int j = 0; j < nd; j++) cnt[j + 1] += cnt[j];
-    dst.insert(dst.end(), cnt.begin(), cnt.end());
-    std::vector<int> pos(cnt.begin(), cnt.end() - 1);
-    int *e = &ent[(size_t)ang * 2 * P];
-    for (int x = 0; x < 2 * P; x++) {
-      const int p = x >> 1, isH = x & 1;
       const Seg &sg = ct.s[side[p]];
-      const int arr = isH ? sg.wa : sg.ua;
-      e[pos[dv[x] - dmin]++] = p | (arr << 28) | (isH << 30);
+      const int dE = (int)floor(ts[p] + 0.5), dH = (int)floor(ts[p]) + 1;
+      by[sg.ua].push_back({dE, p, (0.5 + ts[p]) - (double)dE});
+      by[sg.wa].push_back({dH, p | (1 << 30), ts[p] - (double)(dH - 1)});
+    }
+    for (int arr = 0; arr < 3; arr++) {
+      std::vector<Smp> &v = by[arr];
+      std::stable_sort(v.begin(), v.end(), [](const Smp &x, const Smp &y) { return x.d < y.d; });
+      size_t x0 = 0;
+      while (x0 < v.size()) {
+        const int dlo = v[x0].d;
+        size_t x1 = x0;
+        while (x1 < v.size() && x1 - x0 < NTFF_GRP * NTFF_RUN && v[x1].d < dlo + NTFF_WIN) x1++;
+        BinPart W;
+        W.rec0 = (int)rec.size();
+        W.dlo = dlo;
+        W.nd = v[x1 - 1].d - dlo + 1;
+        W.row = ang;
+        W.arr = arr;
+        const int cnt = (int)(x1 - x0);
+        W.per = (cnt + NTFF_GRP - 1) / NTFF_GRP;
+        rec.resize(rec.size() + (size_t)W.per * NTFF_GRP, BinRec{-1, 0, 0.0});
+        for (int t = 0; t < NTFF_GRP; t++)
+          for (int i = 0; i < W.per; i++) {
+            const size_t x = x0 + (size_t)t * W.per + i;
+            if (x >= x1) break;
+            rec[(size_t)W.rec0 + (size_t)i * NTFF_GRP + t] = BinRec{v[x].p, v[x].d - dlo, v[x].a};
+          }
+        parts.push_back(W);
+        x0 = x1;
+      }
     }
   }
-  HIPCHK(hipMalloc(&g->binEnt, ent.size() * sizeof(int)));
-  HIPCHK(hipMalloc(&g->binStart, dst.size() * sizeof(int)));
-  HIPCHK(hipMalloc(&g->binAng, ai.size() * sizeof(BinAng)));
-  HIPCHK(hipMemcpy(g->binEnt, ent.data(), ent.size() * sizeof(int), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(g->binStart, dst.data(), dst.size() * sizeof(int), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(g->binAng, ai.data(), ai.size() * sizeof(BinAng), hipMemcpyHostToDevice));
+  if (rec.size() > (size_t)INT_MAX) return fdtd_fail(FDTD_ENOMEM, "NTFF binning records exceed 2^31");
+  g->nparts = (int)parts.size();
+  HIPCHK(hipMalloc(&g->binPart, std::max<size_t>(1, parts.size()) * sizeof(BinPart)));
+  HIPCHK(hipMalloc(&g->binRec, std::max<size_t>(1, rec.size()) * sizeof(BinRec)));
+  if (!parts.empty()) HIPCHK(hipMemcpy(g->binPart, parts.data(), parts.size() * sizeof(BinPart), hipMemcpyHostToDevice));
+  if (!rec.empty()) HIPCHK(hipMemcpy(g->binRec, rec.data(), rec.size() * sizeof(BinRec), hipMemcpyHostToDevice));
   return FDTD_OK;
 }
 
+// n steps of samples in cE / cH ([cell][n]) into the accumulators
 static void launch_bin(fdtd_grid *g, int n, double time0)
 {
   const fdtd_grid_desc &d = g->d;
-  hipLaunchKernelGGL(k_ntff_bin_n, dim3(360), dim3(256), 0, 0, g->ct, g->cE, g->cH, n, g->tsTab, g->binEnt,
-                     g->binStart, g->binAng, g->nt[0], g->nt[1], g->nt[2], d.array_size, (long)llround(time0) - 1,
-                     g->coef, d.flavour == FDTD_FLAVOUR_PLAIN ? 0 : 1);
+  const long tb0 = (long)llround(time0) - 1;
+  const int useCoef = d.flavour == FDTD_FLAVOUR_PLAIN ? 0 : 1;
+  if (g->nparts <= 0) return;
+#define NTFF_BIN(NS)                                                                                              \
+  hipLaunchKernelGGL(k_ntff_bin<NS>, dim3(g->nparts), dim3(NTFF_GRP * NS), 0, 0, g->cE, g->cH, n, g->binPart, g->binRec, \
+                     g->nt[0], g->nt[1], g->nt[2], d.array_size, tb0, g->coef, useCoef)
+  if (n <= 2)
+    NTFF_BIN(2);
+  else if (n <= 4)
+    NTFF_BIN(4);
+  else if (n <= 8)
+    NTFF_BIN(8);
+  else
+    NTFF_BIN(16);
+#undef NTFF_BIN
 }
 
-// contour samples of one inner step of the FP32 kernel ([3][total] float2: signed
-// E, H on the side, H one cell back) -> the packing of k_ntff_gather
-__global__ __launch_bounds__(256) void k_ctr_combine(const float2 *smp, Contour ct, double2 *cE, double2 *cH)
+// n steps of samples in cE / cH ([cell][n]) into the running DFT
+static void launch_dft(fdtd_grid *g, int n, double time0)
 {
-  const int p = blockIdx.x * 256 + threadIdx.x;
+  const long tb0 = (long)llround(time0) - 1;
+  hipLaunchKernelGGL(k_ctr_dft, dim3((g->ct.total + 255) / 256, g->nf), dim3(256), 0, 0, g->cE, g->cH, n, g->ct.total,
+                     g->dftOmega, tb0, g->dftE, g->dftH);
+}
+
+// contour samples of n inner steps of the FP32 kernel ([n][3][total] float2:
+// signed E, H on the side, H one cell back) -> the packing of k_ntff_gather,
+// step-minor ([cell][n])
+__global__ __launch_bounds__(256) void k_ctr_combine(const float2 *smp, Contour ct, int n, double2 *cE, double2 *cH)
+{
+  const int p = blockIdx.x * 256 + threadIdx.x, s = blockIdx.y;
   if (p >= ct.total) return;
   int q = 0;
   while (q < 3 && p >= ct.s[q + 1].off) q++;
   const double hs = ct.s[q].hs;
-  const float2 e = smp[p], h0 = smp[ct.total + p], h1 = smp[2 * ct.total + p];
-  cE[p] = make_double2((double)e.x, (double)e.y);
-  cH[p] = make_double2(hs * ((double)h0.x + (double)h1.x), hs * ((double)h0.y + (double)h1.y));
+  const float2 *sm = smp + (size_t)s * 3 * ct.total;
+  const float2 e = sm[p], h0 = sm[ct.total + p], h1 = sm[2 * ct.total + p];
+  cE[(size_t)p * n + s] = make_double2((double)e.x, (double)e.y);
+  cH[(size_t)p * n + s] = make_double2(hs * ((double)h0.x + (double)h1.x), hs * ((double)h0.y + (double)h1.y));
 }
 
 // after a launch of n steps of an FP32 grid: bin every step's samples at its time
@@ -963,13 +1065,12 @@ static int grid_bin_samples(void *ctx, const float2 *samples, int n, const doubl
 {
   fdtd_grid *g = (fdtd_grid *)ctx;
   if (n > g->nbuf) return fdtd_fail(FDTD_EINVAL, "%d steps exceed the %d-step sample buffers", n, g->nbuf);
-  for (int s = 0; s < n; s++) {
-    if (s > 0 && times[s] != times[0] + s) return fdtd_fail(FDTD_ESTATE, "steps of a block are not consecutive");
-    hipLaunchKernelGGL(k_ctr_combine, dim3((g->ct.total + 255) / 256), dim3(256), 0, 0,
-                       samples + (size_t)s * 3 * g->ct.total, g->ct, g->cE + (size_t)s * g->ct.total,
-                       g->cH + (size_t)s * g->ct.total);
-  }
-  launch_bin(g, n, times[0]);
+  for (int s = 1; s < n; s++)
+    if (times[s] != times[0] + s) return fdtd_fail(FDTD_ESTATE, "steps of a block are not consecutive");
+  hipLaunchKernelGGL(k_ctr_combine, dim3((g->ct.total + 255) / 256, n), dim3(256), 0, 0, samples, g->ct, n, g->cE,
+                     g->cH);
+  if (g->ntn) launch_bin(g, n, times[0]);
+  if (g->nf) launch_dft(g, n, times[0]);
   HIPCHK(hipGetLastError());
   return FDTD_OK;
 }
@@ -1026,15 +1127,13 @@ extern "C" int fdtd_hip_grid_create(const fdtd_grid_desc *d, fdtd_grid **out)
   build_time_shift(g, tab, mn, mx, ca, sa, cb, sb);
   const int P = std::max(1, g->ct.total);
   // FP32 grids bin a whole block of steps at once: sample buffers for FAST_MAX_T steps
-  g->nbuf = (d->ntff_time && d->precision == FDTD_FP32) ? FAST_MAX_T : 1;
+  g->nbuf = d->precision == FDTD_FP32 ? FAST_MAX_T : 1;
   HIPCHK(hipMalloc(&g->cE, (size_t)g->nbuf * P * sizeof(double2)));
   HIPCHK(hipMalloc(&g->cH, (size_t)g->nbuf * P * sizeof(double2)));
   if (d->ntff_time && g->ct.total > 0) {
     int rc = build_bin_groups(g, tab);
     if (rc) return rc;
   }
-  HIPCHK(hipMalloc(&g->tsTab, std::max<size_t>(1, tab.size()) * sizeof(double)));
-  if (!tab.empty()) HIPCHK(hipMemcpy(g->tsTab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMalloc(&g->cosA, 360 * sizeof(double)));
   HIPCHK(hipMalloc(&g->sinA, 360 * sizeof(double)));
   HIPCHK(hipMemcpy(g->cosA, ca.data(), 360 * sizeof(double), hipMemcpyHostToDevice));
@@ -1054,8 +1153,10 @@ extern "C" int fdtd_hip_grid_create(const fdtd_grid_desc *d, fdtd_grid **out)
     if (rc) return rc;
     // the per-step time-domain NTFF: the fused kernel samples the contour after
     // every inner step, and each block's samples are binned step by step
-    if (d->ntff_time && g->ct.total > 0 && (rc = fast_state_set_contour(g->fs, g->ct, g->off, grid_bin_samples, g)))
-      return rc;
+    if (d->ntff_time && g->ct.total > 0) {
+      if ((rc = fast_state_set_contour(g->fs, g->ct, g->off, grid_bin_samples, g))) return rc;
+      g->ctrSet = 1;
+    }
   }
   return FDTD_OK;
 }
@@ -1094,10 +1195,13 @@ extern "C" int fdtd_hip_grid_destroy(fdtd_grid *g)
   for (int a = 0; a < 3; a++) (void)hipFree(g->ntb[a]);
   (void)hipFree(g->cE);
   (void)hipFree(g->cH);
-  (void)hipFree(g->binEnt);
-  (void)hipFree(g->binStart);
-  (void)hipFree(g->binAng);
-  (void)hipFree(g->tsTab);
+  (void)hipFree(g->binPart);
+  (void)hipFree(g->binRec);
+  (void)hipFree(g->dftOmega);
+  (void)hipFree(g->dftE);
+  (void)hipFree(g->dftH);
+  (void)hipFree(g->dftTs);
+  (void)hipFree(g->dftOut);
   (void)hipFree(g->cosA);
   (void)hipFree(g->sinA);
   (void)hipFree(g->cosB);
@@ -1229,10 +1333,11 @@ extern "C" int fdtd_hip_grid_step(fdtd_grid *g, const fdtd_step_params *p)
     }
   }
   HIPCHK(hipGetLastError());
-  if (d.ntff_time && g->ct.total > 0) {
+  if ((g->ntn || g->nf) && g->ct.total > 0) {
     hipLaunchKernelGGL(k_ntff_gather, dim3((g->ct.total + 255) / 256), dim3(256), 0, 0, (const double2 *const *)g->Fdev,
                        g->ct, g->cE, g->cH);
-    launch_bin(g, 1, p->time);
+    if (g->ntn) launch_bin(g, 1, p->time);
+    if (g->nf) launch_dft(g, 1, p->time);
     HIPCHK(hipGetLastError());
   }
   return FDTD_OK;
@@ -1272,6 +1377,10 @@ extern "C" int fdtd_hip_grid_reset(fdtd_grid *g)
   if (!g) return fdtd_fail(FDTD_EINVAL, "null argument");
   for (int f = 0; f < 9; f++) HIPCHK(hipMemset(g->F[f], 0, g->ncell * sizeof(double2)));
   for (int a = 0; a < 3 && g->ntn; a++) HIPCHK(hipMemset(g->ntb[a], 0, g->ntn * sizeof(double2)));
+  if (g->nf) {
+    HIPCHK(hipMemset(g->dftE, 0, (size_t)g->nf * g->ct.total * sizeof(double2)));
+    HIPCHK(hipMemset(g->dftH, 0, (size_t)g->nf * g->ct.total * sizeof(double2)));
+  }
   HIPCHK(hipDeviceSynchronize());
   if (g->fs) return fast_state_reset(g->fs);
   return FDTD_OK;
@@ -1321,6 +1430,58 @@ extern "C" int fdtd_hip_grid_time_spectra(fdtd_grid *g, double coef_re, double c
                      tmPlain ? g->sinB : g->sinA, g->tw, d.c0 * h_u_nm, g->spec);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpy(out, g->spec, 321 * 360 * sizeof(double), hipMemcpyDeviceToHost));
+  return FDTD_OK;
+}
+
+extern "C" int fdtd_hip_grid_set_dft(fdtd_grid *g, const double *omega, int nf)
+{
+  if (!g || (nf > 0 && !omega) || nf < 0) return fdtd_fail(FDTD_EINVAL, "bad argument");
+  if (nf > 0 && g->ct.total <= 0) return fdtd_fail(FDTD_ESTATE, "the grid has no NTFF contour");
+  int rc;
+  if (g->fs && (rc = fast_state_flush(g->fs))) return rc;
+  HIPCHK(hipDeviceSynchronize());
+  (void)hipFree(g->dftOmega);
+  (void)hipFree(g->dftE);
+  (void)hipFree(g->dftH);
+  (void)hipFree(g->dftTs);
+  (void)hipFree(g->dftOut);
+  g->dftOmega = nullptr;
+  g->dftE = g->dftH = g->dftOut = nullptr;
+  g->dftTs = nullptr;
+  g->nf = 0;
+  if (nf == 0) return FDTD_OK;
+  const int P = g->ct.total;
+  HIPCHK(hipMalloc(&g->dftOmega, nf * sizeof(double)));
+  HIPCHK(hipMemcpy(g->dftOmega, omega, nf * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc(&g->dftE, (size_t)nf * P * sizeof(double2)));
+  HIPCHK(hipMalloc(&g->dftH, (size_t)nf * P * sizeof(double2)));
+  HIPCHK(hipMemset(g->dftE, 0, (size_t)nf * P * sizeof(double2)));
+  HIPCHK(hipMemset(g->dftH, 0, (size_t)nf * P * sizeof(double2)));
+  HIPCHK(hipMalloc(&g->dftOut, (size_t)nf * 3 * 360 * sizeof(double2)));
+  std::vector<double> tab, mn, mx, ca, sa, cb, sb;
+  build_time_shift(g, tab, mn, mx, ca, sa, cb, sb);
+  HIPCHK(hipMalloc(&g->dftTs, tab.size() * sizeof(double)));
+  HIPCHK(hipMemcpy(g->dftTs, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+  g->nf = nf;
+  // the FP32 fused kernel samples the contour after every inner step
+  if (g->fs && !g->ctrSet) {
+    if ((rc = fast_state_set_contour(g->fs, g->ct, g->off, grid_bin_samples, g))) return rc;
+    g->ctrSet = 1;
+  }
+  return FDTD_OK;
+}
+
+extern "C" int fdtd_hip_grid_get_dft(fdtd_grid *g, double *out)
+{
+  if (!g || !out) return fdtd_fail(FDTD_EINVAL, "null argument");
+  if (!g->nf) return fdtd_fail(FDTD_ESTATE, "no running DFT on this grid (fdtd_hip_grid_set_dft)");
+  int rc;
+  if (g->fs && (rc = fast_state_flush(g->fs))) return rc;
+  const double cf = g->d.flavour == FDTD_FLAVOUR_PLAIN ? 1.0 : g->coef;
+  hipLaunchKernelGGL(k_dft_far, dim3(360, g->nf), dim3(256), 0, 0, g->dftE, g->dftH, g->ct, g->dftTs, g->dftOmega, cf,
+                     g->dftOut);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpy(out, g->dftOut, (size_t)g->nf * 3 * 360 * sizeof(double2), hipMemcpyDeviceToHost));
   return FDTD_OK;
 }
 

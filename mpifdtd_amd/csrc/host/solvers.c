@@ -36,6 +36,7 @@ typedef struct {
   double *eps_mirror;
   int blocked;    /* FP32 grid: steps are queued and run T per launch */
   int ntff_time;  /* the per-step time-domain NTFF runs (its outputs exist) */
+  int ndft;       /* frequencies of the running DFT (fdtd_solver_setDFT) */
 } solver_t;
 
 static solver_t slv_[SLV_N];
@@ -210,6 +211,7 @@ static void solver_init(int kind)
   s->arraySize = dims[3];
   s->blocked = precision_ == FDTD_FP32;
   s->ntff_time = ntff_time_;
+  s->ndft = 0;
   size_t n = (size_t)s->NPXa * s->NPYa;
   for (int q = 0; q < 3; q++) s->mirror[q] = (double complex *)calloc(n, sizeof(double complex));
   s->eps_mirror = (double *)calloc(n, sizeof(double));
@@ -302,6 +304,60 @@ int fdtd_solver_timeSpectra(double *out)
   twiddles(tw);
   FieldInfo fi = field_getFieldInfo();
   return fdtd_hip_grid_time_spectra(slv_[active_].g, creal(coef), cimag(coef), tw, fi.h_u_nm, out);
+}
+
+int fdtd_solver_setDFT(const double *lambda_nm, int n)
+{
+  if (active_ < 0) return FDTD_ESTATE;
+  if (n < 0 || (n > 0 && !lambda_nm)) return FDTD_EINVAL;
+  /* w_s = C_0_S 2 pi / lambda_s, lambda_s = lambda / h (field.c:89-143) */
+  const double C = field_getLightSpeed();
+  FieldInfo fi = field_getFieldInfo();
+  double *w = (double *)malloc((size_t)(n > 0 ? n : 1) * sizeof(double));
+  for (int f = 0; f < n; f++) w[f] = C * 2 * M_PI / (lambda_nm[f] / fi.h_u_nm);
+  int rc = fdtd_hip_grid_set_dft(slv_[active_].g, w, n);
+  free(w);
+  if (rc == FDTD_OK) slv_[active_].ndft = n;
+  return rc;
+}
+
+int fdtd_solver_getDFT(double *out)
+{
+  if (active_ < 0) return FDTD_ESTATE;
+  return fdtd_hip_grid_get_dft(slv_[active_].g, out);
+}
+
+int fdtd_solver_dftSpectra(double *out)
+{
+  if (active_ < 0) return FDTD_ESTATE;
+  const int n = slv_[active_].ndft;
+  if (n <= 0) return FDTD_ESTATE;
+  double complex *U = (double complex *)malloc((size_t)n * 3 * 360 * sizeof(double complex));
+  int rc = fdtd_hip_grid_get_dft(slv_[active_].g, (double *)U);
+  if (rc) {
+    free(U);
+    return rc;
+  }
+  /* the translation and normalisation of fdtd_solver_timeSpectra, on the DFT of
+     the accumulator rows at each frequency instead of the interpolated FFT */
+  const double C = field_getLightSpeed(), w_s = field_getOmega(), Z0 = field_getZ0();
+  const double complex coef = is_mpi(active_) ? csqrt(2 * M_PI * C / (I * w_s))
+                                              : 1.0 / (4 * M_PI * C) * csqrt(2 * M_PI * C / (I * w_s));
+  const int tm = is_tm(active_);
+  for (int f = 0; f < n; f++)
+    for (int ang = 0; ang < 360; ang++) {
+      const double phi = ang * M_PI / 180.0, sp = sin(phi), cp = cos(phi);
+      const double complex u0 = U[((size_t)f * 3 + 0) * 360 + ang], u1 = U[((size_t)f * 3 + 1) * 360 + ang],
+                           u2 = U[((size_t)f * 3 + 2) * 360 + ang];
+      double complex e;
+      if (tm) /* Eth = coef (-Z0 WTH - UPH), WTH = -Wz, UPH = -Ux sin + Uy cos */
+        e = coef * (-Z0 * (-u2) - (-u0 * sp + u1 * cp));
+      else /* Eph = coef (-Z0 WPH + UTH), WPH = -Wx sin + Wy cos, UTH = -Uz */
+        e = coef * (-Z0 * (-u0 * sp + u1 * cp) + (-u2));
+      out[(size_t)f * 360 + ang] = (creal(e) * creal(e) + cimag(e) * cimag(e)) / NTFF_NUM;
+    }
+  free(U);
+  return FDTD_OK;
 }
 
 static void write_spectra(void)

@@ -128,6 +128,26 @@ class Simulator:
         check(N.lib().fdtd_solver_timeSpectra(N.dptr(out)), "fdtd_solver_timeSpectra")
         return out
 
+    def set_dft(self, lambdas_nm):
+        """Running DFT of the NTFF contour at these wavelengths (nm), from the next step on."""
+        lam = np.ascontiguousarray(np.asarray(lambdas_nm, np.float64).reshape(-1))
+        check(N.lib().fdtd_solver_setDFT(N.dptr(lam), lam.size), "fdtd_solver_setDFT")
+        self._ndft = lam.size
+
+    def dft(self):
+        """[n, 3, 360] complex: the DFT of the three time-domain NTFF accumulator rows
+        (TM Ux Uy Wz, TE Wx Wy Uz) of every angle at each wavelength of set_dft."""
+        out = np.empty((self._ndft, 3, 360), np.complex128)
+        check(N.lib().fdtd_solver_getDFT(N.dptr(out)), "fdtd_solver_getDFT")
+        return out
+
+    def dft_spectra(self):
+        """[n, 360]: far-field spectra at the set_dft wavelengths, as time_spectra
+        translates and normalises them, at the exact frequencies."""
+        out = np.empty((self._ndft, 360), np.float64)
+        check(N.lib().fdtd_solver_dftSpectra(N.dptr(out)), "fdtd_solver_dftSpectra")
+        return out
+
     def reset(self, angle_deg=None):
         N.lib().simulator_reset()
         if angle_deg is not None:

@@ -1,0 +1,313 @@
+"""Reflectance surrogate: an ensemble of MLPs trained on batched-FDTD spectra
+(mpifdtd_surrogate.h).
+
+    spec = morpho_spec(seed=1, n=2000)                  # seeded geometries + wavelengths
+    X, Y = generate(spec)                               # batched FDTD: features, reflectance
+    ens = Ensemble([13, 512, 512, 512, 180], k=4)
+    ens.fit(X, Y, epochs=200)                           # BF16 matrix-core training
+    R, used_fdtd = predict_with_fallback(ens, spec_q)   # FDTD where the members disagree
+
+The reference sweeps geometries one FDTD run at a time (main.c:114-138,
+models_isFinish); here a network maps 12 morpho-scale parameters plus the
+wavelength to the far-field reflectance at the 180 back-hemisphere angles, and
+the FDTD stays both its teacher and its fallback.  The forward and backward
+passes, the loss and Adam run in the library's own kernels (BF16 products on
+the matrix cores, FP32 accumulation and master weights); torch only holds the
+device buffers.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _native as N
+from ._native import check
+from .batch import SplitMix64, item
+
+# the 12 geometry parameters (morphoScaleModel.c:10-99) and their sampling ranges
+GEOMETRY = (
+    ("mo_width_nm", 200, 1500, int),
+    ("mo_thick_nm0", 50, 100, int),
+    ("mo_thick_nm1", 50, 100, int),
+    ("mo_layer_num", 4, 10, int),
+    ("mo_branch_nm", 0, 50, int),
+    ("mo_left_gap_nm", 0, 50, int),    # lateral shelf offset (the left side's gap)
+    ("mo_edge_rate", 0.0, 1.0, float),
+    ("mo_curve", 0.0, 0.5, float),
+    ("mo_n", 1.3, 1.8, float),
+    ("mo_nx", 1.0, 1.3, float),
+    ("mo_uniaxial", 0, 1, int),
+    ("mo_asymmetry", 0, 1, int),
+)
+LAMBDA_RANGE = (380, 780)
+N_FEATURES = len(GEOMETRY) + 1
+N_ANGLES = 180
+# the grid every instance runs on: 4.4 um square at h = 10 nm with a 16-cell
+# UPML, normal incidence along +y (angle 90); far-field angles 180..359 face back
+GRID = dict(side_nm=4400, h_nm=10, pml=16, angle_deg=90, steps=2500)
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("the surrogate runs on the GPU only (no HIP device visible)")
+    return torch
+
+
+def _vp(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class MLP:
+    """One network of the library (mlp_create); inputs and targets are FP32
+    torch tensors on the GPU, [B, dims[0]] and [B, dims[-1]]."""
+
+    def __init__(self, dims, batch_max=1024, seed=0):
+        torch = _torch()
+        self.dims = list(dims)
+        self.batch_max = int(batch_max)
+        arr = (ctypes.c_int * len(dims))(*dims)
+        h = ctypes.c_void_p()
+        check(N.lib().mlp_create(len(dims) - 1, arr, self.batch_max, ctypes.c_ulonglong(seed), ctypes.byref(h)),
+              "mlp_create")
+        self._h = h
+        self._torch = torch
+
+    def forward(self, x):
+        torch = self._torch
+        x = x.contiguous().float()
+        out = torch.empty((x.shape[0], self.dims[-1]), dtype=torch.float32, device=x.device)
+        for i in range(0, x.shape[0], self.batch_max):
+            xb, yb = x[i:i + self.batch_max], out[i:i + self.batch_max]
+            check(N.lib().mlp_forward(self._h, _vp(xb), xb.shape[0], _vp(yb)), "mlp_forward")
+        return out
+
+    def grads(self, x, t):
+        loss = ctypes.c_float()
+        x, t = x.contiguous().float(), t.contiguous().float()
+        check(N.lib().mlp_grads(self._h, _vp(x), _vp(t), x.shape[0], ctypes.byref(loss)), "mlp_grads")
+        return loss.value
+
+    def train_step(self, x, t, lr):
+        loss = ctypes.c_float()
+        x, t = x.contiguous().float(), t.contiguous().float()
+        check(N.lib().mlp_train_step(self._h, _vp(x), _vp(t), x.shape[0], ctypes.c_float(lr), ctypes.byref(loss)),
+              "mlp_train_step")
+        return loss.value
+
+    def _layer(self, fn, l):
+        w = np.empty((self.dims[l + 1], self.dims[l]), np.float32)
+        b = np.empty(self.dims[l + 1], np.float32)
+        fp = ctypes.POINTER(ctypes.c_float)
+        check(fn(self._h, l, w.ctypes.data_as(fp), b.ctypes.data_as(fp)), "mlp layer copy")
+        return w, b
+
+    def params(self, l):
+        return self._layer(N.lib().mlp_get_params, l)
+
+    def layer_grads(self, l):
+        return self._layer(N.lib().mlp_get_grads, l)
+
+    def set_params(self, l, w, b):
+        w = np.ascontiguousarray(w, np.float32)
+        b = np.ascontiguousarray(b, np.float32)
+        fp = ctypes.POINTER(ctypes.c_float)
+        check(N.lib().mlp_set_params(self._h, l, w.ctypes.data_as(fp), b.ctypes.data_as(fp)), "mlp_set_params")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            N.lib().mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+class Ensemble:
+    """k networks of the same shape from different seeds and data orders; the
+    prediction is their mean and the disagreement their standard deviation."""
+
+    def __init__(self, dims, k=4, batch=1024, seed=0):
+        self.nets = [MLP(dims, batch, seed * 1000 + i) for i in range(k)]
+        self.batch = batch
+        self.seed = seed
+
+    def fit(self, X, Y, epochs=200, lr=1e-3, log=None):
+        """Adam at a fixed rate over `epochs` passes of shuffled batches."""
+        torch = _torch()
+        n = X.shape[0]
+        hist = []
+        for e in range(epochs):
+            tot = 0.0
+            for i, net in enumerate(self.nets):
+                g = torch.Generator(device=X.device)
+                g.manual_seed(self.seed * 100003 + e * 131 + i)
+                perm = torch.randperm(n, device=X.device, generator=g)
+                for s in range(0, n, self.batch):
+                    idx = perm[s:s + self.batch]
+                    tot += net.train_step(X[idx], Y[idx], lr) * idx.numel()
+            hist.append(tot / (n * len(self.nets)))
+            if log and (e % max(1, epochs // 10) == 0 or e == epochs - 1):
+                log(f"epoch {e}: mse {hist[-1]:.3e}")
+        return hist
+
+    def predict(self, X):
+        torch = _torch()
+        P = torch.stack([net.forward(X) for net in self.nets])
+        return P.mean(0), P.std(0, unbiased=False)
+
+
+def morpho_spec(seed=1, n=64):
+    """Seeded morpho-scale geometries and wavelengths (integer nm): plain dicts."""
+    g = SplitMix64(seed)
+    out = []
+    for _ in range(n):
+        d = {}
+        for name, lo, hi, kind in GEOMETRY:
+            d[name] = g.integer(lo, hi) if kind is int else g.uniform(lo, hi)
+        d["lambda_nm"] = g.integer(*LAMBDA_RANGE)
+        out.append(d)
+    return out
+
+
+def features(spec):
+    """[n, 13] in [0, 1]: the 12 parameters and the wavelength over their ranges."""
+    X = np.empty((len(spec), N_FEATURES), np.float32)
+    for r, d in enumerate(spec):
+        for c, (name, lo, hi, _) in enumerate(GEOMETRY):
+            X[r, c] = (d[name] - lo) / (hi - lo)
+        X[r, -1] = (d["lambda_nm"] - LAMBDA_RANGE[0]) / (LAMBDA_RANGE[1] - LAMBDA_RANGE[0])
+    return X
+
+
+def reflectance(ff):
+    """|E|^2 of the far field at the back-hemisphere angles 180..359 (incidence along +y)."""
+    ff = np.asarray(ff)
+    return (np.abs(ff[..., 180:360]) ** 2).astype(np.float64)
+
+
+def generate(spec, T=8, R=16, chunk=128, log=None, grid=None):
+    """Batched FDTD (MPI TM, morpho scale, CW at each wavelength) for every
+    entry of spec on `grid` (default GRID): returns the features [n, 13] and
+    reflectances [n, 180]."""
+    from .batch import Batch
+    G = grid or GRID
+    w = G["side_nm"] + 2 * G["pml"] * G["h_nm"]
+    Y = np.empty((len(spec), N_ANGLES))
+    for s0 in range(0, len(spec), chunk):
+        part = spec[s0:s0 + chunk]
+        items = []
+        for d in part:
+            params = {k: d[k] for k, *_ in GEOMETRY}
+            items.append(item(N.MPI_TM_UPML_2D, N.MORPHO_SCALE, w, w, G["h_nm"], G["pml"], d["lambda_nm"],
+                              G["angle_deg"], G["steps"], mo_randomness=0, mo_seed=0, **params))
+        with Batch(items, T=T, R=R) as b:
+            b.run()
+            for i in range(len(part)):
+                Y[s0 + i] = reflectance(b.far_field(i))
+        if log:
+            log(f"generated {min(s0 + chunk, len(spec))}/{len(spec)}")
+    return features(spec), Y
+
+
+def predict_with_fallback(ens, spec, scale, threshold=0.02, fdtd=None):
+    """Ensemble prediction for each query; where the members' spread (std over
+    the angles, relative to the mean's norm) exceeds `threshold`, the query is
+    answered by FDTD instead.  Returns the reflectances [n, 180] and the mask of
+    queries that went to FDTD."""
+    torch = _torch()
+    X = torch.tensor(features(spec), device="cuda")
+    mean, std = ens.predict(X)
+    mean = mean.double().cpu().numpy() * scale
+    std = std.double().cpu().numpy() * scale
+    rel = np.linalg.norm(std, axis=1) / np.maximum(np.linalg.norm(mean, axis=1), 1e-300)
+    use = rel > threshold
+    out = mean.copy()
+    if use.any():
+        idx = np.nonzero(use)[0]
+        _, Yf = (fdtd or generate)([spec[i] for i in idx])
+        out[idx] = Yf
+    return out, use
+
+
+def rel_err(pred, ref):
+    """Per-query relative L2 error over the angles."""
+    return np.linalg.norm(pred - ref, axis=1) / np.maximum(np.linalg.norm(ref, axis=1), 1e-300)
+
+
+def nearest_neighbour(Xtr, Ytr, Xq):
+    """The method without a network: the training spectrum of the nearest
+    geometry (Euclidean over the scaled features)."""
+    d2 = (Xq ** 2).sum(1)[:, None] - 2 * Xq @ Xtr.T + (Xtr ** 2).sum(1)[None, :]
+    return Ytr[np.argmin(d2, axis=1)]
+
+
+def experiment(n_train=20000, n_test=2000, epochs=200, k=4, seed=1, threshold=0.02, log=print):
+    """Generate, train, evaluate: the numbers of the surrogate's acceptance
+    (held-out error, fallback rate, error after fallback, throughput against
+    the batched FDTD), as a dict."""
+    import time
+    torch = _torch()
+    spec_tr = morpho_spec(seed=seed, n=n_train)
+    spec_te = morpho_spec(seed=seed + 1, n=n_test)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    Xtr, Ytr = generate(spec_tr, log=log)
+    Xte, Yte = generate(spec_te, log=log)
+    torch.cuda.synchronize()
+    t_gen = time.perf_counter() - t0
+    fdtd_rate = (n_train + n_test) / t_gen
+    scale = float(Ytr.mean())
+    X = torch.tensor(Xtr, device="cuda")
+    Y = torch.tensor(Ytr / scale, dtype=torch.float32, device="cuda")
+    ens = Ensemble([N_FEATURES, 512, 512, 512, N_ANGLES], k=k, batch=1024, seed=seed)
+    t0 = time.perf_counter()
+    hist = ens.fit(X, Y, epochs=epochs, lr=1e-3, log=log)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    Xq = torch.tensor(Xte, device="cuda")
+    mean, std = ens.predict(Xq)
+    pred = mean.double().cpu().numpy() * scale
+    spread = np.linalg.norm(std.double().cpu().numpy(), axis=1) / np.maximum(
+        np.linalg.norm(mean.double().cpu().numpy(), axis=1), 1e-300)
+    err = rel_err(pred, Yte)
+    use = spread > threshold
+    err_fb = np.where(use, 0.0, err)  # the flagged queries are answered by the FDTD itself
+    nn = rel_err(nearest_neighbour(Xtr, Ytr, Xte), Yte)
+    # inference throughput: 64 Ki queries through the whole ensemble
+    Xb = torch.rand((65536, N_FEATURES), device="cuda")
+    ens.predict(Xb)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reps = 5
+    for _ in range(reps):
+        ens.predict(Xb)
+    torch.cuda.synchronize()
+    sur_rate = reps * 65536 / (time.perf_counter() - t0)
+    return {
+        "n_train": n_train, "n_test": n_test, "epochs": epochs, "ensemble": k,
+        "grid": GRID, "fdtd_spectra_per_s": fdtd_rate, "fdtd_seconds": t_gen,
+        "train_seconds": t_train, "final_train_mse": hist[-1],
+        "heldout_rel_err_mean": float(err.mean()), "heldout_rel_err_median": float(np.median(err)),
+        "heldout_rel_err_p95": float(np.quantile(err, 0.95)),
+        "heldout_frac_within_1pct": float((err <= 0.01).mean()),
+        "fallback_threshold": threshold, "fallback_frac": float(use.mean()),
+        "rel_err_after_fallback_mean": float(err_fb.mean()),
+        "frac_within_1pct_after_fallback": float((err_fb <= 0.01).mean()),
+        "nearest_neighbour_rel_err_mean": float(nn.mean()),
+        "nearest_neighbour_rel_err_median": float(np.median(nn)),
+        "surrogate_spectra_per_s": sur_rate, "speedup_vs_fdtd": sur_rate / fdtd_rate,
+    }
+
+
+if __name__ == "__main__":
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(description="reflectance surrogate: generate, train, evaluate")
+    ap.add_argument("--train", type=int, default=20000)
+    ap.add_argument("--test", type=int, default=2000)
+    ap.add_argument("--epochs", type=int, default=200)
+    ap.add_argument("--k", type=int, default=4)
+    ap.add_argument("--seed", type=int, default=1)
+    a = ap.parse_args()
+    res = experiment(a.train, a.test, a.epochs, a.k, a.seed, log=lambda s: print(s, flush=True))
+    print(json.dumps(res), flush=True)

@@ -1,0 +1,118 @@
+"""The surrogate's BF16 matrix-core kernels against the CPU FP64 restatement
+(oracle/mlp.py): the product in every operand layout, the forward pass, the
+gradients of every layer, the Adam update, and a regression that the
+training must fit.
+
+Tolerances: a product of BF16 operands accumulated in FP32 is exact up to the
+FP32 accumulation (1e-5 relative here, K <= 512); the network's activations
+and gradients pass through BF16 between layers (8 significant bits), so the
+forward pass and gradients are held to 3e-2 relative L2 against FP64.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _bf16_bits(a):
+    """round-to-nearest-even FP32 -> BF16 bit patterns, and their values"""
+    u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    b = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+    return b, (b.astype(np.uint32) << 16).view(np.float32)
+
+
+def _mods():
+    import torch
+    import mpifdtd_amd._native as N
+    from mpifdtd_amd import surrogate as S
+    from oracle import mlp as om
+    return torch, N, S, om
+
+
+@pytest.mark.parametrize("ta", [0, 1])
+@pytest.mark.parametrize("tb", [0, 1])
+def test_gemm_layouts(ta, tb):
+    torch, N, S, om = _mods()
+    import ctypes
+    rng = np.random.default_rng(10 + 2 * ta + tb)
+    M, Nn, K = 128, 192, 96
+    a_bits, a_val = _bf16_bits(rng.standard_normal((M, K)))
+    b_bits, b_val = _bf16_bits(rng.standard_normal((K, Nn)))
+    a_store = a_bits.T.copy() if ta else a_bits      # a[K][M] or a[M][K]
+    b_store = b_bits.copy() if tb else b_bits.T.copy()  # b[K][N] or b[N][K]
+    da = torch.tensor(a_store.view(np.int16), device="cuda")
+    db = torch.tensor(b_store.view(np.int16), device="cuda")
+    dc = torch.empty((M, Nn), dtype=torch.float32, device="cuda")
+    vp = ctypes.c_void_p
+    assert N.lib().mlp_gemm_bf16(vp(da.data_ptr()), vp(db.data_ptr()), vp(dc.data_ptr()), M, Nn, K, ta, tb) == 0
+    ref = a_val.astype(np.float64) @ b_val.astype(np.float64)
+    assert rel_l2(dc.cpu().numpy(), ref) <= 1e-5
+
+
+def _net_and_data(S, torch, dims, B, seed):
+    rng = np.random.default_rng(seed)
+    net = S.MLP(dims, batch_max=256, seed=seed)
+    params = [(rng.standard_normal((dims[i + 1], dims[i])).astype(np.float32) / np.sqrt(dims[i]),
+               (0.1 * rng.standard_normal(dims[i + 1])).astype(np.float32)) for i in range(len(dims) - 1)]
+    for l, (w, b) in enumerate(params):
+        net.set_params(l, w, b)
+    x = rng.random((B, dims[0])).astype(np.float32)
+    t = rng.standard_normal((B, dims[-1])).astype(np.float32)
+    return net, params, x, t
+
+
+def test_forward_and_gradients_against_fp64_oracle():
+    torch, N, S, om = _mods()
+    dims, B = [13, 96, 64, 20], 100  # widths and batch off the 64 padding on purpose
+    net, params, x, t = _net_and_data(S, torch, dims, B, 5)
+    for l, (w, b) in enumerate(params):
+        w2, b2 = net.params(l)
+        assert np.array_equal(w2, w) and np.array_equal(b2, b)
+    y = net.forward(torch.tensor(x, device="cuda")).cpu().numpy()
+    y_ref, _ = om.forward(params, x)
+    assert rel_l2(y, y_ref) <= 3e-2, rel_l2(y, y_ref)
+    loss = net.grads(torch.tensor(x, device="cuda"), torch.tensor(t, device="cuda"))
+    g_ref, loss_ref = om.grads(params, x, t)
+    assert abs(loss - loss_ref) <= 3e-2 * loss_ref
+    for l in range(len(params)):
+        dw, db = net.layer_grads(l)
+        assert rel_l2(dw, g_ref[l][0]) <= 3e-2, (l, rel_l2(dw, g_ref[l][0]))
+        assert rel_l2(db, g_ref[l][1]) <= 3e-2, (l, rel_l2(db, g_ref[l][1]))
+    net.close()
+
+
+def test_adam_step_against_oracle():
+    """One train step: the update equals Adam applied in FP64 to the step's own gradients."""
+    torch, N, S, om = _mods()
+    dims, B = [13, 64, 20], 64
+    net, params, x, t = _net_and_data(S, torch, dims, B, 6)
+    lr = 1e-3
+    net.train_step(torch.tensor(x, device="cuda"), torch.tensor(t, device="cuda"), lr)
+    gr = [tuple(np.float64(v) for v in net.layer_grads(l)) for l in range(len(dims) - 1)]
+    st = [[np.zeros_like(w, np.float64), np.zeros_like(w, np.float64), np.zeros_like(b, np.float64),
+           np.zeros_like(b, np.float64)] for w, b in params]
+    ref = om.adam_step(params, gr, st, 1, lr)
+    for l in range(len(params)):
+        w, b = net.params(l)
+        # FP32 rounding of the weights (|w| < 4): 5e-7, against a step of lr = 1e-3
+        assert np.max(np.abs(w - ref[l][0])) <= 5e-7
+        assert np.max(np.abs(b - ref[l][1])) <= 5e-7
+    net.close()
+
+
+def test_training_fits_a_smooth_map():
+    """300 Adam steps on a smooth 13 -> 20 map cut the loss by 20x or more."""
+    torch, N, S, om = _mods()
+    rng = np.random.default_rng(7)
+    A = rng.standard_normal((13, 20))
+    x = rng.random((512, 13)).astype(np.float32)
+    t = np.sin(x @ A).astype(np.float32)
+    X, T = torch.tensor(x, device="cuda"), torch.tensor(t, device="cuda")
+    net = S.MLP([13, 128, 128, 20], batch_max=512, seed=3)
+    first = net.train_step(X, T, 1e-3)
+    for _ in range(300):
+        last = net.train_step(X, T, 1e-3)
+    assert last <= first / 20, (first, last)
+    net.close()
