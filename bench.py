@@ -71,6 +71,8 @@ def parse():
 
 
 def copy_calibration(mib):
+    # a plain device copy of known size, for scaling the PMC byte counters (its
+    # own process: torch's runtime only, the library is not loaded)
     import torch
     n = mib * (1 << 20) // 4
     a = torch.ones(n, dtype=torch.float32, device="cuda")
@@ -140,7 +142,6 @@ def log(msg):
 def run_sweep(args, m):
     """The batched sweep: every instance to completion, timed as one job."""
     import numpy as np
-    import torch
     from mpifdtd_amd import batch as B
     T, R = args.T or 8, args.R or 16
     spec = B.sweep_spec(args.seed, 64)
@@ -150,12 +151,12 @@ def run_sweep(args, m):
     log(f"sweep64: {len(spec)} instances, {sum(cells) / 1e6:.1f} M cells, {upd / 1e9:.1f} G cell-updates, T={T} R={R}")
     t0 = time.perf_counter()
     b = B.Batch(items, T=T, R=R)
-    torch.cuda.synchronize()
+    m.lib().fdtd_hip_synchronize()
     t_init = time.perf_counter() - t0
     log(f"created in {t_init:.1f} s")
     t0 = time.perf_counter()
     b.run()
-    torch.cuda.synchronize()
+    m.lib().fdtd_hip_synchronize()
     dt = time.perf_counter() - t0
     done = b.sync()
     assert done == [s["steps"] for s in spec], "an instance did not finish"
@@ -211,13 +212,13 @@ def main():
         sys.exit(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU sharding is "
                  "described in DESIGN.md and not built")
     import numpy as np
-    import torch
 
     import mpifdtd_amd as m
     from mpifdtd_amd import autotune
     L = m.lib()
-    assert torch.cuda.is_available(), "no GPU"
-    torch.cuda.set_device(0)
+    # the library's own HIP runtime does the synchronising (fdtd_hip_synchronize =
+    # hipDeviceSynchronize): a GPU runtime loaded beside it would not see its queues
+    assert L.fdtd_hip_device_count() >= 1, "no GPU"
     if args.workload == "sweep64":
         line = run_sweep(args, m)
         libc.fflush(None)
@@ -244,14 +245,14 @@ def main():
     sim.calc(W)
     if args.precision == "fp32":
         L.fdtd_solver_setBlocking(T, R)  # launches the queued warm-up steps
-    torch.cuda.synchronize()
+    L.fdtd_hip_synchronize()
     L.fdtd_hip_profile_enable(1)
-    torch.cuda.synchronize()
+    L.fdtd_hip_synchronize()
     t0 = time.perf_counter()
     sim.calc(K)
     if args.precision == "fp32":
         L.fdtd_solver_setBlocking(T, R)  # launches the last partial block inside the timed region
-    torch.cuda.synchronize()
+    L.fdtd_hip_synchronize()
     dt = time.perf_counter() - t0
     prof = prof_read(L)
     L.fdtd_hip_profile_enable(0)

@@ -14,6 +14,8 @@
 #ifndef FDTD_HIP_H
 #define FDTD_HIP_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -103,6 +105,13 @@ typedef struct fdtd_grid fdtd_grid;
 
 const char *fdtd_hip_last_error(void);
 int fdtd_hip_device_count(void);
+/* device memory and synchronisation through the library's own HIP runtime, for
+ * callers that hold no other GPU runtime (no counterpart in the reference) */
+int fdtd_hip_alloc(size_t bytes, void **out);
+int fdtd_hip_free(void *p);
+int fdtd_hip_upload(void *dst, const void *src, size_t bytes);
+int fdtd_hip_download(void *dst, const void *src, size_t bytes);
+int fdtd_hip_synchronize(void);
 
 int fdtd_hip_grid_create(const fdtd_grid_desc *d, fdtd_grid **out);
 int fdtd_hip_grid_destroy(fdtd_grid *g);

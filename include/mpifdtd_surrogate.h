@@ -12,9 +12,9 @@
  * Its products run in BF16 on the matrix cores with FP32 accumulation; master
  * weights, Adam moments, biases and the loss are FP32.  Widths and the batch
  * are padded to multiples of 64 inside the library; the caller passes real
- * sizes.  Data arguments marked "device" are HIP device
- * pointers; the others are host memory.  All calls are on the null stream
- * and synchronous unless stated.
+ * sizes.  Data arguments marked "device" are HIP device pointers (from
+ * fdtd_hip_alloc); the others are host memory.  All calls are on the null
+ * stream and return when their work is done.
  */
 #ifndef MPIFDTD_SURROGATE_H
 #define MPIFDTD_SURROGATE_H
@@ -38,6 +38,13 @@ int mlp_forward(mlp_net *net, const float *x, int B, float *y);
 int mlp_grads(mlp_net *net, const float *x, const float *t, int B, float *loss);
 /* mlp_grads followed by one Adam step (beta1 0.9, beta2 0.999, eps 1e-8) */
 int mlp_train_step(mlp_net *net, const float *x, const float *t, int B, float lr, float *loss);
+/* one pass over rows x rows of x[rows][dims[0]], t[rows][dims[L]] (device,
+ * FP32) in shuffled batches of `batch` (a Fisher-Yates permutation from a
+ * SplitMix64 stream of `seed`), one Adam step per batch, the steps queued
+ * without a host round trip; *loss (host, if not NULL) = the epoch's mean batch
+ * loss weighted by batch size */
+int mlp_train_epoch(mlp_net *net, const float *x, const float *t, int rows, int batch, unsigned long long seed,
+                    float lr, float *loss);
 /* layer l's weights w[dims[l+1]][dims[l]] and bias b[dims[l+1]] (host, FP32) */
 int mlp_get_params(mlp_net *net, int l, float *w, float *b);
 int mlp_set_params(mlp_net *net, int l, const float *w, const float *b);

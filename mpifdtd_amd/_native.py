@@ -168,6 +168,12 @@ def lib():
     sig("mlp_set_params", I, P, I, fp, fp)
     sig("mlp_get_grads", I, P, I, fp, fp)
     sig("mlp_gemm_bf16", I, vp, vp, vp, I, I, I, I, I)
+    sig("mlp_train_epoch", I, P, vp, vp, I, I, ctypes.c_ulonglong, ctypes.c_float, fp)
+    sig("fdtd_hip_alloc", I, ctypes.c_size_t, ctypes.POINTER(vp))
+    sig("fdtd_hip_free", I, vp)
+    sig("fdtd_hip_upload", I, vp, vp, ctypes.c_size_t)
+    sig("fdtd_hip_download", I, vp, vp, ctypes.c_size_t)
+    sig("fdtd_hip_synchronize", I)
     sig("fdtd_hip_profile_enable", V, I)
     sig("fdtd_hip_profile_read", V, dp)
     sig("fdtd_hip_last_error", ctypes.c_char_p)

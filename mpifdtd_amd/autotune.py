@@ -39,7 +39,6 @@ def _save():
 
 def tune(m, make_sim, probe_steps=240, candidates=CANDIDATES):
     """make_sim() -> a fresh FP32 Simulator; returns {"T", "R", "us_per_step": {...}}."""
-    import torch
     _load()
     sim = make_sim()
     key = _key(sim)
@@ -54,11 +53,11 @@ def tune(m, make_sim, probe_steps=240, candidates=CANDIDATES):
             sim.set_blocking(T, R)
             sim.calc(2 * T)  # warm the variant
             sim.set_blocking(T, R)
-            torch.cuda.synchronize()
+            L.fdtd_hip_synchronize()
             t0 = time.perf_counter()
             sim.calc(probe_steps)
             sim.set_blocking(T, R)  # flush the partial block
-            torch.cuda.synchronize()
+            L.fdtd_hip_synchronize()
             timings[f"T{T}R{R}"] = (time.perf_counter() - t0) * 1e6 / probe_steps
     finally:
         sim.finish()

@@ -52,6 +52,41 @@ extern "C" int fdtd_hip_device_count(void)
   return n;
 }
 
+extern "C" int fdtd_hip_alloc(size_t bytes, void **out)
+{
+  if (!out) return fdtd_fail(FDTD_EINVAL, "null argument");
+  *out = nullptr;
+  hipError_t e = hipMalloc(out, bytes ? bytes : 1);
+  if (e != hipSuccess) return fdtd_fail(FDTD_ENOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  return FDTD_OK;
+}
+
+extern "C" int fdtd_hip_free(void *p)
+{
+  hipError_t e = hipFree(p);
+  return e == hipSuccess ? FDTD_OK : fdtd_fail(FDTD_EHIP, "hipFree failed: %s", hipGetErrorString(e));
+}
+
+extern "C" int fdtd_hip_upload(void *dst, const void *src, size_t bytes)
+{
+  if ((!dst || !src) && bytes) return fdtd_fail(FDTD_EINVAL, "null argument");
+  hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+  return e == hipSuccess ? FDTD_OK : fdtd_fail(FDTD_EHIP, "upload failed: %s", hipGetErrorString(e));
+}
+
+extern "C" int fdtd_hip_download(void *dst, const void *src, size_t bytes)
+{
+  if ((!dst || !src) && bytes) return fdtd_fail(FDTD_EINVAL, "null argument");
+  hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+  return e == hipSuccess ? FDTD_OK : fdtd_fail(FDTD_EHIP, "download failed: %s", hipGetErrorString(e));
+}
+
+extern "C" int fdtd_hip_synchronize(void)
+{
+  hipError_t e = hipDeviceSynchronize();
+  return e == hipSuccess ? FDTD_OK : fdtd_fail(FDTD_EHIP, "hipDeviceSynchronize failed: %s", hipGetErrorString(e));
+}
+
 // ---------------------------------------------------------------- kernels
 
 struct GridPtrs {

@@ -156,6 +156,14 @@ __global__ __launch_bounds__(256) void k_unpack(const float *Y, int dp, int B, i
   y[i] = Y[(i / d) * dp + (i % d)];
 }
 
+// rows perm[0..B) of x[.][d] -> xs[B][d]
+__global__ __launch_bounds__(256) void k_gather_rows(const float *x, int d, const int *perm, int B, float *xs)
+{
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (size_t)B * d) return;
+  xs[i] = x[(size_t)perm[i / d] * d + (i % d)];
+}
+
 // dL/dY of the mean squared error over the B x d real outputs, FP32 and BF16
 // (zero on the pads); the loss itself into *loss (zeroed by the caller)
 __global__ __launch_bounds__(256) void k_mse_grad(const float *Y, int dp, const float *t, int B, int d, int Bp,
@@ -232,7 +240,11 @@ struct mlp_net {
   bf16_t *gh;    // its BF16 copy
   float *gtmp;   // FP32 gradient of the layer below
   bf16_t *ghtmp;
-  float *loss;   // device scalar
+  float *loss;   // device losses, one per batch of the current epoch
+  int nloss;
+  float *xs, *ts;  // staging of a shuffled batch [Bmax][dims[0]], [Bmax][dims[L]]
+  int *perm;       // device permutation of an epoch
+  int nperm;
   long step;
 };
 
@@ -274,6 +286,9 @@ static void free_net(mlp_net *n)
   (void)hipFree(n->gtmp);
   (void)hipFree(n->ghtmp);
   (void)hipFree(n->loss);
+  (void)hipFree(n->xs);
+  (void)hipFree(n->ts);
+  (void)hipFree(n->perm);
   delete n;
 }
 
@@ -316,8 +331,10 @@ extern "C" int mlp_create(int nlayers, const int *dims, int batch_max, unsigned 
   }
   if (rc == FDTD_OK) {
     const size_t ng = (size_t)n->Bmax * maxW;
+    n->nloss = 1;
     if (!((rc = dalloc(&n->g, ng)) || (rc = dalloc(&n->gh, ng)) || (rc = dalloc(&n->gtmp, ng)) ||
-          (rc = dalloc(&n->ghtmp, ng)) || (rc = dalloc(&n->loss, 1))) &&
+          (rc = dalloc(&n->ghtmp, ng)) || (rc = dalloc(&n->loss, 1)) ||
+          (rc = dalloc(&n->xs, (size_t)n->Bmax * dims[0])) || (rc = dalloc(&n->ts, (size_t)n->Bmax * dims[nlayers]))) &&
         hipDeviceSynchronize() != hipSuccess)
       rc = fdtd_fail(FDTD_EHIP, "mlp_create: init failed");
   }
@@ -375,17 +392,17 @@ extern "C" int mlp_forward(mlp_net *n, const float *x, int B, float *y)
   return FDTD_OK;
 }
 
-extern "C" int mlp_grads(mlp_net *n, const float *x, const float *t, int B, float *loss)
+// gradients of the batch into the layers' dW / db, the batch's loss into
+// *loss_dev (accumulated: the caller zeroes it); no synchronisation
+static int grads_async(mlp_net *n, const float *x, const float *t, int B, float *loss_dev)
 {
-  if (!n || !x || !t || B < 1 || B > n->Bmax) return fdtd_fail(FDTD_EINVAL, "mlp_grads: bad argument");
   const int Bp = pad_to(B, 64);
   int rc = forward(n, x, B, Bp);
   if (rc) return rc;
   MlpLayer &lo = n->l[n->L - 1];
-  HIPCHK(hipMemsetAsync(n->loss, 0, sizeof(float), 0));
   const size_t ng = (size_t)Bp * lo.Np;
   hipLaunchKernelGGL(k_mse_grad, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, 0, lo.z, lo.Np, t, B, lo.N, Bp, n->g,
-                     n->gh, n->loss);
+                     n->gh, loss_dev);
   float *G = n->g, *Gt = n->gtmp;
   bf16_t *Gh = n->gh, *Ght = n->ghtmp;
   for (int i = n->L - 1; i >= 0; i--) {
@@ -411,15 +428,11 @@ extern "C" int mlp_grads(mlp_net *n, const float *x, const float *t, int B, floa
     }
   }
   HIPCHK(hipGetLastError());
-  if (loss) HIPCHK(hipMemcpy(loss, n->loss, sizeof(float), hipMemcpyDeviceToHost));
-  HIPCHK(hipDeviceSynchronize());
   return FDTD_OK;
 }
 
-extern "C" int mlp_train_step(mlp_net *n, const float *x, const float *t, int B, float lr, float *loss)
+static int adam_async(mlp_net *n, float lr)
 {
-  int rc = mlp_grads(n, x, t, B, loss);
-  if (rc) return rc;
   n->step++;
   const float c1 = (float)(1.0 - std::pow(0.9, (double)n->step)), c2 = (float)(1.0 - std::pow(0.999, (double)n->step));
   for (int i = 0; i < n->L; i++) {
@@ -431,7 +444,76 @@ extern "C" int mlp_train_step(mlp_net *n, const float *x, const float *t, int B,
                        (size_t)y.Np, lr, c1, c2);
   }
   HIPCHK(hipGetLastError());
+  return FDTD_OK;
+}
+
+extern "C" int mlp_grads(mlp_net *n, const float *x, const float *t, int B, float *loss)
+{
+  if (!n || !x || !t || B < 1 || B > n->Bmax) return fdtd_fail(FDTD_EINVAL, "mlp_grads: bad argument");
+  HIPCHK(hipMemset(n->loss, 0, sizeof(float)));
+  int rc = grads_async(n, x, t, B, n->loss);
+  if (rc) return rc;
+  if (loss) HIPCHK(hipMemcpy(loss, n->loss, sizeof(float), hipMemcpyDeviceToHost));
   HIPCHK(hipDeviceSynchronize());
+  return FDTD_OK;
+}
+
+extern "C" int mlp_train_step(mlp_net *n, const float *x, const float *t, int B, float lr, float *loss)
+{
+  if (!n || !x || !t || B < 1 || B > n->Bmax) return fdtd_fail(FDTD_EINVAL, "mlp_train_step: bad argument");
+  HIPCHK(hipMemset(n->loss, 0, sizeof(float)));
+  int rc = grads_async(n, x, t, B, n->loss);
+  if (rc || (rc = adam_async(n, lr))) return rc;
+  if (loss) HIPCHK(hipMemcpy(loss, n->loss, sizeof(float), hipMemcpyDeviceToHost));
+  HIPCHK(hipDeviceSynchronize());
+  return FDTD_OK;
+}
+
+extern "C" int mlp_train_epoch(mlp_net *n, const float *x, const float *t, int rows, int batch, unsigned long long seed,
+                               float lr, float *loss)
+{
+  if (!n || !x || !t || rows < 1 || batch < 1 || batch > n->Bmax) return fdtd_fail(FDTD_EINVAL, "mlp_train_epoch: bad argument");
+  // Fisher-Yates shuffle from a SplitMix64 stream of `seed`
+  std::vector<int> perm(rows);
+  for (int i = 0; i < rows; i++) perm[i] = i;
+  uint64_t s = seed;
+  for (int i = rows - 1; i > 0; i--) std::swap(perm[i], perm[(int)(sm64(s) % (uint64_t)(i + 1))]);
+  if (n->nperm < rows) {
+    (void)hipFree(n->perm);
+    n->perm = nullptr;
+    n->nperm = 0;
+    HIPCHK(hipMalloc(&n->perm, (size_t)rows * sizeof(int)));
+    n->nperm = rows;
+  }
+  const int nb = (rows + batch - 1) / batch;
+  if (n->nloss < nb) {
+    (void)hipFree(n->loss);
+    n->loss = nullptr;
+    n->nloss = 0;
+    HIPCHK(hipMalloc(&n->loss, (size_t)nb * sizeof(float)));
+    n->nloss = nb;
+  }
+  HIPCHK(hipMemcpy(n->perm, perm.data(), (size_t)rows * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(n->loss, 0, (size_t)nb * sizeof(float)));
+  const int d0 = n->l[0].K, dL = n->l[n->L - 1].N;
+  for (int b = 0; b < nb; b++) {
+    const int B = std::min(batch, rows - b * batch);
+    const size_t nx = (size_t)B * d0, nt = (size_t)B * dL;
+    hipLaunchKernelGGL(k_gather_rows, dim3((unsigned)((nx + 255) / 256)), dim3(256), 0, 0, x, d0, n->perm + b * batch, B,
+                       n->xs);
+    hipLaunchKernelGGL(k_gather_rows, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, 0, t, dL, n->perm + b * batch, B,
+                       n->ts);
+    int rc = grads_async(n, n->xs, n->ts, B, n->loss + b);
+    if (rc || (rc = adam_async(n, lr))) return rc;
+  }
+  std::vector<float> lb(nb);
+  HIPCHK(hipMemcpy(lb.data(), n->loss, (size_t)nb * sizeof(float), hipMemcpyDeviceToHost));
+  HIPCHK(hipDeviceSynchronize());
+  if (loss) {
+    double acc = 0.0;
+    for (int b = 0; b < nb; b++) acc += (double)lb[b] * std::min(batch, rows - b * batch);
+    *loss = (float)(acc / rows);
+  }
   return FDTD_OK;
 }
 

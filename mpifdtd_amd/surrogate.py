@@ -12,8 +12,8 @@ models_isFinish); here a network maps 12 morpho-scale parameters plus the
 wavelength to the far-field reflectance at the 180 back-hemisphere angles, and
 the FDTD stays both its teacher and its fallback.  The forward and backward
 passes, the loss and Adam run in the library's own kernels (BF16 products on
-the matrix cores, FP32 accumulation and master weights); torch only holds the
-device buffers.
+the matrix cores, FP32 accumulation and master weights), on device buffers of
+the library's own HIP runtime (DeviceArray).
 """
 import ctypes
 
@@ -46,52 +46,101 @@ N_ANGLES = 180
 GRID = dict(side_nm=4400, h_nm=10, pml=16, angle_deg=90, steps=2500)
 
 
-def _torch():
-    import torch
-    if not torch.cuda.is_available():
+def _lib():
+    L = N.lib()
+    if L.fdtd_hip_device_count() < 1:
         raise RuntimeError("the surrogate runs on the GPU only (no HIP device visible)")
-    return torch
+    return L
 
 
-def _vp(t):
-    return ctypes.c_void_p(t.data_ptr())
+class DeviceArray:
+    """A dense array in device memory of the library's own HIP runtime
+    (fdtd_hip_alloc), uploaded from and downloaded to numpy."""
+
+    def __init__(self, host=None, shape=None, dtype=np.float32, _base=None, _ptr=None):
+        if host is not None:
+            host = np.ascontiguousarray(host, dtype)
+            shape = host.shape
+        self.shape, self.dtype = tuple(int(v) for v in shape), np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self._base = _base
+        if _ptr is not None:
+            self.ptr = _ptr
+            return
+        p = ctypes.c_void_p()
+        check(_lib().fdtd_hip_alloc(self.nbytes, ctypes.byref(p)), "fdtd_hip_alloc")
+        self.ptr = p
+        if host is not None:
+            check(N.lib().fdtd_hip_upload(self.ptr, host.ctypes.data_as(ctypes.c_void_p), self.nbytes), "upload")
+
+    def rows(self, i0, i1):
+        """rows [i0, i1) as a view (no copy)"""
+        row = self.nbytes // self.shape[0]
+        return DeviceArray(shape=(i1 - i0,) + self.shape[1:], dtype=self.dtype, _base=self,
+                           _ptr=ctypes.c_void_p(self.ptr.value + i0 * row))
+
+    def numpy(self):
+        out = np.empty(self.shape, self.dtype)
+        check(N.lib().fdtd_hip_download(out.ctypes.data_as(ctypes.c_void_p), self.ptr, self.nbytes), "download")
+        return out
+
+    def free(self):
+        if self._base is None and getattr(self, "ptr", None) is not None:
+            N.lib().fdtd_hip_free(self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        self.free()
+
+
+def _dev(x):
+    return x if isinstance(x, DeviceArray) else DeviceArray(np.asarray(x, np.float32))
 
 
 class MLP:
-    """One network of the library (mlp_create); inputs and targets are FP32
-    torch tensors on the GPU, [B, dims[0]] and [B, dims[-1]]."""
+    """One network of the library (mlp_create).  Inputs and targets are FP32
+    rows, [B, dims[0]] and [B, dims[-1]], as numpy arrays or DeviceArrays."""
 
     def __init__(self, dims, batch_max=1024, seed=0):
-        torch = _torch()
+        L = _lib()
         self.dims = list(dims)
         self.batch_max = int(batch_max)
         arr = (ctypes.c_int * len(dims))(*dims)
         h = ctypes.c_void_p()
-        check(N.lib().mlp_create(len(dims) - 1, arr, self.batch_max, ctypes.c_ulonglong(seed), ctypes.byref(h)),
+        check(L.mlp_create(len(dims) - 1, arr, self.batch_max, ctypes.c_ulonglong(seed), ctypes.byref(h)),
               "mlp_create")
         self._h = h
-        self._torch = torch
+
+    def forward_into(self, x, y):
+        """device to device: y[B, dims[-1]] = net(x[B, dims[0]])"""
+        for i in range(0, x.shape[0], self.batch_max):
+            i1 = min(i + self.batch_max, x.shape[0])
+            check(N.lib().mlp_forward(self._h, x.rows(i, i1).ptr, i1 - i, y.rows(i, i1).ptr), "mlp_forward")
 
     def forward(self, x):
-        torch = self._torch
-        x = x.contiguous().float()
-        out = torch.empty((x.shape[0], self.dims[-1]), dtype=torch.float32, device=x.device)
-        for i in range(0, x.shape[0], self.batch_max):
-            xb, yb = x[i:i + self.batch_max], out[i:i + self.batch_max]
-            check(N.lib().mlp_forward(self._h, _vp(xb), xb.shape[0], _vp(yb)), "mlp_forward")
-        return out
+        x = _dev(x)
+        y = DeviceArray(shape=(x.shape[0], self.dims[-1]))
+        self.forward_into(x, y)
+        return y.numpy()
 
     def grads(self, x, t):
         loss = ctypes.c_float()
-        x, t = x.contiguous().float(), t.contiguous().float()
-        check(N.lib().mlp_grads(self._h, _vp(x), _vp(t), x.shape[0], ctypes.byref(loss)), "mlp_grads")
+        x, t = _dev(x), _dev(t)
+        check(N.lib().mlp_grads(self._h, x.ptr, t.ptr, x.shape[0], ctypes.byref(loss)), "mlp_grads")
         return loss.value
 
     def train_step(self, x, t, lr):
         loss = ctypes.c_float()
-        x, t = x.contiguous().float(), t.contiguous().float()
-        check(N.lib().mlp_train_step(self._h, _vp(x), _vp(t), x.shape[0], ctypes.c_float(lr), ctypes.byref(loss)),
+        x, t = _dev(x), _dev(t)
+        check(N.lib().mlp_train_step(self._h, x.ptr, t.ptr, x.shape[0], ctypes.c_float(lr), ctypes.byref(loss)),
               "mlp_train_step")
+        return loss.value
+
+    def train_epoch(self, x, t, batch, seed, lr):
+        """one shuffled pass over device rows x, t; returns the mean batch loss"""
+        loss = ctypes.c_float()
+        check(N.lib().mlp_train_epoch(self._h, x.ptr, t.ptr, x.shape[0], int(batch), ctypes.c_ulonglong(seed),
+                                      ctypes.c_float(lr), ctypes.byref(loss)), "mlp_train_epoch")
         return loss.value
 
     def _layer(self, fn, l):
@@ -132,28 +181,22 @@ class Ensemble:
         self.seed = seed
 
     def fit(self, X, Y, epochs=200, lr=1e-3, log=None):
-        """Adam at a fixed rate over `epochs` passes of shuffled batches."""
-        torch = _torch()
-        n = X.shape[0]
+        """Adam at a fixed rate over `epochs` shuffled passes (one library call per pass and network)."""
+        Xd, Yd = _dev(X), _dev(Y)
         hist = []
         for e in range(epochs):
             tot = 0.0
             for i, net in enumerate(self.nets):
-                g = torch.Generator(device=X.device)
-                g.manual_seed(self.seed * 100003 + e * 131 + i)
-                perm = torch.randperm(n, device=X.device, generator=g)
-                for s in range(0, n, self.batch):
-                    idx = perm[s:s + self.batch]
-                    tot += net.train_step(X[idx], Y[idx], lr) * idx.numel()
-            hist.append(tot / (n * len(self.nets)))
+                tot += net.train_epoch(Xd, Yd, self.batch, self.seed * 1000003 + e * 131 + i, lr)
+            hist.append(tot / len(self.nets))
             if log and (e % max(1, epochs // 10) == 0 or e == epochs - 1):
                 log(f"epoch {e}: mse {hist[-1]:.3e}")
         return hist
 
     def predict(self, X):
-        torch = _torch()
-        P = torch.stack([net.forward(X) for net in self.nets])
-        return P.mean(0), P.std(0, unbiased=False)
+        """(mean, std) over the members, numpy float64 [B, dims[-1]]"""
+        P = np.stack([net.forward(X).astype(np.float64) for net in self.nets])
+        return P.mean(0), P.std(0)
 
 
 def morpho_spec(seed=1, n=64):
@@ -214,11 +257,8 @@ def predict_with_fallback(ens, spec, scale, threshold=0.02, fdtd=None):
     the angles, relative to the mean's norm) exceeds `threshold`, the query is
     answered by FDTD instead.  Returns the reflectances [n, 180] and the mask of
     queries that went to FDTD."""
-    torch = _torch()
-    X = torch.tensor(features(spec), device="cuda")
-    mean, std = ens.predict(X)
-    mean = mean.double().cpu().numpy() * scale
-    std = std.double().cpu().numpy() * scale
+    mean, std = ens.predict(features(spec))
+    mean, std = mean * scale, std * scale
     rel = np.linalg.norm(std, axis=1) / np.maximum(np.linalg.norm(mean, axis=1), 1e-300)
     use = rel > threshold
     out = mean.copy()
@@ -246,42 +286,40 @@ def experiment(n_train=20000, n_test=2000, epochs=200, k=4, seed=1, threshold=0.
     (held-out error, fallback rate, error after fallback, throughput against
     the batched FDTD), as a dict."""
     import time
-    torch = _torch()
+    L = _lib()
     spec_tr = morpho_spec(seed=seed, n=n_train)
     spec_te = morpho_spec(seed=seed + 1, n=n_test)
-    torch.cuda.synchronize()
     t0 = time.perf_counter()
     Xtr, Ytr = generate(spec_tr, log=log)
     Xte, Yte = generate(spec_te, log=log)
-    torch.cuda.synchronize()
+    L.fdtd_hip_synchronize()
     t_gen = time.perf_counter() - t0
     fdtd_rate = (n_train + n_test) / t_gen
     scale = float(Ytr.mean())
-    X = torch.tensor(Xtr, device="cuda")
-    Y = torch.tensor(Ytr / scale, dtype=torch.float32, device="cuda")
     ens = Ensemble([N_FEATURES, 512, 512, 512, N_ANGLES], k=k, batch=1024, seed=seed)
     t0 = time.perf_counter()
-    hist = ens.fit(X, Y, epochs=epochs, lr=1e-3, log=log)
-    torch.cuda.synchronize()
+    hist = ens.fit(Xtr, (Ytr / scale).astype(np.float32), epochs=epochs, lr=1e-3, log=log)
     t_train = time.perf_counter() - t0
-    Xq = torch.tensor(Xte, device="cuda")
-    mean, std = ens.predict(Xq)
-    pred = mean.double().cpu().numpy() * scale
-    spread = np.linalg.norm(std.double().cpu().numpy(), axis=1) / np.maximum(
-        np.linalg.norm(mean.double().cpu().numpy(), axis=1), 1e-300)
+    mean, std = ens.predict(Xte)
+    pred = mean * scale
+    spread = np.linalg.norm(std, axis=1) / np.maximum(np.linalg.norm(mean, axis=1), 1e-300)
     err = rel_err(pred, Yte)
     use = spread > threshold
     err_fb = np.where(use, 0.0, err)  # the flagged queries are answered by the FDTD itself
     nn = rel_err(nearest_neighbour(Xtr, Ytr, Xte), Yte)
-    # inference throughput: 64 Ki queries through the whole ensemble
-    Xb = torch.rand((65536, N_FEATURES), device="cuda")
-    ens.predict(Xb)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
+    # inference throughput: 64 Ki queries through the whole ensemble, device to device
+    rng = np.random.default_rng(seed)
+    Xb = DeviceArray(rng.random((65536, N_FEATURES)).astype(np.float32))
+    outs = [DeviceArray(shape=(65536, N_ANGLES)) for _ in ens.nets]
+    for net, o in zip(ens.nets, outs):
+        net.forward_into(Xb, o)
+    L.fdtd_hip_synchronize()
     reps = 5
+    t0 = time.perf_counter()
     for _ in range(reps):
-        ens.predict(Xb)
-    torch.cuda.synchronize()
+        for net, o in zip(ens.nets, outs):
+            net.forward_into(Xb, o)
+    L.fdtd_hip_synchronize()
     sur_rate = reps * 65536 / (time.perf_counter() - t0)
     return {
         "n_train": n_train, "n_test": n_test, "epochs": epochs, "ensemble": k,

@@ -10,7 +10,6 @@ pytestmark = pytest.mark.gpu
 
 
 def test_surrogate_gain_and_fallback():
-    import torch
     from mpifdtd_amd import surrogate as S
     spec_tr = S.morpho_spec(seed=11, n=512)
     spec_te = S.morpho_spec(seed=12, n=48)
@@ -19,11 +18,10 @@ def test_surrogate_gain_and_fallback():
     assert np.all(np.isfinite(Ytr)) and Ytr.min() >= 0 and Ytr.max() > 0
     scale = float(Ytr.mean())
     ens = S.Ensemble([S.N_FEATURES, 256, 256, S.N_ANGLES], k=2, batch=512, seed=3)
-    hist = ens.fit(torch.tensor(Xtr, device="cuda"), torch.tensor(Ytr / scale, dtype=torch.float32, device="cuda"),
-                   epochs=300, lr=1e-3)
+    hist = ens.fit(Xtr, (Ytr / scale).astype(np.float32), epochs=300, lr=1e-3)
     assert hist[-1] < hist[0] / 5
-    mean, _ = ens.predict(torch.tensor(Xte, device="cuda"))
-    err = S.rel_err(mean.double().cpu().numpy() * scale, Yte)
+    mean, _ = ens.predict(Xte)
+    err = S.rel_err(mean * scale, Yte)
     err_nn = S.rel_err(S.nearest_neighbour(Xtr, Ytr, Xte), Yte)
     print(f"held-out rel err: surrogate mean {err.mean():.4f} median {np.median(err):.4f}; "
           f"nearest neighbour mean {err_nn.mean():.4f} median {np.median(err_nn):.4f}")
@@ -34,4 +32,4 @@ def test_surrogate_gain_and_fallback():
     # an unreachable threshold: none does, and the answer is the ensemble mean
     out, use = S.predict_with_fallback(ens, spec_te[:8], scale, threshold=1e30)
     assert not use.any()
-    assert np.allclose(out, mean[:8].double().cpu().numpy() * scale, rtol=1e-6)
+    assert np.allclose(out, mean[:8] * scale, rtol=1e-6)
