@@ -112,6 +112,9 @@ int fdtd_hip_free(void *p);
 int fdtd_hip_upload(void *dst, const void *src, size_t bytes);
 int fdtd_hip_download(void *dst, const void *src, size_t bytes);
 int fdtd_hip_synchronize(void);
+/* the solver objects keep freed device blocks for reuse by later objects of
+ * the same sizes; this returns them to the runtime */
+int fdtd_hip_trim(void);
 
 int fdtd_hip_grid_create(const fdtd_grid_desc *d, fdtd_grid **out);
 int fdtd_hip_grid_destroy(fdtd_grid *g);

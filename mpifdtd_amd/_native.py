@@ -174,6 +174,7 @@ def lib():
     sig("fdtd_hip_upload", I, vp, vp, ctypes.c_size_t)
     sig("fdtd_hip_download", I, vp, vp, ctypes.c_size_t)
     sig("fdtd_hip_synchronize", I)
+    sig("fdtd_hip_trim", I)
     sig("fdtd_hip_profile_enable", V, I)
     sig("fdtd_hip_profile_read", V, dp)
     sig("fdtd_hip_last_error", ctypes.c_char_p)

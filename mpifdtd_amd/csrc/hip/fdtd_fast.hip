@@ -952,7 +952,7 @@ void fast_profile_read(double *ms, long *launches, double *bytes, double *cells)
 template <class T>
 static int dev_alloc(FastState *s, T **p, size_t n)
 {
-  HIPCHK(hipMalloc((void **)p, std::max<size_t>(1, n) * sizeof(T)));
+  HIPCHK(dev_malloc((void **)p, std::max<size_t>(1, n) * sizeof(T)));
   HIPCHK(hipMemset(*p, 0, std::max<size_t>(1, n) * sizeof(T)));
   s->allocs.push_back((void *)*p);
   return FDTD_OK;
@@ -1007,9 +1007,9 @@ int fast_state_create(const fdtd_grid_desc &d, FastState **out)
 void fast_state_destroy(FastState *s)
 {
   if (!s) return;
-  for (void *p : s->allocs) (void)hipFree(p);
-  if (s->dtiles) (void)hipFree(s->dtiles);
-  if (s->csamp) (void)hipFree(s->csamp);
+  for (void *p : s->allocs) (void)dev_free(p);
+  if (s->dtiles) (void)dev_free(s->dtiles);
+  if (s->csamp) (void)dev_free(s->csamp);
   delete s;
 }
 
@@ -1120,9 +1120,9 @@ static int build_tiles(FastState *s)
   s->n_pml = (int)tp.size();
   std::vector<FastTile> all(ti);
   all.insert(all.end(), tp.begin(), tp.end());
-  if (s->dtiles) (void)hipFree(s->dtiles);
+  if (s->dtiles) (void)dev_free(s->dtiles);
   s->dtiles = nullptr;
-  HIPCHK(hipMalloc(&s->dtiles, std::max<size_t>(1, all.size()) * sizeof(FastTile)));
+  HIPCHK(dev_malloc(&s->dtiles, std::max<size_t>(1, all.size()) * sizeof(FastTile)));
   if (!all.empty()) HIPCHK(hipMemcpy(s->dtiles, all.data(), all.size() * sizeof(FastTile), hipMemcpyHostToDevice));
   split_queues(s->qint, 0, s->n_int);
   split_queues(s->qpml, s->n_int, s->n_pml);
@@ -1437,10 +1437,10 @@ void fast_batch_destroy(FastBatch *b)
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
-  if (b->dinst) (void)hipFree(b->dinst);
-  if (b->dbinst) (void)hipFree(b->dbinst);
-  if (b->dtiles) (void)hipFree(b->dtiles);
-  if (b->dqctr) (void)hipFree(b->dqctr);
+  if (b->dinst) (void)dev_free(b->dinst);
+  if (b->dbinst) (void)dev_free(b->dbinst);
+  if (b->dtiles) (void)dev_free(b->dtiles);
+  if (b->dqctr) (void)dev_free(b->dqctr);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
 }
@@ -1516,10 +1516,10 @@ int fast_batch_create(FastState *const *st, const fdtd_step_params *p0, const in
     b->nblk[m] = std::min(ntile, 512);
   }
   HIPCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
-  HIPCHK(hipMalloc(&b->dinst, n * sizeof(FastInst)));
-  HIPCHK(hipMalloc(&b->dbinst, n * sizeof(FastBatchInst)));
-  HIPCHK(hipMalloc(&b->dtiles, std::max<size_t>(1, all.size()) * sizeof(FastTile)));
-  HIPCHK(hipMalloc(&b->dqctr, 4 * sizeof(int)));
+  HIPCHK(dev_malloc(&b->dinst, n * sizeof(FastInst)));
+  HIPCHK(dev_malloc(&b->dbinst, n * sizeof(FastBatchInst)));
+  HIPCHK(dev_malloc(&b->dtiles, std::max<size_t>(1, all.size()) * sizeof(FastTile)));
+  HIPCHK(dev_malloc(&b->dqctr, 4 * sizeof(int)));
   HIPCHK(hipMemcpy(b->dinst, hinst.data(), n * sizeof(FastInst), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(b->dbinst, hb.data(), n * sizeof(FastBatchInst), hipMemcpyHostToDevice));
   if (!all.empty()) HIPCHK(hipMemcpy(b->dtiles, all.data(), all.size() * sizeof(FastTile), hipMemcpyHostToDevice));
@@ -1629,9 +1629,9 @@ int fast_state_set_contour(FastState *s, const Contour &ct, int off, fast_post_f
     if (nc == 2 && abs(fl[0] - fl[1]) < 2)
       return fdtd_fail(FDTD_EINVAL, "NTFF contour %d cells across: too small for the FP32 path", abs(fl[0] - fl[1]));
   }
-  if (s->csamp) (void)hipFree(s->csamp);
+  if (s->csamp) (void)dev_free(s->csamp);
   s->csamp = nullptr;
-  if (ct.total > 0) HIPCHK(hipMalloc(&s->csamp, (size_t)FAST_MAX_T * 3 * ct.total * sizeof(float2)));
+  if (ct.total > 0) HIPCHK(dev_malloc(&s->csamp, (size_t)FAST_MAX_T * 3 * ct.total * sizeof(float2)));
   s->post = post;
   s->post_ctx = ctx;
   return FDTD_OK;

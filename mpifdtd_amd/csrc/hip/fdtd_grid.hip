@@ -23,6 +23,9 @@
 #include <cstring>
 #include <string>
 #include <climits>
+#include <map>
+#include <mutex>
+#include <unordered_map>
 #include <vector>
 #include <algorithm>
 
@@ -52,12 +55,70 @@ extern "C" int fdtd_hip_device_count(void)
   return n;
 }
 
+namespace {
+std::mutex g_amu;
+std::unordered_map<void *, size_t> g_live;   // block -> rounded size
+std::multimap<size_t, void *> g_kept;        // freed blocks by size
+size_t g_kept_bytes = 0;
+const size_t KEEP_MAX = (size_t)64 << 30;
+}  // namespace
+
+hipError_t dev_malloc_bytes(void **p, size_t bytes)
+{
+  const size_t b = ((bytes ? bytes : 1) + 255) & ~(size_t)255;
+  std::lock_guard<std::mutex> lk(g_amu);
+  auto it = g_kept.find(b);
+  if (it != g_kept.end()) {
+    *p = it->second;
+    g_kept.erase(it);
+    g_kept_bytes -= b;
+    g_live[*p] = b;
+    return hipSuccess;
+  }
+  hipError_t e = hipMalloc(p, b);
+  if (e == hipErrorOutOfMemory && !g_kept.empty()) {  // return the kept blocks and try again
+    (void)hipGetLastError();
+    for (auto &kv : g_kept) (void)hipFree(kv.second);
+    g_kept.clear();
+    g_kept_bytes = 0;
+    e = hipMalloc(p, b);
+  }
+  if (e == hipSuccess) g_live[*p] = b;
+  return e;
+}
+
+hipError_t dev_free(void *p)
+{
+  if (!p) return hipSuccess;
+  hipError_t e = hipDeviceSynchronize();
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> lk(g_amu);
+  auto it = g_live.find(p);
+  if (it == g_live.end()) return hipFree(p);
+  const size_t b = it->second;
+  g_live.erase(it);
+  if (g_kept_bytes + b > KEEP_MAX) return hipFree(p);
+  g_kept.emplace(b, p);
+  g_kept_bytes += b;
+  return hipSuccess;
+}
+
+extern "C" int fdtd_hip_trim(void)
+{
+  hipError_t e = hipDeviceSynchronize();
+  std::lock_guard<std::mutex> lk(g_amu);
+  for (auto &kv : g_kept) (void)hipFree(kv.second);
+  g_kept.clear();
+  g_kept_bytes = 0;
+  return e == hipSuccess ? FDTD_OK : fdtd_fail(FDTD_EHIP, "hipDeviceSynchronize failed: %s", hipGetErrorString(e));
+}
+
 extern "C" int fdtd_hip_alloc(size_t bytes, void **out)
 {
   if (!out) return fdtd_fail(FDTD_EINVAL, "null argument");
   *out = nullptr;
-  hipError_t e = hipMalloc(out, bytes ? bytes : 1);
-  if (e != hipSuccess) return fdtd_fail(FDTD_ENOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  hipError_t e = dev_malloc(out, bytes ? bytes : 1);
+  if (e != hipSuccess) return fdtd_fail(FDTD_ENOMEM, "dev_malloc(%zu) failed: %s", bytes, hipGetErrorString(e));
   return FDTD_OK;
 }
 
@@ -938,13 +999,16 @@ static void build_contour(fdtd_grid *g)
 // field and ntffTE_TimeTranslate (angle table A), but as ang*ToRad with
 // ToRad = M_PI/180.0 in ntffTM_TimeCalc / ntffTE_TimeCalc / ntffTM_TimeTranslate
 // (table B, ntffTM.c:314-318, ntffTE.c:89-92, ntffTM.c:169-173); the two differ in the last bit.
+// the per-(angle, contour cell) time shifts (tab, only when with_tab) and the
+// angle tables
 static void build_time_shift(const fdtd_grid *g, std::vector<double> &tab, std::vector<double> &mn, std::vector<double> &mx,
-                             std::vector<double> &ca, std::vector<double> &sa, std::vector<double> &cb, std::vector<double> &sb)
+                             std::vector<double> &ca, std::vector<double> &sa, std::vector<double> &cb, std::vector<double> &sb,
+                             bool with_tab = true)
 {
   const fdtd_grid_desc &d = g->d;
   const Contour &ct = g->ct;
   const bool mpi = d.flavour == FDTD_FLAVOUR_MPI;
-  tab.assign((size_t)360 * ct.total, 0.0);
+  tab.assign(with_tab ? (size_t)360 * ct.total : 0, 0.0);
   mn.assign(360, 0.0);
   mx.assign(360, 0.0);
   ca.resize(360);
@@ -958,6 +1022,7 @@ static void build_time_shift(const fdtd_grid *g, std::vector<double> &tab, std::
     sa[ang] = sin(radA);
     cb[ang] = cos(radB);
     sb[ang] = sin(radB);
+    if (!with_tab) continue;
     const double r1x = mpi ? ca[ang] : cb[ang], r1y = mpi ? sa[ang] : sb[ang];
     double *t = &tab[(size_t)ang * ct.total];
     for (int q = 0; q < 4; q++) {
@@ -1043,8 +1108,8 @@ static int build_bin_groups(fdtd_grid *g, const std::vector<double> &tab)
   }
   if (rec.size() > (size_t)INT_MAX) return fdtd_fail(FDTD_ENOMEM, "NTFF binning records exceed 2^31");
   g->nparts = (int)parts.size();
-  HIPCHK(hipMalloc(&g->binPart, std::max<size_t>(1, parts.size()) * sizeof(BinPart)));
-  HIPCHK(hipMalloc(&g->binRec, std::max<size_t>(1, rec.size()) * sizeof(BinRec)));
+  HIPCHK(dev_malloc(&g->binPart, std::max<size_t>(1, parts.size()) * sizeof(BinPart)));
+  HIPCHK(dev_malloc(&g->binRec, std::max<size_t>(1, rec.size()) * sizeof(BinRec)));
   if (!parts.empty()) HIPCHK(hipMemcpy(g->binPart, parts.data(), parts.size() * sizeof(BinPart), hipMemcpyHostToDevice));
   if (!rec.empty()) HIPCHK(hipMemcpy(g->binRec, rec.data(), rec.size() * sizeof(BinRec), hipMemcpyHostToDevice));
   return FDTD_OK;
@@ -1131,18 +1196,19 @@ extern "C" int fdtd_hip_grid_create(const fdtd_grid_desc *d, fdtd_grid **out)
   g->NPYa = d->N_PY + 2 * g->off;
   g->ncell = (size_t)g->NPXa * g->NPYa;
   *out = g;
-  for (int f = 0; f < 9; f++) {
-    HIPCHK(hipMalloc(&g->F[f], g->ncell * sizeof(double2)));
+  // the FP64 state; an FP32 grid uses these only as export buffers, made on first use
+  for (int f = 0; f < 9 && d->precision == FDTD_FP64; f++) {
+    HIPCHK(dev_malloc(&g->F[f], g->ncell * sizeof(double2)));
     HIPCHK(hipMemset(g->F[f], 0, g->ncell * sizeof(double2)));
   }
-  HIPCHK(hipMalloc(&g->Fdev, 9 * sizeof(double2 *)));
+  HIPCHK(dev_malloc(&g->Fdev, 9 * sizeof(double2 *)));
   HIPCHK(hipMemcpy(g->Fdev, g->F, 9 * sizeof(double2 *), hipMemcpyHostToDevice));
   for (int e = 0; e < 3; e++) {
-    HIPCHK(hipMalloc(&g->eps[e], g->ncell * sizeof(double)));
+    HIPCHK(dev_malloc(&g->eps[e], g->ncell * sizeof(double)));
     HIPCHK(hipMemset(g->eps[e], 0, g->ncell * sizeof(double)));
   }
   const size_t ntab = 6 * (size_t)d->N_PX + 5 * (size_t)d->N_PY;
-  HIPCHK(hipMalloc(&g->tab, ntab * sizeof(double)));
+  HIPCHK(dev_malloc(&g->tab, ntab * sizeof(double)));
   HIPCHK(hipMemset(g->tab, 0, ntab * sizeof(double)));
   for (int q = 0; q < 5; q++) g->tabs.x[q] = g->tab + q * (size_t)d->N_PX;
   g->tabs.sx = g->tab + 5 * (size_t)d->N_PX;
@@ -1154,35 +1220,35 @@ extern "C" int fdtd_hip_grid_create(const fdtd_grid_desc *d, fdtd_grid **out)
   // complex: 240 MB for a 10000-step run of a 2080-cell grid)
   g->ntn = d->ntff_time ? (size_t)360 * d->array_size + 2 * NT_GUARD : 0;
   for (int a = 0; a < 3 && g->ntn; a++) {
-    HIPCHK(hipMalloc(&g->ntb[a], g->ntn * sizeof(double2)));
+    HIPCHK(dev_malloc(&g->ntb[a], g->ntn * sizeof(double2)));
     HIPCHK(hipMemset(g->ntb[a], 0, g->ntn * sizeof(double2)));
     g->nt[a] = g->ntb[a] + NT_GUARD;
   }
   std::vector<double> tab, mn, mx, ca, sa, cb, sb;
-  build_time_shift(g, tab, mn, mx, ca, sa, cb, sb);
+  build_time_shift(g, tab, mn, mx, ca, sa, cb, sb, d->ntff_time != 0);
   const int P = std::max(1, g->ct.total);
   // FP32 grids bin a whole block of steps at once: sample buffers for FAST_MAX_T steps
   g->nbuf = d->precision == FDTD_FP32 ? FAST_MAX_T : 1;
-  HIPCHK(hipMalloc(&g->cE, (size_t)g->nbuf * P * sizeof(double2)));
-  HIPCHK(hipMalloc(&g->cH, (size_t)g->nbuf * P * sizeof(double2)));
+  HIPCHK(dev_malloc(&g->cE, (size_t)g->nbuf * P * sizeof(double2)));
+  HIPCHK(dev_malloc(&g->cH, (size_t)g->nbuf * P * sizeof(double2)));
   if (d->ntff_time && g->ct.total > 0) {
     int rc = build_bin_groups(g, tab);
     if (rc) return rc;
   }
-  HIPCHK(hipMalloc(&g->cosA, 360 * sizeof(double)));
-  HIPCHK(hipMalloc(&g->sinA, 360 * sizeof(double)));
+  HIPCHK(dev_malloc(&g->cosA, 360 * sizeof(double)));
+  HIPCHK(dev_malloc(&g->sinA, 360 * sizeof(double)));
   HIPCHK(hipMemcpy(g->cosA, ca.data(), 360 * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(g->sinA, sa.data(), 360 * sizeof(double), hipMemcpyHostToDevice));
-  HIPCHK(hipMalloc(&g->cosB, 360 * sizeof(double)));
-  HIPCHK(hipMalloc(&g->sinB, 360 * sizeof(double)));
+  HIPCHK(dev_malloc(&g->cosB, 360 * sizeof(double)));
+  HIPCHK(dev_malloc(&g->sinB, 360 * sizeof(double)));
   HIPCHK(hipMemcpy(g->cosB, cb.data(), 360 * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(g->sinB, sb.data(), 360 * sizeof(double), hipMemcpyHostToDevice));
   // 1/(4 pi c R), R = 1e6, applied per step by the MPI solvers (mpiTM_UPML.c:852-854)
   g->coef = 1.0 / (4 * M_PI * d->c0 * 1.0e6);
-  HIPCHK(hipMalloc(&g->ffout, 360 * sizeof(double2)));
-  HIPCHK(hipMalloc(&g->spec, 321 * 360 * sizeof(double)));
-  HIPCHK(hipMalloc(&g->tw, 4096 * sizeof(double2)));
-  HIPCHK(hipMalloc(&g->mstate, sizeof(fdtd_model_state)));
+  HIPCHK(dev_malloc(&g->ffout, 360 * sizeof(double2)));
+  HIPCHK(dev_malloc(&g->spec, 321 * 360 * sizeof(double)));
+  HIPCHK(dev_malloc(&g->tw, 4096 * sizeof(double2)));
+  HIPCHK(dev_malloc(&g->mstate, sizeof(fdtd_model_state)));
   if (d->precision == FDTD_FP32) {
     int rc = fast_state_create(*d, &g->fs);
     if (rc) return rc;
@@ -1223,28 +1289,28 @@ extern "C" void fdtd_hip_profile_read(double out[6]) { fast_profile_read6(out); 
 extern "C" int fdtd_hip_grid_destroy(fdtd_grid *g)
 {
   if (!g) return FDTD_OK;
-  for (int f = 0; f < 9; f++) (void)hipFree(g->F[f]);
-  (void)hipFree(g->Fdev);
-  for (int e = 0; e < 3; e++) (void)hipFree(g->eps[e]);
-  (void)hipFree(g->tab);
-  for (int a = 0; a < 3; a++) (void)hipFree(g->ntb[a]);
-  (void)hipFree(g->cE);
-  (void)hipFree(g->cH);
-  (void)hipFree(g->binPart);
-  (void)hipFree(g->binRec);
-  (void)hipFree(g->dftOmega);
-  (void)hipFree(g->dftE);
-  (void)hipFree(g->dftH);
-  (void)hipFree(g->dftTs);
-  (void)hipFree(g->dftOut);
-  (void)hipFree(g->cosA);
-  (void)hipFree(g->sinA);
-  (void)hipFree(g->cosB);
-  (void)hipFree(g->sinB);
-  (void)hipFree(g->ffout);
-  (void)hipFree(g->spec);
-  (void)hipFree(g->tw);
-  (void)hipFree(g->mstate);
+  for (int f = 0; f < 9; f++) (void)dev_free(g->F[f]);
+  (void)dev_free(g->Fdev);
+  for (int e = 0; e < 3; e++) (void)dev_free(g->eps[e]);
+  (void)dev_free(g->tab);
+  for (int a = 0; a < 3; a++) (void)dev_free(g->ntb[a]);
+  (void)dev_free(g->cE);
+  (void)dev_free(g->cH);
+  (void)dev_free(g->binPart);
+  (void)dev_free(g->binRec);
+  (void)dev_free(g->dftOmega);
+  (void)dev_free(g->dftE);
+  (void)dev_free(g->dftH);
+  (void)dev_free(g->dftTs);
+  (void)dev_free(g->dftOut);
+  (void)dev_free(g->cosA);
+  (void)dev_free(g->sinA);
+  (void)dev_free(g->cosB);
+  (void)dev_free(g->sinB);
+  (void)dev_free(g->ffout);
+  (void)dev_free(g->spec);
+  (void)dev_free(g->tw);
+  (void)dev_free(g->mstate);
   fast_state_destroy(g->fs);
   delete g;
   return FDTD_OK;
@@ -1378,9 +1444,21 @@ extern "C" int fdtd_hip_grid_step(fdtd_grid *g, const fdtd_step_params *p)
   return FDTD_OK;
 }
 
+// an FP32 grid's FP64 export buffer for slot f, zeroed when first made (the
+// ghost ring and the cells the export does not write stay zero)
+static int ensure_F(fdtd_grid *g, int f)
+{
+  if (g->F[f]) return FDTD_OK;
+  HIPCHK(dev_malloc(&g->F[f], g->ncell * sizeof(double2)));
+  HIPCHK(hipMemset(g->F[f], 0, g->ncell * sizeof(double2)));
+  return FDTD_OK;
+}
+
 extern "C" int fdtd_hip_grid_get_field(fdtd_grid *g, int which, double *host)
 {
   if (!g || !host || which < 0 || which > 8) return fdtd_fail(FDTD_EINVAL, "bad argument");
+  int rc0 = ensure_F(g, which);
+  if (rc0) return rc0;
   if (g->fs) {
     int rc = fast_state_export(g->fs, which, g->F[which], g->NPYa, g->off);
     if (rc) return rc;
@@ -1392,6 +1470,8 @@ extern "C" int fdtd_hip_grid_get_field(fdtd_grid *g, int which, double *host)
 extern "C" int fdtd_hip_grid_set_field(fdtd_grid *g, int which, const double *host)
 {
   if (!g || !host || which < 0 || which > 8) return fdtd_fail(FDTD_EINVAL, "bad argument");
+  int rc0 = ensure_F(g, which);
+  if (rc0) return rc0;
   HIPCHK(hipMemcpy(g->F[which], host, g->ncell * sizeof(double2), hipMemcpyHostToDevice));
   if (g->fs) return fast_state_import(g->fs, which, g->F[which], g->NPYa, g->off);
   return FDTD_OK;
@@ -1410,7 +1490,8 @@ extern "C" int fdtd_hip_grid_get_ntff_time(fdtd_grid *g, int which, double *host
 extern "C" int fdtd_hip_grid_reset(fdtd_grid *g)
 {
   if (!g) return fdtd_fail(FDTD_EINVAL, "null argument");
-  for (int f = 0; f < 9; f++) HIPCHK(hipMemset(g->F[f], 0, g->ncell * sizeof(double2)));
+  for (int f = 0; f < 9; f++)
+    if (g->F[f]) HIPCHK(hipMemset(g->F[f], 0, g->ncell * sizeof(double2)));
   for (int a = 0; a < 3 && g->ntn; a++) HIPCHK(hipMemset(g->ntb[a], 0, g->ntn * sizeof(double2)));
   if (g->nf) {
     HIPCHK(hipMemset(g->dftE, 0, (size_t)g->nf * g->ct.total * sizeof(double2)));
@@ -1427,8 +1508,8 @@ extern "C" int fdtd_hip_grid_far_field(fdtd_grid *g, double k, double coef_re, d
   const fdtd_grid_desc &d = g->d;
   if (g->fs) {  // the far field of the FP32 fields is formed in FP64
     for (int slot : {0, 3, 6}) {
-      int rc = fast_state_export(g->fs, slot, g->F[slot], g->NPYa, g->off);
-      if (rc) return rc;
+      int rc = ensure_F(g, slot);
+      if (rc || (rc = fast_state_export(g->fs, slot, g->F[slot], g->NPYa, g->off))) return rc;
     }
   }
   FFArgs a;
@@ -1475,27 +1556,27 @@ extern "C" int fdtd_hip_grid_set_dft(fdtd_grid *g, const double *omega, int nf)
   int rc;
   if (g->fs && (rc = fast_state_flush(g->fs))) return rc;
   HIPCHK(hipDeviceSynchronize());
-  (void)hipFree(g->dftOmega);
-  (void)hipFree(g->dftE);
-  (void)hipFree(g->dftH);
-  (void)hipFree(g->dftTs);
-  (void)hipFree(g->dftOut);
+  (void)dev_free(g->dftOmega);
+  (void)dev_free(g->dftE);
+  (void)dev_free(g->dftH);
+  (void)dev_free(g->dftTs);
+  (void)dev_free(g->dftOut);
   g->dftOmega = nullptr;
   g->dftE = g->dftH = g->dftOut = nullptr;
   g->dftTs = nullptr;
   g->nf = 0;
   if (nf == 0) return FDTD_OK;
   const int P = g->ct.total;
-  HIPCHK(hipMalloc(&g->dftOmega, nf * sizeof(double)));
+  HIPCHK(dev_malloc(&g->dftOmega, nf * sizeof(double)));
   HIPCHK(hipMemcpy(g->dftOmega, omega, nf * sizeof(double), hipMemcpyHostToDevice));
-  HIPCHK(hipMalloc(&g->dftE, (size_t)nf * P * sizeof(double2)));
-  HIPCHK(hipMalloc(&g->dftH, (size_t)nf * P * sizeof(double2)));
+  HIPCHK(dev_malloc(&g->dftE, (size_t)nf * P * sizeof(double2)));
+  HIPCHK(dev_malloc(&g->dftH, (size_t)nf * P * sizeof(double2)));
   HIPCHK(hipMemset(g->dftE, 0, (size_t)nf * P * sizeof(double2)));
   HIPCHK(hipMemset(g->dftH, 0, (size_t)nf * P * sizeof(double2)));
-  HIPCHK(hipMalloc(&g->dftOut, (size_t)nf * 3 * 360 * sizeof(double2)));
+  HIPCHK(dev_malloc(&g->dftOut, (size_t)nf * 3 * 360 * sizeof(double2)));
   std::vector<double> tab, mn, mx, ca, sa, cb, sb;
   build_time_shift(g, tab, mn, mx, ca, sa, cb, sb);
-  HIPCHK(hipMalloc(&g->dftTs, tab.size() * sizeof(double)));
+  HIPCHK(dev_malloc(&g->dftTs, tab.size() * sizeof(double)));
   HIPCHK(hipMemcpy(g->dftTs, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
   g->nf = nf;
   // the FP32 fused kernel samples the contour after every inner step

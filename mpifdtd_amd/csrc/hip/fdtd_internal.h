@@ -23,6 +23,19 @@ struct Contour {
 
 int fdtd_fail(int code, const char *fmt, ...);
 
+// Device memory of the solver objects.  Freed blocks are kept per exact size
+// (up to 64 GiB) and handed to the next request of that size, so that sweeps
+// creating and destroying many equal grids skip hipMalloc / hipFree; dev_free
+// synchronises the device first, so no queued kernel still uses a block that
+// is handed out again.  fdtd_hip_trim() returns the kept blocks.
+hipError_t dev_malloc_bytes(void **p, size_t bytes);
+hipError_t dev_free(void *p);
+template <typename T>
+static inline hipError_t dev_malloc(T **p, size_t bytes)
+{
+  return dev_malloc_bytes((void **)p, bytes);
+}
+
 // ---------------------------------------------------------------- FP32 fast path
 // One core tile of the fused kernel: cells [i0, i0+ni) x [j0, j0+nj) of instance `inst`.
 struct FastTile {

@@ -16,6 +16,7 @@ the matrix cores, FP32 accumulation and master weights), on device buffers of
 the library's own HIP runtime (DeviceArray).
 """
 import ctypes
+import math
 
 import numpy as np
 
@@ -180,14 +181,16 @@ class Ensemble:
         self.batch = batch
         self.seed = seed
 
-    def fit(self, X, Y, epochs=200, lr=1e-3, log=None):
-        """Adam at a fixed rate over `epochs` shuffled passes (one library call per pass and network)."""
+    def fit(self, X, Y, epochs=200, lr=1e-3, lr_end=None, log=None):
+        """Adam over `epochs` shuffled passes (one library call per pass and
+        network), at a fixed rate lr, or cosine-annealed from lr to lr_end."""
         Xd, Yd = _dev(X), _dev(Y)
         hist = []
         for e in range(epochs):
             tot = 0.0
+            r = lr if lr_end is None else lr_end + 0.5 * (lr - lr_end) * (1 + math.cos(math.pi * e / max(1, epochs - 1)))
             for i, net in enumerate(self.nets):
-                tot += net.train_epoch(Xd, Yd, self.batch, self.seed * 1000003 + e * 131 + i, lr)
+                tot += net.train_epoch(Xd, Yd, self.batch, self.seed * 1000003 + e * 131 + i, r)
             hist.append(tot / len(self.nets))
             if log and (e % max(1, epochs // 10) == 0 or e == epochs - 1):
                 log(f"epoch {e}: mse {hist[-1]:.3e}")
@@ -281,24 +284,36 @@ def nearest_neighbour(Xtr, Ytr, Xq):
     return Ytr[np.argmin(d2, axis=1)]
 
 
-def experiment(n_train=20000, n_test=2000, epochs=200, k=4, seed=1, threshold=0.02, log=print):
+def experiment(n_train=20000, n_test=2000, epochs=200, k=4, seed=1, threshold=0.02, log=print, save=None,
+               load=None, lr=1e-3, lr_end=None):
     """Generate, train, evaluate: the numbers of the surrogate's acceptance
     (held-out error, fallback rate, error after fallback, throughput against
-    the batched FDTD), as a dict."""
+    the batched FDTD), as a dict.  `save` writes the generated spectra to an
+    .npz file, `load` reads them from one instead of running the FDTD (its
+    generation rate is then the one recorded in the file)."""
     import time
     L = _lib()
-    spec_tr = morpho_spec(seed=seed, n=n_train)
-    spec_te = morpho_spec(seed=seed + 1, n=n_test)
-    t0 = time.perf_counter()
-    Xtr, Ytr = generate(spec_tr, log=log)
-    Xte, Yte = generate(spec_te, log=log)
-    L.fdtd_hip_synchronize()
-    t_gen = time.perf_counter() - t0
+    if load:
+        z = np.load(load)
+        Xtr, Ytr, Xte, Yte = z["Xtr"], z["Ytr"], z["Xte"], z["Yte"]
+        t_gen = float(z["t_gen"])
+        n_train, n_test = len(Xtr), len(Xte)
+    else:
+        spec_tr = morpho_spec(seed=seed, n=n_train)
+        spec_te = morpho_spec(seed=seed + 1, n=n_test)
+        t0 = time.perf_counter()
+        Xtr, Ytr = generate(spec_tr, log=log)
+        Xte, Yte = generate(spec_te, log=log)
+        L.fdtd_hip_synchronize()
+        t_gen = time.perf_counter() - t0
+        if save:
+            np.savez_compressed(save, Xtr=Xtr, Ytr=Ytr.astype(np.float32), Xte=Xte, Yte=Yte.astype(np.float32),
+                                t_gen=t_gen)
     fdtd_rate = (n_train + n_test) / t_gen
     scale = float(Ytr.mean())
     ens = Ensemble([N_FEATURES, 512, 512, 512, N_ANGLES], k=k, batch=1024, seed=seed)
     t0 = time.perf_counter()
-    hist = ens.fit(Xtr, (Ytr / scale).astype(np.float32), epochs=epochs, lr=1e-3, log=log)
+    hist = ens.fit(Xtr, (Ytr / scale).astype(np.float32), epochs=epochs, lr=lr, lr_end=lr_end, log=log)
     t_train = time.perf_counter() - t0
     mean, std = ens.predict(Xte)
     pred = mean * scale
@@ -322,7 +337,7 @@ def experiment(n_train=20000, n_test=2000, epochs=200, k=4, seed=1, threshold=0.
     L.fdtd_hip_synchronize()
     sur_rate = reps * 65536 / (time.perf_counter() - t0)
     return {
-        "n_train": n_train, "n_test": n_test, "epochs": epochs, "ensemble": k,
+        "n_train": n_train, "n_test": n_test, "epochs": epochs, "ensemble": k, "lr": lr, "lr_end": lr_end,
         "grid": GRID, "fdtd_spectra_per_s": fdtd_rate, "fdtd_seconds": t_gen,
         "train_seconds": t_train, "final_train_mse": hist[-1],
         "heldout_rel_err_mean": float(err.mean()), "heldout_rel_err_median": float(np.median(err)),
@@ -346,6 +361,11 @@ if __name__ == "__main__":
     ap.add_argument("--epochs", type=int, default=200)
     ap.add_argument("--k", type=int, default=4)
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--lr-end", type=float, default=None, help="cosine-anneal the rate to this value")
+    ap.add_argument("--save", default=None, help="write the generated spectra to this .npz")
+    ap.add_argument("--load", default=None, help="read the spectra from this .npz instead of running the FDTD")
     a = ap.parse_args()
-    res = experiment(a.train, a.test, a.epochs, a.k, a.seed, log=lambda s: print(s, flush=True))
+    res = experiment(a.train, a.test, a.epochs, a.k, a.seed, log=lambda s: print(s, flush=True), save=a.save,
+                     load=a.load, lr=a.lr, lr_end=a.lr_end)
     print(json.dumps(res), flush=True)
