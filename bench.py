@@ -53,6 +53,9 @@ def parse():
     ap.add_argument("--mode", default="tm", choices=("tm", "te"))
     ap.add_argument("--T", type=int, default=0, help="temporal block depth (0: autotune)")
     ap.add_argument("--R", type=int, default=0, help="rows per wavefront (0: autotune)")
+    ap.add_argument("--launch", type=int, default=-1,
+                    help="launch shape of the UPML and interior tiles: 0 auto, 1 merged, 2 split, "
+                         "3 split on two streams (-1: autotune's choice, or auto)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 far-field check")
     ap.add_argument("--ntff", action="store_true",
@@ -239,8 +242,11 @@ def main():
                               probe_steps=probe)
         T = T or tuned["T"]
         R = R or tuned["R"]
-    log(f"timed run: {wl} {args.precision} T={T} R={R} warmup={W} steps={K}")
+    launch = args.launch if args.launch >= 0 else (tuned.get("launch", 0) if tuned else 0)
+    log(f"timed run: {wl} {args.precision} T={T} R={R} launch={launch} warmup={W} steps={K}")
     sim = make_sim(m, args.precision, args.mode, total, T or 4, R or 16, workload=wl, ntff=args.ntff)
+    if args.precision == "fp32":
+        sim.set_launch_mode(launch)
     cells = sim.N_PX * sim.N_PY
     sim.calc(W)
     if args.precision == "fp32":
@@ -311,7 +317,9 @@ def main():
         "ms_per_step": round(ms_step, 5), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32" if args.precision == "fp32" else "f64", "data": "synthetic",
         "config": {"workload": WORKLOADS[wl][8].format(mode=args.mode.upper()), "cells": cells,
-                   "precision": args.precision, "T": T, "R": R, "per_step_ntff": args.ntff, "autotune": tuned},
+                   "precision": args.precision, "T": T, "R": R,
+                   "launch": ["auto", "merged", "split", "two_streams"][launch] if args.precision == "fp32" else None,
+                   "per_step_ntff": args.ntff, "autotune": tuned},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

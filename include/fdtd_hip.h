@@ -162,6 +162,15 @@ int fdtd_hip_grid_sync(fdtd_grid *g);
  * (get_field, far_field, ...) first launches the queued steps. */
 int fdtd_hip_grid_set_blocking(fdtd_grid *g, int T, int R);
 int fdtd_hip_grid_get_blocking(const fdtd_grid *g, int *T, int *R);
+/* FP32 grids: how a block's UPML and interior tiles are launched.  AUTO picks
+ * SPLIT for large grids (over 4096 interior workgroups, 8 per UPML one), else
+ * MERGED; MERGED is one launch of both
+ * (one register budget, the UPML body's); SPLIT is two launches in a row, each
+ * at its own occupancy; TWO_STREAMS runs the two launches side by side on two
+ * streams.  The per-step NTFF sampling always uses MERGED. */
+enum { FAST_LAUNCH_AUTO = 0, FAST_LAUNCH_MERGED = 1, FAST_LAUNCH_SPLIT = 2, FAST_LAUNCH_TWO_STREAMS = 3 };
+int fdtd_hip_grid_set_launch_mode(fdtd_grid *g, int mode);
+int fdtd_hip_grid_get_launch_mode(const fdtd_grid *g, int *mode);
 int fdtd_hip_grid_flush(fdtd_grid *g);
 /* event timing of the FP32 fused launches since the last enable (one launch
  * advances the UPML and the interior tiles together by T steps):

@@ -113,6 +113,9 @@ void fdtd_solver_setPrecision(int precision);
 int fdtd_solver_getPrecision(void);
 /* FP32: steps per fused launch T and rows per wavefront R of the active / next solver */
 int fdtd_solver_setBlocking(int T, int R);
+/* FP32: launch shape of the active solver's UPML and interior tiles
+ * (FAST_LAUNCH_* of fdtd_hip.h); FDTD_ESTATE for FP64 solvers */
+int fdtd_solver_setLaunchMode(int mode);
 /* blocking of the active solver; FDTD_ESTATE when it does not queue steps (FP64) */
 int fdtd_solver_getBlocking(int *T, int *R);
 

@@ -1,9 +1,10 @@
-"""Autotuner of the FP32 fused kernels: temporal block depth T and rows per wavefront R.
+"""Autotuner of the FP32 fused kernels: temporal block depth T, rows per
+wavefront R and the launch shape of the UPML and interior tiles.
 
 For a grid class (mode, flavour, grid size rounded to 256 cells) it times a
-short run of each candidate (T, R) on a live simulation and keeps the fastest.
-Block depth and rows per wavefront change only how the steps are launched,
-not the arithmetic of any cell, so tuning on the live state is safe.  Results
+short run of each candidate (T, R) in each launch shape (merged, split, split
+on two streams) on a live simulation and keeps the fastest.  None of these changes the arithmetic of any cell, so tuning on
+the live state is safe.  Results
 are cached per process and, when MPIFDTD_TUNE_CACHE names a file, on disk.
 """
 import json
@@ -16,7 +17,7 @@ _cache = {}
 
 
 def _key(sim):
-    return f"{sim.solver}:{sim.N_PX // 256}x{sim.N_PY // 256}:{'ntff' if sim.per_step_ntff else 'plain'}"
+    return f"{sim.solver}:{sim.N_PX // 256}x{sim.N_PY // 256}:{'ntff' if sim.per_step_ntff else 'plain'}:v3"
 
 
 def _load():
@@ -47,23 +48,34 @@ def tune(m, make_sim, probe_steps=240, candidates=CANDIDATES):
         return _cache[key]
     L = m.lib()
     timings = {}
+    shapes = {}
+
+    def probe(name, T, R, shape):
+        sim.set_blocking(T, R)
+        sim.set_launch_mode(shape)
+        sim.calc(2 * T)  # warm the variant
+        sim.set_blocking(T, R)
+        L.fdtd_hip_synchronize()
+        t0 = time.perf_counter()
+        sim.calc(probe_steps)
+        sim.set_blocking(T, R)  # flush the partial block
+        L.fdtd_hip_synchronize()
+        timings[name] = (time.perf_counter() - t0) * 1e6 / probe_steps
+        shapes[name] = (T, R, shape)
+
     try:
         sim.calc(40)
+        # the per-step NTFF sampling runs merged only
+        shapes_tried = ((1, "merged"),) if sim.per_step_ntff else ((1, "merged"), (2, "split"), (3, "2streams"))
         for T, R in candidates:
-            sim.set_blocking(T, R)
-            sim.calc(2 * T)  # warm the variant
-            sim.set_blocking(T, R)
-            L.fdtd_hip_synchronize()
-            t0 = time.perf_counter()
-            sim.calc(probe_steps)
-            sim.set_blocking(T, R)  # flush the partial block
-            L.fdtd_hip_synchronize()
-            timings[f"T{T}R{R}"] = (time.perf_counter() - t0) * 1e6 / probe_steps
+            for shape, tag in shapes_tried:
+                probe(f"T{T}R{R}{tag}", T, R, shape)
+        sim.set_launch_mode(0)
     finally:
         sim.finish()
     best = min(timings, key=timings.get)
-    T, R = best[1:].split("R")
-    res = {"T": int(T), "R": int(R), "us_per_step": {k: round(v, 2) for k, v in timings.items()}}
+    T, R, shape = shapes[best]
+    res = {"T": T, "R": R, "launch": shape, "us_per_step": {k: round(v, 2) for k, v in timings.items()}}
     _cache[key] = res
     _save()
     del L

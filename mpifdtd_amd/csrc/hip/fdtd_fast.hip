@@ -873,6 +873,11 @@ struct FastState {
   fast_post_fn post = nullptr;
   void *post_ctx = nullptr;
   double times[FAST_MAX_T];
+  // launch shape (FAST_LAUNCH_*); the second stream and its events exist once the
+  // two-stream shape has been used
+  int lmode = 0;
+  hipStream_t s2 = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
 };
 
 // Event timing of the fused launches.  Events are recorded around every launch
@@ -1007,6 +1012,12 @@ int fast_state_create(const fdtd_grid_desc &d, FastState **out)
 void fast_state_destroy(FastState *s)
 {
   if (!s) return;
+  if (s->s2) {
+    (void)hipStreamSynchronize(s->s2);
+    (void)hipStreamDestroy(s->s2);
+    (void)hipEventDestroy(s->ev_in);
+    (void)hipEventDestroy(s->ev_out);
+  }
   for (void *p : s->allocs) (void)dev_free(p);
   if (s->dtiles) (void)dev_free(s->dtiles);
   if (s->csamp) (void)dev_free(s->csamp);
@@ -1238,14 +1249,32 @@ int fast_state_flush(FastState *s)
   memcpy(a.qb[0], s->qpml, sizeof a.qb[0]);
   memcpy(a.qb[1], s->qint, sizeof a.qb[1]);
   const int nbp = 8 * ((s->n_pml + 7) / 8), nbi = 8 * ((s->n_int + 7) / 8);
-  if (!s->csamp && s->R <= 8 && nbi >= 8 * nbp && nbi > 4096) {
-    // large grid with a small interior register budget (R <= 8: four workgroups per
-    // compute unit, against two at the UPML body's budget): interior and UPML
-    // tiles as two launches, each with its own occupancy.  At R = 16 the interior
-    // body already allows only two, and one launch lets the UPML tiles overlap it.
+  int mode = s->lmode;
+  if (mode == FAST_LAUNCH_AUTO)
+    // large grid: interior and UPML tiles as two launches, each at its own
+    // occupancy (the interior body's register budget is below the UPML body's)
+    mode = (nbi >= 8 * nbp && nbi > 4096) ? FAST_LAUNCH_SPLIT : FAST_LAUNCH_MERGED;
+  if (s->csamp) mode = FAST_LAUNCH_MERGED;  // the contour samples come from the merged instantiation
+  if (mode == FAST_LAUNCH_SPLIT) {
     a.nb_pml = 0;
     if (nbi > 0) fast_launch(s->d.mode, s->d.flavour, s->R, 1, a, nbi, 0);
     if (nbp > 0) fast_launch(s->d.mode, s->d.flavour, s->R, 2, a, nbp, 0);
+  } else if (mode == FAST_LAUNCH_TWO_STREAMS) {
+    // UPML tiles on a second stream beside the interior tiles: both read the same
+    // buffer and write disjoint cores, so the two launches may overlap, each at its
+    // own occupancy; events order them after the previous block and before the next
+    if (!s->s2) {
+      HIPCHK(hipStreamCreateWithFlags(&s->s2, hipStreamNonBlocking));
+      HIPCHK(hipEventCreateWithFlags(&s->ev_in, hipEventDisableTiming));
+      HIPCHK(hipEventCreateWithFlags(&s->ev_out, hipEventDisableTiming));
+    }
+    a.nb_pml = 0;
+    HIPCHK(hipEventRecord(s->ev_in, 0));
+    HIPCHK(hipStreamWaitEvent(s->s2, s->ev_in, 0));
+    if (nbp > 0) fast_launch(s->d.mode, s->d.flavour, s->R, 2, a, nbp, s->s2);
+    if (nbi > 0) fast_launch(s->d.mode, s->d.flavour, s->R, 1, a, nbi, 0);
+    HIPCHK(hipEventRecord(s->ev_out, s->s2));
+    HIPCHK(hipStreamWaitEvent(0, s->ev_out, 0));
   } else {
     a.nb_pml = nbp;
     if (nbp + nbi > 0) fast_launch(s->d.mode, s->d.flavour, s->R, s->csamp ? 3 : 0, a, nbp + nbi, 0);
@@ -1266,6 +1295,17 @@ int fast_state_flush(FastState *s)
   if (s->csamp && s->post) return s->post(s->post_ctx, s->csamp, n, s->times);
   return FDTD_OK;
 }
+
+int fast_state_set_launch_mode(FastState *s, int mode)
+{
+  if (mode < FAST_LAUNCH_AUTO || mode > FAST_LAUNCH_TWO_STREAMS) return fdtd_fail(FDTD_EINVAL, "bad launch mode %d", mode);
+  int rc = fast_state_flush(s);
+  if (rc) return rc;
+  s->lmode = mode;
+  return FDTD_OK;
+}
+
+int fast_state_get_launch_mode(const FastState *s) { return s->lmode; }
 
 int fast_state_step(FastState *s, const fdtd_step_params *p, int *launched)
 {

@@ -1275,6 +1275,19 @@ extern "C" int fdtd_hip_grid_get_blocking(const fdtd_grid *g, int *T, int *R)
   return FDTD_OK;
 }
 
+extern "C" int fdtd_hip_grid_set_launch_mode(fdtd_grid *g, int mode)
+{
+  if (!g || !g->fs) return fdtd_fail(FDTD_EINVAL, "launch modes apply to FP32 grids only");
+  return fast_state_set_launch_mode(g->fs, mode);
+}
+
+extern "C" int fdtd_hip_grid_get_launch_mode(const fdtd_grid *g, int *mode)
+{
+  if (!g || !g->fs || !mode) return fdtd_fail(FDTD_EINVAL, "launch modes apply to FP32 grids only");
+  *mode = fast_state_get_launch_mode(g->fs);
+  return FDTD_OK;
+}
+
 extern "C" int fdtd_hip_grid_flush(fdtd_grid *g)
 {
   if (!g) return fdtd_fail(FDTD_EINVAL, "null argument");

@@ -138,6 +138,8 @@ int fast_state_gather(FastState *s, const Contour &ct, int off, double2 *cE, dou
 typedef int (*fast_post_fn)(void *ctx, const float2 *samples, int n, const double *times);
 int fast_state_set_contour(FastState *s, const Contour &ct, int off, fast_post_fn post, void *ctx);
 int fast_state_set_blocking(FastState *s, int T, int R);
+int fast_state_set_launch_mode(FastState *s, int mode);
+int fast_state_get_launch_mode(const FastState *s);
 
 struct FastBatch;
 int fast_batch_create(FastState *const *st, const fdtd_step_params *p0, const int *steps, int n, int T, int R,

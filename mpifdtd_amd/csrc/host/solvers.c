@@ -71,6 +71,12 @@ int fdtd_solver_setBlocking(int T, int R)
   return FDTD_OK;
 }
 
+int fdtd_solver_setLaunchMode(int mode)
+{
+  if (active_ < 0 || !slv_[active_].blocked) return FDTD_ESTATE;
+  return fdtd_hip_grid_set_launch_mode(slv_[active_].g, mode);
+}
+
 int fdtd_solver_getBlocking(int *T, int *R)
 {
   if (active_ < 0 || !slv_[active_].blocked) return FDTD_ESTATE;

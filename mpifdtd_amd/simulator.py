@@ -81,6 +81,10 @@ class Simulator:
         """FP32: steps per fused launch and rows per wavefront of the interior tiles."""
         check(N.lib().fdtd_solver_setBlocking(int(T), int(R)), "fdtd_solver_setBlocking")
 
+    def set_launch_mode(self, mode):
+        """FP32: 0 auto, 1 merged, 2 split, 3 split on two streams (FAST_LAUNCH_* of fdtd_hip.h)."""
+        check(N.lib().fdtd_solver_setLaunchMode(int(mode)), "fdtd_solver_setLaunchMode")
+
     def blocking(self):
         """(T, R) of this FP32 solver."""
         t, r = ctypes.c_int(), ctypes.c_int()
