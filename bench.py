@@ -122,9 +122,10 @@ def cpu_baseline(mode, workload):
                       f"per-step NTFF ({dt:.1f} s on one host core; gcc -O2 oracle)"}
 
 
-def measured_traffic(mode, T, R, cells):
-    """HBM bytes per launch of this configuration from the committed rocprofv3
-    counter summaries (FETCH_SIZE / WRITE_SIZE scaled by a plain copy), or None."""
+def measured_traffic(mode, T, R, cells, launch):
+    """HBM bytes per launch of this configuration (launch shape included) from
+    the committed rocprofv3 counter summaries (FETCH_SIZE / WRITE_SIZE scaled by
+    a plain copy), or None."""
     import glob
     import re
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*traffic*.md"))):
@@ -133,7 +134,8 @@ def measured_traffic(mode, T, R, cells):
                 rec = json.loads(block)
             except ValueError:
                 continue
-            if (rec.get("mode"), rec.get("T"), rec.get("R"), rec.get("cells")) == (mode, T, R, cells):
+            if (rec.get("mode"), rec.get("T"), rec.get("R"), rec.get("cells"), rec.get("launch")) == \
+                    (mode, T, R, cells, launch):
                 return rec["traffic_bytes_per_launch"], os.path.relpath(path, ROOT)
     return None, None
 
@@ -266,6 +268,8 @@ def main():
     ff = sim.far_field()
     t_ff = time.perf_counter() - t1
     f32_0 = sim.fields()[0]  # TM Ez / TE Ex (a copy)
+    shape = sim.launch_mode() if args.precision == "fp32" else 0
+    shape_name = ["auto", "merged", "split", "two_streams"][shape]
     sim.finish()
     ms_step = dt * 1e3 / K
     mcups = cells * K / dt / 1e6
@@ -274,10 +278,11 @@ def main():
     ms_k, _, nlaunch, b_int, b_pml, cupd = prof
     if args.precision == "fp32" and nlaunch > 0 and ms_k > 0:
         achieved = (b_int + b_pml) / (ms_k * 1e-3) / 1e9
-        traffic, tsrc = measured_traffic(args.mode, T, R, cells)
+        traffic, tsrc = measured_traffic(args.mode, T, R, cells, shape_name)
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": tsrc,
-                "kernel": f"k_fast<{args.mode.upper()},MPI,R={R}>",
+                "kernel": f"k_fast<{args.mode.upper()},MPI,R={R}>" + (
+                    "" if shape_name == "merged" else f" interior + UPML launches ({shape_name.replace('_', ' ')})"),
                 "launch_avg_us": round(ms_k * 1e3 / nlaunch, 2), "steps_per_launch": T,
                 "bytes_per_launch": round((b_int + b_pml) / nlaunch),
                 "bytes_per_cell_update": round((b_int + b_pml) / cupd, 2)}
@@ -318,7 +323,7 @@ def main():
         "dtype": "f32" if args.precision == "fp32" else "f64", "data": "synthetic",
         "config": {"workload": WORKLOADS[wl][8].format(mode=args.mode.upper()), "cells": cells,
                    "precision": args.precision, "T": T, "R": R,
-                   "launch": ["auto", "merged", "split", "two_streams"][launch] if args.precision == "fp32" else None,
+                   "launch": shape_name if args.precision == "fp32" else None,
                    "per_step_ntff": args.ntff, "autotune": tuned},
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -170,6 +170,7 @@ int fdtd_hip_grid_get_blocking(const fdtd_grid *g, int *T, int *R);
  * streams.  The per-step NTFF sampling always uses MERGED. */
 enum { FAST_LAUNCH_AUTO = 0, FAST_LAUNCH_MERGED = 1, FAST_LAUNCH_SPLIT = 2, FAST_LAUNCH_TWO_STREAMS = 3 };
 int fdtd_hip_grid_set_launch_mode(fdtd_grid *g, int mode);
+/* the setting; with AUTO, the shape of the last launch (AUTO before the first) */
 int fdtd_hip_grid_get_launch_mode(const fdtd_grid *g, int *mode);
 int fdtd_hip_grid_flush(fdtd_grid *g);
 /* event timing of the FP32 fused launches since the last enable (one launch

@@ -116,6 +116,8 @@ int fdtd_solver_setBlocking(int T, int R);
 /* FP32: launch shape of the active solver's UPML and interior tiles
  * (FAST_LAUNCH_* of fdtd_hip.h); FDTD_ESTATE for FP64 solvers */
 int fdtd_solver_setLaunchMode(int mode);
+/* the setting, or with AUTO the shape of the last launch */
+int fdtd_solver_getLaunchMode(int *mode);
 /* blocking of the active solver; FDTD_ESTATE when it does not queue steps (FP64) */
 int fdtd_solver_getBlocking(int *T, int *R);
 

@@ -875,7 +875,7 @@ struct FastState {
   double times[FAST_MAX_T];
   // launch shape (FAST_LAUNCH_*); the second stream and its events exist once the
   // two-stream shape has been used
-  int lmode = 0;
+  int lmode = 0, lmode_last = 0;  // setting; shape of the last launch
   hipStream_t s2 = nullptr;
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
 };
@@ -1255,6 +1255,7 @@ int fast_state_flush(FastState *s)
     // occupancy (the interior body's register budget is below the UPML body's)
     mode = (nbi >= 8 * nbp && nbi > 4096) ? FAST_LAUNCH_SPLIT : FAST_LAUNCH_MERGED;
   if (s->csamp) mode = FAST_LAUNCH_MERGED;  // the contour samples come from the merged instantiation
+  s->lmode_last = mode;
   if (mode == FAST_LAUNCH_SPLIT) {
     a.nb_pml = 0;
     if (nbi > 0) fast_launch(s->d.mode, s->d.flavour, s->R, 1, a, nbi, 0);
@@ -1305,7 +1306,8 @@ int fast_state_set_launch_mode(FastState *s, int mode)
   return FDTD_OK;
 }
 
-int fast_state_get_launch_mode(const FastState *s) { return s->lmode; }
+// the setting, or with AUTO the shape of the last launch (AUTO before any)
+int fast_state_get_launch_mode(const FastState *s) { return s->lmode != FAST_LAUNCH_AUTO ? s->lmode : s->lmode_last; }
 
 int fast_state_step(FastState *s, const fdtd_step_params *p, int *launched)
 {

@@ -77,6 +77,12 @@ int fdtd_solver_setLaunchMode(int mode)
   return fdtd_hip_grid_set_launch_mode(slv_[active_].g, mode);
 }
 
+int fdtd_solver_getLaunchMode(int *mode)
+{
+  if (active_ < 0 || !slv_[active_].blocked) return FDTD_ESTATE;
+  return fdtd_hip_grid_get_launch_mode(slv_[active_].g, mode);
+}
+
 int fdtd_solver_getBlocking(int *T, int *R)
 {
   if (active_ < 0 || !slv_[active_].blocked) return FDTD_ESTATE;

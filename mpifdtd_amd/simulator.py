@@ -85,6 +85,12 @@ class Simulator:
         """FP32: 0 auto, 1 merged, 2 split, 3 split on two streams (FAST_LAUNCH_* of fdtd_hip.h)."""
         check(N.lib().fdtd_solver_setLaunchMode(int(mode)), "fdtd_solver_setLaunchMode")
 
+    def launch_mode(self):
+        """FP32: the launch shape setting, or with auto the shape of the last launch."""
+        v = ctypes.c_int()
+        check(N.lib().fdtd_solver_getLaunchMode(ctypes.byref(v)), "fdtd_solver_getLaunchMode")
+        return v.value
+
     def blocking(self):
         """(T, R) of this FP32 solver."""
         t, r = ctypes.c_int(), ctypes.c_int()
