@@ -867,8 +867,14 @@ struct FastState {
   double bytes_per_step = 0, cells_per_step = 0;
   double bytes_int_step = 0, bytes_pml_step = 0;  // algorithmic bytes of the two tile lists
   std::vector<FastTile> htp, hti;  // host copies of the UPML and interior tiles (batch builder)
-  // per-step NTFF samples (null when the contour is not sampled)
+  // per-step NTFF samples (null when the contour is not sampled): two buffers
+  // used in turn, so that a block's samples are consumed (on the consumer's
+  // stream, which records cfree) while the next block runs
   float2 *csamp = nullptr;
+  float2 *csampb[2] = {nullptr, nullptr};
+  hipEvent_t cfree[2] = {nullptr, nullptr};
+  bool cused[2] = {false, false};
+  int cbuf = 0;
   FastContour hct;
   fast_post_fn post = nullptr;
   void *post_ctx = nullptr;
@@ -1020,7 +1026,10 @@ void fast_state_destroy(FastState *s)
   }
   for (void *p : s->allocs) (void)dev_free(p);
   if (s->dtiles) (void)dev_free(s->dtiles);
-  if (s->csamp) (void)dev_free(s->csamp);
+  for (int b = 0; b < 2; b++) {
+    if (s->csampb[b]) (void)dev_free(s->csampb[b]);
+    if (s->cfree[b]) (void)hipEventDestroy(s->cfree[b]);
+  }
   delete s;
 }
 
@@ -1234,6 +1243,13 @@ int fast_state_flush(FastState *s)
   a.src = s->src_on;
   a.cur = s->cur;
   memcpy(a.st, s->hsteps, sizeof a.st);
+  const int sb = s->cbuf;
+  if (s->csamp) {
+    // this block's sample buffer was last read by the consumer two blocks ago
+    if (s->cused[sb]) HIPCHK(hipStreamWaitEvent(0, s->cfree[sb], 0));
+    a.csamp = s->csampb[sb];
+    a.ct = s->hct;
+  }
   ProfRec pr;
   if (g_prof.on) {
     if ((rc = prof_event(&pr.e0)) || (rc = prof_event(&pr.e1))) return rc;
@@ -1242,10 +1258,6 @@ int fast_state_flush(FastState *s)
   // one launch: the UPML tiles' workgroups first, then the interior tiles', each
   // split into 8 XCD partitions of ceil(n/8) workgroups
   a.tiles = s->dtiles;
-  if (s->csamp) {
-    a.csamp = s->csamp;
-    a.ct = s->hct;
-  }
   memcpy(a.qb[0], s->qpml, sizeof a.qb[0]);
   memcpy(a.qb[1], s->qint, sizeof a.qb[1]);
   const int nbp = 8 * ((s->n_pml + 7) / 8), nbi = 8 * ((s->n_int + 7) / 8);
@@ -1293,7 +1305,11 @@ int fast_state_flush(FastState *s)
   const int n = s->pending;
   s->pending = 0;
   s->T = Tsave;
-  if (s->csamp && s->post) return s->post(s->post_ctx, s->csamp, n, s->times);
+  if (s->csamp && s->post) {
+    s->cused[sb] = true;
+    s->cbuf ^= 1;
+    return s->post(s->post_ctx, s->csampb[sb], n, s->times, s->cfree[sb]);
+  }
   return FDTD_OK;
 }
 
@@ -1671,9 +1687,19 @@ int fast_state_set_contour(FastState *s, const Contour &ct, int off, fast_post_f
     if (nc == 2 && abs(fl[0] - fl[1]) < 2)
       return fdtd_fail(FDTD_EINVAL, "NTFF contour %d cells across: too small for the FP32 path", abs(fl[0] - fl[1]));
   }
-  if (s->csamp) (void)dev_free(s->csamp);
+  HIPCHK(hipDeviceSynchronize());
+  for (int b = 0; b < 2; b++) {
+    if (s->csampb[b]) (void)dev_free(s->csampb[b]);
+    s->csampb[b] = nullptr;
+    s->cused[b] = false;
+    if (!s->cfree[b]) HIPCHK(hipEventCreateWithFlags(&s->cfree[b], hipEventDisableTiming));
+  }
   s->csamp = nullptr;
-  if (ct.total > 0) HIPCHK(dev_malloc(&s->csamp, (size_t)FAST_MAX_T * 3 * ct.total * sizeof(float2)));
+  s->cbuf = 0;
+  if (ct.total > 0) {
+    for (int b = 0; b < 2; b++) HIPCHK(dev_malloc(&s->csampb[b], (size_t)FAST_MAX_T * 3 * ct.total * sizeof(float2)));
+    s->csamp = s->csampb[0];
+  }
   s->post = post;
   s->post_ctx = ctx;
   return FDTD_OK;

@@ -904,6 +904,10 @@ struct fdtd_grid {
   double *dftTs;         // [360][total] the binning's time shifts
   double2 *dftOut;       // [nf][3][360]
   int ctrSet;            // FP32: the fused kernel samples the contour
+  // FP32: stream of the sample packing, binning and running DFT, beside the next
+  // block's step kernel; ev_k orders it after the block that took the samples
+  hipStream_t ns;
+  hipEvent_t ev_k;
   double *cosA, *sinA, *cosB, *sinB;
   double coef;
   double2 *ffout;
@@ -1116,14 +1120,14 @@ static int build_bin_groups(fdtd_grid *g, const std::vector<double> &tab)
 }
 
 // n steps of samples in cE / cH ([cell][n]) into the accumulators
-static void launch_bin(fdtd_grid *g, int n, double time0)
+static void launch_bin(fdtd_grid *g, int n, double time0, hipStream_t st)
 {
   const fdtd_grid_desc &d = g->d;
   const long tb0 = (long)llround(time0) - 1;
   const int useCoef = d.flavour == FDTD_FLAVOUR_PLAIN ? 0 : 1;
   if (g->nparts <= 0) return;
 #define NTFF_BIN(NS)                                                                                              \
-  hipLaunchKernelGGL(k_ntff_bin<NS>, dim3(g->nparts), dim3(NTFF_GRP * NS), 0, 0, g->cE, g->cH, n, g->binPart, g->binRec, \
+  hipLaunchKernelGGL(k_ntff_bin<NS>, dim3(g->nparts), dim3(NTFF_GRP * NS), 0, st, g->cE, g->cH, n, g->binPart, g->binRec, \
                      g->nt[0], g->nt[1], g->nt[2], d.array_size, tb0, g->coef, useCoef)
   if (n <= 2)
     NTFF_BIN(2);
@@ -1137,10 +1141,10 @@ static void launch_bin(fdtd_grid *g, int n, double time0)
 }
 
 // n steps of samples in cE / cH ([cell][n]) into the running DFT
-static void launch_dft(fdtd_grid *g, int n, double time0)
+static void launch_dft(fdtd_grid *g, int n, double time0, hipStream_t st)
 {
   const long tb0 = (long)llround(time0) - 1;
-  hipLaunchKernelGGL(k_ctr_dft, dim3((g->ct.total + 255) / 256, g->nf), dim3(256), 0, 0, g->cE, g->cH, n, g->ct.total,
+  hipLaunchKernelGGL(k_ctr_dft, dim3((g->ct.total + 255) / 256, g->nf), dim3(256), 0, st, g->cE, g->cH, n, g->ct.total,
                      g->dftOmega, tb0, g->dftE, g->dftH);
 }
 
@@ -1160,18 +1164,35 @@ __global__ __launch_bounds__(256) void k_ctr_combine(const float2 *smp, Contour 
   cH[(size_t)p * n + s] = make_double2(hs * ((double)h0.x + (double)h1.x), hs * ((double)h0.y + (double)h1.y));
 }
 
-// after a launch of n steps of an FP32 grid: bin every step's samples at its time
-static int grid_bin_samples(void *ctx, const float2 *samples, int n, const double *times)
+// after a launch of n steps of an FP32 grid: pack the samples, bin every step at
+// its time and add them to the running DFT, on the grid's NTFF stream (the
+// next block's step kernel runs meanwhile)
+static int grid_bin_samples(void *ctx, const float2 *samples, int n, const double *times, hipEvent_t consumed)
 {
   fdtd_grid *g = (fdtd_grid *)ctx;
   if (n > g->nbuf) return fdtd_fail(FDTD_EINVAL, "%d steps exceed the %d-step sample buffers", n, g->nbuf);
   for (int s = 1; s < n; s++)
     if (times[s] != times[0] + s) return fdtd_fail(FDTD_ESTATE, "steps of a block are not consecutive");
-  hipLaunchKernelGGL(k_ctr_combine, dim3((g->ct.total + 255) / 256, n), dim3(256), 0, 0, samples, g->ct, n, g->cE,
+  if (!g->ns) {
+    HIPCHK(hipStreamCreateWithFlags(&g->ns, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&g->ev_k, hipEventDisableTiming));
+  }
+  HIPCHK(hipEventRecord(g->ev_k, 0));
+  HIPCHK(hipStreamWaitEvent(g->ns, g->ev_k, 0));
+  hipLaunchKernelGGL(k_ctr_combine, dim3((g->ct.total + 255) / 256, n), dim3(256), 0, g->ns, samples, g->ct, n, g->cE,
                      g->cH);
-  if (g->ntn) launch_bin(g, n, times[0]);
-  if (g->nf) launch_dft(g, n, times[0]);
+  HIPCHK(hipEventRecord(consumed, g->ns));
+  if (g->ntn) launch_bin(g, n, times[0], g->ns);
+  if (g->nf) launch_dft(g, n, times[0], g->ns);
   HIPCHK(hipGetLastError());
+  return FDTD_OK;
+}
+
+// the NTFF stream's queued work done (before the host or the null stream touches
+// the accumulators, the DFT sums or the sample buffers)
+static int ntff_sync(fdtd_grid *g)
+{
+  if (g->ns) HIPCHK(hipStreamSynchronize(g->ns));
   return FDTD_OK;
 }
 
@@ -1302,6 +1323,12 @@ extern "C" void fdtd_hip_profile_read(double out[6]) { fast_profile_read6(out); 
 extern "C" int fdtd_hip_grid_destroy(fdtd_grid *g)
 {
   if (!g) return FDTD_OK;
+  (void)hipDeviceSynchronize();
+  if (g->ns) {
+    (void)hipStreamDestroy(g->ns);
+    (void)hipEventDestroy(g->ev_k);
+    g->ns = nullptr;
+  }
   for (int f = 0; f < 9; f++) (void)dev_free(g->F[f]);
   (void)dev_free(g->Fdev);
   for (int e = 0; e < 3; e++) (void)dev_free(g->eps[e]);
@@ -1450,8 +1477,8 @@ extern "C" int fdtd_hip_grid_step(fdtd_grid *g, const fdtd_step_params *p)
   if ((g->ntn || g->nf) && g->ct.total > 0) {
     hipLaunchKernelGGL(k_ntff_gather, dim3((g->ct.total + 255) / 256), dim3(256), 0, 0, (const double2 *const *)g->Fdev,
                        g->ct, g->cE, g->cH);
-    if (g->ntn) launch_bin(g, 1, p->time);
-    if (g->nf) launch_dft(g, 1, p->time);
+    if (g->ntn) launch_bin(g, 1, p->time, 0);
+    if (g->nf) launch_dft(g, 1, p->time, 0);
     HIPCHK(hipGetLastError());
   }
   return FDTD_OK;
@@ -1496,6 +1523,7 @@ extern "C" int fdtd_hip_grid_get_ntff_time(fdtd_grid *g, int which, double *host
   if (!g->ntn) return fdtd_fail(FDTD_ESTATE, "the grid was created without the per-step time-domain NTFF");
   int rc;
   if (g->fs && (rc = fast_state_flush(g->fs))) return rc;  // bins the queued steps
+  if ((rc = ntff_sync(g))) return rc;
   HIPCHK(hipMemcpy(host, g->nt[which], (size_t)360 * g->d.array_size * sizeof(double2), hipMemcpyDeviceToHost));
   return FDTD_OK;
 }
@@ -1503,6 +1531,8 @@ extern "C" int fdtd_hip_grid_get_ntff_time(fdtd_grid *g, int which, double *host
 extern "C" int fdtd_hip_grid_reset(fdtd_grid *g)
 {
   if (!g) return fdtd_fail(FDTD_EINVAL, "null argument");
+  int rc0 = ntff_sync(g);
+  if (rc0) return rc0;
   for (int f = 0; f < 9; f++)
     if (g->F[f]) HIPCHK(hipMemset(g->F[f], 0, g->ncell * sizeof(double2)));
   for (int a = 0; a < 3 && g->ntn; a++) HIPCHK(hipMemset(g->ntb[a], 0, g->ntn * sizeof(double2)));
@@ -1549,6 +1579,7 @@ extern "C" int fdtd_hip_grid_time_spectra(fdtd_grid *g, double coef_re, double c
   if (!g->ntn) return fdtd_fail(FDTD_ESTATE, "the grid was created without the per-step time-domain NTFF");
   int rc;
   if (g->fs && (rc = fast_state_flush(g->fs))) return rc;  // bins the queued steps
+  if ((rc = ntff_sync(g))) return rc;
   const fdtd_grid_desc &d = g->d;
   HIPCHK(hipMemcpy(g->tw, twiddles, 4096 * sizeof(double2), hipMemcpyHostToDevice));
   const size_t lds = NTFF_NUM * sizeof(double2);
@@ -1606,6 +1637,7 @@ extern "C" int fdtd_hip_grid_get_dft(fdtd_grid *g, double *out)
   if (!g->nf) return fdtd_fail(FDTD_ESTATE, "no running DFT on this grid (fdtd_hip_grid_set_dft)");
   int rc;
   if (g->fs && (rc = fast_state_flush(g->fs))) return rc;
+  if ((rc = ntff_sync(g))) return rc;
   const double cf = g->d.flavour == FDTD_FLAVOUR_PLAIN ? 1.0 : g->coef;
   hipLaunchKernelGGL(k_dft_far, dim3(360, g->nf), dim3(256), 0, 0, g->dftE, g->dftH, g->ct, g->dftTs, g->dftOmega, cf,
                      g->dftOut);

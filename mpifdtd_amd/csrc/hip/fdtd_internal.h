@@ -135,7 +135,10 @@ int fast_state_gather(FastState *s, const Contour &ct, int off, double2 *cE, dou
 // Per-step contour sampling inside the fused kernel: after every launch of n
 // steps, post(ctx, samples, n, times) receives the device buffer
 // [n][3][total] float2 and the time of each step.
-typedef int (*fast_post_fn)(void *ctx, const float2 *samples, int n, const double *times);
+// after each block of n steps with contour samples: the consumer queues its work
+// on its own stream after the block, and records `consumed` there once it has
+// read `samples` (the buffer is written again two blocks later)
+typedef int (*fast_post_fn)(void *ctx, const float2 *samples, int n, const double *times, hipEvent_t consumed);
 int fast_state_set_contour(FastState *s, const Contour &ct, int off, fast_post_fn post, void *ctx);
 int fast_state_set_blocking(FastState *s, int T, int R);
 int fast_state_set_launch_mode(FastState *s, int mode);
