@@ -668,19 +668,22 @@ __global__ __launch_bounds__(256) void k_fast(FastArgs a)
 // each queue ordered by decreasing instance size, until both are empty; every
 // workgroup then exits.  One launch advances every unfinished instance by its
 // own depth; k_batch_advance between launches moves the progress records.
-template <int MODE, int R>
+// KQ: the queues pulled, 0 UPML tiles, 1 interior tiles, 2 both (UPML first).
+// A one-queue instantiation carries one tile body only, so it runs at that
+// body's register budget.
+template <int MODE, int R, int KQ>
 __global__ __launch_bounds__(256) void k_fast_batch(FastArgs a)
 {
   __shared__ int s_tid;
-  for (int k = 0; k < 2; k++) {
+  for (int k = (KQ == 1 ? 1 : 0); k < (KQ == 0 ? 1 : 2); k++) {
     for (;;) {
       __syncthreads();  // every thread has read s_tid of the previous pull
       if (threadIdx.x == 0) s_tid = atomicAdd(&a.qctr[k], 1);
       __syncthreads();
       const int tid = s_tid;
       if (tid >= a.qn[k]) break;
-      if (k == 0) fast_tile<MODE, FDTD_FLAVOUR_MPI, 8, 1, 1, 0>(a, a.qoff[0] + tid);
-      else fast_tile<MODE, FDTD_FLAVOUR_MPI, R, 0, 1, 0>(a, a.qoff[1] + tid);
+      if (KQ != 1 && k == 0) fast_tile<MODE, FDTD_FLAVOUR_MPI, 8, 1, 1, 0>(a, a.qoff[0] + tid);
+      else if (KQ != 0) fast_tile<MODE, FDTD_FLAVOUR_MPI, R, 0, 1, 0>(a, a.qoff[1] + tid);
     }
   }
 }
@@ -1424,9 +1427,11 @@ int fast_state_gather(FastState *s, const Contour &ct, int off, double2 *cE, dou
 
 // ---------------------------------------------------------------- batched instances
 //
-// Several FP32 MPI-flavour instances of uneven size advance together: one
-// k_fast_batch launch per T-step block and mode pulls the UPML and then the
-// interior tiles of every unfinished instance from device queues, and
+// Several FP32 MPI-flavour instances of uneven size advance together: per T-step
+// block and mode, one k_fast_batch launch pulls the interior tiles of every
+// unfinished instance from a device queue and a second the UPML tiles (each at
+// its own register budget: two workgroups per compute unit for the interior
+// body at R = 16, one for the UPML body), and
 // k_batch_advance moves the per-instance progress records on the device.  No
 // host value changes between blocks, so the loop of blocks is captured once as
 // a hipGraph and replayed.
@@ -1442,19 +1447,26 @@ struct FastBatch {
   int *dqctr = nullptr;             // [2 modes][2 queues]
   int nt[2][2] = {{0, 0}, {0, 0}};  // tiles per mode and queue
   int off[2][2] = {{0, 0}, {0, 0}};
-  int nblk[2] = {0, 0};
+  int nblk[2][2] = {{0, 0}, {0, 0}};  // workgroups per mode and queue
   hipStream_t stream = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   int glevels = 0;
 };
 
-template <int MODE>
-static void batch_launch_mode(const FastArgs &a, int R, int nblk, hipStream_t st)
+template <int MODE, int KQ>
+static void batch_launch_q(const FastArgs &a, int R, int nblk, hipStream_t st)
 {
-  if (R == 4) hipLaunchKernelGGL((k_fast_batch<MODE, 4>), dim3(nblk), dim3(256), 0, st, a);
-  else if (R == 8) hipLaunchKernelGGL((k_fast_batch<MODE, 8>), dim3(nblk), dim3(256), 0, st, a);
-  else hipLaunchKernelGGL((k_fast_batch<MODE, 16>), dim3(nblk), dim3(256), 0, st, a);
+  if (R == 4) hipLaunchKernelGGL((k_fast_batch<MODE, 4, KQ>), dim3(nblk), dim3(256), 0, st, a);
+  else if (R == 8) hipLaunchKernelGGL((k_fast_batch<MODE, 8, KQ>), dim3(nblk), dim3(256), 0, st, a);
+  else hipLaunchKernelGGL((k_fast_batch<MODE, 16, KQ>), dim3(nblk), dim3(256), 0, st, a);
+}
+
+template <int MODE>
+static void batch_launch_mode(const FastArgs &a, int R, const int nblk[2], hipStream_t st)
+{
+  if (nblk[1] > 0) batch_launch_q<MODE, 1>(a, R, nblk[1], st);
+  if (nblk[0] > 0) batch_launch_q<MODE, 0>(a, R, nblk[0], st);
 }
 
 static FastArgs batch_args(const FastBatch *b, int m)
@@ -1478,7 +1490,7 @@ static FastArgs batch_args(const FastBatch *b, int m)
 static int batch_level(FastBatch *b)
 {
   for (int m = 0; m < 2; m++) {
-    if (b->nblk[m] == 0) continue;
+    if (b->nblk[m][0] + b->nblk[m][1] == 0) continue;
     const FastArgs a = batch_args(b, m);
     if (m == FDTD_MODE_TM) batch_launch_mode<FDTD_MODE_TM>(a, b->R, b->nblk[m], b->stream);
     else batch_launch_mode<FDTD_MODE_TE>(a, b->R, b->nblk[m], b->stream);
@@ -1569,9 +1581,10 @@ int fast_batch_create(FastState *const *st, const fdtd_step_params *p0, const in
       }
       b->nt[m][k] = (int)all.size() - b->off[m][k];
     }
-    // resident workgroups: 2 per compute unit at the UPML body's register budget
-    const int ntile = b->nt[m][0] + b->nt[m][1];
-    b->nblk[m] = std::min(ntile, 512);
+    // resident workgroups: one per compute unit at the UPML body's register
+    // budget, two at the interior body's
+    b->nblk[m][0] = std::min(b->nt[m][0], 256);
+    b->nblk[m][1] = std::min(b->nt[m][1], 512);
   }
   HIPCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
   HIPCHK(dev_malloc(&b->dinst, n * sizeof(FastInst)));

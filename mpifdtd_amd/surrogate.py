@@ -258,10 +258,17 @@ def generate(spec, T=8, R=16, chunk=128, log=None, grid=None):
 def predict_with_fallback(ens, spec, scale, threshold=0.02, fdtd=None):
     """Ensemble prediction for each query; where the members' spread (std over
     the angles, relative to the mean's norm) exceeds `threshold`, the query is
-    answered by FDTD instead.  Returns the reflectances [n, 180] and the mask of
+    answered by FDTD instead.  `scale` is a Targets, or the float that linear
+    targets were divided by.  Returns the reflectances [n, 180] and the mask of
     queries that went to FDTD."""
-    mean, std = ens.predict(features(spec))
-    mean, std = mean * scale, std * scale
+    tg = scale if isinstance(scale, Targets) else None
+    X = features(spec)
+    if tg is None:
+        mean, std = ens.predict(X)
+        mean, std = mean * scale, std * scale
+    else:
+        P = np.stack([tg.decode(net.forward(X)) for net in ens.nets])
+        mean, std = P.mean(0), P.std(0)
     rel = np.linalg.norm(std, axis=1) / np.maximum(np.linalg.norm(mean, axis=1), 1e-300)
     use = rel > threshold
     out = mean.copy()
@@ -284,8 +291,26 @@ def nearest_neighbour(Xtr, Ytr, Xq):
     return Ytr[np.argmin(d2, axis=1)]
 
 
+class Targets:
+    """The network's view of the reflectances: linear (R / scale) or log
+    (log(R / scale + floor), which turns a relative error into an additive one)."""
+
+    def __init__(self, Y, kind="linear"):
+        self.kind = kind
+        self.scale = float(np.mean(Y))
+        self.floor = 1e-4
+
+    def encode(self, Y):
+        Z = Y / self.scale
+        return (np.log(Z + self.floor) if self.kind == "log" else Z).astype(np.float32)
+
+    def decode(self, Z):
+        Z = np.asarray(Z, np.float64)
+        return (np.exp(Z) - self.floor if self.kind == "log" else Z) * self.scale
+
+
 def experiment(n_train=20000, n_test=2000, epochs=200, k=4, seed=1, threshold=0.02, log=print, save=None,
-               load=None, lr=1e-3, lr_end=None):
+               load=None, lr=1e-3, lr_end=None, target="linear"):
     """Generate, train, evaluate: the numbers of the surrogate's acceptance
     (held-out error, fallback rate, error after fallback, throughput against
     the batched FDTD), as a dict.  `save` writes the generated spectra to an
@@ -310,14 +335,14 @@ def experiment(n_train=20000, n_test=2000, epochs=200, k=4, seed=1, threshold=0.
             np.savez_compressed(save, Xtr=Xtr, Ytr=Ytr.astype(np.float32), Xte=Xte, Yte=Yte.astype(np.float32),
                                 t_gen=t_gen)
     fdtd_rate = (n_train + n_test) / t_gen
-    scale = float(Ytr.mean())
+    tg = Targets(Ytr, target)
     ens = Ensemble([N_FEATURES, 512, 512, 512, N_ANGLES], k=k, batch=1024, seed=seed)
     t0 = time.perf_counter()
-    hist = ens.fit(Xtr, (Ytr / scale).astype(np.float32), epochs=epochs, lr=lr, lr_end=lr_end, log=log)
+    hist = ens.fit(Xtr, tg.encode(Ytr), epochs=epochs, lr=lr, lr_end=lr_end, log=log)
     t_train = time.perf_counter() - t0
-    mean, std = ens.predict(Xte)
-    pred = mean * scale
-    spread = np.linalg.norm(std, axis=1) / np.maximum(np.linalg.norm(mean, axis=1), 1e-300)
+    P = np.stack([tg.decode(net.forward(Xte)) for net in ens.nets])  # members in reflectance units
+    pred = P.mean(0)
+    spread = np.linalg.norm(P.std(0), axis=1) / np.maximum(np.linalg.norm(pred, axis=1), 1e-300)
     err = rel_err(pred, Yte)
     use = spread > threshold
     err_fb = np.where(use, 0.0, err)  # the flagged queries are answered by the FDTD itself
@@ -338,6 +363,7 @@ def experiment(n_train=20000, n_test=2000, epochs=200, k=4, seed=1, threshold=0.
     sur_rate = reps * 65536 / (time.perf_counter() - t0)
     return {
         "n_train": n_train, "n_test": n_test, "epochs": epochs, "ensemble": k, "lr": lr, "lr_end": lr_end,
+        "target": target,
         "grid": GRID, "fdtd_spectra_per_s": fdtd_rate, "fdtd_seconds": t_gen,
         "train_seconds": t_train, "final_train_mse": hist[-1],
         "heldout_rel_err_mean": float(err.mean()), "heldout_rel_err_median": float(np.median(err)),
@@ -363,9 +389,10 @@ if __name__ == "__main__":
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--lr-end", type=float, default=None, help="cosine-anneal the rate to this value")
+    ap.add_argument("--target", default="linear", choices=("linear", "log"))
     ap.add_argument("--save", default=None, help="write the generated spectra to this .npz")
     ap.add_argument("--load", default=None, help="read the spectra from this .npz instead of running the FDTD")
     a = ap.parse_args()
     res = experiment(a.train, a.test, a.epochs, a.k, a.seed, log=lambda s: print(s, flush=True), save=a.save,
-                     load=a.load, lr=a.lr, lr_end=a.lr_end)
+                     load=a.load, lr=a.lr, lr_end=a.lr_end, target=a.target)
     print(json.dumps(res), flush=True)
