@@ -156,6 +156,8 @@ def run_sweep(args, m):
     log(f"sweep64: {len(spec)} instances, {sum(cells) / 1e6:.1f} M cells, {upd / 1e9:.1f} G cell-updates, T={T} R={R}")
     t0 = time.perf_counter()
     b = B.Batch(items, T=T, R=R)
+    if args.launch >= 0:
+        b.set_launch_mode(args.launch)
     m.lib().fdtd_hip_synchronize()
     t_init = time.perf_counter() - t0
     log(f"created in {t_init:.1f} s")
@@ -190,7 +192,8 @@ def run_sweep(args, m):
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"batched sweep of 64 seeded instances (seed {args.seed}): multilayer/zigzag, "
                                "sides 512-2048 + 16-cell UPML, TM/TE, lambda 380-780 nm, 6000-10000 steps",
-                   "cells": sum(cells), "cell_updates": upd, "T": T, "R": R, "create_s": round(t_init, 2)},
+                   "cells": sum(cells), "cell_updates": upd, "T": T, "R": R,
+                   "launch": "two_streams" if args.launch == 3 else "split", "create_s": round(t_init, 2)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "note": "algorithmic bytes (interior-cell 52/56 B) of the whole sweep over its wall time"},

@@ -197,6 +197,10 @@ int fdtd_hip_batch_create(fdtd_grid *const *grids, const fdtd_step_params *p0, c
 /* blocks >= 0: queue that many blocks (asynchronous); blocks < 0: run every
  * instance to its step count and synchronise */
 int fdtd_hip_batch_run(fdtd_hip_batch *b, int blocks);
+/* the interior tiles of a block are launched, then the UPML tiles
+ * (FAST_LAUNCH_SPLIT, the default; AUTO and MERGED mean the same here), or
+ * the two side by side on two streams (FAST_LAUNCH_TWO_STREAMS) */
+int fdtd_hip_batch_set_launch_mode(fdtd_hip_batch *b, int mode);
 int fdtd_hip_batch_sync(fdtd_hip_batch *b, int *steps_done);
 int fdtd_hip_batch_destroy(fdtd_hip_batch *b);
 

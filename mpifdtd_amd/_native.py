@@ -153,6 +153,7 @@ def lib():
     sig("fdtd_batch_run", I, P)
     sig("fdtd_batch_run_blocks", I, P, I)
     sig("fdtd_batch_sync", I, P, ip)
+    sig("fdtd_batch_set_launch_mode", I, P, I)
     sig("fdtd_batch_size", I, P)
     sig("fdtd_batch_dims", I, P, I, ip)
     sig("fdtd_batch_field", I, P, I, I, dp)

@@ -43,6 +43,10 @@ class Batch:
         self.n = len(items)
         self.T, self.R = T, R
 
+    def set_launch_mode(self, mode):
+        """2 (default): interior tiles then UPML tiles; 3: the two side by side on two streams."""
+        check(N.lib().fdtd_batch_set_launch_mode(self._h, int(mode)), "fdtd_batch_set_launch_mode")
+
     def run(self):
         """Every instance to its step count (synchronised)."""
         check(N.lib().fdtd_batch_run(self._h), "fdtd_batch_run")

@@ -1449,6 +1449,11 @@ struct FastBatch {
   int off[2][2] = {{0, 0}, {0, 0}};
   int nblk[2][2] = {{0, 0}, {0, 0}};  // workgroups per mode and queue
   hipStream_t stream = nullptr;
+  // two_streams: the UPML launches of a block on s2 beside the interior launches
+  // (fork and join by events, captured into the graph as dependencies)
+  int two_streams = 0;
+  hipStream_t s2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   int glevels = 0;
@@ -1463,10 +1468,10 @@ static void batch_launch_q(const FastArgs &a, int R, int nblk, hipStream_t st)
 }
 
 template <int MODE>
-static void batch_launch_mode(const FastArgs &a, int R, const int nblk[2], hipStream_t st)
+static void batch_launch_mode(const FastArgs &a, int R, const int nblk[2], hipStream_t st, hipStream_t st_pml)
 {
   if (nblk[1] > 0) batch_launch_q<MODE, 1>(a, R, nblk[1], st);
-  if (nblk[0] > 0) batch_launch_q<MODE, 0>(a, R, nblk[0], st);
+  if (nblk[0] > 0) batch_launch_q<MODE, 0>(a, R, nblk[0], st_pml);
 }
 
 static FastArgs batch_args(const FastBatch *b, int m)
@@ -1489,11 +1494,21 @@ static FastArgs batch_args(const FastBatch *b, int m)
 // one block of T steps for every unfinished instance, then the progress update
 static int batch_level(FastBatch *b)
 {
+  hipStream_t sp = b->stream;
+  if (b->two_streams) {
+    HIPCHK(hipEventRecord(b->ev_fork, b->stream));
+    HIPCHK(hipStreamWaitEvent(b->s2, b->ev_fork, 0));
+    sp = b->s2;
+  }
   for (int m = 0; m < 2; m++) {
     if (b->nblk[m][0] + b->nblk[m][1] == 0) continue;
     const FastArgs a = batch_args(b, m);
-    if (m == FDTD_MODE_TM) batch_launch_mode<FDTD_MODE_TM>(a, b->R, b->nblk[m], b->stream);
-    else batch_launch_mode<FDTD_MODE_TE>(a, b->R, b->nblk[m], b->stream);
+    if (m == FDTD_MODE_TM) batch_launch_mode<FDTD_MODE_TM>(a, b->R, b->nblk[m], b->stream, sp);
+    else batch_launch_mode<FDTD_MODE_TE>(a, b->R, b->nblk[m], b->stream, sp);
+  }
+  if (b->two_streams) {
+    HIPCHK(hipEventRecord(b->ev_join, b->s2));
+    HIPCHK(hipStreamWaitEvent(b->stream, b->ev_join, 0));
   }
   hipLaunchKernelGGL(k_batch_advance, dim3((std::max(b->n, 4) + 255) / 256), dim3(256), 0, b->stream, b->dbinst, b->n,
                      b->T, b->dqctr, 4, 0);
@@ -1512,6 +1527,12 @@ void fast_batch_destroy(FastBatch *b)
   if (b->dtiles) (void)dev_free(b->dtiles);
   if (b->dqctr) (void)dev_free(b->dqctr);
   if (b->stream) (void)hipStreamDestroy(b->stream);
+  if (b->s2) {
+    (void)hipStreamSynchronize(b->s2);
+    (void)hipStreamDestroy(b->s2);
+    (void)hipEventDestroy(b->ev_fork);
+    (void)hipEventDestroy(b->ev_join);
+  }
   delete b;
 }
 
@@ -1587,6 +1608,9 @@ int fast_batch_create(FastState *const *st, const fdtd_step_params *p0, const in
     b->nblk[m][1] = std::min(b->nt[m][1], 512);
   }
   HIPCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  HIPCHK(hipStreamCreateWithFlags(&b->s2, hipStreamNonBlocking));
+  HIPCHK(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming));
   HIPCHK(dev_malloc(&b->dinst, n * sizeof(FastInst)));
   HIPCHK(dev_malloc(&b->dbinst, n * sizeof(FastBatchInst)));
   HIPCHK(dev_malloc(&b->dtiles, std::max<size_t>(1, all.size()) * sizeof(FastTile)));
@@ -1626,6 +1650,19 @@ int fast_batch_run(FastBatch *b, int blocks, int use_graph)
     for (; done + b->glevels <= blocks; done += b->glevels) HIPCHK(hipGraphLaunch(b->gexec, b->stream));
   for (; done < blocks; done++)
     if ((rc = batch_level(b))) return rc;
+  return FDTD_OK;
+}
+
+// the UPML launches beside the interior launches on a second stream (1) or after them (0);
+// applies to blocks queued afterwards (the captured graph is rebuilt)
+int fast_batch_set_two_streams(FastBatch *b, int on)
+{
+  HIPCHK(hipStreamSynchronize(b->stream));
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->gexec = nullptr;
+  b->graph = nullptr;
+  b->two_streams = on ? 1 : 0;
   return FDTD_OK;
 }
 

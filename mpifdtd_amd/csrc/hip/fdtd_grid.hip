@@ -1703,6 +1703,12 @@ extern "C" int fdtd_hip_batch_run(fdtd_hip_batch *b, int blocks)
   return fast_batch_run(b->fb, blocks, 1);
 }
 
+extern "C" int fdtd_hip_batch_set_launch_mode(fdtd_hip_batch *b, int mode)
+{
+  if (!b || mode < FAST_LAUNCH_AUTO || mode > FAST_LAUNCH_TWO_STREAMS) return fdtd_fail(FDTD_EINVAL, "bad argument");
+  return fast_batch_set_two_streams(b->fb, mode == FAST_LAUNCH_TWO_STREAMS);
+}
+
 extern "C" int fdtd_hip_batch_sync(fdtd_hip_batch *b, int *steps_done)
 {
   if (!b) return fdtd_fail(FDTD_EINVAL, "null batch");

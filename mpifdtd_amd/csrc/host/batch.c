@@ -99,6 +99,10 @@ int fdtd_batch_run_blocks(fdtd_batch *b, int blocks)
 }
 int fdtd_batch_sync(fdtd_batch *b, int *steps_done) { return b ? fdtd_hip_batch_sync(b->hb, steps_done) : FDTD_EINVAL; }
 int fdtd_batch_size(const fdtd_batch *b) { return b ? b->n : 0; }
+int fdtd_batch_set_launch_mode(fdtd_batch *b, int mode)
+{
+  return b ? fdtd_hip_batch_set_launch_mode(b->hb, mode) : FDTD_EINVAL;
+}
 
 int fdtd_batch_dims(const fdtd_batch *b, int i, int out[4])
 {
