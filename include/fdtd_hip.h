@@ -167,7 +167,7 @@ int fdtd_hip_grid_get_blocking(const fdtd_grid *g, int *T, int *R);
  * MERGED; MERGED is one launch of both
  * (one register budget, the UPML body's); SPLIT is two launches in a row, each
  * at its own occupancy; TWO_STREAMS runs the two launches side by side on two
- * streams.  The per-step NTFF sampling always uses MERGED. */
+ * streams. */
 enum { FAST_LAUNCH_AUTO = 0, FAST_LAUNCH_MERGED = 1, FAST_LAUNCH_SPLIT = 2, FAST_LAUNCH_TWO_STREAMS = 3 };
 int fdtd_hip_grid_set_launch_mode(fdtd_grid *g, int mode);
 /* the setting; with AUTO, the shape of the last launch (AUTO before the first) */

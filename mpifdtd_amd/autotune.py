@@ -17,7 +17,7 @@ _cache = {}
 
 
 def _key(sim):
-    return f"{sim.solver}:{sim.N_PX // 256}x{sim.N_PY // 256}:{'ntff' if sim.per_step_ntff else 'plain'}:v3"
+    return f"{sim.solver}:{sim.N_PX // 256}x{sim.N_PY // 256}:{'ntff' if sim.per_step_ntff else 'plain'}:v4"
 
 
 def _load():
@@ -65,8 +65,7 @@ def tune(m, make_sim, probe_steps=240, candidates=CANDIDATES):
 
     try:
         sim.calc(40)
-        # the per-step NTFF sampling runs merged only
-        shapes_tried = ((1, "merged"),) if sim.per_step_ntff else ((1, "merged"), (2, "split"), (3, "2streams"))
+        shapes_tried = ((1, "merged"), (2, "split"), (3, "2streams"))
         for T, R in candidates:
             for shape, tag in shapes_tried:
                 probe(f"T{T}R{R}{tag}", T, R, shape)

@@ -657,6 +657,10 @@ __global__ __launch_bounds__(256) void k_fast(FastArgs a)
     fast_tiles<MODE, FLAV, R, 0>(a, a.qb[1], blockIdx.x, gridDim.x);
   } else if (KIND == 2) {
     fast_tiles<MODE, FLAV, 8, 1>(a, a.qb[0], blockIdx.x, gridDim.x);
+  } else if (KIND == 4) {  // KIND 1 with the per-step NTFF samples
+    fast_tiles<MODE, FLAV, R, 0, 1>(a, a.qb[1], blockIdx.x, gridDim.x);
+  } else if (KIND == 5) {  // KIND 2 with the per-step NTFF samples
+    fast_tiles<MODE, FLAV, 8, 1, 1>(a, a.qb[0], blockIdx.x, gridDim.x);
   } else {
     if ((int)blockIdx.x < a.nb_pml) fast_tiles<MODE, FLAV, 8, 1>(a, a.qb[0], blockIdx.x, a.nb_pml);
     else fast_tiles<MODE, FLAV, R, 0>(a, a.qb[1], (int)blockIdx.x - a.nb_pml, (int)gridDim.x - a.nb_pml);
@@ -788,6 +792,12 @@ template <int MODE, int FLAV>
 static fast_launcher pick(int R, int kind)
 {
   if (kind == 2) return launch_one<MODE, FLAV, 8, 2>;
+  if (kind == 5) return launch_one<MODE, FLAV, 8, 5>;
+  if (kind == 4) {
+    if (R == 4) return launch_one<MODE, FLAV, 4, 4>;
+    if (R == 8) return launch_one<MODE, FLAV, 8, 4>;
+    return launch_one<MODE, FLAV, 16, 4>;
+  }
   if (kind == 3) {
     if (R == 4) return launch_one<MODE, FLAV, 4, 3>;
     if (R == 8) return launch_one<MODE, FLAV, 8, 3>;
@@ -804,7 +814,7 @@ static fast_launcher pick(int R, int kind)
 }
 
 // kind 0 (3 with NTFF samples): nblocks = a.nb_pml UPML workgroups + the interior workgroups;
-// kind 1: interior workgroups only; kind 2: UPML workgroups only
+// kind 1 (4 with samples): interior workgroups only; kind 2 (5 with samples): UPML workgroups only
 int fast_launch(int mode, int flavour, int R, int kind, const FastArgs &a, int nblocks, hipStream_t st)
 {
   fast_launcher f;
@@ -1269,12 +1279,13 @@ int fast_state_flush(FastState *s)
     // large grid: interior and UPML tiles as two launches, each at its own
     // occupancy (the interior body's register budget is below the UPML body's)
     mode = (nbi >= 8 * nbp && nbi > 4096) ? FAST_LAUNCH_SPLIT : FAST_LAUNCH_MERGED;
-  if (s->csamp) mode = FAST_LAUNCH_MERGED;  // the contour samples come from the merged instantiation
   s->lmode_last = mode;
+  // kinds of the tile launches, with the per-step NTFF samples or without
+  const int ki = s->csamp ? 4 : 1, kp = s->csamp ? 5 : 2, km = s->csamp ? 3 : 0;
   if (mode == FAST_LAUNCH_SPLIT) {
     a.nb_pml = 0;
-    if (nbi > 0) fast_launch(s->d.mode, s->d.flavour, s->R, 1, a, nbi, 0);
-    if (nbp > 0) fast_launch(s->d.mode, s->d.flavour, s->R, 2, a, nbp, 0);
+    if (nbi > 0) fast_launch(s->d.mode, s->d.flavour, s->R, ki, a, nbi, 0);
+    if (nbp > 0) fast_launch(s->d.mode, s->d.flavour, s->R, kp, a, nbp, 0);
   } else if (mode == FAST_LAUNCH_TWO_STREAMS) {
     // UPML tiles on a second stream beside the interior tiles: both read the same
     // buffer and write disjoint cores, so the two launches may overlap, each at its
@@ -1287,13 +1298,13 @@ int fast_state_flush(FastState *s)
     a.nb_pml = 0;
     HIPCHK(hipEventRecord(s->ev_in, 0));
     HIPCHK(hipStreamWaitEvent(s->s2, s->ev_in, 0));
-    if (nbp > 0) fast_launch(s->d.mode, s->d.flavour, s->R, 2, a, nbp, s->s2);
-    if (nbi > 0) fast_launch(s->d.mode, s->d.flavour, s->R, 1, a, nbi, 0);
+    if (nbp > 0) fast_launch(s->d.mode, s->d.flavour, s->R, kp, a, nbp, s->s2);
+    if (nbi > 0) fast_launch(s->d.mode, s->d.flavour, s->R, ki, a, nbi, 0);
     HIPCHK(hipEventRecord(s->ev_out, s->s2));
     HIPCHK(hipStreamWaitEvent(0, s->ev_out, 0));
   } else {
     a.nb_pml = nbp;
-    if (nbp + nbi > 0) fast_launch(s->d.mode, s->d.flavour, s->R, s->csamp ? 3 : 0, a, nbp + nbi, 0);
+    if (nbp + nbi > 0) fast_launch(s->d.mode, s->d.flavour, s->R, km, a, nbp + nbi, 0);
   }
   HIPCHK(hipGetLastError());
   if (g_prof.on) {
