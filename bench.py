@@ -47,7 +47,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=0, help="timed steps (default 8000; 400 for morpho8256)")
     ap.add_argument("--warmup", type=int, default=-1, help="untimed steps (default 400; 40 for morpho8256)")
-    ap.add_argument("--workload", default="mie1056", choices=sorted(WORKLOADS) + ["sweep64"])
+    ap.add_argument("--workload", default="mie1056", choices=sorted(WORKLOADS) + ["sweep64", "morpho8256_sweep81"])
     ap.add_argument("--seed", type=int, default=1, help="sweep64: generator seed")
     ap.add_argument("--precision", default="fp32", choices=("fp32", "fp64"))
     ap.add_argument("--mode", default="tm", choices=("tm", "te"))
@@ -66,6 +66,7 @@ def parse():
                          "that scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters to bytes")
     a = ap.parse_args()
     big = a.workload != "mie1056"
+    a.steps_given = a.steps > 0
     if a.steps <= 0:
         a.steps = 400 if big else 8000
     if a.warmup < 0:
@@ -142,6 +143,120 @@ def measured_traffic(mode, T, R, cells, launch):
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def run_sweep81(args, m):
+    """Config 4's wavelength sweep (380-780 nm every 5 nm: 81 wavelengths) on the
+    8256 x 4160 Morpho-scale grid as one run: the reference's plain pulse solver
+    (TM_UPML_2D / TE_UPML_2D, a Gaussian plane-wave pulse at normal incidence
+    whose band covers the sweep) with the running DFT of the NTFF contour at the
+    81 wavelengths.  Each wavelength's far-field spectrum (the reference's
+    |Eth|^2 / 8192 or |Eph|^2 / 8192, ntffTM.c:197-233) comes from the DFT, the
+    back-hemisphere angles 180-359 being its reflectance."""
+    import numpy as np
+    L = m.lib()
+    model, w, h, hu, pml, lam, ang, params, _ = WORKLOADS["morpho8256"]
+    lams = np.arange(380.0, 781.0, 5.0)
+    steps = args.steps if args.steps_given else 20000
+    solver = m.TM_UPML_2D if args.mode == "tm" else m.TE_UPML_2D
+    T, R = args.T or 8, args.R or 16
+    launch = args.launch if args.launch >= 0 else 3
+
+    def make(precision, n):
+        return m.Simulator(solver, getattr(m, model), m.field_info(w, h, hu, pml, lam, ang, n),
+                           params=m.model_params(**params), ntff_time=False, precision=precision, T=T, R=R)
+
+    tune = {}
+    if not (args.T and args.R and args.launch >= 0):
+        # a short tune of depth, rows and launch shape on the same grid with the DFT on
+        probe = make("fp32", 4000)
+        probe.set_dft(lams)
+        probe.calc(40)
+        for Tc, Rc in ((8, 16), (8, 8), (6, 8)):
+            for sh in (2, 3):
+                probe.set_blocking(Tc, Rc)
+                probe.set_launch_mode(sh)
+                probe.calc(16)
+                probe.set_blocking(Tc, Rc)
+                L.fdtd_hip_synchronize()
+                t0 = time.perf_counter()
+                probe.calc(48)
+                probe.set_blocking(Tc, Rc)
+                L.fdtd_hip_synchronize()
+                tune[(Tc, Rc, sh)] = (time.perf_counter() - t0) * 1e6 / 48
+        probe.finish()
+        T, R, launch = min(tune, key=tune.get)
+    log(f"sweep81: {len(lams)} wavelengths, {steps} steps, T={T} R={R} launch={launch}")
+    sim = make("fp32", steps)
+    cells = sim.N_PX * sim.N_PY
+    sim.set_launch_mode(launch)
+    sim.set_dft(lams)
+    L.fdtd_hip_synchronize()
+    L.fdtd_hip_profile_enable(1)
+    t0 = time.perf_counter()
+    sim.calc(steps)
+    L.fdtd_solver_setBlocking(T, R)  # launches the last partial block
+    L.fdtd_hip_synchronize()
+    dt = time.perf_counter() - t0
+    prof = prof_read(L)
+    L.fdtd_hip_profile_enable(0)
+    t1 = time.perf_counter()
+    spec = sim.dft_spectra()
+    t_spec = time.perf_counter() - t1
+    shape = ["auto", "merged", "split", "two_streams"][sim.launch_mode()]
+    sim.finish()
+    assert np.all(np.isfinite(spec)) and spec[:, 180:360].max() > 0
+    check = {}
+    if not args.no_check:
+        # FP32 against FP64 on a shorter run of the same grid and source: the 81
+        # transforms of the three accumulator rows, and the FP64 path's step rate
+        nchk = 4000  # the scattered pulse reaches the nearest contour side after about 3000 steps
+        log(f"fp32 / fp64 check over {nchk} steps")
+        res = {}
+        for prec in ("fp32", "fp64"):
+            c = make(prec, nchk)
+            if prec == "fp32":
+                c.set_launch_mode(launch)
+            c.set_dft(lams)
+            L.fdtd_hip_synchronize()
+            t2 = time.perf_counter()
+            c.calc(nchk)
+            if prec == "fp32":
+                L.fdtd_solver_setBlocking(T, R)
+            L.fdtd_hip_synchronize()
+            res[prec] = (c.dft(), time.perf_counter() - t2)
+            c.finish()
+        d32, d64 = res["fp32"][0], res["fp64"][0]
+        assert np.linalg.norm(d64) > 0, "the check run ended before the pulse reached the contour"
+        check["dft_rel_err_vs_fp64"] = float(np.linalg.norm(d32 - d64) / np.linalg.norm(d64))
+        ms64 = res["fp64"][1] * 1e3 / nchk
+        check["fp64_mcups"] = round(cells / (ms64 * 1e-3) / 1e6, 1)
+        check["fp64_spectra_per_s"] = round(len(lams) / (steps * ms64 * 1e-3), 4)
+    ms_k, _, nlaunch, b_int, b_pml, cupd = prof
+    achieved = (b_int + b_pml) / (ms_k * 1e-3) / 1e9 if nlaunch and ms_k > 0 else None
+    line = {
+        "metric": "Mcell-updates/s + % HBM roofline; spectra/s at <=1e-4 far-field err",
+        "value": round(cells * steps / dt / 1e6, 1), "unit": "Mcell-updates/s", "n_gpus": 1, "steps": steps,
+        "warmup": 0, "ms_per_step": round(dt * 1e3 / steps, 5), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"Morpho scale {args.mode.upper()}, 8192x4096 + 32-cell UPML (8256x4160), h=5 nm, "
+                               "plain pulse solver, normal incidence (+y), running DFT at 380-780 nm every 5 nm "
+                               "(81 wavelengths) in one run",
+                   "cells": cells, "precision": "fp32", "T": T, "R": R, "launch": shape, "wavelengths": len(lams),
+                   "tune_us_per_step": {f"T{k[0]}R{k[1]}{'split' if k[2] == 2 else '2streams'}": round(v, 2)
+                                        for k, v in tune.items()}},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+                     "traffic": None, "kernel": f"k_fast<{args.mode.upper()},PLAIN,R={R}> with contour samples",
+                     "launch_avg_us": round(ms_k * 1e3 / nlaunch, 2) if nlaunch else None,
+                     "bytes_per_cell_update": round((b_int + b_pml) / cupd, 2) if cupd else None},
+        "cpu_baseline": None,
+        "spectra_per_s": round(len(lams) / (dt + t_spec), 4),
+    }
+    if check:
+        line.update(check)
+        line["speedup_vs_fp64_spectra"] = round(line["spectra_per_s"] / check["fp64_spectra_per_s"], 2)
+    return line
 
 
 def run_sweep(args, m):
@@ -227,8 +342,8 @@ def main():
     # the library's own HIP runtime does the synchronising (fdtd_hip_synchronize =
     # hipDeviceSynchronize): a GPU runtime loaded beside it would not see its queues
     assert L.fdtd_hip_device_count() >= 1, "no GPU"
-    if args.workload == "sweep64":
-        line = run_sweep(args, m)
+    if args.workload in ("sweep64", "morpho8256_sweep81"):
+        line = run_sweep(args, m) if args.workload == "sweep64" else run_sweep81(args, m)
         libc.fflush(None)
         sys.stdout.flush()
         os.dup2(real_stdout, 1)

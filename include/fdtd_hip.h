@@ -170,6 +170,11 @@ int fdtd_hip_grid_get_blocking(const fdtd_grid *g, int *T, int *R);
  * streams. */
 enum { FAST_LAUNCH_AUTO = 0, FAST_LAUNCH_MERGED = 1, FAST_LAUNCH_SPLIT = 2, FAST_LAUNCH_TWO_STREAMS = 3 };
 int fdtd_hip_grid_set_launch_mode(fdtd_grid *g, int mode);
+/* FP32 grids with contour samples: the sample packing, binning and running DFT
+ * of a block run beside the next block on a stream of their own (1) or after
+ * it on the step stream (0); -1 (default) chooses by size: beside it on grids
+ * up to 8 Mi cells, whose step kernel leaves the GPU partly idle */
+int fdtd_hip_grid_set_ntff_overlap(fdtd_grid *g, int on);
 /* the setting; with AUTO, the shape of the last launch (AUTO before the first) */
 int fdtd_hip_grid_get_launch_mode(const fdtd_grid *g, int *mode);
 int fdtd_hip_grid_flush(fdtd_grid *g);

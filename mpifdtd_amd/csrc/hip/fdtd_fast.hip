@@ -869,6 +869,10 @@ struct FastState {
   FastTile *dtiles = nullptr;
   int n_int = 0, n_pml = 0;
   int qint[9], qpml[9];
+  // with contour samples each list is [tiles off the contour, tiles on it]: the
+  // sampling instantiations run on the second part only
+  int n_int_c = 0, n_pml_c = 0;
+  int qint_p[9], qint_c[9], qpml_p[9], qpml_c[9];
   FastStep hsteps[FAST_MAX_T];
   int T = 4, R = 16, tilesT = -1, tilesR = -1;
   int cur = 0, pending = 0;
@@ -1147,6 +1151,28 @@ static int build_tiles(FastState *s)
   auto byrow = [](const FastTile &a, const FastTile &b) { return a.i0 != b.i0 ? a.i0 < b.i0 : a.j0 < b.j0; };
   std::sort(ti.begin(), ti.end(), byrow);
   std::sort(tp.begin(), tp.end(), byrow);
+  // tiles whose core holds a contour cell the kernel samples (a side's line fi or
+  // its neighbour line fi - 1, within the side's range) go last in their list
+  auto on_contour = [&](const FastTile &t) {
+    if (!s->csamp) return false;
+    for (int q = 0; q < 4; q++) {
+      const FastCtrSide &g = s->hct.s[q];
+      if (g.n <= 0) continue;
+      const int a0 = g.horiz ? t.j0 : t.i0, a1 = a0 + (g.horiz ? t.nj : t.ni);  // across the side
+      const int b0 = g.horiz ? t.i0 : t.j0, b1 = b0 + (g.horiz ? t.ni : t.nj);  // along it
+      const bool line = (g.fi >= a0 && g.fi < a1) || (g.fi - 1 >= a0 && g.fi - 1 < a1);
+      if (line && g.lo < b1 && g.lo + g.n > b0) return true;
+    }
+    return false;
+  };
+  auto off_first = [&](std::vector<FastTile> &v) {
+    std::stable_partition(v.begin(), v.end(), [&](const FastTile &t) { return !on_contour(t); });
+    int c = 0;
+    for (const FastTile &t : v) c += on_contour(t) ? 1 : 0;
+    return c;
+  };
+  s->n_int_c = off_first(ti);
+  s->n_pml_c = off_first(tp);
   s->htp = tp;
   s->hti = ti;
   s->n_int = (int)ti.size();
@@ -1159,6 +1185,10 @@ static int build_tiles(FastState *s)
   if (!all.empty()) HIPCHK(hipMemcpy(s->dtiles, all.data(), all.size() * sizeof(FastTile), hipMemcpyHostToDevice));
   split_queues(s->qint, 0, s->n_int);
   split_queues(s->qpml, s->n_int, s->n_pml);
+  split_queues(s->qint_p, 0, s->n_int - s->n_int_c);
+  split_queues(s->qint_c, s->n_int - s->n_int_c, s->n_int_c);
+  split_queues(s->qpml_p, s->n_int, s->n_pml - s->n_pml_c);
+  split_queues(s->qpml_c, s->n_int + s->n_pml - s->n_pml_c, s->n_pml_c);
   // algorithmic bytes per step (4-byte reals): complex E,H read+write everywhere, UPML
   // auxiliaries read+write in UPML cells, 1/eps read (TM 1, TE 2)
   // updated cells (the plain solvers skip their edge cells) and how many of them
@@ -1280,12 +1310,25 @@ int fast_state_flush(FastState *s)
     // occupancy (the interior body's register budget is below the UPML body's)
     mode = (nbi >= 8 * nbp && nbi > 4096) ? FAST_LAUNCH_SPLIT : FAST_LAUNCH_MERGED;
   s->lmode_last = mode;
-  // kinds of the tile launches, with the per-step NTFF samples or without
-  const int ki = s->csamp ? 4 : 1, kp = s->csamp ? 5 : 2, km = s->csamp ? 3 : 0;
+  // the interior (UPML) tiles: those off the contour with the plain kind, those
+  // on it with the sampling kind (without samples, all of a list is off it)
+  auto launch_list = [&](bool pml, hipStream_t st) {
+    const int nt = pml ? s->n_pml : s->n_int, nc = pml ? s->n_pml_c : s->n_int_c;
+    FastArgs b = a;
+    b.nb_pml = 0;
+    if (nt - nc > 0) {
+      memcpy(pml ? b.qb[0] : b.qb[1], pml ? s->qpml_p : s->qint_p, sizeof b.qb[0]);
+      fast_launch(s->d.mode, s->d.flavour, s->R, pml ? 2 : 1, b, 8 * ((nt - nc + 7) / 8), st);
+    }
+    if (nc > 0) {
+      memcpy(pml ? b.qb[0] : b.qb[1], pml ? s->qpml_c : s->qint_c, sizeof b.qb[0]);
+      fast_launch(s->d.mode, s->d.flavour, s->R, pml ? 5 : 4, b, 8 * ((nc + 7) / 8), st);
+    }
+  };
+  const int km = s->csamp ? 3 : 0;
   if (mode == FAST_LAUNCH_SPLIT) {
-    a.nb_pml = 0;
-    if (nbi > 0) fast_launch(s->d.mode, s->d.flavour, s->R, ki, a, nbi, 0);
-    if (nbp > 0) fast_launch(s->d.mode, s->d.flavour, s->R, kp, a, nbp, 0);
+    launch_list(false, 0);
+    launch_list(true, 0);
   } else if (mode == FAST_LAUNCH_TWO_STREAMS) {
     // UPML tiles on a second stream beside the interior tiles: both read the same
     // buffer and write disjoint cores, so the two launches may overlap, each at its
@@ -1295,11 +1338,10 @@ int fast_state_flush(FastState *s)
       HIPCHK(hipEventCreateWithFlags(&s->ev_in, hipEventDisableTiming));
       HIPCHK(hipEventCreateWithFlags(&s->ev_out, hipEventDisableTiming));
     }
-    a.nb_pml = 0;
     HIPCHK(hipEventRecord(s->ev_in, 0));
     HIPCHK(hipStreamWaitEvent(s->s2, s->ev_in, 0));
-    if (nbp > 0) fast_launch(s->d.mode, s->d.flavour, s->R, kp, a, nbp, s->s2);
-    if (nbi > 0) fast_launch(s->d.mode, s->d.flavour, s->R, ki, a, nbi, 0);
+    launch_list(true, s->s2);
+    launch_list(false, 0);
     HIPCHK(hipEventRecord(s->ev_out, s->s2));
     HIPCHK(hipStreamWaitEvent(0, s->ev_out, 0));
   } else {
@@ -1757,6 +1799,7 @@ int fast_state_set_contour(FastState *s, const Contour &ct, int off, fast_post_f
   }
   s->csamp = nullptr;
   s->cbuf = 0;
+  s->tilesT = -1;  // the tile lists are split by the contour when next built
   if (ct.total > 0) {
     for (int b = 0; b < 2; b++) HIPCHK(dev_malloc(&s->csampb[b], (size_t)FAST_MAX_T * 3 * ct.total * sizeof(float2)));
     s->csamp = s->csampb[0];

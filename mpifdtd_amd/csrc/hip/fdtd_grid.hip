@@ -540,31 +540,51 @@ __global__ __launch_bounds__(NTFF_GRP * NS) void k_ntff_bin(const double2 *cE, c
 // ---------------------------------------------------------------- running DFT of the contour
 
 // V_f[c] += sum_s val_s[c] e^{i w_f tb_s}, tb_s = tb0 + s the binning's base time
-// of step s: n steps of the [cell][n] sample buffers per launch, one workgroup
-// per (256 cells, frequency); the n phase factors come from FP64 sincos.
+// of step s: n steps of the [cell][n] sample buffers per launch.  A workgroup
+// covers DFT_CC cells x DFT_FB frequencies: lane (c, j) loads cell c's n samples
+// (one cache line) and adds them to frequencies j, j + 8, ... of the group,
+// whose n phase factors each (FP64 sincos) sit in LDS.  The accumulators are
+// [f][cell], so the 32 cells of a frequency update together.
+#define DFT_CC 32
+#define DFT_FB 16
+template <int NS>  // NS >= n
 __global__ __launch_bounds__(256) void k_ctr_dft(const double2 *cE, const double2 *cH, int n, int P, const double *omega,
-                                                 long tb0, double2 *VE, double2 *VH)
+                                                 int nf, long tb0, double2 *VE, double2 *VH)
 {
-  __shared__ double2 ph[FAST_MAX_T];
-  const int f = blockIdx.y;
-  if ((int)threadIdx.x < n) {
+  __shared__ double2 ph[DFT_FB][FAST_MAX_T];
+  const int f0 = blockIdx.y * DFT_FB, nfl = min(DFT_FB, nf - f0);
+  for (int q = threadIdx.x; q < nfl * n; q += 256) {
+    const int f = q / n, s = q % n;
     double sn, cs;
-    sincos(omega[f] * (double)(tb0 + (long)threadIdx.x), &sn, &cs);
-    ph[threadIdx.x] = make_double2(cs, sn);
+    sincos(omega[f0 + f] * (double)(tb0 + s), &sn, &cs);
+    ph[f][s] = make_double2(cs, sn);
   }
   __syncthreads();
-  const int p = blockIdx.x * 256 + threadIdx.x;
+  const int c = threadIdx.x % DFT_CC, fl = threadIdx.x / DFT_CC;
+  const int p = blockIdx.x * DFT_CC + c;
   if (p >= P) return;
-  double2 ae = VE[(size_t)f * P + p], ah = VH[(size_t)f * P + p];
-  for (int s = 0; s < n; s++) {
-    const double2 e = cE[(size_t)p * n + s], h = cH[(size_t)p * n + s], w = ph[s];
-    ae.x += e.x * w.x - e.y * w.y;
-    ae.y += e.x * w.y + e.y * w.x;
-    ah.x += h.x * w.x - h.y * w.y;
-    ah.y += h.x * w.y + h.y * w.x;
+  double2 e[NS], h[NS];
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    e[s] = s < n ? cE[(size_t)p * n + s] : make_double2(0.0, 0.0);
+    h[s] = s < n ? cH[(size_t)p * n + s] : make_double2(0.0, 0.0);
   }
-  VE[(size_t)f * P + p] = ae;
-  VH[(size_t)f * P + p] = ah;
+  for (int f = fl; f < nfl; f += 256 / DFT_CC) {
+    const size_t k = (size_t)(f0 + f) * P + p;
+    double2 ae = VE[k], ah = VH[k];
+    // every s < NS unrolled (the samples stay in registers); steps past n hold
+    // zeros and add nothing
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      const double2 w = s < n ? ph[f][s] : make_double2(0.0, 0.0);
+      ae.x += e[s].x * w.x - e[s].y * w.y;
+      ae.y += e[s].x * w.y + e[s].y * w.x;
+      ah.x += h[s].x * w.x - h[s].y * w.y;
+      ah.y += h[s].x * w.y + h[s].y * w.x;
+    }
+    VE[k] = ae;
+    VH[k] = ah;
+  }
 }
 
 // The DFT at w_f of the three time-domain accumulator rows of every angle,
@@ -908,6 +928,7 @@ struct fdtd_grid {
   // block's step kernel; ev_k orders it after the block that took the samples
   hipStream_t ns;
   hipEvent_t ev_k;
+  int overlap;  // -1: by grid size (on up to NTFF_OVERLAP_CELLS cells), 0: in order on the step stream, 1: on ns
   double *cosA, *sinA, *cosB, *sinB;
   double coef;
   double2 *ffout;
@@ -1144,8 +1165,19 @@ static void launch_bin(fdtd_grid *g, int n, double time0, hipStream_t st)
 static void launch_dft(fdtd_grid *g, int n, double time0, hipStream_t st)
 {
   const long tb0 = (long)llround(time0) - 1;
-  hipLaunchKernelGGL(k_ctr_dft, dim3((g->ct.total + 255) / 256, g->nf), dim3(256), 0, st, g->cE, g->cH, n, g->ct.total,
-                     g->dftOmega, tb0, g->dftE, g->dftH);
+  const dim3 grid((g->ct.total + DFT_CC - 1) / DFT_CC, (g->nf + DFT_FB - 1) / DFT_FB);
+#define CTR_DFT(NS)                                                                                                 \
+  hipLaunchKernelGGL(k_ctr_dft<NS>, grid, dim3(256), 0, st, g->cE, g->cH, n, g->ct.total, g->dftOmega, g->nf, tb0, \
+                     g->dftE, g->dftH)
+  if (n <= 1)
+    CTR_DFT(1);
+  else if (n <= 4)
+    CTR_DFT(4);
+  else if (n <= 8)
+    CTR_DFT(8);
+  else
+    CTR_DFT(16);
+#undef CTR_DFT
 }
 
 // contour samples of n inner steps of the FP32 kernel ([n][3][total] float2:
@@ -1164,6 +1196,8 @@ __global__ __launch_bounds__(256) void k_ctr_combine(const float2 *smp, Contour 
   cH[(size_t)p * n + s] = make_double2(hs * ((double)h0.x + (double)h1.x), hs * ((double)h0.y + (double)h1.y));
 }
 
+#define NTFF_OVERLAP_CELLS (8L << 20)
+
 // after a launch of n steps of an FP32 grid: pack the samples, bin every step at
 // its time and add them to the running DFT, on the grid's NTFF stream (the
 // next block's step kernel runs meanwhile)
@@ -1173,17 +1207,25 @@ static int grid_bin_samples(void *ctx, const float2 *samples, int n, const doubl
   if (n > g->nbuf) return fdtd_fail(FDTD_EINVAL, "%d steps exceed the %d-step sample buffers", n, g->nbuf);
   for (int s = 1; s < n; s++)
     if (times[s] != times[0] + s) return fdtd_fail(FDTD_ESTATE, "steps of a block are not consecutive");
-  if (!g->ns) {
-    HIPCHK(hipStreamCreateWithFlags(&g->ns, hipStreamNonBlocking));
-    HIPCHK(hipEventCreateWithFlags(&g->ev_k, hipEventDisableTiming));
+  // beside the next block on small grids, whose step kernel leaves the GPU
+  // partly idle; in order on large ones, whose step kernel fills it (there the
+  // overlap only stretches both)
+  const bool ovl = g->overlap < 0 ? g->ncell <= NTFF_OVERLAP_CELLS : g->overlap != 0;
+  hipStream_t st = 0;
+  if (ovl) {
+    if (!g->ns) {
+      HIPCHK(hipStreamCreateWithFlags(&g->ns, hipStreamNonBlocking));
+      HIPCHK(hipEventCreateWithFlags(&g->ev_k, hipEventDisableTiming));
+    }
+    HIPCHK(hipEventRecord(g->ev_k, 0));
+    HIPCHK(hipStreamWaitEvent(g->ns, g->ev_k, 0));
+    st = g->ns;
   }
-  HIPCHK(hipEventRecord(g->ev_k, 0));
-  HIPCHK(hipStreamWaitEvent(g->ns, g->ev_k, 0));
-  hipLaunchKernelGGL(k_ctr_combine, dim3((g->ct.total + 255) / 256, n), dim3(256), 0, g->ns, samples, g->ct, n, g->cE,
+  hipLaunchKernelGGL(k_ctr_combine, dim3((g->ct.total + 255) / 256, n), dim3(256), 0, st, samples, g->ct, n, g->cE,
                      g->cH);
-  HIPCHK(hipEventRecord(consumed, g->ns));
-  if (g->ntn) launch_bin(g, n, times[0], g->ns);
-  if (g->nf) launch_dft(g, n, times[0], g->ns);
+  HIPCHK(hipEventRecord(consumed, st));
+  if (g->ntn) launch_bin(g, n, times[0], st);
+  if (g->nf) launch_dft(g, n, times[0], st);
   HIPCHK(hipGetLastError());
   return FDTD_OK;
 }
@@ -1213,6 +1255,7 @@ extern "C" int fdtd_hip_grid_create(const fdtd_grid_desc *d, fdtd_grid **out)
   memset((void *)g, 0, sizeof(*g));
   g->d = *d;
   g->off = d->flavour == FDTD_FLAVOUR_MPI ? 1 : 0;
+  g->overlap = -1;
   g->NPXa = d->N_PX + 2 * g->off;
   g->NPYa = d->N_PY + 2 * g->off;
   g->ncell = (size_t)g->NPXa * g->NPYa;
@@ -1293,6 +1336,15 @@ extern "C" int fdtd_hip_grid_get_blocking(const fdtd_grid *g, int *T, int *R)
 {
   if (!g || !g->fs) return fdtd_fail(FDTD_EINVAL, "temporal blocking applies to FP32 grids only");
   fast_state_get_blocking(g->fs, T, R);
+  return FDTD_OK;
+}
+
+extern "C" int fdtd_hip_grid_set_ntff_overlap(fdtd_grid *g, int on)
+{
+  if (!g || on < -1 || on > 1) return fdtd_fail(FDTD_EINVAL, "bad argument");
+  int rc;
+  if ((g->fs && (rc = fast_state_flush(g->fs))) || (rc = ntff_sync(g))) return rc;
+  g->overlap = on;
   return FDTD_OK;
 }
 
