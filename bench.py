@@ -31,6 +31,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# launch shapes of the UPML and interior tiles (FAST_LAUNCH_* of fdtd_hip.h)
+SHAPES = ["auto", "merged", "split", "two_streams", "split_pml8", "two_streams_pml8"]
 
 # name -> (model, width_nm, height_nm, h_nm, pml, lambda_nm, angle_deg, model params, description)
 WORKLOADS = {
@@ -55,7 +57,8 @@ def parse():
     ap.add_argument("--R", type=int, default=0, help="rows per wavefront (0: autotune)")
     ap.add_argument("--launch", type=int, default=-1,
                     help="launch shape of the UPML and interior tiles: 0 auto, 1 merged, 2 split, "
-                         "3 split on two streams (-1: autotune's choice, or auto)")
+                         "3 split on two streams, 4 / 5: 2 / 3 with 8-wavefront UPML workgroups "
+                         "(-1: autotune's choice, or auto)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 far-field check")
     ap.add_argument("--ntff", action="store_true",
@@ -173,7 +176,7 @@ def run_sweep81(args, m):
         probe.set_dft(lams)
         probe.calc(40)
         for Tc, Rc in ((8, 16), (8, 8), (6, 8)):
-            for sh in (2, 3):
+            for sh in (2, 3, 4, 5):
                 probe.set_blocking(Tc, Rc)
                 probe.set_launch_mode(sh)
                 probe.calc(16)
@@ -203,7 +206,7 @@ def run_sweep81(args, m):
     t1 = time.perf_counter()
     spec = sim.dft_spectra()
     t_spec = time.perf_counter() - t1
-    shape = ["auto", "merged", "split", "two_streams"][sim.launch_mode()]
+    shape = SHAPES[sim.launch_mode()]
     sim.finish()
     assert np.all(np.isfinite(spec)) and spec[:, 180:360].max() > 0
     check = {}
@@ -243,8 +246,7 @@ def run_sweep81(args, m):
                                "plain pulse solver, normal incidence (+y), running DFT at 380-780 nm every 5 nm "
                                "(81 wavelengths) in one run",
                    "cells": cells, "precision": "fp32", "T": T, "R": R, "launch": shape, "wavelengths": len(lams),
-                   "tune_us_per_step": {f"T{k[0]}R{k[1]}{'split' if k[2] == 2 else '2streams'}": round(v, 2)
-                                        for k, v in tune.items()}},
+                   "tune_us_per_step": {f"T{k[0]}R{k[1]}{SHAPES[k[2]]}": round(v, 2) for k, v in tune.items()}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
                      "traffic": None, "kernel": f"k_fast<{args.mode.upper()},PLAIN,R={R}> with contour samples",
@@ -308,7 +310,7 @@ def run_sweep(args, m):
         "config": {"workload": f"batched sweep of 64 seeded instances (seed {args.seed}): multilayer/zigzag, "
                                "sides 512-2048 + 16-cell UPML, TM/TE, lambda 380-780 nm, 6000-10000 steps",
                    "cells": sum(cells), "cell_updates": upd, "T": T, "R": R,
-                   "launch": "two_streams" if args.launch == 3 else "split", "create_s": round(t_init, 2)},
+                   "launch": "two_streams" if args.launch in (3, 5) else "split", "create_s": round(t_init, 2)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "note": "algorithmic bytes (interior-cell 52/56 B) of the whole sweep over its wall time"},
@@ -387,7 +389,7 @@ def main():
     t_ff = time.perf_counter() - t1
     f32_0 = sim.fields()[0]  # TM Ez / TE Ex (a copy)
     shape = sim.launch_mode() if args.precision == "fp32" else 0
-    shape_name = ["auto", "merged", "split", "two_streams"][shape]
+    shape_name = SHAPES[shape]
     sim.finish()
     ms_step = dt * 1e3 / K
     mcups = cells * K / dt / 1e6

@@ -168,7 +168,16 @@ int fdtd_hip_grid_get_blocking(const fdtd_grid *g, int *T, int *R);
  * (one register budget, the UPML body's); SPLIT is two launches in a row, each
  * at its own occupancy; TWO_STREAMS runs the two launches side by side on two
  * streams. */
-enum { FAST_LAUNCH_AUTO = 0, FAST_LAUNCH_MERGED = 1, FAST_LAUNCH_SPLIT = 2, FAST_LAUNCH_TWO_STREAMS = 3 };
+enum {
+  FAST_LAUNCH_AUTO = 0,
+  FAST_LAUNCH_MERGED = 1,
+  FAST_LAUNCH_SPLIT = 2,
+  FAST_LAUNCH_TWO_STREAMS = 3,
+  /* SPLIT / TWO_STREAMS with the UPML tiles as 8 wavefronts of 4 rows (512
+   * threads): half the registers per wavefront, two wavefronts per SIMD */
+  FAST_LAUNCH_SPLIT_PML8 = 4,
+  FAST_LAUNCH_TWO_STREAMS_PML8 = 5
+};
 int fdtd_hip_grid_set_launch_mode(fdtd_grid *g, int mode);
 /* FP32 grids with contour samples: the sample packing, binning and running DFT
  * of a block run beside the next block on a stream of their own (1) or after

@@ -17,7 +17,7 @@ _cache = {}
 
 
 def _key(sim):
-    return f"{sim.solver}:{sim.N_PX // 256}x{sim.N_PY // 256}:{'ntff' if sim.per_step_ntff else 'plain'}:v4"
+    return f"{sim.solver}:{sim.N_PX // 256}x{sim.N_PY // 256}:{'ntff' if sim.per_step_ntff else 'plain'}:v5"
 
 
 def _load():
@@ -65,7 +65,7 @@ def tune(m, make_sim, probe_steps=240, candidates=CANDIDATES):
 
     try:
         sim.calc(40)
-        shapes_tried = ((1, "merged"), (2, "split"), (3, "2streams"))
+        shapes_tried = ((1, "merged"), (2, "split"), (3, "2streams"), (4, "split8"), (5, "2streams8"))
         for T, R in candidates:
             for shape, tag in shapes_tried:
                 probe(f"T{T}R{R}{tag}", T, R, shape)

@@ -142,12 +142,12 @@ __device__ __forceinline__ FastStep batch_step(const FastBatchInst &B, int s)
 // One tile: load the block, run its T steps, write back the core.  BATCH: the
 // instance's depth, ping-pong index and step scalars come from its device
 // progress record instead of the launch arguments.
-template <int MODE, int FLAV, int R, int PML, int BATCH, int SMP>
+template <int MODE, int FLAV, int R, int PML, int BATCH, int SMP, int W = 4>
 __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
 {
-  constexpr int NR = 4 * R;  // rows of the loaded block
-  __shared__ float2 xE[4][64];
-  __shared__ float2 xH[4][64];
+  constexpr int NR = W * R;  // rows of the loaded block (W wavefronts of R rows)
+  __shared__ float2 xE[W][64];
+  __shared__ float2 xH[W][64];
   __shared__ FastStep sst[FAST_MAX_T];
   __shared__ float2 sex[NR];  // e^{i kx x} of the block rows (MPI scattered wave)
   // UPML x tables of the block rows: tx0..tx4, 1/(2 + sigma_x)
@@ -206,7 +206,7 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
     const int jc = min(max(jj, 0), NPY - 1);  // clamped column: every load below stays in bounds
     const float c2 = uni_f(I.c2);
     if ((int)threadIdx.x < T) sst[threadIdx.x] = BATCH ? batch_step(a.binst[t.inst], threadIdx.x) : a.st[threadIdx.x];
-    for (int r = threadIdx.x; r < NR; r += 256) {
+    for (int r = threadIdx.x; r < NR; r += 64 * W) {
       const int i = ib0 + r;
       const bool ok = i >= 0 && i < NPX;
       const int ic = min(max(i, 0), NPX - 1);
@@ -441,7 +441,7 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
         const unsigned bit = 1u << r;
         if (MODE == FDTD_MODE_TM) {
           const float2 ejp = shfl_dn1(F0[r]);
-          const float2 eip = r < R - 1 ? F0[r + 1] : (w < 3 ? xE[w + 1][lane] : make_float2(0.f, 0.f));
+          const float2 eip = r < R - 1 ? F0[r + 1] : (w < W - 1 ? xE[w + 1][lane] : make_float2(0.f, 0.f));
           const float2 dEy = c_sub(ejp, F0[r]);  // Ez(j+1) - Ez
           const float2 dEx = c_sub(eip, F0[r]);  // Ez(i+1) - Ez
           if (!PML) {
@@ -466,7 +466,7 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
           }
         } else {
           const float2 exj = shfl_dn1(F0[r]);
-          const float2 eyi = r < R - 1 ? F1[r + 1] : (w < 3 ? xE[w + 1][lane] : make_float2(0.f, 0.f));
+          const float2 eyi = r < R - 1 ? F1[r + 1] : (w < W - 1 ? xE[w + 1][lane] : make_float2(0.f, 0.f));
           // ((Ey(i+1) - Ey) - Ex(j+1)) + Ex
           const float2 ce = c_add(c_sub(c_sub(eyi, F1[r]), exj), F0[r]);
           if (!PML) {
@@ -630,12 +630,12 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
 // XCD b % 8 and walks that XCD's spatially contiguous partition of the tile
 // list [qbeg[q], qbeg[q+1]) with stride nblk/8.  Neighbouring tiles (which read
 // each other's halos) share an L2, and no device-scope atomic is needed.
-template <int MODE, int FLAV, int R, int PML, int SMP = 0>
+template <int MODE, int FLAV, int R, int PML, int SMP = 0, int W = 4>
 __device__ __forceinline__ void fast_tiles(const FastArgs &a, const int *qbeg, int b, int nblk)
 {
   const int q = b & 7;
   const int nb = nblk >> 3;
-  for (int tid = qbeg[q] + (b >> 3); tid < qbeg[q + 1]; tid += nb) fast_tile<MODE, FLAV, R, PML, 0, SMP>(a, tid);
+  for (int tid = qbeg[q] + (b >> 3); tid < qbeg[q + 1]; tid += nb) fast_tile<MODE, FLAV, R, PML, 0, SMP, W>(a, tid);
 }
 
 // KIND 0: one launch per block of T steps, workgroups [0, nb_pml) take the UPML
@@ -665,6 +665,14 @@ __global__ __launch_bounds__(256) void k_fast(FastArgs a)
     if ((int)blockIdx.x < a.nb_pml) fast_tiles<MODE, FLAV, 8, 1>(a, a.qb[0], blockIdx.x, a.nb_pml);
     else fast_tiles<MODE, FLAV, R, 0>(a, a.qb[1], (int)blockIdx.x - a.nb_pml, (int)gridDim.x - a.nb_pml);
   }
+}
+
+// UPML tiles (32 rows) as 8 wavefronts of 4 rows: half the registers of the
+// 4 x 8 form per wavefront, so that two wavefronts fit each SIMD
+template <int MODE, int FLAV, int SMP>
+__global__ __launch_bounds__(512) void k_fast_pml8(FastArgs a)
+{
+  fast_tiles<MODE, FLAV, 4, 1, SMP, 8>(a, a.qb[0], blockIdx.x, gridDim.x);
 }
 
 // Batched instances: every workgroup pulls tiles from the launch's two device
@@ -786,6 +794,12 @@ static void launch_one(const FastArgs &a, int nblocks, hipStream_t st)
   hipLaunchKernelGGL((k_fast<MODE, FLAV, R, KIND>), dim3(nblocks), dim3(256), 0, st, a);
 }
 
+template <int MODE, int FLAV, int SMP>
+static void launch_pml8(const FastArgs &a, int nblocks, hipStream_t st)
+{
+  hipLaunchKernelGGL((k_fast_pml8<MODE, FLAV, SMP>), dim3(nblocks), dim3(512), 0, st, a);
+}
+
 typedef void (*fast_launcher)(const FastArgs &, int, hipStream_t);
 
 template <int MODE, int FLAV>
@@ -793,6 +807,8 @@ static fast_launcher pick(int R, int kind)
 {
   if (kind == 2) return launch_one<MODE, FLAV, 8, 2>;
   if (kind == 5) return launch_one<MODE, FLAV, 8, 5>;
+  if (kind == 6) return launch_pml8<MODE, FLAV, 0>;
+  if (kind == 7) return launch_pml8<MODE, FLAV, 1>;
   if (kind == 4) {
     if (R == 4) return launch_one<MODE, FLAV, 4, 4>;
     if (R == 8) return launch_one<MODE, FLAV, 8, 4>;
@@ -814,7 +830,8 @@ static fast_launcher pick(int R, int kind)
 }
 
 // kind 0 (3 with NTFF samples): nblocks = a.nb_pml UPML workgroups + the interior workgroups;
-// kind 1 (4 with samples): interior workgroups only; kind 2 (5 with samples): UPML workgroups only
+// kind 1 (4 with samples): interior workgroups only; kind 2 (5 with samples): UPML workgroups only;
+// kind 6 (7 with samples): UPML workgroups of 8 wavefronts x 4 rows (512 threads)
 int fast_launch(int mode, int flavour, int R, int kind, const FastArgs &a, int nblocks, hipStream_t st)
 {
   fast_launcher f;
@@ -1312,24 +1329,26 @@ int fast_state_flush(FastState *s)
   s->lmode_last = mode;
   // the interior (UPML) tiles: those off the contour with the plain kind, those
   // on it with the sampling kind (without samples, all of a list is off it)
+  // UPML tiles as 8 wavefronts of 4 rows (512 threads) in the *_PML8 shapes
+  const bool w8 = mode == FAST_LAUNCH_SPLIT_PML8 || mode == FAST_LAUNCH_TWO_STREAMS_PML8;
   auto launch_list = [&](bool pml, hipStream_t st) {
     const int nt = pml ? s->n_pml : s->n_int, nc = pml ? s->n_pml_c : s->n_int_c;
     FastArgs b = a;
     b.nb_pml = 0;
     if (nt - nc > 0) {
       memcpy(pml ? b.qb[0] : b.qb[1], pml ? s->qpml_p : s->qint_p, sizeof b.qb[0]);
-      fast_launch(s->d.mode, s->d.flavour, s->R, pml ? 2 : 1, b, 8 * ((nt - nc + 7) / 8), st);
+      fast_launch(s->d.mode, s->d.flavour, s->R, pml ? (w8 ? 6 : 2) : 1, b, 8 * ((nt - nc + 7) / 8), st);
     }
     if (nc > 0) {
       memcpy(pml ? b.qb[0] : b.qb[1], pml ? s->qpml_c : s->qint_c, sizeof b.qb[0]);
-      fast_launch(s->d.mode, s->d.flavour, s->R, pml ? 5 : 4, b, 8 * ((nc + 7) / 8), st);
+      fast_launch(s->d.mode, s->d.flavour, s->R, pml ? (w8 ? 7 : 5) : 4, b, 8 * ((nc + 7) / 8), st);
     }
   };
   const int km = s->csamp ? 3 : 0;
-  if (mode == FAST_LAUNCH_SPLIT) {
+  if (mode == FAST_LAUNCH_SPLIT || mode == FAST_LAUNCH_SPLIT_PML8) {
     launch_list(false, 0);
     launch_list(true, 0);
-  } else if (mode == FAST_LAUNCH_TWO_STREAMS) {
+  } else if (mode == FAST_LAUNCH_TWO_STREAMS || mode == FAST_LAUNCH_TWO_STREAMS_PML8) {
     // UPML tiles on a second stream beside the interior tiles: both read the same
     // buffer and write disjoint cores, so the two launches may overlap, each at its
     // own occupancy; events order them after the previous block and before the next
@@ -1371,7 +1390,7 @@ int fast_state_flush(FastState *s)
 
 int fast_state_set_launch_mode(FastState *s, int mode)
 {
-  if (mode < FAST_LAUNCH_AUTO || mode > FAST_LAUNCH_TWO_STREAMS) return fdtd_fail(FDTD_EINVAL, "bad launch mode %d", mode);
+  if (mode < FAST_LAUNCH_AUTO || mode > FAST_LAUNCH_TWO_STREAMS_PML8) return fdtd_fail(FDTD_EINVAL, "bad launch mode %d", mode);
   int rc = fast_state_flush(s);
   if (rc) return rc;
   s->lmode = mode;
