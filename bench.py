@@ -32,7 +32,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 # launch shapes of the UPML and interior tiles (FAST_LAUNCH_* of fdtd_hip.h)
-SHAPES = ["auto", "merged", "split", "two_streams", "split_pml8", "two_streams_pml8"]
+SHAPES = ["auto", "merged", "split", "two_streams", "split_pml8", "two_streams_pml8", "split_w8", "two_streams_w8"]
 
 # name -> (model, width_nm, height_nm, h_nm, pml, lambda_nm, angle_deg, model params, description)
 WORKLOADS = {
@@ -57,7 +57,8 @@ def parse():
     ap.add_argument("--R", type=int, default=0, help="rows per wavefront (0: autotune)")
     ap.add_argument("--launch", type=int, default=-1,
                     help="launch shape of the UPML and interior tiles: 0 auto, 1 merged, 2 split, "
-                         "3 split on two streams, 4 / 5: 2 / 3 with 8-wavefront UPML workgroups "
+                         "3 split on two streams, 4 / 5: 2 / 3 with 8-wavefront UPML workgroups, "
+                         "6 / 7: 4 / 5 with 8-wavefront interior workgroups "
                          "(-1: autotune's choice, or auto)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 far-field check")
@@ -176,7 +177,7 @@ def run_sweep81(args, m):
         probe.set_dft(lams)
         probe.calc(40)
         for Tc, Rc in ((8, 16), (8, 8), (6, 8)):
-            for sh in (2, 3, 4, 5):
+            for sh in (2, 3, 4, 5, 6, 7):
                 probe.set_blocking(Tc, Rc)
                 probe.set_launch_mode(sh)
                 probe.calc(16)
@@ -310,7 +311,7 @@ def run_sweep(args, m):
         "config": {"workload": f"batched sweep of 64 seeded instances (seed {args.seed}): multilayer/zigzag, "
                                "sides 512-2048 + 16-cell UPML, TM/TE, lambda 380-780 nm, 6000-10000 steps",
                    "cells": sum(cells), "cell_updates": upd, "T": T, "R": R,
-                   "launch": "two_streams" if args.launch in (3, 5) else "split", "create_s": round(t_init, 2)},
+                   "launch": "two_streams" if args.launch in (3, 5, 7) else "split", "create_s": round(t_init, 2)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "note": "algorithmic bytes (interior-cell 52/56 B) of the whole sweep over its wall time"},

@@ -176,7 +176,12 @@ enum {
   /* SPLIT / TWO_STREAMS with the UPML tiles as 8 wavefronts of 4 rows (512
    * threads): half the registers per wavefront, two wavefronts per SIMD */
   FAST_LAUNCH_SPLIT_PML8 = 4,
-  FAST_LAUNCH_TWO_STREAMS_PML8 = 5
+  FAST_LAUNCH_TWO_STREAMS_PML8 = 5,
+  /* SPLIT_PML8 / TWO_STREAMS_PML8 with the interior tiles (4R rows) as 8
+   * wavefronts of R/2 rows: twice the wavefronts per SIMD (R = 8 or 16; with
+   * R = 4 the interior tiles keep 4 wavefronts) */
+  FAST_LAUNCH_SPLIT_W8 = 6,
+  FAST_LAUNCH_TWO_STREAMS_W8 = 7
 };
 int fdtd_hip_grid_set_launch_mode(fdtd_grid *g, int mode);
 /* FP32 grids with contour samples: the sample packing, binning and running DFT
@@ -212,8 +217,9 @@ int fdtd_hip_batch_create(fdtd_grid *const *grids, const fdtd_step_params *p0, c
  * instance to its step count and synchronise */
 int fdtd_hip_batch_run(fdtd_hip_batch *b, int blocks);
 /* the interior tiles of a block are launched, then the UPML tiles
- * (FAST_LAUNCH_SPLIT, the default; AUTO and MERGED mean the same here), or
- * the two side by side on two streams (FAST_LAUNCH_TWO_STREAMS) */
+ * (FAST_LAUNCH_SPLIT, the default; AUTO, MERGED and the other split shapes
+ * mean the same here), or the two side by side on two streams
+ * (FAST_LAUNCH_TWO_STREAMS and the other two-stream shapes) */
 int fdtd_hip_batch_set_launch_mode(fdtd_hip_batch *b, int mode);
 int fdtd_hip_batch_sync(fdtd_hip_batch *b, int *steps_done);
 int fdtd_hip_batch_destroy(fdtd_hip_batch *b);

@@ -3,7 +3,9 @@ wavefront R and the launch shape of the UPML and interior tiles.
 
 For a grid class (mode, flavour, grid size rounded to 256 cells) it times a
 short run of each candidate (T, R) in each launch shape (merged, split, split
-on two streams) on a live simulation and keeps the fastest.  None of these changes the arithmetic of any cell, so tuning on
+on two streams; the last two also with 8-wavefront UPML workgroups, and with
+8-wavefront interior workgroups as well) on a live simulation and keeps the
+fastest.  None of these changes the arithmetic of any cell, so tuning on
 the live state is safe.  Results
 are cached per process and, when MPIFDTD_TUNE_CACHE names a file, on disk.
 """
@@ -17,7 +19,7 @@ _cache = {}
 
 
 def _key(sim):
-    return f"{sim.solver}:{sim.N_PX // 256}x{sim.N_PY // 256}:{'ntff' if sim.per_step_ntff else 'plain'}:v5"
+    return f"{sim.solver}:{sim.N_PX // 256}x{sim.N_PY // 256}:{'ntff' if sim.per_step_ntff else 'plain'}:v6"
 
 
 def _load():
@@ -65,9 +67,12 @@ def tune(m, make_sim, probe_steps=240, candidates=CANDIDATES):
 
     try:
         sim.calc(40)
-        shapes_tried = ((1, "merged"), (2, "split"), (3, "2streams"), (4, "split8"), (5, "2streams8"))
+        shapes_tried = ((1, "merged"), (2, "split"), (3, "2streams"), (4, "split8"), (5, "2streams8"),
+                        (6, "splitw8"), (7, "2streamsw8"))
         for T, R in candidates:
             for shape, tag in shapes_tried:
+                if shape >= 6 and R < 8:
+                    continue  # R = 4 interior tiles keep 4 wavefronts: the same as shapes 4 / 5
                 probe(f"T{T}R{R}{tag}", T, R, shape)
         sim.set_launch_mode(0)
     finally:

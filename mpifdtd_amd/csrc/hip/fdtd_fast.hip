@@ -146,8 +146,12 @@ template <int MODE, int FLAV, int R, int PML, int BATCH, int SMP, int W = 4>
 __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
 {
   constexpr int NR = W * R;  // rows of the loaded block (W wavefronts of R rows)
-  __shared__ float2 xE[W][64];
-  __shared__ float2 xH[W][64];
+  // Rows exchanged between wavefronts: wave w writes its first E row to xE[w]
+  // and its last H row to xH[w + 1]; xE[W] and xH[0] stay zero (the block's
+  // outer halo rows), so every read is unconditional and the step code has no
+  // branch that would split it.
+  __shared__ float2 xE[W + 1][64];
+  __shared__ float2 xH[W + 1][64];
   __shared__ FastStep sst[FAST_MAX_T];
   __shared__ float2 sex[NR];  // e^{i kx x} of the block rows (MPI scattered wave)
   // UPML x tables of the block rows: tx0..tx4, 1/(2 + sigma_x)
@@ -165,6 +169,10 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
     const int H0 = a.T + 1;  // halo of the tiling depth; T <= a.T
     __syncthreads();  // the previous tile's last reads of the shared arrays are done
     if (BATCH && T == 0) return;  // instance finished (uniform)
+    if (w == 0) {
+      xE[W][lane] = make_float2(0.f, 0.f);
+      xH[0][lane] = make_float2(0.f, 0.f);
+    }
     // Instance scalars and pointers, read once into scalar registers.  The
     // readfirstlane is convergent, so the compiler cannot sink these loads into
     // the per-row branches below (where each row would wait on its own round trip).
@@ -326,7 +334,23 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
     const unsigned smask = MODE == FDTD_MODE_TM ? scat0 : (scat0 | scat1);
     const bool wsrc = __ballot(smask != 0) != 0;  // wavefront-uniform
     auto src_g = [&](int r, float ie) -> float2 { return c_scale(c_mul(sex[w * R + r], eyl), ie - 1.0f); };
-    auto src_mpi = [&](int r, float ie, const FastStep &st) -> float2 { return c_mul(src_g(r, ie), st.dc); };
+    // MPI flavour, interior tiles without sampling: g of each cell, formed once
+    // per tile in wavefronts holding a scatterer cell (TM: on Ez with 1/eps_z;
+    // TE: on Ey with 1/eps_y) and kept in LDS (each wavefront writes and reads
+    // only its own rows; registers would cost the second wavefront per SIMD);
+    // other tiles form it at each step.  The change g * dc is four FMAs.
+    constexpr bool GROWS = FLAV == FDTD_FLAVOUR_MPI && !PML && !SMP;
+    __shared__ float2 sg[GROWS ? NR : 1][64];
+    auto add_src = [&](float2 e, int r, float ie, const FastStep &st) -> float2 {
+      float2 g;
+      if constexpr (GROWS) g = sg[w * R + r][lane];
+      else g = src_g(r, ie);
+      e.x = fmaf(g.x, st.dc.x, e.x);
+      e.x = fmaf(-g.y, st.dc.y, e.x);
+      e.y = fmaf(g.x, st.dc.y, e.y);
+      e.y = fmaf(g.y, st.dc.x, e.y);
+      return e;
+    };
     // the source of component c in row r: value now (sn) and change (sd); zero in vacuum
     auto src_any = [&](int c, int r, float ie, const FastStep &st, auto SRCT, float2 &sn, float2 &sd) {
       constexpr bool SRC = decltype(SRCT)::value;
@@ -353,13 +377,13 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
         const int rr = w * R + r;  // row within the block (LDS tables)
         const unsigned bit = 1u << r;
         if (MODE == FDTD_MODE_TM) {
-          const float2 hym = r > 0 ? F2[r - 1] : (w > 0 ? xH[w - 1][lane] : make_float2(0.f, 0.f));
+          const float2 hym = r > 0 ? F2[r - 1] : xH[w][lane];
           const float2 hxm = shfl_up1(F1[r]);
           const float2 curl = c_sub(c_sub(F2[r], hym), c_sub(F1[r], hxm));
           if (!PML) {
             float2 e = c_add(F0[r], c_scale(curl, ie0[r]));
             if (FLAV == FDTD_FLAVOUR_MPI) {
-              if (SRC) e = c_add(e, src_mpi(r, ie0[r], st));
+              if (SRC) e = add_src(e, r, ie0[r], st);
             } else if (scat0 & bit) {
               float2 sn, sd;
               source(0, r, ie0[r], st, sn, sd);
@@ -379,14 +403,14 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
           }
         } else {
           const float2 hzj = shfl_up1(F2[r]);
-          const float2 hzi = r > 0 ? F2[r - 1] : (w > 0 ? xH[w - 1][lane] : make_float2(0.f, 0.f));
+          const float2 hzi = r > 0 ? F2[r - 1] : xH[w][lane];
           const float2 cx = c_sub(F2[r], hzj);   // Hz - Hz(j-1)
           const float2 cyv = c_sub(hzi, F2[r]);  // Hz(i-1) - Hz
           if (!PML) {
             float2 ex = c_add(F0[r], c_scale(cx, ie0[r]));
             float2 ey = c_add(F1[r], c_scale(cyv, ie1[r]));
             if (FLAV == FDTD_FLAVOUR_MPI) {
-              if (SRC) ey = c_add(ey, src_mpi(r, ie1[r], st));
+              if (SRC) ey = add_src(ey, r, ie1[r], st);
             } else {
               if (scat0 & bit) {
                 float2 sn, sd;
@@ -425,13 +449,8 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
           }
         }
       }
-      // first row of E for the wave above (TM Ez, TE Ey)
+      // first row of E (TM Ez, TE Ey) for wave w - 1
       xE[w][lane] = MODE == FDTD_MODE_TM ? F0[0] : F1[0];
-    };
-    auto e_phase = [&](int s) {
-      const FastStep st = sst[s];
-      if (FLAV == FDTD_FLAVOUR_MPI && wsrc) e_rows(st, std::true_type());
-      else e_rows(st, std::false_type());
     };
 
     auto h_phase = [&]() {
@@ -441,7 +460,7 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
         const unsigned bit = 1u << r;
         if (MODE == FDTD_MODE_TM) {
           const float2 ejp = shfl_dn1(F0[r]);
-          const float2 eip = r < R - 1 ? F0[r + 1] : (w < W - 1 ? xE[w + 1][lane] : make_float2(0.f, 0.f));
+          const float2 eip = r < R - 1 ? F0[r + 1] : xE[w + 1][lane];
           const float2 dEy = c_sub(ejp, F0[r]);  // Ez(j+1) - Ez
           const float2 dEx = c_sub(eip, F0[r]);  // Ez(i+1) - Ez
           if (!PML) {
@@ -466,7 +485,7 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
           }
         } else {
           const float2 exj = shfl_dn1(F0[r]);
-          const float2 eyi = r < R - 1 ? F1[r + 1] : (w < W - 1 ? xE[w + 1][lane] : make_float2(0.f, 0.f));
+          const float2 eyi = r < R - 1 ? F1[r + 1] : xE[w + 1][lane];
           // ((Ey(i+1) - Ey) - Ex(j+1)) + Ex
           const float2 ce = c_add(c_sub(c_sub(eyi, F1[r]), exj), F0[r]);
           if (!PML) {
@@ -481,7 +500,7 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
           }
         }
       }
-      xH[w][lane] = F2[R - 1];  // last row of Hy (TM) / Hz (TE) for the wave below
+      xH[w + 1][lane] = F2[R - 1];  // last row of Hy (TM) / Hz (TE) for wave w + 1
     };
 
     // per-step NTFF samples of the contour cells in this tile's core (the state
@@ -573,14 +592,38 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
     };
 
     if (FLAV == FDTD_FLAVOUR_MPI) {
-      xH[w][lane] = F2[R - 1];
+      xH[w + 1][lane] = F2[R - 1];
       __syncthreads();
-      for (int s = 0; s < T; s++) {
-        e_phase(s);
-        __syncthreads();
-        h_phase();
-        __syncthreads();
-        if (SMP) sample(s);
+      if constexpr (GROWS) {
+        // interior tiles: two copies of the step loop, chosen per wavefront, with
+        // the scattered wave (g formed once here) and without it (vacuum)
+        auto steps = [&](auto SRCT) {
+          constexpr bool SRC = decltype(SRCT)::value;
+          if (SRC) {
+#pragma unroll
+            for (int r = 0; r < R; r++) sg[w * R + r][lane] = src_g(r, MODE == FDTD_MODE_TM ? ie0[r] : ie1[r]);
+          }
+          for (int s = 0; s < T; s++) {
+            e_rows(sst[s], SRCT);
+            __syncthreads();
+            h_phase();
+            __syncthreads();
+            if (SMP) sample(s);
+          }
+        };
+        if (wsrc) steps(std::true_type());
+        else steps(std::false_type());
+      } else {
+        // UPML and sampling tiles (their registers are full with the auxiliaries
+        // or the sampling plan): one loop
+        for (int s = 0; s < T; s++) {
+          if (wsrc) e_rows(sst[s], std::true_type());
+          else e_rows(sst[s], std::false_type());
+          __syncthreads();
+          h_phase();
+          __syncthreads();
+          if (SMP) sample(s);
+        }
       }
     } else {
       xE[w][lane] = MODE == FDTD_MODE_TM ? F0[0] : F1[0];
@@ -588,7 +631,7 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
       for (int s = 0; s < T; s++) {
         h_phase();
         __syncthreads();
-        e_phase(s);
+        e_rows(sst[s], std::false_type());
         __syncthreads();
         if (SMP) sample(s);
       }
@@ -647,8 +690,15 @@ __device__ __forceinline__ void fast_tiles(const FastArgs &a, const int *qbeg, i
 // interior kernel keeps the occupancy of its own, smaller register budget.
 // KIND 3: KIND 0 with the per-step NTFF samples (a separate instantiation, so
 // that the other kinds do not carry the sampling code's registers).
+// Interior tile bodies are held to the occupancy their registers allow (R = 16:
+// two wavefronts per SIMD, R = 8: four, R = 4: six): the step loop is
+// straight-line code, and unconstrained scheduling spends registers on overlap
+// past the point where a further wavefront would no longer fit.
+__host__ __device__ constexpr int fast_interior_waves(int R) { return R >= 16 ? 2 : (R >= 8 ? 4 : 6); }
+
 template <int MODE, int FLAV, int R, int KIND>
-__global__ __launch_bounds__(256) void k_fast(FastArgs a)
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu((KIND == 1 || KIND == 4) ? fast_interior_waves(R) : 1))) void k_fast(FastArgs a)
 {
   if (KIND == 3) {
     if ((int)blockIdx.x < a.nb_pml) fast_tiles<MODE, FLAV, 8, 1, 1>(a, a.qb[0], blockIdx.x, a.nb_pml);
@@ -675,6 +725,15 @@ __global__ __launch_bounds__(512) void k_fast_pml8(FastArgs a)
   fast_tiles<MODE, FLAV, 4, 1, SMP, 8>(a, a.qb[0], blockIdx.x, gridDim.x);
 }
 
+// Interior tiles (4R rows) as 8 wavefronts of R/2 rows: the same tiles and
+// per-cell code as KIND 1 / 4, with half the registers per wavefront, so that
+// twice as many wavefronts share each SIMD (R = 16: four instead of two).
+template <int MODE, int FLAV, int RH, int SMP>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(fast_interior_waves(RH)))) void k_fast_int8(FastArgs a)
+{
+  fast_tiles<MODE, FLAV, RH, 0, SMP, 8>(a, a.qb[1], blockIdx.x, gridDim.x);
+}
+
 // Batched instances: every workgroup pulls tiles from the launch's two device
 // queues, first the UPML tiles (the longest per tile), then the interior tiles,
 // each queue ordered by decreasing instance size, until both are empty; every
@@ -684,7 +743,8 @@ __global__ __launch_bounds__(512) void k_fast_pml8(FastArgs a)
 // A one-queue instantiation carries one tile body only, so it runs at that
 // body's register budget.
 template <int MODE, int R, int KQ>
-__global__ __launch_bounds__(256) void k_fast_batch(FastArgs a)
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(KQ == 1 ? fast_interior_waves(R) : 1))) void k_fast_batch(FastArgs a)
 {
   __shared__ int s_tid;
   for (int k = (KQ == 1 ? 1 : 0); k < (KQ == 0 ? 1 : 2); k++) {
@@ -800,6 +860,12 @@ static void launch_pml8(const FastArgs &a, int nblocks, hipStream_t st)
   hipLaunchKernelGGL((k_fast_pml8<MODE, FLAV, SMP>), dim3(nblocks), dim3(512), 0, st, a);
 }
 
+template <int MODE, int FLAV, int RH, int SMP>
+static void launch_int8(const FastArgs &a, int nblocks, hipStream_t st)
+{
+  hipLaunchKernelGGL((k_fast_int8<MODE, FLAV, RH, SMP>), dim3(nblocks), dim3(512), 0, st, a);
+}
+
 typedef void (*fast_launcher)(const FastArgs &, int, hipStream_t);
 
 template <int MODE, int FLAV>
@@ -809,6 +875,9 @@ static fast_launcher pick(int R, int kind)
   if (kind == 5) return launch_one<MODE, FLAV, 8, 5>;
   if (kind == 6) return launch_pml8<MODE, FLAV, 0>;
   if (kind == 7) return launch_pml8<MODE, FLAV, 1>;
+  // interior tiles of 4R rows as 8 wavefronts of R/2 rows (R = 8, 16)
+  if (kind == 8) return R == 8 ? launch_int8<MODE, FLAV, 4, 0> : launch_int8<MODE, FLAV, 8, 0>;
+  if (kind == 9) return R == 8 ? launch_int8<MODE, FLAV, 4, 1> : launch_int8<MODE, FLAV, 8, 1>;
   if (kind == 4) {
     if (R == 4) return launch_one<MODE, FLAV, 4, 4>;
     if (R == 8) return launch_one<MODE, FLAV, 8, 4>;
@@ -1330,25 +1399,29 @@ int fast_state_flush(FastState *s)
   // the interior (UPML) tiles: those off the contour with the plain kind, those
   // on it with the sampling kind (without samples, all of a list is off it)
   // UPML tiles as 8 wavefronts of 4 rows (512 threads) in the *_PML8 shapes
-  const bool w8 = mode == FAST_LAUNCH_SPLIT_PML8 || mode == FAST_LAUNCH_TWO_STREAMS_PML8;
+  const bool w8 = mode == FAST_LAUNCH_SPLIT_PML8 || mode == FAST_LAUNCH_TWO_STREAMS_PML8 ||
+                  mode == FAST_LAUNCH_SPLIT_W8 || mode == FAST_LAUNCH_TWO_STREAMS_W8;
+  // and in the *_W8 shapes the interior tiles as 8 wavefronts of R/2 rows (R >= 8)
+  const bool i8 = (mode == FAST_LAUNCH_SPLIT_W8 || mode == FAST_LAUNCH_TWO_STREAMS_W8) && s->R >= 8;
   auto launch_list = [&](bool pml, hipStream_t st) {
     const int nt = pml ? s->n_pml : s->n_int, nc = pml ? s->n_pml_c : s->n_int_c;
     FastArgs b = a;
     b.nb_pml = 0;
     if (nt - nc > 0) {
       memcpy(pml ? b.qb[0] : b.qb[1], pml ? s->qpml_p : s->qint_p, sizeof b.qb[0]);
-      fast_launch(s->d.mode, s->d.flavour, s->R, pml ? (w8 ? 6 : 2) : 1, b, 8 * ((nt - nc + 7) / 8), st);
+      fast_launch(s->d.mode, s->d.flavour, s->R, pml ? (w8 ? 6 : 2) : (i8 ? 8 : 1), b, 8 * ((nt - nc + 7) / 8), st);
     }
     if (nc > 0) {
       memcpy(pml ? b.qb[0] : b.qb[1], pml ? s->qpml_c : s->qint_c, sizeof b.qb[0]);
-      fast_launch(s->d.mode, s->d.flavour, s->R, pml ? (w8 ? 7 : 5) : 4, b, 8 * ((nc + 7) / 8), st);
+      fast_launch(s->d.mode, s->d.flavour, s->R, pml ? (w8 ? 7 : 5) : (i8 ? 9 : 4), b, 8 * ((nc + 7) / 8), st);
     }
   };
   const int km = s->csamp ? 3 : 0;
-  if (mode == FAST_LAUNCH_SPLIT || mode == FAST_LAUNCH_SPLIT_PML8) {
+  if (mode == FAST_LAUNCH_SPLIT || mode == FAST_LAUNCH_SPLIT_PML8 || mode == FAST_LAUNCH_SPLIT_W8) {
     launch_list(false, 0);
     launch_list(true, 0);
-  } else if (mode == FAST_LAUNCH_TWO_STREAMS || mode == FAST_LAUNCH_TWO_STREAMS_PML8) {
+  } else if (mode == FAST_LAUNCH_TWO_STREAMS || mode == FAST_LAUNCH_TWO_STREAMS_PML8 ||
+             mode == FAST_LAUNCH_TWO_STREAMS_W8) {
     // UPML tiles on a second stream beside the interior tiles: both read the same
     // buffer and write disjoint cores, so the two launches may overlap, each at its
     // own occupancy; events order them after the previous block and before the next
@@ -1390,7 +1463,7 @@ int fast_state_flush(FastState *s)
 
 int fast_state_set_launch_mode(FastState *s, int mode)
 {
-  if (mode < FAST_LAUNCH_AUTO || mode > FAST_LAUNCH_TWO_STREAMS_PML8) return fdtd_fail(FDTD_EINVAL, "bad launch mode %d", mode);
+  if (mode < FAST_LAUNCH_AUTO || mode > FAST_LAUNCH_TWO_STREAMS_W8) return fdtd_fail(FDTD_EINVAL, "bad launch mode %d", mode);
   int rc = fast_state_flush(s);
   if (rc) return rc;
   s->lmode = mode;
