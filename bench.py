@@ -311,7 +311,8 @@ def run_sweep(args, m):
         "config": {"workload": f"batched sweep of 64 seeded instances (seed {args.seed}): multilayer/zigzag, "
                                "sides 512-2048 + 16-cell UPML, TM/TE, lambda 380-780 nm, 6000-10000 steps",
                    "cells": sum(cells), "cell_updates": upd, "T": T, "R": R,
-                   "launch": "two_streams" if args.launch in (3, 5, 7) else "split", "create_s": round(t_init, 2)},
+                   "launch": SHAPES[args.launch] if args.launch >= 2 else "two_streams_pml8",
+                   "create_s": round(t_init, 2)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "note": "algorithmic bytes (interior-cell 52/56 B) of the whole sweep over its wall time"},

@@ -217,9 +217,10 @@ int fdtd_hip_batch_create(fdtd_grid *const *grids, const fdtd_step_params *p0, c
  * instance to its step count and synchronise */
 int fdtd_hip_batch_run(fdtd_hip_batch *b, int blocks);
 /* the interior tiles of a block are launched, then the UPML tiles
- * (FAST_LAUNCH_SPLIT, the default; AUTO, MERGED and the other split shapes
- * mean the same here), or the two side by side on two streams
- * (FAST_LAUNCH_TWO_STREAMS and the other two-stream shapes) */
+ * (FAST_LAUNCH_SPLIT; AUTO and MERGED mean the same here), or the two side by
+ * side on two streams (FAST_LAUNCH_TWO_STREAMS); the *_PML8 shapes run the
+ * UPML tiles as 8 wavefronts, the *_W8 shapes the interior tiles as well
+ * (R >= 8).  Default: FAST_LAUNCH_TWO_STREAMS_PML8. */
 int fdtd_hip_batch_set_launch_mode(fdtd_hip_batch *b, int mode);
 int fdtd_hip_batch_sync(fdtd_hip_batch *b, int *steps_done);
 int fdtd_hip_batch_destroy(fdtd_hip_batch *b);

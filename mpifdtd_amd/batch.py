@@ -44,7 +44,9 @@ class Batch:
         self.T, self.R = T, R
 
     def set_launch_mode(self, mode):
-        """2 (default): interior tiles then UPML tiles; 3: the two side by side on two streams."""
+        """2: interior tiles then UPML tiles; 3: the two side by side on two streams; 4 / 5: 2 / 3
+        with the UPML tiles as 8 wavefronts (5 is the default); 6 / 7: 4 / 5 with the interior
+        tiles as 8 wavefronts as well (FAST_LAUNCH_* of fdtd_hip.h)."""
         check(N.lib().fdtd_batch_set_launch_mode(self._h, int(mode)), "fdtd_batch_set_launch_mode")
 
     def run(self):

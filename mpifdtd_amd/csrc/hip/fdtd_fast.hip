@@ -742,10 +742,13 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(fast_interi
 // KQ: the queues pulled, 0 UPML tiles, 1 interior tiles, 2 both (UPML first).
 // A one-queue instantiation carries one tile body only, so it runs at that
 // body's register budget.
-template <int MODE, int R, int KQ>
-__global__ __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(KQ == 1 ? fast_interior_waves(R) : 1))) void k_fast_batch(FastArgs a)
+// W8 (one-queue instantiations): the tiles as 8 wavefronts (512 threads), UPML
+// tiles of 32 rows as 8 x 4 rows, interior tiles of 4R rows as 8 x R/2 rows.
+template <int MODE, int R, int KQ, int W8 = 0>
+__global__ __launch_bounds__(W8 ? 512 : 256)
+__attribute__((amdgpu_waves_per_eu(KQ == 1 ? fast_interior_waves(W8 ? R / 2 : R) : 1))) void k_fast_batch(FastArgs a)
 {
+  static_assert(!W8 || (KQ != 2 && R >= 8), "8-wavefront bodies: one queue, R >= 8");
   __shared__ int s_tid;
   for (int k = (KQ == 1 ? 1 : 0); k < (KQ == 0 ? 1 : 2); k++) {
     for (;;) {
@@ -754,8 +757,13 @@ __attribute__((amdgpu_waves_per_eu(KQ == 1 ? fast_interior_waves(R) : 1))) void 
       __syncthreads();
       const int tid = s_tid;
       if (tid >= a.qn[k]) break;
-      if (KQ != 1 && k == 0) fast_tile<MODE, FDTD_FLAVOUR_MPI, 8, 1, 1, 0>(a, a.qoff[0] + tid);
-      else if (KQ != 0) fast_tile<MODE, FDTD_FLAVOUR_MPI, R, 0, 1, 0>(a, a.qoff[1] + tid);
+      if constexpr (W8) {
+        if (KQ == 0) fast_tile<MODE, FDTD_FLAVOUR_MPI, 4, 1, 1, 0, 8>(a, a.qoff[0] + tid);
+        else fast_tile<MODE, FDTD_FLAVOUR_MPI, R / 2, 0, 1, 0, 8>(a, a.qoff[1] + tid);
+      } else {
+        if (KQ != 1 && k == 0) fast_tile<MODE, FDTD_FLAVOUR_MPI, 8, 1, 1, 0>(a, a.qoff[0] + tid);
+        else if (KQ != 0) fast_tile<MODE, FDTD_FLAVOUR_MPI, R, 0, 1, 0>(a, a.qoff[1] + tid);
+      }
     }
   }
 }
@@ -1595,8 +1603,11 @@ struct FastBatch {
   int nblk[2][2] = {{0, 0}, {0, 0}};  // workgroups per mode and queue
   hipStream_t stream = nullptr;
   // two_streams: the UPML launches of a block on s2 beside the interior launches
-  // (fork and join by events, captured into the graph as dependencies)
-  int two_streams = 0;
+  // (fork and join by events, captured into the graph as dependencies);
+  // pml8 / int8: the UPML / interior tiles as 8 wavefronts (the *_PML8, *_W8 shapes)
+  // Default: TWO_STREAMS_PML8, the fastest on the 64-instance sweep (260.7 G cell
+  // updates/s against 243.5 for SPLIT; every shape gives the same bits)
+  int two_streams = 1, pml8 = 1, int8 = 0;
   hipStream_t s2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   hipGraph_t graph = nullptr;
@@ -1605,18 +1616,22 @@ struct FastBatch {
 };
 
 template <int MODE, int KQ>
-static void batch_launch_q(const FastArgs &a, int R, int nblk, hipStream_t st)
+static void batch_launch_q(const FastArgs &a, int R, int nblk, bool w8, hipStream_t st)
 {
-  if (R == 4) hipLaunchKernelGGL((k_fast_batch<MODE, 4, KQ>), dim3(nblk), dim3(256), 0, st, a);
+  if (w8 && KQ == 0) hipLaunchKernelGGL((k_fast_batch<MODE, 8, 0, 1>), dim3(nblk), dim3(512), 0, st, a);
+  else if (w8 && R == 8) hipLaunchKernelGGL((k_fast_batch<MODE, 8, KQ, 1>), dim3(nblk), dim3(512), 0, st, a);
+  else if (w8 && R == 16) hipLaunchKernelGGL((k_fast_batch<MODE, 16, KQ, 1>), dim3(nblk), dim3(512), 0, st, a);
+  else if (R == 4) hipLaunchKernelGGL((k_fast_batch<MODE, 4, KQ>), dim3(nblk), dim3(256), 0, st, a);
   else if (R == 8) hipLaunchKernelGGL((k_fast_batch<MODE, 8, KQ>), dim3(nblk), dim3(256), 0, st, a);
   else hipLaunchKernelGGL((k_fast_batch<MODE, 16, KQ>), dim3(nblk), dim3(256), 0, st, a);
 }
 
 template <int MODE>
-static void batch_launch_mode(const FastArgs &a, int R, const int nblk[2], hipStream_t st, hipStream_t st_pml)
+static void batch_launch_mode(const FastBatch *b, const FastArgs &a, const int nblk[2], hipStream_t st,
+                              hipStream_t st_pml)
 {
-  if (nblk[1] > 0) batch_launch_q<MODE, 1>(a, R, nblk[1], st);
-  if (nblk[0] > 0) batch_launch_q<MODE, 0>(a, R, nblk[0], st_pml);
+  if (nblk[1] > 0) batch_launch_q<MODE, 1>(a, b->R, nblk[1], b->int8 != 0, st);
+  if (nblk[0] > 0) batch_launch_q<MODE, 0>(a, b->R, nblk[0], b->pml8 != 0, st_pml);
 }
 
 static FastArgs batch_args(const FastBatch *b, int m)
@@ -1648,8 +1663,8 @@ static int batch_level(FastBatch *b)
   for (int m = 0; m < 2; m++) {
     if (b->nblk[m][0] + b->nblk[m][1] == 0) continue;
     const FastArgs a = batch_args(b, m);
-    if (m == FDTD_MODE_TM) batch_launch_mode<FDTD_MODE_TM>(a, b->R, b->nblk[m], b->stream, sp);
-    else batch_launch_mode<FDTD_MODE_TE>(a, b->R, b->nblk[m], b->stream, sp);
+    if (m == FDTD_MODE_TM) batch_launch_mode<FDTD_MODE_TM>(b, a, b->nblk[m], b->stream, sp);
+    else batch_launch_mode<FDTD_MODE_TE>(b, a, b->nblk[m], b->stream, sp);
   }
   if (b->two_streams) {
     HIPCHK(hipEventRecord(b->ev_join, b->s2));
@@ -1747,8 +1762,8 @@ int fast_batch_create(FastState *const *st, const fdtd_step_params *p0, const in
       }
       b->nt[m][k] = (int)all.size() - b->off[m][k];
     }
-    // resident workgroups: one per compute unit at the UPML body's register
-    // budget, two at the interior body's
+    // resident workgroups: at most one per compute unit for the UPML bodies,
+    // two for the interior body at R = 16
     b->nblk[m][0] = std::min(b->nt[m][0], 256);
     b->nblk[m][1] = std::min(b->nt[m][1], 512);
   }
@@ -1798,16 +1813,21 @@ int fast_batch_run(FastBatch *b, int blocks, int use_graph)
   return FDTD_OK;
 }
 
-// the UPML launches beside the interior launches on a second stream (1) or after them (0);
-// applies to blocks queued afterwards (the captured graph is rebuilt)
-int fast_batch_set_two_streams(FastBatch *b, int on)
+// launch shape (FAST_LAUNCH_*): the UPML launches beside the interior launches on a
+// second stream or after them, each tile kind as 4 or 8 wavefronts (interior tiles
+// as 8 wavefronts only for R >= 8); applies to blocks queued afterwards (the
+// captured graph is rebuilt)
+int fast_batch_set_launch(FastBatch *b, int mode)
 {
   HIPCHK(hipStreamSynchronize(b->stream));
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   b->gexec = nullptr;
   b->graph = nullptr;
-  b->two_streams = on ? 1 : 0;
+  b->two_streams = mode == FAST_LAUNCH_TWO_STREAMS || mode == FAST_LAUNCH_TWO_STREAMS_PML8 ||
+                   mode == FAST_LAUNCH_TWO_STREAMS_W8;
+  b->pml8 = mode >= FAST_LAUNCH_SPLIT_PML8;
+  b->int8 = (mode == FAST_LAUNCH_SPLIT_W8 || mode == FAST_LAUNCH_TWO_STREAMS_W8) && b->R >= 8;
   return FDTD_OK;
 }
 

@@ -148,7 +148,7 @@ struct FastBatch;
 int fast_batch_create(FastState *const *st, const fdtd_step_params *p0, const int *steps, int n, int T, int R,
                       FastBatch **out);
 int fast_batch_run(FastBatch *b, int blocks, int use_graph);
-int fast_batch_set_two_streams(FastBatch *b, int on);
+int fast_batch_set_launch(FastBatch *b, int mode);
 int fast_batch_blocks_left(const FastBatch *b);
 int fast_batch_sync(FastBatch *b, int *steps_done);
 void fast_batch_destroy(FastBatch *b);

@@ -91,17 +91,19 @@ def test_batch_chunks_and_graph_replays_agree():
         assert np.array_equal(a, c)
 
 
-def test_batch_two_streams_equal_split():
-    """The UPML launches beside the interior launches (a fork and join captured into
-    the replayed graph) give the same state, bit for bit, as the launches in a row;
-    the run is chunked so that eager blocks and graph replays both occur."""
+@pytest.mark.parametrize("shape", [2, 3, 4, 6, 7])
+def test_batch_launch_shapes_equal_default(shape):
+    """The launches in a row or side by side (a fork and join captured into the
+    replayed graph), with either tile kind as 4 or 8 wavefronts, give the same
+    state, bit for bit, as the default shape (two streams, 8-wavefront UPML
+    tiles); the run is chunked so that eager blocks and graph replays both occur."""
     import mpifdtd_amd as m
     from mpifdtd_amd import batch as B
     with B.Batch(_items(m, B), T=T, R=R) as b:
         b.run()
         ref = [[b.field(i, k) for k in (0, 3, 6)] for i in range(len(CASES))]
     with B.Batch(_items(m, B), T=T, R=R) as b:
-        b.set_launch_mode(3)
+        b.set_launch_mode(shape)
         b.run_blocks(5)
         b.sync()
         b.run_blocks(20)
