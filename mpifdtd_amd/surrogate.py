@@ -378,6 +378,31 @@ def experiment(n_train=20000, n_test=2000, epochs=200, k=4, seed=1, threshold=0.
     }
 
 
+def data_scaling(load, sizes=(2500, 5000, 10000, 20000), steps=4000, k=4, seed=1, lr=1e-3, lr_end=1e-5, log=print):
+    """Held-out error against the number of training spectra: the ensemble is
+    trained on nested prefixes of the training set of `load` (an .npz written by
+    experiment(save=...)) for about the same number of Adam steps each (epochs
+    = steps * batch / n, cosine-annealed), and evaluated on the same held-out
+    set.  Returns [{"n", "epochs", "median", "mean", "p95", "train_mse"}]."""
+    z = np.load(load)
+    Xtr, Ytr, Xte, Yte = z["Xtr"], z["Ytr"].astype(np.float64), z["Xte"], z["Yte"].astype(np.float64)
+    out = []
+    for n in sizes:
+        n = min(int(n), len(Xtr))
+        tg = Targets(Ytr[:n])
+        ens = Ensemble([N_FEATURES, 512, 512, 512, N_ANGLES], k=k, batch=1024, seed=seed)
+        epochs = max(1, int(round(steps * 1024 / n)))
+        hist = ens.fit(Xtr[:n], tg.encode(Ytr[:n]), epochs=epochs, lr=lr, lr_end=lr_end)
+        pred = np.stack([tg.decode(net.forward(Xte)) for net in ens.nets]).mean(0)
+        err = rel_err(pred, Yte)
+        row = {"n": n, "epochs": epochs, "median": float(np.median(err)), "mean": float(err.mean()),
+               "p95": float(np.quantile(err, 0.95)), "train_mse": hist[-1]}
+        if log:
+            log(row)
+        out.append(row)
+    return out
+
+
 if __name__ == "__main__":
     import argparse
     import json
@@ -392,7 +417,21 @@ if __name__ == "__main__":
     ap.add_argument("--target", default="linear", choices=("linear", "log"))
     ap.add_argument("--save", default=None, help="write the generated spectra to this .npz")
     ap.add_argument("--load", default=None, help="read the spectra from this .npz instead of running the FDTD")
+    ap.add_argument("--subsets", default=None,
+                    help="comma-separated training-set sizes: held-out error against the number of spectra "
+                         "(needs --load)")
+    ap.add_argument("--steps", type=int, default=4000, help="--subsets: Adam steps per size")
     a = ap.parse_args()
+    if a.subsets:
+        if not a.load:
+            ap.error("--subsets needs --load")
+        rows = data_scaling(a.load, [int(x) for x in a.subsets.split(",")], steps=a.steps, k=a.k, seed=a.seed,
+                            lr=a.lr, lr_end=a.lr_end if a.lr_end is not None else 1e-5,
+                            log=lambda r: print(json.dumps(r), flush=True))
+        ns = np.log([r["n"] for r in rows])
+        slope = float(np.polyfit(ns, np.log([r["median"] for r in rows]), 1)[0]) if len(rows) > 1 else None
+        print(json.dumps({"data_scaling": rows, "median_exponent": slope}), flush=True)
+        raise SystemExit(0)
     res = experiment(a.train, a.test, a.epochs, a.k, a.seed, log=lambda s: print(s, flush=True), save=a.save,
                      load=a.load, lr=a.lr, lr_end=a.lr_end, target=a.target)
     print(json.dumps(res), flush=True)
