@@ -729,7 +729,7 @@ __global__ __launch_bounds__(512) void k_fast_pml8(FastArgs a)
 // per-cell code as KIND 1 / 4, with half the registers per wavefront, so that
 // twice as many wavefronts share each SIMD (R = 16: four instead of two).
 template <int MODE, int FLAV, int RH, int SMP>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(fast_interior_waves(RH)))) void k_fast_int8(FastArgs a)
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(fast_interior_waves(RH)))) void k_fast_interior8(FastArgs a)
 {
   fast_tiles<MODE, FLAV, RH, 0, SMP, 8>(a, a.qb[1], blockIdx.x, gridDim.x);
 }
@@ -869,9 +869,9 @@ static void launch_pml8(const FastArgs &a, int nblocks, hipStream_t st)
 }
 
 template <int MODE, int FLAV, int RH, int SMP>
-static void launch_int8(const FastArgs &a, int nblocks, hipStream_t st)
+static void launch_interior8(const FastArgs &a, int nblocks, hipStream_t st)
 {
-  hipLaunchKernelGGL((k_fast_int8<MODE, FLAV, RH, SMP>), dim3(nblocks), dim3(512), 0, st, a);
+  hipLaunchKernelGGL((k_fast_interior8<MODE, FLAV, RH, SMP>), dim3(nblocks), dim3(512), 0, st, a);
 }
 
 typedef void (*fast_launcher)(const FastArgs &, int, hipStream_t);
@@ -884,8 +884,8 @@ static fast_launcher pick(int R, int kind)
   if (kind == 6) return launch_pml8<MODE, FLAV, 0>;
   if (kind == 7) return launch_pml8<MODE, FLAV, 1>;
   // interior tiles of 4R rows as 8 wavefronts of R/2 rows (R = 8, 16)
-  if (kind == 8) return R == 8 ? launch_int8<MODE, FLAV, 4, 0> : launch_int8<MODE, FLAV, 8, 0>;
-  if (kind == 9) return R == 8 ? launch_int8<MODE, FLAV, 4, 1> : launch_int8<MODE, FLAV, 8, 1>;
+  if (kind == 8) return R == 8 ? launch_interior8<MODE, FLAV, 4, 0> : launch_interior8<MODE, FLAV, 8, 0>;
+  if (kind == 9) return R == 8 ? launch_interior8<MODE, FLAV, 4, 1> : launch_interior8<MODE, FLAV, 8, 1>;
   if (kind == 4) {
     if (R == 4) return launch_one<MODE, FLAV, 4, 4>;
     if (R == 8) return launch_one<MODE, FLAV, 8, 4>;
@@ -1604,10 +1604,10 @@ struct FastBatch {
   hipStream_t stream = nullptr;
   // two_streams: the UPML launches of a block on s2 beside the interior launches
   // (fork and join by events, captured into the graph as dependencies);
-  // pml8 / int8: the UPML / interior tiles as 8 wavefronts (the *_PML8, *_W8 shapes)
+  // pml8 / inner8: the UPML / interior tiles as 8 wavefronts (the *_PML8, *_W8 shapes)
   // Default: TWO_STREAMS_PML8, the fastest on the 64-instance sweep (260.7 G cell
   // updates/s against 243.5 for SPLIT; every shape gives the same bits)
-  int two_streams = 1, pml8 = 1, int8 = 0;
+  int two_streams = 1, pml8 = 1, inner8 = 0;
   hipStream_t s2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   hipGraph_t graph = nullptr;
@@ -1630,7 +1630,7 @@ template <int MODE>
 static void batch_launch_mode(const FastBatch *b, const FastArgs &a, const int nblk[2], hipStream_t st,
                               hipStream_t st_pml)
 {
-  if (nblk[1] > 0) batch_launch_q<MODE, 1>(a, b->R, nblk[1], b->int8 != 0, st);
+  if (nblk[1] > 0) batch_launch_q<MODE, 1>(a, b->R, nblk[1], b->inner8 != 0, st);
   if (nblk[0] > 0) batch_launch_q<MODE, 0>(a, b->R, nblk[0], b->pml8 != 0, st_pml);
 }
 
@@ -1827,7 +1827,7 @@ int fast_batch_set_launch(FastBatch *b, int mode)
   b->two_streams = mode == FAST_LAUNCH_TWO_STREAMS || mode == FAST_LAUNCH_TWO_STREAMS_PML8 ||
                    mode == FAST_LAUNCH_TWO_STREAMS_W8;
   b->pml8 = mode >= FAST_LAUNCH_SPLIT_PML8;
-  b->int8 = (mode == FAST_LAUNCH_SPLIT_W8 || mode == FAST_LAUNCH_TWO_STREAMS_W8) && b->R >= 8;
+  b->inner8 = (mode == FAST_LAUNCH_SPLIT_W8 || mode == FAST_LAUNCH_TWO_STREAMS_W8) && b->R >= 8;
   return FDTD_OK;
 }
 
