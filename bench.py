@@ -107,6 +107,34 @@ def prof_read(L):
     return list(out)
 
 
+def cpu_reference(mode, workload, steps=120):
+    """The reference itself on a bounded sample of the same workload: its MPI-TM /
+    TE UPML solver compiled from its own sources (oracle/_ref/ref_harness, one MPI
+    rank, one core), `steps` simulator_calc() calls including the per-step NTFF,
+    timed inside the harness.  Runs as a child process before this process touches
+    the GPU.  None when the harness is not built or does not run."""
+    import subprocess
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    if not os.access(exe, os.X_OK):
+        return None
+    model, w, h, hu, pml, lam, ang, params, _ = WORKLOADS[workload]
+    if model != "MIE_CYLINDER" or params != {"circle_radius_nm": 500, "circle_n": 1.6}:
+        return None  # the reference's cylinder is r = 500 nm, n = 1.6 (circleModel.c:6-10)
+    cmd = [exe, "mode=time", "solver=%d" % (4 if mode == "tm" else 5), "model=1", f"width={w}", f"height={h}",
+           f"h={hu}", f"pml={pml}", f"lambda={lam}", f"angle={ang}", f"steps={steps}", "warm=2"]
+    try:
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=240)
+        res = json.loads(r.stdout.strip().splitlines()[-1])
+    except (OSError, ValueError, IndexError, subprocess.SubprocessError):
+        return None
+    cells = res["N_PX"] * res["N_PY"]
+    return {"value": round(cells * res["steps"] / res["seconds"] / 1e6, 3), "unit": "Mcell-updates/s", "cores": 1,
+            "kind": "reference",
+            "sample": f"{res['steps']} steps of the same {res['N_PX']}x{res['N_PY']} {workload} {mode.upper()} grid, "
+                      f"the reference's own MPI {mode.upper()} UPML solver (FP64, per-step NTFF) built from its "
+                      f"sources with gcc -O2, one MPI rank ({res['seconds']:.1f} s on one host core)"}
+
+
 def cpu_baseline(mode, workload):
     """The FP64 CPU oracle (the reference's MPI-TM/TE step restated in C, gcc -O2,
     one core) on a bounded sample of the same workload, including the per-step
@@ -338,6 +366,12 @@ def main():
     if args.gpus != 1:
         sys.exit(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU sharding is "
                  "described in DESIGN.md and not built")
+    # the reference's CPU path, timed first: a child process started before anything
+    # here has touched the GPU
+    cpu_ref = None
+    if not args.no_cpu and args.workload == "mie1056":
+        log("cpu baseline (the reference, one core)")
+        cpu_ref = cpu_reference(args.mode, args.workload)
     import numpy as np
 
     import mpifdtd_amd as m
@@ -434,10 +468,10 @@ def main():
         extra["speedup_vs_fp64_spectra"] = round(extra["spectra_per_s"] / extra["fp64_spectra_per_s"], 2)
     # the CPU leg runs on the default workload only: the one-core oracle needs minutes
     # just to sample the Morpho model's eps on 34 M cells
-    run_cpu = not args.no_cpu and wl == "mie1056"
+    run_cpu = not args.no_cpu and wl == "mie1056" and cpu_ref is None
     if run_cpu:
         log("cpu baseline (oracle, one core)")
-    cpu = cpu_baseline(args.mode, wl) if run_cpu else None
+    cpu = cpu_ref if cpu_ref is not None else (cpu_baseline(args.mode, wl) if run_cpu else None)
     line = {
         "metric": "Mcell-updates/s + % HBM roofline; spectra/s at <=1e-4 far-field err",
         "value": round(mcups, 1), "unit": "Mcell-updates/s", "n_gpus": 1, "steps": K, "warmup": W,

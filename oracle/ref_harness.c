@@ -5,8 +5,8 @@
  * main.c, whose model/solver choice is hard-coded (main.c:155-156).
  *
  * TEST INFRASTRUCTURE ONLY: used by tests/golden/make_golden.py to write the
- * golden fixtures that pin oracle/fdtd_oracle.c.  Never shipped or called by
- * the product.
+ * golden fixtures that pin oracle/fdtd_oracle.c, and timed (mode=time) as the
+ * CPU baseline of bench.py.  Never shipped or called by the product.
  *
  * usage: ref_harness key=value ...
  *   mode=run|eps   solver=<enum SOLVER>  model=<enum MODEL>  sweep=<models_isFinish calls>
@@ -21,11 +21,16 @@
  * mode=eps : field_init + models_initModel, dumps eps_xy.bin (D_XY at (x,y)),
  *            eps_dy.bin (D_Y at (x+.5,y)), eps_dx.bin (D_X at (x,y+.5)) for all
  *            N_PX x N_PY cells.
+ * mode=time: simulator_init, `warm` x simulator_calc, then `steps` x
+ *            simulator_calc timed with CLOCK_MONOTONIC; prints one JSON line
+ *            {"N_PX", "N_PY", "steps", "seconds"} and writes nothing.  The CPU
+ *            baseline of bench.py (the reference timed beside the GPU run).
  */
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #include <complex.h>
+#include <time.h>
 #include <unistd.h>
 #include <sys/stat.h>
 #include <mpi.h>
@@ -102,6 +107,19 @@ int main(int argc, char **argv)
 
   simulator_setSolver((enum SOLVER)solver);
   simulator_init(fi);
+  if (strcmp(mode, "time") == 0) {
+    const int warm = atoi(arg(argc, argv, "warm", "2"));
+    for (int n = 0; n < warm; n++) simulator_calc();
+    struct timespec t0, t1;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    for (int n = 0; n < fi.stepNum; n++) simulator_calc();
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    FieldInfo_S st = field_getFieldInfo_S();
+    printf("{\"N_PX\": %d, \"N_PY\": %d, \"steps\": %d, \"seconds\": %.6f}\n", st.N_PX, st.N_PY, fi.stepNum,
+           (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec));
+    fflush(stdout);
+    _exit(0);  /* no finish(): it would write the reference's output files */
+  }
   for (int n = 0; n < fi.stepNum; n++) simulator_calc();
 
   FieldInfo_S s = field_getFieldInfo_S();
