@@ -61,6 +61,8 @@ def parse():
                          "6 / 7: 4 / 5 with 8-wavefront interior workgroups "
                          "(-1: autotune's choice, or auto)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--ntff-overlap", type=int, default=-1,
+                    help="contour sample work beside the next block (1), after it (0), by grid size (-1)")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 far-field check")
     ap.add_argument("--ntff", action="store_true",
                     help="run the reference's per-step time-domain NTFF inside the timed steps (the MPI solvers "
@@ -222,6 +224,7 @@ def run_sweep81(args, m):
     sim = make("fp32", steps)
     cells = sim.N_PX * sim.N_PY
     sim.set_launch_mode(launch)
+    sim.set_ntff_overlap(args.ntff_overlap)
     sim.set_dft(lams)
     L.fdtd_hip_synchronize()
     L.fdtd_hip_profile_enable(1)
@@ -405,6 +408,7 @@ def main():
     sim = make_sim(m, args.precision, args.mode, total, T or 4, R or 16, workload=wl, ntff=args.ntff)
     if args.precision == "fp32":
         sim.set_launch_mode(launch)
+        sim.set_ntff_overlap(args.ntff_overlap)
     cells = sim.N_PX * sim.N_PY
     sim.calc(W)
     if args.precision == "fp32":

@@ -116,6 +116,10 @@ int fdtd_solver_setBlocking(int T, int R);
 /* FP32: launch shape of the active solver's UPML and interior tiles
  * (FAST_LAUNCH_* of fdtd_hip.h); FDTD_ESTATE for FP64 solvers */
 int fdtd_solver_setLaunchMode(int mode);
+/* FP32 with contour samples: the sample packing, binning and running DFT of a
+ * block beside the next block on a stream of their own (1), in order after it
+ * (0), or by grid size (-1, the default; fdtd_hip_grid_set_ntff_overlap) */
+int fdtd_solver_setNTFFOverlap(int on);
 /* the setting, or with AUTO the shape of the last launch */
 int fdtd_solver_getLaunchMode(int *mode);
 /* blocking of the active solver; FDTD_ESTATE when it does not queue steps (FP64) */

@@ -147,6 +147,7 @@ def lib():
     sig("fdtd_solver_setBlocking", I, I, I)
     sig("fdtd_solver_getBlocking", I, ip, ip)
     sig("fdtd_solver_setLaunchMode", I, I)
+    sig("fdtd_solver_setNTFFOverlap", I, I)
     sig("fdtd_solver_getLaunchMode", I, ip)
     # mpifdtd_batch.h
     sig("fdtd_batch_create", I, ctypes.POINTER(BatchItem), I, I, I, ctypes.POINTER(P))

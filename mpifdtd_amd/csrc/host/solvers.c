@@ -77,6 +77,12 @@ int fdtd_solver_setLaunchMode(int mode)
   return fdtd_hip_grid_set_launch_mode(slv_[active_].g, mode);
 }
 
+int fdtd_solver_setNTFFOverlap(int on)
+{
+  if (active_ < 0 || !slv_[active_].blocked) return FDTD_ESTATE;
+  return fdtd_hip_grid_set_ntff_overlap(slv_[active_].g, on);
+}
+
 int fdtd_solver_getLaunchMode(int *mode)
 {
   if (active_ < 0 || !slv_[active_].blocked) return FDTD_ESTATE;

@@ -87,6 +87,11 @@ class Simulator:
         of R/2 rows (FAST_LAUNCH_* of fdtd_hip.h)."""
         check(N.lib().fdtd_solver_setLaunchMode(int(mode)), "fdtd_solver_setLaunchMode")
 
+    def set_ntff_overlap(self, on):
+        """FP32 with contour samples: 1 the packing, binning and running DFT of a block beside
+        the next block on a stream of their own, 0 in order after it, -1 by grid size."""
+        check(N.lib().fdtd_solver_setNTFFOverlap(int(on)), "fdtd_solver_setNTFFOverlap")
+
     def launch_mode(self):
         """FP32: the launch shape setting, or with auto the shape of the last launch."""
         v = ctypes.c_int()
