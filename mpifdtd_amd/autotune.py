@@ -13,13 +13,14 @@ import json
 import os
 import time
 
-CANDIDATES = [(1, 16), (2, 16), (3, 16), (4, 16), (5, 16), (6, 16), (8, 16), (4, 8), (6, 8), (8, 8), (10, 8), (4, 4)]
+CANDIDATES = [(1, 16), (2, 16), (3, 16), (4, 16), (5, 16), (6, 16), (8, 16), (10, 16), (4, 8), (6, 8), (8, 8), (10, 8),
+              (4, 4)]
 
 _cache = {}
 
 
 def _key(sim):
-    return f"{sim.solver}:{sim.N_PX // 256}x{sim.N_PY // 256}:{'ntff' if sim.per_step_ntff else 'plain'}:v6"
+    return f"{sim.solver}:{sim.N_PX // 256}x{sim.N_PY // 256}:{'ntff' if sim.per_step_ntff else 'plain'}:v7"
 
 
 def _load():
