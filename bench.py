@@ -206,18 +206,19 @@ def run_sweep81(args, m):
         probe = make("fp32", 4000)
         probe.set_dft(lams)
         probe.calc(40)
-        for Tc, Rc in ((8, 16), (8, 8), (6, 8)):
+        for Tc, Rc in ((8, 16), (10, 16), (8, 8), (6, 8)):
             for sh in (2, 3, 4, 5, 6, 7):
+                n = 6 * Tc  # whole blocks only
                 probe.set_blocking(Tc, Rc)
                 probe.set_launch_mode(sh)
-                probe.calc(16)
+                probe.calc(2 * Tc)
                 probe.set_blocking(Tc, Rc)
                 L.fdtd_hip_synchronize()
                 t0 = time.perf_counter()
-                probe.calc(48)
+                probe.calc(n)
                 probe.set_blocking(Tc, Rc)
                 L.fdtd_hip_synchronize()
-                tune[(Tc, Rc, sh)] = (time.perf_counter() - t0) * 1e6 / 48
+                tune[(Tc, Rc, sh)] = (time.perf_counter() - t0) * 1e6 / n
         probe.finish()
         T, R, launch = min(tune, key=tune.get)
     log(f"sweep81: {len(lams)} wavelengths, {steps} steps, T={T} R={R} launch={launch}")

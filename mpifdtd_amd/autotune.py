@@ -20,7 +20,7 @@ _cache = {}
 
 
 def _key(sim):
-    return f"{sim.solver}:{sim.N_PX // 256}x{sim.N_PY // 256}:{'ntff' if sim.per_step_ntff else 'plain'}:v7"
+    return f"{sim.solver}:{sim.N_PX // 256}x{sim.N_PY // 256}:{'ntff' if sim.per_step_ntff else 'plain'}:v8"
 
 
 def _load():
@@ -54,16 +54,17 @@ def tune(m, make_sim, probe_steps=240, candidates=CANDIDATES):
     shapes = {}
 
     def probe(name, T, R, shape):
+        n = T * max(1, round(probe_steps / T))  # whole blocks only: no partial block in the timing
         sim.set_blocking(T, R)
         sim.set_launch_mode(shape)
         sim.calc(2 * T)  # warm the variant
         sim.set_blocking(T, R)
         L.fdtd_hip_synchronize()
         t0 = time.perf_counter()
-        sim.calc(probe_steps)
-        sim.set_blocking(T, R)  # flush the partial block
+        sim.calc(n)
+        sim.set_blocking(T, R)  # flushes nothing: n is a multiple of T
         L.fdtd_hip_synchronize()
-        timings[name] = (time.perf_counter() - t0) * 1e6 / probe_steps
+        timings[name] = (time.perf_counter() - t0) * 1e6 / n
         shapes[name] = (T, R, shape)
 
     try:
