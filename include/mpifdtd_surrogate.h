@@ -5,7 +5,7 @@
  * time (main.c:114-138, models_isFinish).  The surrogate maps 12 morpho-scale
  * geometry parameters plus the wavelength (each scaled to [0, 1]) to the
  * far-field reflectance at 180 back-hemisphere angles; FDTD spectra train it
- * and judge it, and FDTD answers the queries on which an ensemble of such
+ * and measure its error, and FDTD answers the queries on which an ensemble of such
  * networks disagrees.
  *
  * The network is a stack of dense layers, GELU between them, linear output.

@@ -3,8 +3,8 @@
 Builds oracle/_ref/ref_harness (the reference's own C sources, compiled where
 they lie under /root/reference, driven by oracle/ref_harness.c) and runs it on
 small configurations; the outputs are stored as compressed .npz fixtures next
-to this script.  Run in the build container only (the GPU box has no
-/root/reference):
+to this script.  Run only where /root/reference exists (the fixtures, not
+the reference, go wherever the tests run):
 
     python tests/golden/make_golden.py
 """
