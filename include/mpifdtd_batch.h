@@ -47,6 +47,20 @@ int fdtd_batch_field(fdtd_batch *b, int i, int which, double *out);
 int fdtd_batch_far_field(fdtd_batch *b, int i, double *out360);
 void fdtd_batch_destroy(fdtd_batch *b);
 
+/* The reference's parameter sweep as batch items (main.c:114-138 nextSimulation,
+ * main.c:71-88 calcFieldSize): for the current parameters of `model`
+ * (models_setParams, or the reference's start values), one item per incidence
+ * angle from start_deg to end_deg by delta_deg; then models_isFinish() steps the
+ * model to its next structure, until it reports the sweep finished.  With
+ * fit_size, each structure's field is the model's models_needSize() plus
+ * 2 h (pml + 5) + 200 nm per axis, as calcFieldSize sets it; otherwise base's
+ * width_nm / height_nm.  Writes at most max items (items may be NULL with max 0)
+ * and returns how many the sweep has, or an FDTD_* error (< 0).  The model is
+ * left selected, with its parameters at the end of the sweep, as after the
+ * reference's loop. */
+int fdtd_batch_sweep(FieldInfo base, int solver, int model, int start_deg, int end_deg, int delta_deg, int fit_size,
+                     fdtd_batch_item *items, int max);
+
 #ifdef __cplusplus
 }
 #endif

@@ -160,6 +160,7 @@ def lib():
     sig("fdtd_batch_field", I, P, I, I, dp)
     sig("fdtd_batch_far_field", I, P, I, dp)
     sig("fdtd_batch_destroy", V, P)
+    sig("fdtd_batch_sweep", I, FieldInfo, I, I, I, I, I, I, ctypes.POINTER(BatchItem), I)
     # mpifdtd_surrogate.h
     vp = ctypes.c_void_p
     fp = ctypes.POINTER(ctypes.c_float)

@@ -31,6 +31,30 @@ def item(solver, model, width_nm, height_nm, h_u_nm, pml, lambda_nm, angle_deg, 
     return it
 
 
+def reference_sweep(solver, model, h_u_nm, pml, lambda_nm, steps, start_deg=0, end_deg=0, delta_deg=1,
+                    fit_size=True, width_nm=0, height_nm=0, **params):
+    """The reference's parameter sweep (main.c:114-138 nextSimulation) as batch items:
+    every incidence angle from start_deg to end_deg by delta_deg for each structure of
+    `model`, from the given parameters (the reference's start values by default) until
+    models_isFinish() ends the sweep; each structure's field sized as calcFieldSize
+    does (main.c:71-88) when fit_size, else width_nm x height_nm."""
+    L = N.lib()
+    L.models_setModel(model)
+    p = N.ModelParams()
+    L.models_defaultParams(ctypes.byref(p))
+    for k, v in params.items():
+        setattr(p, k, v)
+    L.models_setParams(ctypes.byref(p))
+    base = N.FieldInfo(width_nm, height_nm, h_u_nm, pml, lambda_nm, start_deg, steps)
+    args = (base, solver, model, start_deg, end_deg, delta_deg, 1 if fit_size else 0)
+    n = L.fdtd_batch_sweep(*args, None, 0)
+    check(n if n < 0 else 0, "fdtd_batch_sweep")
+    L.models_setParams(ctypes.byref(p))  # the same sweep again, written this time
+    arr = (BatchItem * n)()
+    check(min(0, L.fdtd_batch_sweep(*args, arr, n)), "fdtd_batch_sweep")
+    return list(arr)
+
+
 class Batch:
     def __init__(self, items, T=8, R=16):
         L = N.lib()

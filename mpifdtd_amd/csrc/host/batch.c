@@ -124,3 +124,39 @@ int fdtd_batch_far_field(fdtd_batch *b, int i, double *out360)
 }
 
 void fdtd_batch_destroy(fdtd_batch *b) { batch_free(b); }
+
+/* main.c:71-88 (calcFieldSize) and main.c:114-138 (nextSimulation): the angles of
+ * one structure, then the next structure, each rank's stride being 1 here */
+int fdtd_batch_sweep(FieldInfo base, int solver, int model, int start_deg, int end_deg, int delta_deg, int fit_size,
+                     fdtd_batch_item *items, int max)
+{
+  if (solver != MPI_TM_UPML_2D && solver != MPI_TE_UPML_2D) return FDTD_EINVAL;
+  if (model != NO_MODEL && model != MIE_CYLINDER && model != LAYER && model != MORPHO_SCALE && model != ZIGZAG)
+    return FDTD_EINVAL;
+  if (max < 0 || (max > 0 && !items)) return FDTD_EINVAL;
+  if (end_deg < start_deg || (delta_deg <= 0 && end_deg > start_deg)) return FDTD_EINVAL;
+  const int per_structure = delta_deg > 0 ? (end_deg - start_deg) / delta_deg + 1 : 1;
+  fdtd_model_params p;
+  models_setModel((enum MODEL)model);  /* keeps the model's current parameters */
+  int n = 0;
+  for (int guard = 0; guard < 1000000; guard++) {
+    models_getParams(&p);
+    FieldInfo fi = base;
+    if (fit_size) {
+      int x_nm, y_nm;
+      models_needSize(&x_nm, &y_nm);
+      fi.width_nm = x_nm + fi.h_u_nm * (fi.pml + 5) * 2 + 200;
+      fi.height_nm = y_nm + fi.h_u_nm * (fi.pml + 5) * 2 + 200;
+    }
+    for (int a = 0; a < per_structure; a++, n++) {
+      if (n >= max) continue;
+      fi.angle_deg = start_deg + a * (delta_deg > 0 ? delta_deg : 0);
+      items[n].field = fi;
+      items[n].solver = solver;
+      items[n].model = model;
+      items[n].params = p;
+    }
+    if (models_isFinish()) return n;
+  }
+  return FDTD_EINVAL;  /* a model whose sweep never ends */
+}

@@ -114,3 +114,20 @@ def test_batch_launch_shapes_equal_default(shape):
     for i in range(len(CASES)):
         for x, y in zip(got[i], ref[i]):
             assert np.array_equal(x, y), i
+
+
+def test_reference_sweep_runs_as_a_batch():
+    """The first structures of the reference's zigzag sweep (fitted field sizes, two
+    angles each) run as one batch; every instance reaches its step count and has a
+    finite, non-zero far field."""
+    import mpifdtd_amd as m
+    from mpifdtd_amd import batch as B
+    items = B.reference_sweep(5, 5, h_u_nm=20, pml=10, lambda_nm=500, steps=120, start_deg=0, end_deg=30,
+                              delta_deg=30)[:6]
+    assert len(items) == 6 and len({(it.field.width_nm, it.params.zz_degree) for it in items}) == 3
+    with B.Batch(items, T=T, R=R) as b:
+        b.run()
+        assert b.sync() == [120] * 6
+        for i in range(6):
+            ff = b.far_field(i)
+            assert np.all(np.isfinite(ff)) and np.abs(ff).max() > 0, i

@@ -124,3 +124,57 @@ def test_model_sweeps_match_reference_rules():
             L.models_getParams(ctypes.byref(p))
             want = swept_params(model, k)
             assert all(getattr(p, key) == want[key] for key in keys), (model, k)
+
+
+def test_reference_sweep_enumerates_structures_then_angles():
+    """fdtd_batch_sweep follows nextSimulation (main.c:114-138): every angle of a
+    structure, then models_isFinish() to the next structure, until the sweep ends;
+    fitted field sizes follow calcFieldSize (main.c:71-88)."""
+    import mpifdtd_amd as m
+    from conftest import swept_params
+    from mpifdtd_amd import _native as N
+    from mpifdtd_amd.batch import reference_sweep
+    L = N.lib()
+    # the number of structures of the multilayer sweep from the reference's start values
+    L.models_setModel(2)
+    L.models_setParams(ctypes.byref(m.model_params()))
+    structures = 1
+    while not L.models_isFinish():
+        structures += 1
+    assert structures > 1
+    items = reference_sweep(4, 2, h_u_nm=10, pml=10, lambda_nm=500, steps=100, start_deg=0, end_deg=30,
+                            delta_deg=10)
+    assert len(items) == 4 * structures
+    keys = ("ml_thick_nm0", "ml_thick_nm1", "ml_edge_rate", "ml_branch_nm", "ml_layer_num")
+    for k, it in enumerate(items):
+        s, a = divmod(k, 4)
+        assert it.field.angle_deg == 10 * a and it.solver == 4 and it.model == 2
+        want = swept_params(2, s)
+        assert all(getattr(it.params, key) == want[key] for key in keys), k
+        L.models_setParams(ctypes.byref(it.params))
+        x, y = ctypes.c_int(), ctypes.c_int()
+        L.models_needSize(ctypes.byref(x), ctypes.byref(y))
+        assert (it.field.width_nm, it.field.height_nm) == (x.value + 10 * 15 * 2 + 200, y.value + 10 * 15 * 2 + 200)
+        assert (it.field.h_u_nm, it.field.pml, it.field.lambda_nm, it.field.stepNum) == (10, 10, 500, 100)
+
+
+def test_reference_sweep_limits_and_errors():
+    import mpifdtd_amd as m
+    from mpifdtd_amd import _native as N
+    L = N.lib()
+    fi = N.FieldInfo(2000, 2000, 10, 10, 500, 0, 100)
+    L.models_setModel(5)
+    L.models_setParams(ctypes.byref(m.model_params()))
+    n = L.fdtd_batch_sweep(fi, 5, 5, 0, 0, 1, 0, None, 0)  # zigzag, one angle, fixed size
+    assert n > 1
+    L.models_setParams(ctypes.byref(m.model_params()))
+    arr = (N.BatchItem * 3)()
+    assert L.fdtd_batch_sweep(fi, 5, 5, 0, 0, 1, 0, arr, 3) == n  # writes 3, reports all
+    assert [(a.field.width_nm, a.field.height_nm, a.model) for a in arr] == [(2000, 2000, 5)] * 3
+    assert arr[0].params.zz_degree != arr[1].params.zz_degree
+    for bad in ((fi, 2, 5, 0, 0, 1, 0, None, 0),      # not an MPI solver
+                (fi, 4, 4, 0, 0, 1, 0, None, 0),      # concentric circles: out of scope, refused
+                (fi, 4, 5, 0, 30, 0, 0, None, 0),     # no angle step
+                (fi, 4, 5, 30, 0, 10, 0, None, 0),    # end before start
+                (fi, 4, 5, 0, 0, 1, 0, None, 2)):     # no output array
+        assert L.fdtd_batch_sweep(*bad) < 0, bad
