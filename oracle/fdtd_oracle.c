@@ -500,6 +500,38 @@ static void set_coef_tm(orc_sim *s)
   }
 }
 
+/* Split-field PML TM (fdtdTM.c:184-243): per-cell coefficients from eps and the
+   x / y conductivities, C[0] C_EZX, C[1] C_EZXLX, C[2] C_EZY, C[3] C_EZYLY,
+   C[4] C_HX, C[5] C_HXLY, C[6] C_HY, C[7] C_HYLX; eps maps EPS_EZ (D_XY),
+   EPS_HX (j + 0.5, D_Y), EPS_HY (i + 0.5, D_X) */
+static double pml_coef(double ep_mu, double sig) { return (1.0 - sig / ep_mu) / (1.0 + sig / ep_mu); } /* field.c:288-291 */
+static double pml_coef_lxy(double ep_mu, double sig) { return 1.0 / (ep_mu + sig); }                 /* field.c:292-295 */
+static void set_coef_tm_split(orc_sim *s)
+{
+  const double R = 1.0e-8, M = 2.0;
+  const double sig_max = -(M + 1.0) * s->EPS0 * s->C0 / 2.0 / s->N_PML * log(R);
+  for (int i = 0; i < s->NPXa; i++) {
+    for (int j = 0; j < s->NPYa; j++) {
+      int k = i * s->NPYa + j;
+      s->EPS[0][k] = models_eps(s, i, j, ORC_D_XY);
+      s->EPS[1][k] = models_eps(s, i, j + 0.5, ORC_D_Y);
+      s->EPS[2][k] = models_eps(s, i + 0.5, j, ORC_D_X);
+      double sig_ez_x = sig_max * sigma_x(s, i);
+      double sig_ez_y = sig_max * sigma_y(s, j);
+      double sig_hx_yy = s->MU0 / s->EPS0 * (sig_max * sigma_y(s, j + 0.5));
+      double sig_hy_xx = s->MU0 / s->EPS0 * (sig_max * sigma_x(s, i + 0.5));
+      s->C[0][k] = pml_coef(s->EPS[0][k], sig_ez_x);
+      s->C[1][k] = pml_coef_lxy(s->EPS[0][k], sig_ez_x);
+      s->C[2][k] = pml_coef(s->EPS[0][k], sig_ez_y);
+      s->C[3][k] = pml_coef_lxy(s->EPS[0][k], sig_ez_y);
+      s->C[4][k] = pml_coef(s->MU0, sig_hx_yy);
+      s->C[5][k] = pml_coef_lxy(s->MU0, sig_hx_yy);
+      s->C[6][k] = pml_coef(s->MU0, sig_hy_xx);
+      s->C[7][k] = pml_coef_lxy(s->MU0, sig_hy_xx);
+    }
+  }
+}
+
 /* TE: mpiTE_UPML.c:489-536 (MPI) and fdtdTE_upml.c:361-412 (plain); no /cos(pi/3) (Q7) */
 static void set_coef_te(orc_sim *s)
 {
@@ -547,7 +579,7 @@ orc_sim *orc_create(const orc_config *cfg)
   field_init(s);
   models_init(s);  /* models_initModel runs after field_init (simulator.c:229-231) */
   s->mpi = (cfg->solver == ORC_MPI_TM_UPML || cfg->solver == ORC_MPI_TE_UPML);
-  s->tm = (cfg->solver == ORC_MPI_TM_UPML || cfg->solver == ORC_TM_UPML);
+  s->tm = (cfg->solver == ORC_MPI_TM_UPML || cfg->solver == ORC_TM_UPML || cfg->solver == ORC_TM_2D);
   /* MPI single rank: SUB_N_PX = N_PX + 2 with a ghost ring (mpiTM_UPML.c:737-743) */
   s->off = s->mpi ? 1 : 0;
   s->NPXa = s->N_PX + 2 * s->off;
@@ -561,7 +593,9 @@ orc_sim *orc_create(const orc_config *cfg)
     s->NTb[a] = (dcx *)calloc(nn, sizeof(dcx));
     s->NT[a] = s->NTb[a] + NT_GUARD;
   }
-  if (s->tm) set_coef_tm(s); else set_coef_te(s);
+  if (cfg->solver == ORC_TM_2D) set_coef_tm_split(s);
+  else if (s->tm) set_coef_tm(s);
+  else set_coef_te(s);
   s->R0 = 1.0e6 * to_cell(s, 500);  /* ntffTM_init (ntffTM.c:31), ntffTE_init (ntffTE.c:16) */
   return s;
 }
@@ -1069,9 +1103,74 @@ static void plain_te_ntff(orc_sim *s)
 }
 
 /* ------------------------------------------------------------------ steps */
+/* Split-field PML TM step (fdtdTM.c:269-327): calcH, calcE (Ezx, Ezy, then
+   Ez = Ezx + Ezy), then the scattered wave on Ezx (field.c:179-199, gap 0);
+   slots F[0] Ez, F[1] Ezx, F[2] Ezy, F[3] Hx, F[6] Hy */
+static void split_tm_step(orc_sim *s)
+{
+  const int NPX = s->NPXa, NPY = s->NPYa;
+  dcx *Ez = s->F[0], *Ezx = s->F[1], *Ezy = s->F[2], *Hx = s->F[3], *Hy = s->F[6];
+  double **C = s->C;
+  for (int i = 1; i < NPX - 1; i++)
+    for (int j = 1; j < NPY - 1; j++) {
+      int k = IDX(i, j);
+      Hx[k] = C[4][k] * Hx[k] - C[5][k] * (Ezx[k + 1] - Ezx[k] + Ezy[k + 1] - Ezy[k]);
+    }
+  for (int i = 1; i < NPX - 1; i++)
+    for (int j = 1; j < NPY - 1; j++) {
+      int k = IDX(i, j);
+      Hy[k] = C[6][k] * Hy[k] + C[7][k] * (Ezx[k + NPY] - Ezx[k] + Ezy[k + NPY] - Ezy[k]);
+    }
+  for (int i = 1; i < NPX - 1; i++)
+    for (int j = 1; j < NPY - 1; j++) {
+      int k = IDX(i, j);
+      Ezx[k] = C[0][k] * Ezx[k] + C[1][k] * (Hy[k] - Hy[k - NPY]);
+    }
+  for (int i = 1; i < NPX - 1; i++)
+    for (int j = 1; j < NPY - 1; j++) {
+      int k = IDX(i, j);
+      Ezy[k] = C[2][k] * Ezy[k] - C[3][k] * (Hx[k] - Hx[k - 1]);
+    }
+  for (int i = 1; i < NPX - 1; i++)
+    for (int j = 1; j < NPY - 1; j++) {
+      int k = IDX(i, j);
+      Ez[k] = Ezx[k] + Ezy[k];
+    }
+  double time = s->time, w_s = s->Omega_s, ray_coef = s->ray_coef, k_s = s->K_s;
+  double rad = s->Angle_deg * M_PI / 180.0;
+  double ks_cos = cos(rad) * k_s, ks_sin = sin(rad) * k_s;
+  for (int i = 1; i < NPX - 1; i++)
+    for (int j = 1; j < NPY - 1; j++) {
+      int k = IDX(i, j);
+      double kr = (i + 0.0) * ks_cos + (j + 0.0) * ks_sin;
+      Ezx[k] += ray_coef * (s->EPS0 / s->EPS[0][k] - 1.0) *
+                (cexp(I * (kr - w_s * (time + 0.5))) - cexp(I * (kr - w_s * (time - 0.5))));
+    }
+}
+
+void orc_elliptic(const orc_sim *s, double *out181)
+{
+  const dcx *Ez = s->F[0];
+  const int NPY = s->NPYa;
+  for (int ang = 180; ang >= 0; ang--) {
+    double rad = ang * M_PI / 180.0;
+    double x = 1.2 * s->lambda_s * cos(rad) + s->N_PX / 2.0;
+    double y = 1.2 * s->lambda_s * sin(rad) + s->N_PY / 2.0;
+    int i = (int)floor(x), j = (int)floor(y);
+    double dx = x - i, dy = y - j;
+    int k = i * NPY + j;
+    dcx v = Ez[k] * (1.0 - dx) * (1.0 - dy) + Ez[k + NPY] * dx * (1.0 - dy) + Ez[k + 1] * (1.0 - dx) * dy +
+            Ez[k + NPY + 1] * dx * dy;
+    out181[180 - ang] = creal(v) * creal(v) + cimag(v) * cimag(v);
+  }
+}
+
 void orc_step(orc_sim *s)
 {
   switch (s->cfg.solver) {
+  case ORC_TM_2D:
+    split_tm_step(s);
+    break;
   case ORC_MPI_TM_UPML: /* update (mpiTM_UPML.c:196-217) */
     tm_calc_jd_e(s);
     mpi_tm_scattered_wave(s, s->F[TM_EZ], s->EPS[0]);

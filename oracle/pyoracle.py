@@ -12,6 +12,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "liboracle.so")
 
+TM_2D = 0  # split-field PML TM (fdtdTM.c)
 TM_UPML, TE_UPML, MPI_TM_UPML, MPI_TE_UPML = 2, 3, 4, 5
 NO_MODEL, MIE_CYLINDER, LAYER, MORPHO_SCALE, ZIGZAG = 0, 1, 2, 3, 5
 D_X, D_Y, D_XY = 0, 1, 2
@@ -95,6 +96,7 @@ def lib():
         L.orc_sigma_y.argtypes = [P, D, D]
         L.orc_sigma_y.restype = D
         L.orc_cfft.argtypes = [dp, I]
+        L.orc_elliptic.argtypes = [P, dp]
         _lib = L
     return _lib
 
@@ -183,6 +185,12 @@ class Sim:
     def far_field(self):
         out = np.empty(360, np.complex128)
         lib().orc_ntff_frequency(self.h, _dp(out))
+        return out
+
+    def elliptic(self):
+        """TM_2D: the 181 values field_outputElliptic writes (field.c:322-341), for 0..180."""
+        out = np.empty(181, np.float64)
+        lib().orc_elliptic(self.h, _dp(out))
         return out
 
     def time_spectra(self):

@@ -12,7 +12,8 @@
  *   mode=run|eps   solver=<enum SOLVER>  model=<enum MODEL>  sweep=<models_isFinish calls>
  *   width=<nm> height=<nm> h=<nm> pml=<cells> lambda=<nm> angle=<deg> steps=<n>
  *   out=<directory>
- * mode=run : simulator_init, `steps` x simulator_calc, then dumps
+ * mode=run : simulator_init, `steps` x simulator_calc, then dumps (solvers TM_2D,
+ *            TM/TE_UPML_2D, MPI_TM/TE_UPML_2D)
  *            f0/f1/f2.bin (TM: Ez,Hx,Hy  TE: Ex,Ey,Hz; solver's own layout),
  *            eps.bin (getEps), meta.txt, ff.bin (TM: ntffTM_Frequency on the
  *            fields repacked to N_PX x N_PY, after ntffTM_init), then
@@ -42,6 +43,7 @@
 #include "mpiTE_UPML.h"
 #include "fdtdTM_upml.h"
 #include "fdtdTE_upml.h"
+#include "fdtdTM.h"
 
 static const char *arg(int argc, char **argv, const char *key, const char *def)
 {
@@ -124,7 +126,7 @@ int main(int argc, char **argv)
 
   FieldInfo_S s = field_getFieldInfo_S();
   int mpi = (solver == MPI_TM_UPML_2D || solver == MPI_TE_UPML_2D);
-  int tm = (solver == MPI_TM_UPML_2D || solver == TM_UPML_2D);
+  int tm = (solver == MPI_TM_UPML_2D || solver == TM_UPML_2D || solver == TM_2D);
   int npx = mpi ? s.N_PX + 2 : s.N_PX, npy = mpi ? s.N_PY + 2 : s.N_PY;
   size_t n = (size_t)npx * npy;
   double complex *f[3];
@@ -138,6 +140,9 @@ int main(int argc, char **argv)
   } else if (solver == TM_UPML_2D) {
     f[0] = fdtdTM_upml_getEz(); f[1] = fdtdTM_upml_getHx(); f[2] = fdtdTM_upml_getHy();
     eps = fdtdTM_upml_getEps();
+  } else if (solver == TM_2D) {
+    f[0] = fdtdTM_getEz(); f[1] = fdtdTM_getHx(); f[2] = fdtdTM_getHy();
+    eps = fdtdTM_getEps();
   } else {
     f[0] = fdtdTE_upml_getEx(); f[1] = fdtdTE_upml_getEy(); f[2] = fdtdTE_upml_getHz();
     eps = fdtdTE_upml_getEps();
@@ -178,7 +183,7 @@ int main(int argc, char **argv)
 
   if (chdir(out) != 0) { perror(out); return 3; }
   if (!mpi) {
-    simulator_reset(); /* plain solvers write the spectra files here */
+    simulator_reset(); /* plain UPML solvers write the spectra files here, TM_2D tm_<h>nm.txt */
   } else if (!tm) {
     mkdir("MPI_TE_UPML", 0755);
     void (*fin)(void) = mpi_fdtdTE_upml_getFinish();

@@ -6,7 +6,7 @@ small configurations; the outputs are stored as compressed .npz fixtures next
 to this script.  Run only where /root/reference exists (the fixtures, not
 the reference, go wherever the tests run):
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [fixture names: only these]
 """
 import os
 import shutil
@@ -30,6 +30,8 @@ RUN_CASES = [
     ("te_mie_pulse", 3, 1, 0, 4000, 4000, 50, 10, 500, 0, 700),
     ("mpitm_morpho", 4, 3, 0, 1600, 1800, 20, 8, 500, 0, 150),
     ("mpite_zigzag", 5, 5, 0, 1200, 1500, 20, 8, 450, 0, 150),
+    ("tm2d_mie_a0", 0, 1, 0, 4000, 4000, 50, 10, 500, 0, 300),      # split-field PML TM (fdtdTM.c)
+    ("tm2d_zigzag_a30_rect", 0, 5, 0, 1200, 1500, 20, 8, 450, 30, 150),
 ]
 
 
@@ -68,11 +70,14 @@ def read_text_rows(path, rows, ncols):
     return np.stack(out)
 
 
-def main():
+def main(only=()):
+    """Writes every fixture, or only the named ones."""
     subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True, capture_output=True)
     tmp = tempfile.mkdtemp(prefix="golden_")
     try:
         for (name, solver, model, sweep, w, h_nm, hu, pml, lam, ang, steps) in RUN_CASES:
+            if only and name not in only:
+                continue
             out = os.path.join(tmp, name)
             run([f"mode=run", f"solver={solver}", f"model={model}", f"sweep={sweep}", f"width={w}",
                  f"height={h_nm}", f"h={hu}", f"pml={pml}", f"lambda={lam}", f"angle={ang}",
@@ -90,6 +95,9 @@ def main():
             if os.path.exists(spec):
                 full = np.fromfile(spec, np.float64).reshape(321, 360)
                 d["spectra"] = full[::4]           # lambda = 380, 384, ..., 700
+            ell = os.path.join(out, f"tm_{hu}nm.txt")  # TM_2D reset(): field_outputElliptic of Ez
+            if os.path.exists(ell):
+                d["elliptic"] = np.loadtxt(ell)[:, 1]
             tdir = os.path.join(out, "MPI_TE_UPML")
             if os.path.isdir(tdir):
                 rows = set(range(0, 360, 15))
@@ -100,6 +108,8 @@ def main():
             np.savez_compressed(os.path.join(HERE, f"ref_{name}.npz"), **d)
             print("wrote", name, {k: v.shape for k, v in d.items()})
         for (name, model, sweep, w, h_nm, hu) in EPS_CASES:
+            if only and name not in only:
+                continue
             out = os.path.join(tmp, name)
             run([f"mode=eps", f"model={model}", f"sweep={sweep}", f"width={w}", f"height={h_nm}",
                  f"h={hu}", "pml=10", "lambda=500", "steps=10", f"out={out}"])
@@ -114,4 +124,4 @@ def main():
 
 
 if __name__ == "__main__":
-    sys.exit(main())
+    sys.exit(main(tuple(sys.argv[1:])))

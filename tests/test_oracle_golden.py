@@ -2,7 +2,7 @@
 
 The fixtures under tests/golden/ were written by tests/golden/make_golden.py,
 which runs the reference's C sources (mpiTM_UPML.c, mpiTE_UPML.c, fdtdTM_upml.c,
-fdtdTE_upml.c, field.c, ntffTM.c, ntffTE.c, cfft.c, ntff.c, models.c and the
+fdtdTE_upml.c, the split-field fdtdTM.c, field.c, ntffTM.c, ntffTE.c, cfft.c, ntff.c, models.c and the
 model files) as one MPI rank.  The oracle restates the same arithmetic in the
 same order, so most comparisons are bit-exact.
 """
@@ -13,7 +13,8 @@ from conftest import load_golden, rel_l2, swept_params
 from oracle import pyoracle as po
 
 RUN_CASES = ["mpitm_mie_a0", "mpitm_mie_a30_rect", "mpite_mie_a0", "mpite_mie_a60_rect",
-             "tm_mie_pulse", "te_mie_pulse", "mpitm_morpho", "mpite_zigzag"]
+             "tm_mie_pulse", "te_mie_pulse", "mpitm_morpho", "mpite_zigzag",
+             "tm2d_mie_a0", "tm2d_zigzag_a30_rect"]
 
 
 def oracle_for(g):
@@ -28,7 +29,7 @@ def test_fields_eps_farfield_match_reference(name):
     g = load_golden(name)
     sim, steps = oracle_for(g)
     sim.step(steps)
-    tm = sim.cfg.solver in (po.MPI_TM_UPML, po.TM_UPML)
+    tm = sim.cfg.solver in (po.MPI_TM_UPML, po.TM_UPML, po.TM_2D)
     which = (0, 3, 6)  # Ez,Hx,Hy / Ex,Ey,Hz
     for k, w in enumerate(which):
         ref = g[f"f{k}"]
@@ -56,6 +57,11 @@ def test_fields_eps_farfield_match_reference(name):
             ref = g[nm]
             mine = arr[rows, :steps]
             assert np.max(np.abs(mine - ref)) < 1e-19 + 1e-12 * np.max(np.abs(ref)), nm
+    if "elliptic" in g:
+        # field_outputElliptic prints "%lf": absolute resolution 5e-7
+        ell = sim.elliptic()
+        assert np.abs(g["elliptic"]).max() > 1e-3
+        assert np.max(np.abs(ell - g["elliptic"])) <= 5.0000001e-7, np.max(np.abs(ell - g["elliptic"]))
 
 
 def test_plain_solver_layout_is_unpadded():
