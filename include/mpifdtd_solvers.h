@@ -1,11 +1,14 @@
 /*
- * mpifdtd_solvers.h -- the four UPML solvers' entry points, backed by HIP.
+ * mpifdtd_solvers.h -- the four UPML solvers' and the split-field PML TM
+ * solver's entry points, backed by HIP.
  *
  * Same symbols and signatures as the reference's solver headers:
  *   mpiTM_UPML.h:5-18   (mpiTM_UPML.c:88-136)   MPI TMz UPML, run as one rank on one GPU
  *   mpiTE_UPML.h:5-19   (mpiTE_UPML.c:72-126)   MPI TEz UPML, run as one rank on one GPU
  *   fdtdTM_upml.h:5-14  (fdtdTM_upml.c:117-152) plain TMz UPML
  *   fdtdTE_upml.h:5-14                          plain TEz UPML
+ *   fdtdTM.h:5-16       (fdtdTM.c:39-78)        split-field PML TMz (FP64 only; its
+ *                                               reset() writes tm_<h>nm.txt)
  * get{Update,Init,Finish,Reset} return the solver's void(void) methods; the
  * field getters return borrowed host pointers (valid from init to finish) that
  * are refreshed from device memory on every getter call; getEps returns the
@@ -63,6 +66,18 @@ fdtd_dcomplex *fdtdTM_upml_getHx(void);
 fdtd_dcomplex *fdtdTM_upml_getHy(void);
 fdtd_dcomplex *fdtdTM_upml_getEz(void);
 double *fdtdTM_upml_getEps(void);
+
+/* fdtdTM.h:5-16 (split-field PML TMz; getEps returns EPS_EZ) */
+fdtd_method fdtdTM_getUpdate(void);
+fdtd_method fdtdTM_getFinish(void);
+fdtd_method fdtdTM_getReset(void);
+fdtd_method fdtdTM_getInit(void);
+fdtd_dcomplex *fdtdTM_getEzx(void);
+fdtd_dcomplex *fdtdTM_getEzy(void);
+fdtd_dcomplex *fdtdTM_getEz(void);
+fdtd_dcomplex *fdtdTM_getHy(void);
+fdtd_dcomplex *fdtdTM_getHx(void);
+double *fdtdTM_getEps(void);
 
 /* fdtdTE_upml.h:5-14 */
 fdtd_method fdtdTE_upml_getUpdate(void);

@@ -187,7 +187,8 @@ def lib():
     for n in ("mpi_fdtdTM_upml_getEz", "mpi_fdtdTM_upml_getHx", "mpi_fdtdTM_upml_getHy", "mpi_fdtdTM_upml_getEps",
               "mpi_fdtdTE_upml_getEx", "mpi_fdtdTE_upml_getEy", "mpi_fdtdTE_upml_getHz", "mpi_fdtdTE_upml_getEps",
               "fdtdTM_upml_getEz", "fdtdTM_upml_getHx", "fdtdTM_upml_getHy", "fdtdTM_upml_getEps",
-              "fdtdTE_upml_getEx", "fdtdTE_upml_getEy", "fdtdTE_upml_getHz", "fdtdTE_upml_getEps"):
+              "fdtdTE_upml_getEx", "fdtdTE_upml_getEy", "fdtdTE_upml_getHz", "fdtdTE_upml_getEps",
+              "fdtdTM_getEz", "fdtdTM_getHx", "fdtdTM_getHy", "fdtdTM_getEzx", "fdtdTM_getEzy", "fdtdTM_getEps"):
         sig(n, P)
     _lib = L
     return L

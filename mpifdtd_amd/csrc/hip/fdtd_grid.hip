@@ -283,6 +283,62 @@ __global__ __launch_bounds__(256) void k_tm_h(GridPtrs g, Tabs t)
   Hy[k] = hy;
 }
 
+// Split-field PML TM (fdtdTM.c): coefficients of field_pmlCoef / field_pmlCoef_LXY
+// (field.c:288-295), formed per cell in the reference's operation order
+__device__ __forceinline__ double pml_coef(double ep_mu, double sig) { return (1.0 - sig / ep_mu) / (1.0 + sig / ep_mu); }
+__device__ __forceinline__ double pml_coef_lxy(double ep_mu, double sig) { return 1.0 / (ep_mu + sig); }
+
+// calcH (fdtdTM.c:313-327).  x: 1 sig_hy_xx; y: 1 sig_hx_yy
+__global__ __launch_bounds__(256) void k_split_tm_h(GridPtrs g, Tabs t)
+{
+  FOR_CELL();
+  const double2 *Ezx = g.F[1], *Ezy = g.F[2];
+  double2 *Hx = g.F[3], *Hy = g.F[6];
+  const double2 zx = Ezx[k], zy = Ezy[k];
+  const double2 zxt = Ezx[k + 1], zyt = Ezy[k + 1], zxr = Ezx[k + NPY], zyr = Ezy[k + NPY];
+  const double shx = t.y[1][y], shy = t.x[1][x];
+  const double chx = pml_coef(g.mu0, shx), chxly = pml_coef_lxy(g.mu0, shx);
+  const double chy = pml_coef(g.mu0, shy), chylx = pml_coef_lxy(g.mu0, shy);
+  const double2 hx = Hx[k], hy = Hy[k];
+  // EZX(i,j+1)-EZX(i,j) + EZY(i,j+1)-EZY(i,j), left to right
+  const double dxr = ((zxt.x - zx.x) + zyt.x) - zy.x, dxi = ((zxt.y - zx.y) + zyt.y) - zy.y;
+  const double dyr = ((zxr.x - zx.x) + zyr.x) - zy.x, dyi = ((zxr.y - zx.y) + zyr.y) - zy.y;
+  Hx[k] = make_double2(chx * hx.x - chxly * dxr, chx * hx.y - chxly * dxi);
+  Hy[k] = make_double2(chy * hy.x + chylx * dyr, chy * hy.y + chylx * dyi);
+}
+
+// calcE (fdtdTM.c:294-311) and the scattered wave on Ezx after Ez is formed
+// (fdtdTM.c:283-288, field.c:179-199 with gap 0).  x: 0 sig_ez_x; y: 0 sig_ez_y
+__global__ __launch_bounds__(256) void k_split_tm_e(GridPtrs g, Tabs t, StepArgs s)
+{
+  FOR_CELL();
+  double2 *Ez = g.F[0], *Ezx = g.F[1], *Ezy = g.F[2];
+  const double2 *Hx = g.F[3], *Hy = g.F[6];
+  const double e = g.eps[0][k];
+  const double sx = t.x[0][x], sy = t.y[0][y];
+  const double cezx = pml_coef(e, sx), cezxlx = pml_coef_lxy(e, sx);
+  const double cezy = pml_coef(e, sy), cezyly = pml_coef_lxy(e, sy);
+  const double2 zx = Ezx[k], zy = Ezy[k];
+  const double2 hy = Hy[k], hyl = Hy[k - NPY], hx = Hx[k], hxb = Hx[k - 1];
+  double2 nzx, nzy;
+  nzx.x = cezx * zx.x + cezxlx * (hy.x - hyl.x);
+  nzx.y = cezx * zx.y + cezxlx * (hy.y - hyl.y);
+  nzy.x = cezy * zy.x - cezyly * (hx.x - hxb.x);
+  nzy.y = cezy * zy.y - cezyly * (hx.y - hxb.y);
+  Ez[k] = make_double2(nzx.x + nzy.x, nzx.y + nzy.y);
+  if (e != 1.0) {  // vacuum cells add an exact zero
+    const double kr = (i + 0.0) * s.ks_cos + (j + 0.0) * s.ks_sin;
+    const double amp = s.ray_coef * (1.0 / e - 1.0);
+    double s1, c1, s0, c0;
+    sincos(kr - s.w_s * (s.time + 0.5), &s1, &c1);
+    sincos(kr - s.w_s * (s.time - 0.5), &s0, &c0);
+    nzx.x += amp * (c1 - c0);
+    nzx.y += amp * (s1 - s0);
+  }
+  Ezx[k] = nzx;
+  Ezy[k] = nzy;
+}
+
 // TE E half.  x: 0 C_DXJX1 1 C_DXJX0 2 C_DY, sx; y: 0 C_JX 1 C_JXHZ, sy
 template <int PLAIN>
 __global__ __launch_bounds__(256) void k_te_e(GridPtrs g, Tabs t, StepArgs s)
@@ -1244,7 +1300,10 @@ extern "C" int fdtd_hip_grid_create(const fdtd_grid_desc *d, fdtd_grid **out)
   if (d->N_PX < 4 || d->N_PY < 4) return fdtd_fail(FDTD_EINVAL, "grid %dx%d too small", d->N_PX, d->N_PY);
   if (d->precision != FDTD_FP64 && d->precision != FDTD_FP32) return fdtd_fail(FDTD_EINVAL, "bad precision %d", d->precision);
   if (d->mode != FDTD_MODE_TM && d->mode != FDTD_MODE_TE) return fdtd_fail(FDTD_EINVAL, "bad mode %d", d->mode);
-  if (d->flavour != FDTD_FLAVOUR_MPI && d->flavour != FDTD_FLAVOUR_PLAIN) return fdtd_fail(FDTD_EINVAL, "bad flavour");
+  if (d->flavour != FDTD_FLAVOUR_MPI && d->flavour != FDTD_FLAVOUR_PLAIN && d->flavour != FDTD_FLAVOUR_SPLIT)
+    return fdtd_fail(FDTD_EINVAL, "bad flavour");
+  if (d->flavour == FDTD_FLAVOUR_SPLIT && (d->precision != FDTD_FP64 || d->mode != FDTD_MODE_TM || d->ntff_time))
+    return fdtd_fail(FDTD_EINVAL, "the split-field PML solver runs in FP64, TM, without the per-step NTFF");
   if (d->ntff_left < 1 || d->ntff_bottom < 1 || d->ntff_right >= d->N_PX || d->ntff_top >= d->N_PY ||
       d->ntff_left >= d->ntff_right || d->ntff_bottom >= d->ntff_top)
     return fdtd_fail(FDTD_EINVAL, "NTFF contour (%d..%d x %d..%d) outside the %dx%d grid", d->ntff_left, d->ntff_right,
@@ -1508,7 +1567,13 @@ extern "C" int fdtd_hip_grid_step(fdtd_grid *g, const fdtd_step_params *p)
     }
   }
   dim3 blk(64, 4), grd((g->NPYa - 2 + 63) / 64, (g->NPXa - 2 + 3) / 4);
-  if (d.mode == FDTD_MODE_TM) {
+  if (d.flavour == FDTD_FLAVOUR_SPLIT) {  // update (fdtdTM.c:283-288): calcH, calcE, scattered wave
+    s.ps_on = 0;
+    s.time = p->time;
+    s.w_s = p->omega;
+    hipLaunchKernelGGL(k_split_tm_h, grd, blk, 0, 0, gp, g->tabs);
+    hipLaunchKernelGGL(k_split_tm_e, grd, blk, 0, 0, gp, g->tabs, s);
+  } else if (d.mode == FDTD_MODE_TM) {
     if (plain) {
       hipLaunchKernelGGL(k_tm_h, grd, blk, 0, 0, gp, g->tabs);
       hipLaunchKernelGGL(k_tm_e<1>, grd, blk, 0, 0, gp, g->tabs, s);

@@ -1,10 +1,11 @@
 /*
  * simulator.c -- solver dispatcher (simulator.c:19-279 of the reference).
  *
- * Selects one of the four UPML solvers by enum SOLVER, wires its methods into
- * the update / init / finish / reset table and drives field_nextStep after
- * every update.  The split-field and NS-FDTD solvers are outside this
- * library; selecting them is an error, as for unknown solvers in the reference.
+ * Selects one of the four UPML solvers or the split-field PML TM solver by enum
+ * SOLVER, wires its methods into the update / init / finish / reset table and
+ * drives field_nextStep after every update.  The split-field TE and NS-FDTD
+ * solvers are outside this library; selecting them is an error, as for unknown
+ * solvers in the reference.
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -26,6 +27,13 @@ static struct timeval t1_;
 void simulator_setSolver(enum SOLVER solver)
 {
   switch (solver) {
+  case TM_2D:  /* setTM (simulator.c:32-47) */
+    update_ = fdtdTM_getUpdate(); init_ = fdtdTM_getInit();
+    finish_ = fdtdTM_getFinish(); reset_ = fdtdTM_getReset();
+    eps_ = fdtdTM_getEps; draw_ = fdtdTM_getEz;
+    printf("TM mode \n");
+    solverDir_ = "TM";
+    break;
   case TM_UPML_2D:
     update_ = fdtdTM_upml_getUpdate(); init_ = fdtdTM_upml_getInit();
     finish_ = fdtdTM_upml_getFinish(); reset_ = fdtdTM_upml_getReset();

@@ -1,10 +1,12 @@
 /*
- * solvers.c -- the four UPML solvers' entry points on top of the HIP grid.
+ * solvers.c -- the four UPML solvers' and the split-field PML TM solver's
+ * entry points on top of the HIP grid.
  *
  * Each solver keeps the reference's lifecycle (init -> update* -> reset /
  * finish) and getter ABI:
  *   MPI TM  mpiTM_UPML.c:88-250      MPI TE  mpiTE_UPML.c:72-335
  *   TM      fdtdTM_upml.c:54-152     TE      fdtdTE_upml.c:101-192
+ *   split-field PML TM  fdtdTM.c:1-327 (FP64 only)
  * but the state lives in device memory.  setCoefficient (mpiTM_UPML.c:663-716,
  * mpiTE_UPML.c:489-536, fdtdTM_upml.c:224-274, fdtdTE_upml.c:361-412) is
  * restated as 1-D tables per axis, because every UPML coefficient depends on
@@ -33,6 +35,7 @@ typedef struct {
   fdtd_grid *g;
   int NPXa, NPYa, off, arraySize;
   double complex *mirror[3];  /* TM: Ez Hx Hy   TE: Ex Ey Hz */
+  double complex *split_mirror[2];  /* split-field TM: Ezx Ezy */
   double *eps_mirror;
   int blocked;    /* FP32 grid: steps are queued and run T per launch */
   int ntff_time;  /* the per-step time-domain NTFF runs (its outputs exist) */
@@ -55,7 +58,7 @@ static void die(const char *what)
 }
 #define CHECK(call) do { if ((call) != FDTD_OK) die(#call); } while (0)
 
-static int is_tm(int kind) { return kind == SLV_MPI_TM || kind == SLV_TM; }
+static int is_tm(int kind) { return kind == SLV_MPI_TM || kind == SLV_TM || kind == SLV_SPLIT_TM; }
 static int is_mpi(int kind) { return kind == SLV_MPI_TM || kind == SLV_MPI_TE; }
 
 void fdtd_solver_setNTFFTime(int on) { ntff_time_ = on ? 1 : 0; }
@@ -106,7 +109,20 @@ static void build_coefs(int kind, fdtd_grid *g)
   for (int q = 0; q < 4; q++) Y[q] = buf + (size_t)6 * nx + (size_t)q * ny;
   SY = buf + (size_t)6 * nx + (size_t)4 * ny;
   const double sig_z = 0;
-  if (is_tm(kind)) {
+  if (kind == SLV_SPLIT_TM) {
+    /* fdtdTM.c:184-243: the conductivities; the coefficients are formed per cell on
+       the device with eps (field_pmlCoef / field_pmlCoef_LXY of EPS_EZ and MU_0_S) */
+    const double sig_max = -(M + 1.0) * eps * C / 2.0 / N_PML * log(R);
+    const double mu_eps = field_getMu0() / eps;  /* MU_0_S / EPSILON_0_S */
+    for (int x = 0; x < nx; x++) {
+      X[0][x] = sig_max * field_sigmaX(x, 0);                 /* sig_ez_x */
+      X[1][x] = mu_eps * (sig_max * field_sigmaX(x + 0.5, 0)); /* sig_hy_xx */
+    }
+    for (int y = 0; y < ny; y++) {
+      Y[0][y] = sig_max * field_sigmaY(0, y);                 /* sig_ez_y */
+      Y[1][y] = mu_eps * (sig_max * field_sigmaY(0, y + 0.5)); /* sig_hx_yy */
+    }
+  } else if (is_tm(kind)) {
     const double sig_max = -(M + 1.0) * eps * C / 2.0 / N_PML / cos(M_PI / 3) * log(R);
     for (int x = 0; x < nx; x++) {
       double sig_ez_x = sig_max * field_sigmaX(x, 0);
@@ -167,6 +183,7 @@ static void solver_free(int kind)
   solver_t *s = &slv_[kind];
   if (s->g) fdtd_hip_grid_destroy(s->g);
   for (int q = 0; q < 3; q++) free(s->mirror[q]);
+  for (int q = 0; q < 2; q++) free(s->split_mirror[q]);
   free(s->eps_mirror);
   memset(s, 0, sizeof *s);
   if (active_ == kind) active_ = -1;
@@ -182,7 +199,11 @@ int solver_make_grid(int kind, int precision, int ntff_time, int T, int R, fdtd_
   fdtd_grid_desc d;
   memset(&d, 0, sizeof d);
   d.mode = is_tm(kind) ? FDTD_MODE_TM : FDTD_MODE_TE;
-  d.flavour = is_mpi(kind) ? FDTD_FLAVOUR_MPI : FDTD_FLAVOUR_PLAIN;
+  d.flavour = kind == SLV_SPLIT_TM ? FDTD_FLAVOUR_SPLIT : (is_mpi(kind) ? FDTD_FLAVOUR_MPI : FDTD_FLAVOUR_PLAIN);
+  if (kind == SLV_SPLIT_TM) {  /* FP64 only; the reference's fdtdTM.c has no NTFF */
+    precision = FDTD_FP64;
+    ntff_time = 0;
+  }
   d.precision = precision;
   d.N_PX = fs.N_PX;
   d.N_PY = fs.N_PY;
@@ -221,17 +242,19 @@ static void solver_init(int kind)
   solver_t *s = &slv_[kind];
   if (s->g) solver_free(kind);
   CHECK(solver_make_grid(kind, precision_, ntff_time_, block_T_, block_R_, &s->g));
+  const int split = kind == SLV_SPLIT_TM;
   int dims[4];
   CHECK(fdtd_hip_grid_dims(s->g, dims));
   s->NPXa = dims[0];
   s->NPYa = dims[1];
   s->off = dims[2];
   s->arraySize = dims[3];
-  s->blocked = precision_ == FDTD_FP32;
-  s->ntff_time = ntff_time_;
+  s->blocked = !split && precision_ == FDTD_FP32;
+  s->ntff_time = split ? 0 : ntff_time_;
   s->ndft = 0;
   size_t n = (size_t)s->NPXa * s->NPYa;
   for (int q = 0; q < 3; q++) s->mirror[q] = (double complex *)calloc(n, sizeof(double complex));
+  for (int q = 0; q < 2 && split; q++) s->split_mirror[q] = (double complex *)calloc(n, sizeof(double complex));
   s->eps_mirror = (double *)calloc(n, sizeof(double));
   active_ = kind;
 }
@@ -441,10 +464,42 @@ static void mpi_te_ntff_output(void)
   }
 }
 
+/* field_outputElliptic (field.c:322-341) of Ez: |Ez|^2 interpolated bilinearly
+   (myComplex.c:40-51) on the circle of radius 1.2 lambda around the grid centre,
+   the split-field TM solver's reset output (fdtdTM.c:167-173) */
+static void write_elliptic(int kind)
+{
+  FieldInfo fi = field_getFieldInfo();
+  char name[128];
+  snprintf(name, sizeof name, "tm_%dnm.txt", fi.h_u_nm);
+  const double complex *ez = solver_field(kind, 0);
+  const double lam = field_getLambda();
+  printf("output start\n");
+  FILE *fp = fopen(name, "w");
+  if (!fp) {
+    printf("cannot open file %s \n", name);
+    exit(2);
+  }
+  for (int ang = 180; ang >= 0; ang--) {
+    double rad = ang * M_PI / 180.0;
+    double x = 1.2 * lam * cos(rad) + N_PX / 2.0;
+    double y = 1.2 * lam * sin(rad) + N_PY / 2.0;
+    int i = (int)floor(x), j = (int)floor(y);
+    double dx = x - i, dy = y - j;
+    size_t k = (size_t)i * N_PY + j;
+    double complex v = ez[k] * (1.0 - dx) * (1.0 - dy) + ez[k + N_PY] * dx * (1.0 - dy) + ez[k + 1] * (1.0 - dx) * dy +
+                       ez[k + N_PY + 1] * dx * dy;
+    fprintf(fp, "%d %lf \n", 180 - ang, creal(v) * creal(v) + cimag(v) * cimag(v));
+  }
+  fclose(fp);
+  printf("output to %s end\n", name);
+}
+
 static void solver_reset(int kind)
 {
   solver_t *s = &slv_[kind];
   if (!s->g) return;
+  if (kind == SLV_SPLIT_TM && file_output_) write_elliptic(kind);     /* fdtdTM.c:167-173 */
   if (!is_mpi(kind) && file_output_ && s->ntff_time) write_spectra();  /* fdtdTM_upml.c:87, fdtdTE_upml.c:117 */
   CHECK(fdtd_hip_grid_reset(s->g));
 }
@@ -535,6 +590,29 @@ fdtd_dcomplex *fdtdTM_upml_getEz(void) { return (fdtd_dcomplex *)solver_field(SL
 fdtd_dcomplex *fdtdTM_upml_getHx(void) { return (fdtd_dcomplex *)solver_field(SLV_TM, 1); }
 fdtd_dcomplex *fdtdTM_upml_getHy(void) { return (fdtd_dcomplex *)solver_field(SLV_TM, 2); }
 double *fdtdTM_upml_getEps(void) { return solver_eps(SLV_TM); }
+
+/* ---------------------------------------------------------------- split-field PML TM (fdtdTM.c) */
+static void split_tm_init(void) { solver_init(SLV_SPLIT_TM); }
+static void split_tm_update(void) { solver_update(SLV_SPLIT_TM); }
+static void split_tm_reset(void) { solver_reset(SLV_SPLIT_TM); }
+static void split_tm_finish(void) { solver_finish(SLV_SPLIT_TM); }
+static double complex *split_field(int q)  /* 0 Ezx (slot 1), 1 Ezy (slot 2) */
+{
+  solver_t *s = &slv_[SLV_SPLIT_TM];
+  if (!s->g) return NULL;
+  CHECK(fdtd_hip_grid_get_field(s->g, 1 + q, (double *)s->split_mirror[q]));
+  return s->split_mirror[q];
+}
+fdtd_method fdtdTM_getUpdate(void) { return split_tm_update; }
+fdtd_method fdtdTM_getFinish(void) { return split_tm_finish; }
+fdtd_method fdtdTM_getReset(void) { return split_tm_reset; }
+fdtd_method fdtdTM_getInit(void) { return split_tm_init; }
+fdtd_dcomplex *fdtdTM_getEzx(void) { return (fdtd_dcomplex *)split_field(0); }
+fdtd_dcomplex *fdtdTM_getEzy(void) { return (fdtd_dcomplex *)split_field(1); }
+fdtd_dcomplex *fdtdTM_getEz(void) { return (fdtd_dcomplex *)solver_field(SLV_SPLIT_TM, 0); }
+fdtd_dcomplex *fdtdTM_getHy(void) { return (fdtd_dcomplex *)solver_field(SLV_SPLIT_TM, 2); }
+fdtd_dcomplex *fdtdTM_getHx(void) { return (fdtd_dcomplex *)solver_field(SLV_SPLIT_TM, 1); }
+double *fdtdTM_getEps(void) { return solver_eps(SLV_SPLIT_TM); }
 
 /* ---------------------------------------------------------------- plain TE */
 static void te_init(void) { solver_init(SLV_TE); }

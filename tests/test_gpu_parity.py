@@ -185,3 +185,36 @@ def test_far_field_follows_mie_series_at_100_cells_per_wavelength(solver, pol, p
     assert err < 0.05, err
     assert other > 0.3, other
     assert asym < 0.01, asym
+
+
+@pytest.mark.parametrize("name", ["tm2d_mie_a0", "tm2d_zigzag_a30_rect"])
+def test_split_field_tm_parity(name, tmp_path, monkeypatch):
+    """The split-field PML TM solver (fdtdTM.c, solver TM_2D): Ez, Hx, Hy, eps and the
+    far field against the compiled reference's golden run, the split components Ezx,
+    Ezy and the whole state against the oracle, and the tm_<h>nm.txt file that
+    finish() writes through reset() against the reference's (%lf resolution)."""
+    m, po = _mods()
+    g = load_golden(name)
+    solver, model, sweep, w, h, hu, pml, lam, ang, steps = [int(v) for v in g["cfg"]]
+    params = swept_params(model, sweep)
+    print(f"case {name}: split-field TM model {model} {w}x{h} h={hu} steps={steps}", flush=True)
+    monkeypatch.chdir(tmp_path)
+    sim = run_gpu(m, solver, model, w, h, hu, pml, lam, ang, steps, params, file_output=True)
+    o = run_oracle(po, solver, model, w, h, hu, pml, lam, ang, steps, params)
+    try:
+        f = sim.fields()
+        for k in range(3):
+            assert f[k].shape == g[f"f{k}"].shape
+            assert rel_l2(f[k], g[f"f{k}"]) <= TOL, (name, k, rel_l2(f[k], g[f"f{k}"]))
+        assert np.array_equal(sim.eps(), g["eps"]), name
+        assert rel_l2(sim.far_field(), g["ff"]) <= TOL
+        for which in (0, 1, 2, 3, 6):  # Ez, Ezx, Ezy, Hx, Hy
+            assert np.abs(o.field(which)).max() > 0
+            assert rel_l2(sim.state(which), o.field(which)) <= TOL, (name, which)
+        ell_oracle = o.elliptic()
+    finally:
+        sim.finish()
+    rows = np.loadtxt(tmp_path / f"tm_{hu}nm.txt")
+    assert np.array_equal(rows[:, 0], np.arange(181))
+    assert np.max(np.abs(rows[:, 1] - g["elliptic"])) <= 1.0000001e-6
+    assert np.max(np.abs(rows[:, 1] - ell_oracle)) <= 5.0000001e-7
