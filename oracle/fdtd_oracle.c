@@ -52,7 +52,7 @@ struct orc_sim {
   double RFperC;
   /* array layout */
   int NPXa, NPYa, off;
-  bool mpi, tm;
+  bool mpi, tm, split;
   dcx *F[9];
   double *C[15];
   double *EPS[3];
@@ -532,6 +532,36 @@ static void set_coef_tm_split(orc_sim *s)
   }
 }
 
+/* Split-field PML TE (fdtdTE.c:200-241): C[0] C_EX, C[1] C_EXLY, C[2] C_EY,
+   C[3] C_EYLX, C[4] C_HZX, C[5] C_HZXLX, C[6] C_HZY, C[7] C_HZYLY; eps maps
+   EPS_EX (i + 0.5, D_Y), EPS_EY (j + 0.5, D_X), EPS_HZ (mean of D_X and D_Y
+   at (i + 0.5, j + 0.5)) */
+static void set_coef_te_split(orc_sim *s)
+{
+  const double R = 1.0e-8, M = 2.0;
+  const double sig_max = -(M + 1.0) * s->EPS0 * s->C0 / 2.0 / s->N_PML * log(R);
+  for (int i = 0; i < s->NPXa; i++) {
+    for (int j = 0; j < s->NPYa; j++) {
+      int k = i * s->NPYa + j;
+      s->EPS[0][k] = models_eps(s, i + 0.5, j, ORC_D_Y);
+      s->EPS[1][k] = models_eps(s, i, j + 0.5, ORC_D_X);
+      s->EPS[2][k] = 0.5 * (models_eps(s, i + 0.5, j + 0.5, ORC_D_X) + models_eps(s, i + 0.5, j + 0.5, ORC_D_Y));
+      double sig_ex_y = sig_max * sigma_y(s, j);
+      double sig_ey_x = sig_max * sigma_x(s, i);
+      double sig_hz_xx = s->MU0 / s->EPS0 * (sig_max * sigma_x(s, i + 0.5));
+      double sig_hz_yy = s->MU0 / s->EPS0 * (sig_max * sigma_y(s, j + 0.5));
+      s->C[0][k] = pml_coef(s->EPS[0][k], sig_ex_y);
+      s->C[1][k] = pml_coef_lxy(s->EPS[0][k], sig_ex_y);
+      s->C[2][k] = pml_coef(s->EPS[1][k], sig_ey_x);
+      s->C[3][k] = pml_coef_lxy(s->EPS[1][k], sig_ey_x);
+      s->C[4][k] = pml_coef(s->MU0, sig_hz_xx);
+      s->C[5][k] = pml_coef_lxy(s->MU0, sig_hz_xx);
+      s->C[6][k] = pml_coef(s->MU0, sig_hz_yy);
+      s->C[7][k] = pml_coef_lxy(s->MU0, sig_hz_yy);
+    }
+  }
+}
+
 /* TE: mpiTE_UPML.c:489-536 (MPI) and fdtdTE_upml.c:361-412 (plain); no /cos(pi/3) (Q7) */
 static void set_coef_te(orc_sim *s)
 {
@@ -580,6 +610,7 @@ orc_sim *orc_create(const orc_config *cfg)
   models_init(s);  /* models_initModel runs after field_init (simulator.c:229-231) */
   s->mpi = (cfg->solver == ORC_MPI_TM_UPML || cfg->solver == ORC_MPI_TE_UPML);
   s->tm = (cfg->solver == ORC_MPI_TM_UPML || cfg->solver == ORC_TM_UPML || cfg->solver == ORC_TM_2D);
+  s->split = (cfg->solver == ORC_TM_2D || cfg->solver == ORC_TE_2D);
   /* MPI single rank: SUB_N_PX = N_PX + 2 with a ghost ring (mpiTM_UPML.c:737-743) */
   s->off = s->mpi ? 1 : 0;
   s->NPXa = s->N_PX + 2 * s->off;
@@ -594,6 +625,7 @@ orc_sim *orc_create(const orc_config *cfg)
     s->NT[a] = s->NTb[a] + NT_GUARD;
   }
   if (cfg->solver == ORC_TM_2D) set_coef_tm_split(s);
+  else if (cfg->solver == ORC_TE_2D) set_coef_te_split(s);
   else if (s->tm) set_coef_tm(s);
   else set_coef_te(s);
   s->R0 = 1.0e6 * to_cell(s, 500);  /* ntffTM_init (ntffTM.c:31), ntffTE_init (ntffTE.c:16) */
@@ -1148,9 +1180,54 @@ static void split_tm_step(orc_sim *s)
     }
 }
 
+/* Split-field PML TE step (fdtdTE.c:281-321): calcE (Ex, Ey), the scattered wave
+   on Ey (field.c:179-199, gap (0, 0.5)), calcH (Hzx, Hzy, then Hz = Hzx + Hzy);
+   slots F[0] Ex, F[3] Ey, F[6] Hz, F[7] Hzx, F[8] Hzy */
+static void split_te_step(orc_sim *s)
+{
+  const int NPX = s->NPXa, NPY = s->NPYa;
+  dcx *Ex = s->F[0], *Ey = s->F[3], *Hz = s->F[6], *Hzx = s->F[7], *Hzy = s->F[8];
+  double **C = s->C;
+  for (int i = 1; i < NPX - 1; i++)
+    for (int j = 1; j < NPY - 1; j++) {
+      int k = IDX(i, j);
+      Ex[k] = C[0][k] * Ex[k] + C[1][k] * (Hzx[k] - Hzx[k - 1] + Hzy[k] - Hzy[k - 1]);
+    }
+  for (int i = 1; i < NPX - 1; i++)
+    for (int j = 1; j < NPY - 1; j++) {
+      int k = IDX(i, j);
+      Ey[k] = C[2][k] * Ey[k] - C[3][k] * (Hzx[k] - Hzx[k - NPY] + Hzy[k] - Hzy[k - NPY]);
+    }
+  double time = s->time, w_s = s->Omega_s, ray_coef = s->ray_coef, k_s = s->K_s;
+  double rad = s->Angle_deg * M_PI / 180.0;
+  double ks_cos = cos(rad) * k_s, ks_sin = sin(rad) * k_s;
+  for (int i = 1; i < NPX - 1; i++)
+    for (int j = 1; j < NPY - 1; j++) {
+      int k = IDX(i, j);
+      double kr = (i + 0.0) * ks_cos + (j + 0.5) * ks_sin;
+      Ey[k] += ray_coef * (s->EPS0 / s->EPS[1][k] - 1.0) *
+               (cexp(I * (kr - w_s * (time + 0.5))) - cexp(I * (kr - w_s * (time - 0.5))));
+    }
+  for (int i = 1; i < NPX - 1; i++)
+    for (int j = 1; j < NPY - 1; j++) {
+      int k = IDX(i, j);
+      Hzx[k] = C[4][k] * Hzx[k] - C[5][k] * (Ey[k + NPY] - Ey[k]);
+    }
+  for (int i = 1; i < NPX - 1; i++)
+    for (int j = 1; j < NPY - 1; j++) {
+      int k = IDX(i, j);
+      Hzy[k] = C[6][k] * Hzy[k] + C[7][k] * (Ex[k + 1] - Ex[k]);
+    }
+  for (int i = 1; i < NPX - 1; i++)
+    for (int j = 1; j < NPY - 1; j++) {
+      int k = IDX(i, j);
+      Hz[k] = Hzx[k] + Hzy[k];
+    }
+}
+
 void orc_elliptic(const orc_sim *s, double *out181)
 {
-  const dcx *Ez = s->F[0];
+  const dcx *Ez = s->cfg.solver == ORC_TE_2D ? s->F[3] : s->F[0];  /* TE_2D: Ey (fdtdTE.c:269-274) */
   const int NPY = s->NPYa;
   for (int ang = 180; ang >= 0; ang--) {
     double rad = ang * M_PI / 180.0;
@@ -1170,6 +1247,9 @@ void orc_step(orc_sim *s)
   switch (s->cfg.solver) {
   case ORC_TM_2D:
     split_tm_step(s);
+    break;
+  case ORC_TE_2D:
+    split_te_step(s);
     break;
   case ORC_MPI_TM_UPML: /* update (mpiTM_UPML.c:196-217) */
     tm_calc_jd_e(s);

@@ -12,7 +12,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "liboracle.so")
 
-TM_2D = 0  # split-field PML TM (fdtdTM.c)
+TM_2D, TE_2D = 0, 1  # split-field PML TM / TE (fdtdTM.c, fdtdTE.c)
 TM_UPML, TE_UPML, MPI_TM_UPML, MPI_TE_UPML = 2, 3, 4, 5
 NO_MODEL, MIE_CYLINDER, LAYER, MORPHO_SCALE, ZIGZAG = 0, 1, 2, 3, 5
 D_X, D_Y, D_XY = 0, 1, 2
@@ -188,7 +188,7 @@ class Sim:
         return out
 
     def elliptic(self):
-        """TM_2D: the 181 values field_outputElliptic writes (field.c:322-341), for 0..180."""
+        """TM_2D / TE_2D: the 181 values field_outputElliptic writes (field.c:322-341), for 0..180."""
         out = np.empty(181, np.float64)
         lib().orc_elliptic(self.h, _dp(out))
         return out

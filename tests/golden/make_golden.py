@@ -32,6 +32,8 @@ RUN_CASES = [
     ("mpite_zigzag", 5, 5, 0, 1200, 1500, 20, 8, 450, 0, 150),
     ("tm2d_mie_a0", 0, 1, 0, 4000, 4000, 50, 10, 500, 0, 300),      # split-field PML TM (fdtdTM.c)
     ("tm2d_zigzag_a30_rect", 0, 5, 0, 1200, 1500, 20, 8, 450, 30, 150),
+    ("te2d_mie_a0", 1, 1, 0, 4000, 4000, 50, 10, 500, 0, 300),       # split-field PML TE (fdtdTE.c)
+    ("te2d_layer_a45_rect", 1, 2, 3, 1200, 1600, 20, 8, 550, 45, 150),
 ]
 
 
@@ -95,9 +97,10 @@ def main(only=()):
             if os.path.exists(spec):
                 full = np.fromfile(spec, np.float64).reshape(321, 360)
                 d["spectra"] = full[::4]           # lambda = 380, 384, ..., 700
-            ell = os.path.join(out, f"tm_{hu}nm.txt")  # TM_2D reset(): field_outputElliptic of Ez
-            if os.path.exists(ell):
-                d["elliptic"] = np.loadtxt(ell)[:, 1]
+            for pol in ("tm", "te"):  # TM_2D / TE_2D reset(): field_outputElliptic of Ez / Ey
+                ell = os.path.join(out, f"{pol}_{hu}nm.txt")
+                if os.path.exists(ell):
+                    d["elliptic"] = np.loadtxt(ell)[:, 1]
             tdir = os.path.join(out, "MPI_TE_UPML")
             if os.path.isdir(tdir):
                 rows = set(range(0, 360, 15))
