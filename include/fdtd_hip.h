@@ -27,9 +27,11 @@ enum { FDTD_MODE_TM = 0, FDTD_MODE_TE = 1 };
  *              (mpiTM_UPML.c / mpiTE_UPML.c single rank).
  * plain flavour: N_PX x N_PY arrays, H half first, Gaussian pulse
  *              (fdtdTM_upml.c / fdtdTE_upml.c).
- * split flavour: the split-field PML TM solver (fdtdTM.c): N_PX x N_PY arrays,
- *              H half first, CW scattered wave on Ezx; FP64, TM, no per-step
- *              NTFF; slots Ez 0, Ezx 1, Ezy 2, Hx 3, Hy 6. */
+ * split flavour: the split-field PML solvers (fdtdTM.c, fdtdTE.c): N_PX x N_PY
+ *              arrays, CW scattered wave; FP64, no per-step NTFF.  TM: H half
+ *              first, the wave on Ezx, slots Ez 0, Ezx 1, Ezy 2, Hx 3, Hy 6;
+ *              TE: E half first, the wave on Ey, slots Ex 0, Ey 3, Hz 6, Hzx 7,
+ *              Hzy 8. */
 enum { FDTD_FLAVOUR_MPI = 0, FDTD_FLAVOUR_PLAIN = 1, FDTD_FLAVOUR_SPLIT = 2 };
 enum { FDTD_FP64 = 0, FDTD_FP32 = 1 };
 
@@ -82,10 +84,13 @@ typedef struct fdtd_grid_desc {
  *       y0 C_DZ  y1 C_DZJZ1(=C_DZJZ0)  y2 C_MX  y3 C_MXEZ         sy = sig_hy_y
  *   TE: x0 C_DXJX1 x1 C_DXJX0 x2 C_DY  x3 C_MZ   x4 C_MZEXEY     sx = sig_ey_x
  *       y0 C_JX  y1 C_JXHZ  y2 C_BZ  y3 C_BZMZ1(=C_BZMZ0)         sy = sig_ey_y
- * Split flavour (fdtdTM.c:184-243; the coefficients are formed per cell on the
- * device from these conductivities and eps, as field_pmlCoef / _LXY do):
- *       x0 sig_ez_x(i)  x1 sig_hy_xx(i) = mu0/eps0 sig(i + 1/2)
- *       y0 sig_ez_y(j)  y1 sig_hx_yy(j) = mu0/eps0 sig(j + 1/2)
+ * Split flavour (fdtdTM.c:184-243, fdtdTE.c:200-241; the coefficients are formed
+ * per cell on the device from these conductivities and eps, as field_pmlCoef /
+ * _LXY do; TM and TE use the same four tables):
+ *       x0 sig_max sigma_x(i)  (TM sig_ez_x, TE sig_ey_x)
+ *       x1 mu0/eps0 sig_max sigma_x(i + 1/2)  (TM sig_hy_xx, TE sig_hz_xx)
+ *       y0 sig_max sigma_y(j)  (TM sig_ez_y, TE sig_ex_y)
+ *       y1 mu0/eps0 sig_max sigma_y(j + 1/2)  (TM sig_hx_yy, TE sig_hz_yy)
  * Each x table has N_PX entries, each y table N_PY. */
 typedef struct fdtd_coef_tables {
   const double *x[5];

@@ -1,6 +1,6 @@
 /*
- * mpifdtd_solvers.h -- the four UPML solvers' and the split-field PML TM
- * solver's entry points, backed by HIP.
+ * mpifdtd_solvers.h -- the four UPML solvers' and the two split-field PML
+ * solvers' entry points, backed by HIP.
  *
  * Same symbols and signatures as the reference's solver headers:
  *   mpiTM_UPML.h:5-18   (mpiTM_UPML.c:88-136)   MPI TMz UPML, run as one rank on one GPU
@@ -9,6 +9,8 @@
  *   fdtdTE_upml.h:5-14                          plain TEz UPML
  *   fdtdTM.h:5-16       (fdtdTM.c:39-78)        split-field PML TMz (FP64 only; its
  *                                               reset() writes tm_<h>nm.txt)
+ *   fdtdTE.h:5-16       (fdtdTE.c:29-66)        split-field PML TEz (FP64 only; its
+ *                                               reset() writes te_<h>nm.txt)
  * get{Update,Init,Finish,Reset} return the solver's void(void) methods; the
  * field getters return borrowed host pointers (valid from init to finish) that
  * are refreshed from device memory on every getter call; getEps returns the
@@ -78,6 +80,18 @@ fdtd_dcomplex *fdtdTM_getEz(void);
 fdtd_dcomplex *fdtdTM_getHy(void);
 fdtd_dcomplex *fdtdTM_getHx(void);
 double *fdtdTM_getEps(void);
+
+/* fdtdTE.h:5-16 (split-field PML TEz; getEps returns EPS_EY) */
+fdtd_method fdtdTE_getUpdate(void);
+fdtd_method fdtdTE_getFinish(void);
+fdtd_method fdtdTE_getReset(void);
+fdtd_method fdtdTE_getInit(void);
+fdtd_dcomplex *fdtdTE_getEx(void);
+fdtd_dcomplex *fdtdTE_getEy(void);
+fdtd_dcomplex *fdtdTE_getHzx(void);
+fdtd_dcomplex *fdtdTE_getHzy(void);
+fdtd_dcomplex *fdtdTE_getHz(void);
+double *fdtdTE_getEps(void);
 
 /* fdtdTE_upml.h:5-14 */
 fdtd_method fdtdTE_upml_getUpdate(void);

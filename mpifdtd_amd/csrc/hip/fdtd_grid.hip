@@ -339,6 +339,62 @@ __global__ __launch_bounds__(256) void k_split_tm_e(GridPtrs g, Tabs t, StepArgs
   Ezy[k] = nzy;
 }
 
+// Split-field PML TE (fdtdTE.c): calcE (fdtdTE.c:290-303), then the scattered wave
+// on Ey (fdtdTE.c:283-288, field.c:179-199 with gap (0, 0.5)).  x: 0 sig_ey_x; y: 0 sig_ex_y
+__global__ __launch_bounds__(256) void k_split_te_e(GridPtrs g, Tabs t, StepArgs s)
+{
+  FOR_CELL();
+  double2 *Ex = g.F[0], *Ey = g.F[3];
+  const double2 *Hzx = g.F[7], *Hzy = g.F[8];
+  const double eex = g.eps[0][k], eey = g.eps[1][k];
+  const double sy = t.y[0][y], sx = t.x[0][x];
+  const double cex = pml_coef(eex, sy), cexly = pml_coef_lxy(eex, sy);
+  const double cey = pml_coef(eey, sx), ceylx = pml_coef_lxy(eey, sx);
+  const double2 zx = Hzx[k], zy = Hzy[k];
+  const double2 zxb = Hzx[k - 1], zyb = Hzy[k - 1], zxl = Hzx[k - NPY], zyl = Hzy[k - NPY];
+  const double2 ex = Ex[k], ey = Ey[k];
+  // HZX(i,j) - HZX(i,j-1) + HZY(i,j) - HZY(i,j-1), left to right (and i-1 for Ey)
+  const double ar = ((zx.x - zxb.x) + zy.x) - zyb.x, ai = ((zx.y - zxb.y) + zy.y) - zyb.y;
+  const double br = ((zx.x - zxl.x) + zy.x) - zyl.x, bi = ((zx.y - zxl.y) + zy.y) - zyl.y;
+  double2 nex, ney;
+  nex.x = cex * ex.x + cexly * ar;
+  nex.y = cex * ex.y + cexly * ai;
+  ney.x = cey * ey.x - ceylx * br;
+  ney.y = cey * ey.y - ceylx * bi;
+  if (eey != 1.0) {  // vacuum cells add an exact zero
+    const double kr = (i + 0.0) * s.ks_cos + (j + 0.5) * s.ks_sin;
+    const double amp = s.ray_coef * (1.0 / eey - 1.0);
+    double s1, c1, s0, c0;
+    sincos(kr - s.w_s * (s.time + 0.5), &s1, &c1);
+    sincos(kr - s.w_s * (s.time - 0.5), &s0, &c0);
+    ney.x += amp * (c1 - c0);
+    ney.y += amp * (s1 - s0);
+  }
+  Ex[k] = nex;
+  Ey[k] = ney;
+}
+
+// calcH (fdtdTE.c:305-321): Hzx, Hzy, then Hz = Hzx + Hzy.  x: 1 sig_hz_xx; y: 1 sig_hz_yy
+__global__ __launch_bounds__(256) void k_split_te_h(GridPtrs g, Tabs t)
+{
+  FOR_CELL();
+  const double2 *Ex = g.F[0], *Ey = g.F[3];
+  double2 *Hz = g.F[6], *Hzx = g.F[7], *Hzy = g.F[8];
+  const double shx = t.x[1][x], shy = t.y[1][y];
+  const double chzx = pml_coef(g.mu0, shx), chzxlx = pml_coef_lxy(g.mu0, shx);
+  const double chzy = pml_coef(g.mu0, shy), chzyly = pml_coef_lxy(g.mu0, shy);
+  const double2 ey = Ey[k], eyr = Ey[k + NPY], ex = Ex[k], ext = Ex[k + 1];
+  const double2 zx = Hzx[k], zy = Hzy[k];
+  double2 nzx, nzy;
+  nzx.x = chzx * zx.x - chzxlx * (eyr.x - ey.x);
+  nzx.y = chzx * zx.y - chzxlx * (eyr.y - ey.y);
+  nzy.x = chzy * zy.x + chzyly * (ext.x - ex.x);
+  nzy.y = chzy * zy.y + chzyly * (ext.y - ex.y);
+  Hzx[k] = nzx;
+  Hzy[k] = nzy;
+  Hz[k] = make_double2(nzx.x + nzy.x, nzx.y + nzy.y);
+}
+
 // TE E half.  x: 0 C_DXJX1 1 C_DXJX0 2 C_DY, sx; y: 0 C_JX 1 C_JXHZ, sy
 template <int PLAIN>
 __global__ __launch_bounds__(256) void k_te_e(GridPtrs g, Tabs t, StepArgs s)
@@ -1302,8 +1358,8 @@ extern "C" int fdtd_hip_grid_create(const fdtd_grid_desc *d, fdtd_grid **out)
   if (d->mode != FDTD_MODE_TM && d->mode != FDTD_MODE_TE) return fdtd_fail(FDTD_EINVAL, "bad mode %d", d->mode);
   if (d->flavour != FDTD_FLAVOUR_MPI && d->flavour != FDTD_FLAVOUR_PLAIN && d->flavour != FDTD_FLAVOUR_SPLIT)
     return fdtd_fail(FDTD_EINVAL, "bad flavour");
-  if (d->flavour == FDTD_FLAVOUR_SPLIT && (d->precision != FDTD_FP64 || d->mode != FDTD_MODE_TM || d->ntff_time))
-    return fdtd_fail(FDTD_EINVAL, "the split-field PML solver runs in FP64, TM, without the per-step NTFF");
+  if (d->flavour == FDTD_FLAVOUR_SPLIT && (d->precision != FDTD_FP64 || d->ntff_time))
+    return fdtd_fail(FDTD_EINVAL, "the split-field PML solvers run in FP64, without the per-step NTFF");
   if (d->ntff_left < 1 || d->ntff_bottom < 1 || d->ntff_right >= d->N_PX || d->ntff_top >= d->N_PY ||
       d->ntff_left >= d->ntff_right || d->ntff_bottom >= d->ntff_top)
     return fdtd_fail(FDTD_EINVAL, "NTFF contour (%d..%d x %d..%d) outside the %dx%d grid", d->ntff_left, d->ntff_right,
@@ -1567,12 +1623,17 @@ extern "C" int fdtd_hip_grid_step(fdtd_grid *g, const fdtd_step_params *p)
     }
   }
   dim3 blk(64, 4), grd((g->NPYa - 2 + 63) / 64, (g->NPXa - 2 + 3) / 4);
-  if (d.flavour == FDTD_FLAVOUR_SPLIT) {  // update (fdtdTM.c:283-288): calcH, calcE, scattered wave
+  if (d.flavour == FDTD_FLAVOUR_SPLIT) {
     s.ps_on = 0;
     s.time = p->time;
     s.w_s = p->omega;
-    hipLaunchKernelGGL(k_split_tm_h, grd, blk, 0, 0, gp, g->tabs);
-    hipLaunchKernelGGL(k_split_tm_e, grd, blk, 0, 0, gp, g->tabs, s);
+    if (d.mode == FDTD_MODE_TM) {  // update (fdtdTM.c:283-288): calcH, calcE, scattered wave
+      hipLaunchKernelGGL(k_split_tm_h, grd, blk, 0, 0, gp, g->tabs);
+      hipLaunchKernelGGL(k_split_tm_e, grd, blk, 0, 0, gp, g->tabs, s);
+    } else {  // update (fdtdTE.c:283-288): calcE, scattered wave, calcH
+      hipLaunchKernelGGL(k_split_te_e, grd, blk, 0, 0, gp, g->tabs, s);
+      hipLaunchKernelGGL(k_split_te_h, grd, blk, 0, 0, gp, g->tabs);
+    }
   } else if (d.mode == FDTD_MODE_TM) {
     if (plain) {
       hipLaunchKernelGGL(k_tm_h, grd, blk, 0, 0, gp, g->tabs);

@@ -1,11 +1,11 @@
 /*
  * simulator.c -- solver dispatcher (simulator.c:19-279 of the reference).
  *
- * Selects one of the four UPML solvers or the split-field PML TM solver by enum
- * SOLVER, wires its methods into the update / init / finish / reset table and
- * drives field_nextStep after every update.  The split-field TE and NS-FDTD
- * solvers are outside this library; selecting them is an error, as for unknown
- * solvers in the reference.
+ * Selects one of the four UPML solvers or the two split-field PML solvers by
+ * enum SOLVER, wires its methods into the update / init / finish / reset table
+ * and drives field_nextStep after every update.  The NS-FDTD solvers are
+ * outside this library; selecting them is an error, as for unknown solvers in
+ * the reference.
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -33,6 +33,13 @@ void simulator_setSolver(enum SOLVER solver)
     eps_ = fdtdTM_getEps; draw_ = fdtdTM_getEz;
     printf("TM mode \n");
     solverDir_ = "TM";
+    break;
+  case TE_2D:  /* setTE (simulator.c:49-64) */
+    update_ = fdtdTE_getUpdate(); init_ = fdtdTE_getInit();
+    finish_ = fdtdTE_getFinish(); reset_ = fdtdTE_getReset();
+    eps_ = fdtdTE_getEps; draw_ = fdtdTE_getEy;
+    printf("TE mode \n");
+    solverDir_ = "TE";
     break;
   case TM_UPML_2D:
     update_ = fdtdTM_upml_getUpdate(); init_ = fdtdTM_upml_getInit();

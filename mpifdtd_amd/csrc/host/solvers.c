@@ -1,12 +1,12 @@
 /*
- * solvers.c -- the four UPML solvers' and the split-field PML TM solver's
+ * solvers.c -- the four UPML solvers' and the two split-field PML solvers'
  * entry points on top of the HIP grid.
  *
  * Each solver keeps the reference's lifecycle (init -> update* -> reset /
  * finish) and getter ABI:
  *   MPI TM  mpiTM_UPML.c:88-250      MPI TE  mpiTE_UPML.c:72-335
  *   TM      fdtdTM_upml.c:54-152     TE      fdtdTE_upml.c:101-192
- *   split-field PML TM  fdtdTM.c:1-327 (FP64 only)
+ *   split-field PML TM  fdtdTM.c:1-327, TE  fdtdTE.c:1-321 (FP64 only)
  * but the state lives in device memory.  setCoefficient (mpiTM_UPML.c:663-716,
  * mpiTE_UPML.c:489-536, fdtdTM_upml.c:224-274, fdtdTE_upml.c:361-412) is
  * restated as 1-D tables per axis, because every UPML coefficient depends on
@@ -35,7 +35,7 @@ typedef struct {
   fdtd_grid *g;
   int NPXa, NPYa, off, arraySize;
   double complex *mirror[3];  /* TM: Ez Hx Hy   TE: Ex Ey Hz */
-  double complex *split_mirror[2];  /* split-field TM: Ezx Ezy */
+  double complex *split_mirror[2];  /* split-field TM: Ezx Ezy   TE: Hzx Hzy */
   double *eps_mirror;
   int blocked;    /* FP32 grid: steps are queued and run T per launch */
   int ntff_time;  /* the per-step time-domain NTFF runs (its outputs exist) */
@@ -60,6 +60,7 @@ static void die(const char *what)
 
 static int is_tm(int kind) { return kind == SLV_MPI_TM || kind == SLV_TM || kind == SLV_SPLIT_TM; }
 static int is_mpi(int kind) { return kind == SLV_MPI_TM || kind == SLV_MPI_TE; }
+static int is_split(int kind) { return kind == SLV_SPLIT_TM || kind == SLV_SPLIT_TE; }
 
 void fdtd_solver_setNTFFTime(int on) { ntff_time_ = on ? 1 : 0; }
 void fdtd_solver_setFileOutput(int on) { file_output_ = on ? 1 : 0; }
@@ -109,18 +110,19 @@ static void build_coefs(int kind, fdtd_grid *g)
   for (int q = 0; q < 4; q++) Y[q] = buf + (size_t)6 * nx + (size_t)q * ny;
   SY = buf + (size_t)6 * nx + (size_t)4 * ny;
   const double sig_z = 0;
-  if (kind == SLV_SPLIT_TM) {
-    /* fdtdTM.c:184-243: the conductivities; the coefficients are formed per cell on
-       the device with eps (field_pmlCoef / field_pmlCoef_LXY of EPS_EZ and MU_0_S) */
+  if (is_split(kind)) {
+    /* fdtdTM.c:184-243, fdtdTE.c:200-241: the conductivities (the same four tables
+       for TM and TE); the coefficients are formed per cell on the device with eps
+       (field_pmlCoef / field_pmlCoef_LXY of the component's eps map and MU_0_S) */
     const double sig_max = -(M + 1.0) * eps * C / 2.0 / N_PML * log(R);
     const double mu_eps = field_getMu0() / eps;  /* MU_0_S / EPSILON_0_S */
     for (int x = 0; x < nx; x++) {
-      X[0][x] = sig_max * field_sigmaX(x, 0);                 /* sig_ez_x */
-      X[1][x] = mu_eps * (sig_max * field_sigmaX(x + 0.5, 0)); /* sig_hy_xx */
+      X[0][x] = sig_max * field_sigmaX(x, 0);                 /* TM sig_ez_x, TE sig_ey_x */
+      X[1][x] = mu_eps * (sig_max * field_sigmaX(x + 0.5, 0)); /* TM sig_hy_xx, TE sig_hz_xx */
     }
     for (int y = 0; y < ny; y++) {
-      Y[0][y] = sig_max * field_sigmaY(0, y);                 /* sig_ez_y */
-      Y[1][y] = mu_eps * (sig_max * field_sigmaY(0, y + 0.5)); /* sig_hx_yy */
+      Y[0][y] = sig_max * field_sigmaY(0, y);                 /* TM sig_ez_y, TE sig_ex_y */
+      Y[1][y] = mu_eps * (sig_max * field_sigmaY(0, y + 0.5)); /* TM sig_hx_yy, TE sig_hz_yy */
     }
   } else if (is_tm(kind)) {
     const double sig_max = -(M + 1.0) * eps * C / 2.0 / N_PML / cos(M_PI / 3) * log(R);
@@ -199,8 +201,8 @@ int solver_make_grid(int kind, int precision, int ntff_time, int T, int R, fdtd_
   fdtd_grid_desc d;
   memset(&d, 0, sizeof d);
   d.mode = is_tm(kind) ? FDTD_MODE_TM : FDTD_MODE_TE;
-  d.flavour = kind == SLV_SPLIT_TM ? FDTD_FLAVOUR_SPLIT : (is_mpi(kind) ? FDTD_FLAVOUR_MPI : FDTD_FLAVOUR_PLAIN);
-  if (kind == SLV_SPLIT_TM) {  /* FP64 only; the reference's fdtdTM.c has no NTFF */
+  d.flavour = is_split(kind) ? FDTD_FLAVOUR_SPLIT : (is_mpi(kind) ? FDTD_FLAVOUR_MPI : FDTD_FLAVOUR_PLAIN);
+  if (is_split(kind)) {  /* FP64 only; the reference's fdtdTM.c / fdtdTE.c have no NTFF */
     precision = FDTD_FP64;
     ntff_time = 0;
   }
@@ -242,7 +244,7 @@ static void solver_init(int kind)
   solver_t *s = &slv_[kind];
   if (s->g) solver_free(kind);
   CHECK(solver_make_grid(kind, precision_, ntff_time_, block_T_, block_R_, &s->g));
-  const int split = kind == SLV_SPLIT_TM;
+  const int split = is_split(kind);
   int dims[4];
   CHECK(fdtd_hip_grid_dims(s->g, dims));
   s->NPXa = dims[0];
@@ -298,8 +300,9 @@ static double *solver_eps(int kind)
 {
   solver_t *s = &slv_[kind];
   if (!s->g) return NULL;
-  /* MPI TM: EPS_EZ, MPI TE: EPS_EY, plain TM: EPS_EZ, plain TE: EPS_EX */
-  int which = (kind == SLV_MPI_TE) ? 1 : 0;
+  /* MPI TM: EPS_EZ, MPI TE: EPS_EY, plain TM: EPS_EZ, plain TE: EPS_EX, split TM: EPS_EZ,
+     split TE: EPS_EY (fdtdTE.c:62-65) */
+  int which = (kind == SLV_MPI_TE || kind == SLV_SPLIT_TE) ? 1 : 0;
   CHECK(fdtd_hip_grid_get_eps(s->g, which, s->eps_mirror));
   return s->eps_mirror;
 }
@@ -464,15 +467,15 @@ static void mpi_te_ntff_output(void)
   }
 }
 
-/* field_outputElliptic (field.c:322-341) of Ez: |Ez|^2 interpolated bilinearly
-   (myComplex.c:40-51) on the circle of radius 1.2 lambda around the grid centre,
-   the split-field TM solver's reset output (fdtdTM.c:167-173) */
+/* field_outputElliptic (field.c:322-341) of Ez (TM) or Ey (TE): |.|^2 interpolated
+   bilinearly (myComplex.c:40-51) on the circle of radius 1.2 lambda around the grid
+   centre, the split-field solvers' reset output (fdtdTM.c:167-173, fdtdTE.c:269-274) */
 static void write_elliptic(int kind)
 {
   FieldInfo fi = field_getFieldInfo();
   char name[128];
-  snprintf(name, sizeof name, "tm_%dnm.txt", fi.h_u_nm);
-  const double complex *ez = solver_field(kind, 0);
+  snprintf(name, sizeof name, "%s_%dnm.txt", kind == SLV_SPLIT_TM ? "tm" : "te", fi.h_u_nm);
+  const double complex *ez = solver_field(kind, kind == SLV_SPLIT_TM ? 0 : 1);
   const double lam = field_getLambda();
   printf("output start\n");
   FILE *fp = fopen(name, "w");
@@ -485,6 +488,12 @@ static void write_elliptic(int kind)
     double x = 1.2 * lam * cos(rad) + N_PX / 2.0;
     double y = 1.2 * lam * sin(rad) + N_PY / 2.0;
     int i = (int)floor(x), j = (int)floor(y);
+    if (i < 0 || j < 0 || i + 1 >= N_PX || j + 1 >= N_PY) {
+      /* the circle leaves the grid (sides under 2.4 lambda): the reference reads
+         outside its array here; 0 is written instead */
+      fprintf(fp, "%d %lf \n", 180 - ang, 0.0);
+      continue;
+    }
     double dx = x - i, dy = y - j;
     size_t k = (size_t)i * N_PY + j;
     double complex v = ez[k] * (1.0 - dx) * (1.0 - dy) + ez[k + N_PY] * dx * (1.0 - dy) + ez[k + 1] * (1.0 - dx) * dy +
@@ -499,7 +508,7 @@ static void solver_reset(int kind)
 {
   solver_t *s = &slv_[kind];
   if (!s->g) return;
-  if (kind == SLV_SPLIT_TM && file_output_) write_elliptic(kind);     /* fdtdTM.c:167-173 */
+  if (is_split(kind) && file_output_) write_elliptic(kind);            /* fdtdTM.c:167-173, fdtdTE.c:269-274 */
   if (!is_mpi(kind) && file_output_ && s->ntff_time) write_spectra();  /* fdtdTM_upml.c:87, fdtdTE_upml.c:117 */
   CHECK(fdtd_hip_grid_reset(s->g));
 }
@@ -596,23 +605,40 @@ static void split_tm_init(void) { solver_init(SLV_SPLIT_TM); }
 static void split_tm_update(void) { solver_update(SLV_SPLIT_TM); }
 static void split_tm_reset(void) { solver_reset(SLV_SPLIT_TM); }
 static void split_tm_finish(void) { solver_finish(SLV_SPLIT_TM); }
-static double complex *split_field(int q)  /* 0 Ezx (slot 1), 1 Ezy (slot 2) */
+/* the split components: TM q 0 Ezx (slot 1), 1 Ezy (slot 2); TE q 0 Hzx (slot 7), 1 Hzy (slot 8) */
+static double complex *split_field(int kind, int q)
 {
-  solver_t *s = &slv_[SLV_SPLIT_TM];
+  solver_t *s = &slv_[kind];
   if (!s->g) return NULL;
-  CHECK(fdtd_hip_grid_get_field(s->g, 1 + q, (double *)s->split_mirror[q]));
+  CHECK(fdtd_hip_grid_get_field(s->g, (kind == SLV_SPLIT_TM ? 1 : 7) + q, (double *)s->split_mirror[q]));
   return s->split_mirror[q];
 }
 fdtd_method fdtdTM_getUpdate(void) { return split_tm_update; }
 fdtd_method fdtdTM_getFinish(void) { return split_tm_finish; }
 fdtd_method fdtdTM_getReset(void) { return split_tm_reset; }
 fdtd_method fdtdTM_getInit(void) { return split_tm_init; }
-fdtd_dcomplex *fdtdTM_getEzx(void) { return (fdtd_dcomplex *)split_field(0); }
-fdtd_dcomplex *fdtdTM_getEzy(void) { return (fdtd_dcomplex *)split_field(1); }
+fdtd_dcomplex *fdtdTM_getEzx(void) { return (fdtd_dcomplex *)split_field(SLV_SPLIT_TM, 0); }
+fdtd_dcomplex *fdtdTM_getEzy(void) { return (fdtd_dcomplex *)split_field(SLV_SPLIT_TM, 1); }
 fdtd_dcomplex *fdtdTM_getEz(void) { return (fdtd_dcomplex *)solver_field(SLV_SPLIT_TM, 0); }
 fdtd_dcomplex *fdtdTM_getHy(void) { return (fdtd_dcomplex *)solver_field(SLV_SPLIT_TM, 2); }
 fdtd_dcomplex *fdtdTM_getHx(void) { return (fdtd_dcomplex *)solver_field(SLV_SPLIT_TM, 1); }
 double *fdtdTM_getEps(void) { return solver_eps(SLV_SPLIT_TM); }
+
+/* ---------------------------------------------------------------- split-field PML TE (fdtdTE.c) */
+static void split_te_init(void) { solver_init(SLV_SPLIT_TE); }
+static void split_te_update(void) { solver_update(SLV_SPLIT_TE); }
+static void split_te_reset(void) { solver_reset(SLV_SPLIT_TE); }
+static void split_te_finish(void) { solver_finish(SLV_SPLIT_TE); }
+fdtd_method fdtdTE_getUpdate(void) { return split_te_update; }
+fdtd_method fdtdTE_getFinish(void) { return split_te_finish; }
+fdtd_method fdtdTE_getReset(void) { return split_te_reset; }
+fdtd_method fdtdTE_getInit(void) { return split_te_init; }
+fdtd_dcomplex *fdtdTE_getEx(void) { return (fdtd_dcomplex *)solver_field(SLV_SPLIT_TE, 0); }
+fdtd_dcomplex *fdtdTE_getEy(void) { return (fdtd_dcomplex *)solver_field(SLV_SPLIT_TE, 1); }
+fdtd_dcomplex *fdtdTE_getHzx(void) { return (fdtd_dcomplex *)split_field(SLV_SPLIT_TE, 0); }
+fdtd_dcomplex *fdtdTE_getHzy(void) { return (fdtd_dcomplex *)split_field(SLV_SPLIT_TE, 1); }
+fdtd_dcomplex *fdtdTE_getHz(void) { return (fdtd_dcomplex *)solver_field(SLV_SPLIT_TE, 2); }
+double *fdtdTE_getEps(void) { return solver_eps(SLV_SPLIT_TE); }
 
 /* ---------------------------------------------------------------- plain TE */
 static void te_init(void) { solver_init(SLV_TE); }

@@ -21,6 +21,7 @@ _GETTERS = {
     N.TM_UPML_2D: ("fdtdTM_upml_getEz", "fdtdTM_upml_getHx", "fdtdTM_upml_getHy", "fdtdTM_upml_getEps"),
     N.TE_UPML_2D: ("fdtdTE_upml_getEx", "fdtdTE_upml_getEy", "fdtdTE_upml_getHz", "fdtdTE_upml_getEps"),
     N.TM_2D: ("fdtdTM_getEz", "fdtdTM_getHx", "fdtdTM_getHy", "fdtdTM_getEps"),  # split-field PML (FP64)
+    N.TE_2D: ("fdtdTE_getEx", "fdtdTE_getEy", "fdtdTE_getHz", "fdtdTE_getEps"),
 }
 
 

@@ -1234,6 +1234,10 @@ void orc_elliptic(const orc_sim *s, double *out181)
     double x = 1.2 * s->lambda_s * cos(rad) + s->N_PX / 2.0;
     double y = 1.2 * s->lambda_s * sin(rad) + s->N_PY / 2.0;
     int i = (int)floor(x), j = (int)floor(y);
+    if (i < 0 || j < 0 || i + 1 >= s->NPXa || j + 1 >= NPY) {
+      out181[180 - ang] = 0.0;  /* the reference reads outside the array here (grids under 2.4 lambda) */
+      continue;
+    }
     double dx = x - i, dy = y - j;
     int k = i * NPY + j;
     dcx v = Ez[k] * (1.0 - dx) * (1.0 - dy) + Ez[k + NPY] * dx * (1.0 - dy) + Ez[k + 1] * (1.0 - dx) * dy +

@@ -33,7 +33,7 @@ RUN_CASES = [
     ("tm2d_mie_a0", 0, 1, 0, 4000, 4000, 50, 10, 500, 0, 300),      # split-field PML TM (fdtdTM.c)
     ("tm2d_zigzag_a30_rect", 0, 5, 0, 1200, 1500, 20, 8, 450, 30, 150),
     ("te2d_mie_a0", 1, 1, 0, 4000, 4000, 50, 10, 500, 0, 300),       # split-field PML TE (fdtdTE.c)
-    ("te2d_layer_a45_rect", 1, 2, 3, 1200, 1600, 20, 8, 550, 45, 150),
+    ("te2d_layer_a45_rect", 1, 2, 3, 1800, 2000, 20, 8, 550, 45, 150),
 ]
 
 
