@@ -562,6 +562,58 @@ static void set_coef_te_split(orc_sim *s)
   }
 }
 
+/* Non-standard FDTD TM (nsFdtdTM.c:222-306): C[0] C_EZX, C[1] C_EZXLX, C[2] C_EZY,
+   C[3] C_EZYLY, C[4] C_HX, C[5] C_HXLY, C[6] C_HY, C[7] C_HYLX, and C[8] the
+   scattered-wave factor 1/(_n n) - 1 of field_nsScatteredWaveNotUPML
+   (field.c:155-177) with EPS_EZ; eps maps as the split-field TM solver's */
+static double ns_beta(double alpha) { return tanh(alpha) / (1 + pow(tanh(alpha), 2)); } /* nsFdtdTM.c:221-224 */
+static double ns_coef1(double alpha, double beta) { (void)alpha; return (1 - beta) / (1 + beta); }
+static void set_coef_ns_tm(orc_sim *s)
+{
+  const double R = 1.0e-8, M = 2.0;
+  const double sig_max = -(M + 1.0) * s->EPS0 * s->C0 / s->N_PML * log(R);
+  const double w_s = s->Omega_s, k_s = s->K_s;
+  for (int i = 0; i < s->NPXa; i++) {
+    for (int j = 0; j < s->NPYa; j++) {
+      int k = i * s->NPYa + j;
+      s->EPS[0][k] = models_eps(s, i, j, ORC_D_XY);
+      s->EPS[1][k] = models_eps(s, i, j + 0.5, ORC_D_Y);
+      s->EPS[2][k] = models_eps(s, i + 0.5, j, ORC_D_X);
+      double sig_ez_x = sig_max * sigma_x(s, i), sig_ez_y = sig_max * sigma_y(s, j);
+      double sig_hx_y = sig_max * sigma_y(s, j + 0.5), sig_hy_x = sig_max * sigma_x(s, i + 0.5);
+      double a_hx_y = sig_hx_y / (2 * s->EPS0), a_hy_x = sig_hy_x / (2 * s->EPS0);
+      double a_ez_x = sig_ez_x / (2 * s->EPS0), a_ez_y = sig_ez_y / (2 * s->EPS0);
+      double b_hx_y = ns_beta(a_hx_y), b_hy_x = ns_beta(a_hy_x), b_ez_x = ns_beta(a_ez_x), b_ez_y = ns_beta(a_ez_y);
+      double z_ez = sqrt(s->MU0 / s->EPS[0][k]);
+      double n_ez = sqrt(s->EPS[0][k] / s->EPS0);
+      double k_ez_s = k_s * n_ez;
+      double u_ez = sin(w_s * 0.5) / sin(k_ez_s * 0.5);
+      s->C[0][k] = ns_coef1(a_ez_x, b_ez_x);
+      s->C[1][k] = u_ez * z_ez / (1 + b_ez_x);
+      s->C[2][k] = ns_coef1(a_ez_y, b_ez_y);
+      s->C[3][k] = u_ez * z_ez / (1.0 + b_ez_x); /* b_ez_x, as nsFdtdTM.c:282 has it */
+      double z_hx = sqrt(s->MU0 / s->EPS[1][k]);
+      double n_hx = sqrt(s->EPS[1][k] / s->EPS0);
+      double k_hx_s = k_s * n_hx;
+      double u_hx = sin(w_s * 0.5) / sin(k_hx_s * 0.5);
+      s->C[4][k] = ns_coef1(a_hx_y, b_hx_y);
+      s->C[5][k] = u_hx / z_hx / (1.0 + b_hx_y);
+      double z_hy = sqrt(s->MU0 / s->EPS[2][k]);
+      double n_hy = sqrt(s->EPS[2][k] / s->EPS0);
+      double k_hy_s = k_s * n_hy;
+      double u_hy = sin(w_s * 0.5) / sin(k_hy_s * 0.5);
+      s->C[6][k] = ns_coef1(a_hy_x, b_hy_x);
+      s->C[7][k] = u_hy / z_hy / (1.0 + b_hy_x);
+      /* field.c:168-174 */
+      double n = sqrt(s->EPS[0][k] / s->EPS0);
+      double u0 = sin(w_s * 0.5) / sin(k_s * 0.5);
+      double u1 = sin(w_s * 0.5) / sin(n * k_s * 0.5);
+      double _n = u0 / u1;
+      s->C[8][k] = 1.0 / (_n * n) - 1.0;
+    }
+  }
+}
+
 /* TE: mpiTE_UPML.c:489-536 (MPI) and fdtdTE_upml.c:361-412 (plain); no /cos(pi/3) (Q7) */
 static void set_coef_te(orc_sim *s)
 {
@@ -610,7 +662,8 @@ orc_sim *orc_create(const orc_config *cfg)
   models_init(s);  /* models_initModel runs after field_init (simulator.c:229-231) */
   s->mpi = (cfg->solver == ORC_MPI_TM_UPML || cfg->solver == ORC_MPI_TE_UPML);
   s->tm = (cfg->solver == ORC_MPI_TM_UPML || cfg->solver == ORC_TM_UPML || cfg->solver == ORC_TM_2D);
-  s->split = (cfg->solver == ORC_TM_2D || cfg->solver == ORC_TE_2D);
+  s->split = (cfg->solver == ORC_TM_2D || cfg->solver == ORC_TE_2D || cfg->solver == ORC_NS_TM_2D);
+  if (cfg->solver == ORC_NS_TM_2D) s->tm = true;
   /* MPI single rank: SUB_N_PX = N_PX + 2 with a ghost ring (mpiTM_UPML.c:737-743) */
   s->off = s->mpi ? 1 : 0;
   s->NPXa = s->N_PX + 2 * s->off;
@@ -626,6 +679,7 @@ orc_sim *orc_create(const orc_config *cfg)
   }
   if (cfg->solver == ORC_TM_2D) set_coef_tm_split(s);
   else if (cfg->solver == ORC_TE_2D) set_coef_te_split(s);
+  else if (cfg->solver == ORC_NS_TM_2D) set_coef_ns_tm(s);
   else if (s->tm) set_coef_tm(s);
   else set_coef_te(s);
   s->R0 = 1.0e6 * to_cell(s, 500);  /* ntffTM_init (ntffTM.c:31), ntffTE_init (ntffTE.c:16) */
@@ -1225,6 +1279,56 @@ static void split_te_step(orc_sim *s)
     }
 }
 
+/* Non-standard FDTD TM step (nsFdtdTM.c:68-151): calcH with the non-standard
+   second differences (weight r/2, r = 1/6 + k^2/180 - k^4/23040), calcE (Ezx,
+   Ezy), the scattered wave on Ezy (field.c:155-177, gap 0, dot 1), then
+   Ez = Ezx + Ezy; slots as the split-field TM solver's */
+static void ns_tm_step(orc_sim *s)
+{
+  const int NPX = s->NPXa, NPY = s->NPYa, dx = NPY, dy = 1;
+  dcx *Ez = s->F[0], *Ezx = s->F[1], *Ezy = s->F[2], *Hx = s->F[3], *Hy = s->F[6];
+  double **C = s->C;
+  const double k_s = s->K_s;
+  const double r = 1.0 / 6.0 + k_s * k_s / 180.0 - pow(k_s, 4) / 23040;
+  const double r_2 = r / 2.0;
+  for (int i = 1; i < NPX - 1; i++)
+    for (int j = 1; j < NPY - 1; j++) {
+      int k = IDX(i, j);
+      dcx ns = r_2 * ((Ez[k + dy + dx] + Ez[k + dy - dx] - 2 * Ez[k + dy]) - (Ez[k + dx] + Ez[k - dx] - 2 * Ez[k]));
+      Hx[k] = C[4][k] * Hx[k] - C[5][k] * (Ez[k + dy] - Ez[k] + ns);
+    }
+  for (int i = 1; i < NPX - 1; i++)
+    for (int j = 1; j < NPY - 1; j++) {
+      int k = IDX(i, j);
+      dcx ns = r_2 * ((Ez[k + dx + dy] + Ez[k + dx - dy] - 2 * Ez[k + dx]) - (Ez[k + dy] + Ez[k - dy] - 2 * Ez[k]));
+      Hy[k] = C[6][k] * Hy[k] + C[7][k] * (Ez[k + dx] - Ez[k] + ns);
+    }
+  for (int i = 1; i < NPX - 1; i++)
+    for (int j = 1; j < NPY - 1; j++) {
+      int k = IDX(i, j);
+      Ezx[k] = C[0][k] * Ezx[k] + C[1][k] * (Hy[k] - Hy[k - dx]);
+    }
+  for (int i = 1; i < NPX - 1; i++)
+    for (int j = 1; j < NPY - 1; j++) {
+      int k = IDX(i, j);
+      Ezy[k] = C[2][k] * Ezy[k] - C[3][k] * (Hx[k] - Hx[k - dy]);
+    }
+  double time = s->time, w_s = s->Omega_s, ray_coef = s->ray_coef * 1.0;
+  double rad = s->Angle_deg * M_PI / 180.0;
+  double ks_cos = cos(rad) * k_s, ks_sin = sin(rad) * k_s;
+  for (int i = 1; i < NPX - 1; i++)
+    for (int j = 1; j < NPY - 1; j++) {
+      int k = IDX(i, j);
+      double kr = (i + 0) * ks_cos + (j + 0) * ks_sin;
+      Ezy[k] += ray_coef * C[8][k] * (cexp(I * (kr - w_s * (time + 1.0))) - cexp(I * (kr - w_s * (time))));
+    }
+  for (int i = 1; i < NPX - 1; i++)
+    for (int j = 1; j < NPY - 1; j++) {
+      int k = IDX(i, j);
+      Ez[k] = Ezx[k] + Ezy[k];
+    }
+}
+
 void orc_elliptic(const orc_sim *s, double *out181)
 {
   const dcx *Ez = s->cfg.solver == ORC_TE_2D ? s->F[3] : s->F[0];  /* TE_2D: Ey (fdtdTE.c:269-274) */
@@ -1254,6 +1358,9 @@ void orc_step(orc_sim *s)
     break;
   case ORC_TE_2D:
     split_te_step(s);
+    break;
+  case ORC_NS_TM_2D:
+    ns_tm_step(s);
     break;
   case ORC_MPI_TM_UPML: /* update (mpiTM_UPML.c:196-217) */
     tm_calc_jd_e(s);

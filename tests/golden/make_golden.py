@@ -34,6 +34,8 @@ RUN_CASES = [
     ("tm2d_zigzag_a30_rect", 0, 5, 0, 1200, 1500, 20, 8, 450, 30, 150),
     ("te2d_mie_a0", 1, 1, 0, 4000, 4000, 50, 10, 500, 0, 300),       # split-field PML TE (fdtdTE.c)
     ("te2d_layer_a45_rect", 1, 2, 3, 1800, 2000, 20, 8, 550, 45, 150),
+    ("nstm_mie_a0", 6, 1, 0, 4000, 4000, 50, 10, 500, 0, 300),        # non-standard FDTD TM (nsFdtdTM.c)
+    ("nstm_zigzag_a30_rect", 6, 5, 0, 1400, 1800, 20, 8, 450, 30, 150),
 ]
 
 
@@ -97,7 +99,7 @@ def main(only=()):
             if os.path.exists(spec):
                 full = np.fromfile(spec, np.float64).reshape(321, 360)
                 d["spectra"] = full[::4]           # lambda = 380, 384, ..., 700
-            for pol in ("tm", "te"):  # TM_2D / TE_2D reset(): field_outputElliptic of Ez / Ey
+            for pol in ("tm", "te", "ns_tm"):  # TM_2D / TE_2D / NS_TM_2D reset(): field_outputElliptic
                 ell = os.path.join(out, f"{pol}_{hu}nm.txt")
                 if os.path.exists(ell):
                     d["elliptic"] = np.loadtxt(ell)[:, 1]

@@ -2,7 +2,7 @@
 
 The fixtures under tests/golden/ were written by tests/golden/make_golden.py,
 which runs the reference's C sources (mpiTM_UPML.c, mpiTE_UPML.c, fdtdTM_upml.c,
-fdtdTE_upml.c, the split-field fdtdTM.c and fdtdTE.c, field.c, ntffTM.c, ntffTE.c, cfft.c, ntff.c, models.c and the
+fdtdTE_upml.c, the split-field fdtdTM.c and fdtdTE.c, nsFdtdTM.c, field.c, ntffTM.c, ntffTE.c, cfft.c, ntff.c, models.c and the
 model files) as one MPI rank.  The oracle restates the same arithmetic in the
 same order, so most comparisons are bit-exact.
 """
@@ -14,7 +14,8 @@ from oracle import pyoracle as po
 
 RUN_CASES = ["mpitm_mie_a0", "mpitm_mie_a30_rect", "mpite_mie_a0", "mpite_mie_a60_rect",
              "tm_mie_pulse", "te_mie_pulse", "mpitm_morpho", "mpite_zigzag",
-             "tm2d_mie_a0", "tm2d_zigzag_a30_rect", "te2d_mie_a0", "te2d_layer_a45_rect"]
+             "tm2d_mie_a0", "tm2d_zigzag_a30_rect", "te2d_mie_a0", "te2d_layer_a45_rect",
+             "nstm_mie_a0", "nstm_zigzag_a30_rect"]
 
 
 def oracle_for(g):
@@ -29,7 +30,7 @@ def test_fields_eps_farfield_match_reference(name):
     g = load_golden(name)
     sim, steps = oracle_for(g)
     sim.step(steps)
-    tm = sim.cfg.solver in (po.MPI_TM_UPML, po.TM_UPML, po.TM_2D)
+    tm = sim.cfg.solver in (po.MPI_TM_UPML, po.TM_UPML, po.TM_2D, po.NS_TM_2D)
     which = (0, 3, 6)  # Ez,Hx,Hy / Ex,Ey,Hz
     for k, w in enumerate(which):
         ref = g[f"f{k}"]
