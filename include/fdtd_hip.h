@@ -31,8 +31,14 @@ enum { FDTD_MODE_TM = 0, FDTD_MODE_TE = 1 };
  *              arrays, CW scattered wave; FP64, no per-step NTFF.  TM: H half
  *              first, the wave on Ezx, slots Ez 0, Ezx 1, Ezy 2, Hx 3, Hy 6;
  *              TE: E half first, the wave on Ey, slots Ex 0, Ey 3, Hz 6, Hzx 7,
- *              Hzy 8. */
-enum { FDTD_FLAVOUR_MPI = 0, FDTD_FLAVOUR_PLAIN = 1, FDTD_FLAVOUR_SPLIT = 2 };
+ *              Hzy 8.
+ * ns flavour: the non-standard FDTD TM solver (nsFdtdTM.c): as the split
+ *              TM flavour (N_PX x N_PY, FP64, no per-step NTFF, H half first,
+ *              slots Ez 0, Ezx 1, Ezy 2, Hx 3, Hy 6), with the non-standard
+ *              second differences in the H half and the NS scattered wave on
+ *              Ezy; its coefficients come from fdtd_hip_grid_set_coefs (the
+ *              1-D decay tables) and fdtd_hip_grid_set_ns_coefs (per cell). */
+enum { FDTD_FLAVOUR_MPI = 0, FDTD_FLAVOUR_PLAIN = 1, FDTD_FLAVOUR_SPLIT = 2, FDTD_FLAVOUR_NS = 3 };
 enum { FDTD_FP64 = 0, FDTD_FP32 = 1 };
 
 /* Model ids: enum MODEL (models.h:5-14) values */
@@ -91,6 +97,8 @@ typedef struct fdtd_grid_desc {
  *       x1 mu0/eps0 sig_max sigma_x(i + 1/2)  (TM sig_hy_xx, TE sig_hz_xx)
  *       y0 sig_max sigma_y(j)  (TM sig_ez_y, TE sig_ex_y)
  *       y1 mu0/eps0 sig_max sigma_y(j + 1/2)  (TM sig_hx_yy, TE sig_hz_yy)
+ * NS flavour (nsFdtdTM.c:226-306, coef1 (1-b)/(1+b) of the simplified beta b):
+ *       x0 C_EZX(i)  x1 C_HY(i + 1/2)   y0 C_EZY(j)  y1 C_HX(j + 1/2)
  * Each x table has N_PX entries, each y table N_PY. */
 typedef struct fdtd_coef_tables {
   const double *x[5];
@@ -136,6 +144,11 @@ int fdtd_hip_grid_fill_eps(fdtd_grid *g, const fdtd_model_state *m);
 /* which: TM 0 EPS_EZ 1 EPS_HX 2 EPS_HY ; TE 0 EPS_EX 1 EPS_EY 2 EPS_HZ; solver layout */
 int fdtd_hip_grid_set_eps(fdtd_grid *g, int which, const double *host);
 int fdtd_hip_grid_get_eps(fdtd_grid *g, int which, double *host);
+/* NS flavour: the per-cell coefficients, five N_PX*N_PY maps in this order:
+ * C_EZXLX, C_EZYLY, C_HXLY, C_HYLX (nsFdtdTM.c:276-304) and the factor
+ * 1/(_n n) - 1 of the NS scattered wave (field.c:155-177) of EPS_EZ; r_half is
+ * r/2 of the second differences, r = 1/6 + k^2/180 - k^4/23040 (nsFdtdTM.c:117-118) */
+int fdtd_hip_grid_set_ns_coefs(fdtd_grid *g, const double *cell5, double r_half);
 /* one reference update(): E/H halves, source, per-step NTFF */
 int fdtd_hip_grid_step(fdtd_grid *g, const fdtd_step_params *p);
 /* which 0..8: TM Ez Jz Dz Hx Mx Bx Hy My By / TE Ex Jx Dx Ey Jy Dy Hz Mz Bz;

@@ -1,6 +1,6 @@
 /*
- * mpifdtd_solvers.h -- the four UPML solvers' and the two split-field PML
- * solvers' entry points, backed by HIP.
+ * mpifdtd_solvers.h -- the four UPML solvers', the two split-field PML
+ * solvers' and the non-standard FDTD TM solver's entry points, backed by HIP.
  *
  * Same symbols and signatures as the reference's solver headers:
  *   mpiTM_UPML.h:5-18   (mpiTM_UPML.c:88-136)   MPI TMz UPML, run as one rank on one GPU
@@ -11,6 +11,10 @@
  *                                               reset() writes tm_<h>nm.txt)
  *   fdtdTE.h:5-16       (fdtdTE.c:29-66)        split-field PML TEz (FP64 only; its
  *                                               reset() writes te_<h>nm.txt)
+ *   nsFdtdTM.h:6-17     (nsFdtdTM.c:32-42)      non-standard FDTD TMz (FP64 only; its
+ *                                               reset() writes ns_tm_<h>nm.txt;
+ *                                               getEzx / getEzy, declared but not
+ *                                               defined by the reference, are defined)
  * get{Update,Init,Finish,Reset} return the solver's void(void) methods; the
  * field getters return borrowed host pointers (valid from init to finish) that
  * are refreshed from device memory on every getter call; getEps returns the
@@ -92,6 +96,20 @@ fdtd_dcomplex *fdtdTE_getHzx(void);
 fdtd_dcomplex *fdtdTE_getHzy(void);
 fdtd_dcomplex *fdtdTE_getHz(void);
 double *fdtdTE_getEps(void);
+
+/* nsFdtdTM.h:6-17 (non-standard FDTD TMz; getEps returns EPS_EZ).  The reference's
+ * nsFdtdTM_getSolver returns a Solver whose fields it leaves unset; it has no
+ * counterpart here. */
+fdtd_method nsFdtdTM_getUpdate(void);
+fdtd_method nsFdtdTM_getFinish(void);
+fdtd_method nsFdtdTM_getReset(void);
+fdtd_method nsFdtdTM_getInit(void);
+fdtd_dcomplex *nsFdtdTM_getEzx(void);
+fdtd_dcomplex *nsFdtdTM_getEzy(void);
+fdtd_dcomplex *nsFdtdTM_getEz(void);
+fdtd_dcomplex *nsFdtdTM_getHy(void);
+fdtd_dcomplex *nsFdtdTM_getHx(void);
+double *nsFdtdTM_getEps(void);
 
 /* fdtdTE_upml.h:5-14 */
 fdtd_method fdtdTE_upml_getUpdate(void);

@@ -189,7 +189,9 @@ def lib():
               "fdtdTM_upml_getEz", "fdtdTM_upml_getHx", "fdtdTM_upml_getHy", "fdtdTM_upml_getEps",
               "fdtdTE_upml_getEx", "fdtdTE_upml_getEy", "fdtdTE_upml_getHz", "fdtdTE_upml_getEps",
               "fdtdTM_getEz", "fdtdTM_getHx", "fdtdTM_getHy", "fdtdTM_getEzx", "fdtdTM_getEzy", "fdtdTM_getEps",
-              "fdtdTE_getEx", "fdtdTE_getEy", "fdtdTE_getHz", "fdtdTE_getHzx", "fdtdTE_getHzy", "fdtdTE_getEps"):
+              "fdtdTE_getEx", "fdtdTE_getEy", "fdtdTE_getHz", "fdtdTE_getHzx", "fdtdTE_getHzy", "fdtdTE_getEps",
+              "nsFdtdTM_getEz", "nsFdtdTM_getHx", "nsFdtdTM_getHy", "nsFdtdTM_getEzx", "nsFdtdTM_getEzy",
+              "nsFdtdTM_getEps"):
         sig(n, P)
     _lib = L
     return L

@@ -395,6 +395,64 @@ __global__ __launch_bounds__(256) void k_split_te_h(GridPtrs g, Tabs t)
   Hz[k] = make_double2(nzx.x + nzy.x, nzx.y + nzy.y);
 }
 
+// Non-standard FDTD TM (nsFdtdTM.c).  Per-cell coefficient maps nsc[q * ncell + k]:
+// 0 C_EZXLX, 1 C_EZYLY, 2 C_HXLY, 3 C_HYLX, 4 the NS scattered-wave factor.
+// calcH (nsFdtdTM.c:111-151): the Yee difference plus r/2 times the difference of
+// the second differences across the other axis.  x: 1 C_HY; y: 1 C_HX
+__global__ __launch_bounds__(256) void k_ns_tm_h(GridPtrs g, Tabs t, const double *__restrict__ nsc, double r2)
+{
+  FOR_CELL();
+  const size_t n = (size_t)g.NPXa * NPY;
+  const double2 *Ez = g.F[0];
+  double2 *Hx = g.F[3], *Hy = g.F[6];
+  // c (i,j), u (i,j+1), d (i,j-1), r (i+1,j), l (i-1,j), ur (i+1,j+1), ul (i-1,j+1), dr (i+1,j-1)
+  const double2 c = Ez[k], u = Ez[k + 1], d = Ez[k - 1], r = Ez[k + NPY], l = Ez[k - NPY];
+  const double2 ur = Ez[k + NPY + 1], ul = Ez[k - NPY + 1], dr = Ez[k + NPY - 1];
+  const double chx = t.y[1][y], chy = t.x[1][x];
+  const double chxly = nsc[2 * n + k], chylx = nsc[3 * n + k];
+  const double2 hx = Hx[k], hy = Hy[k];
+  // (Ez[k+dy+dx] + Ez[k+dy-dx] - 2 Ez[k+dy]) - (Ez[k+dx] + Ez[k-dx] - 2 Ez[k])
+  const double nxr = r2 * (((ur.x + ul.x) - 2 * u.x) - ((r.x + l.x) - 2 * c.x));
+  const double nxi = r2 * (((ur.y + ul.y) - 2 * u.y) - ((r.y + l.y) - 2 * c.y));
+  // (Ez[k+dx+dy] + Ez[k+dx-dy] - 2 Ez[k+dx]) - (Ez[k+dy] + Ez[k-dy] - 2 Ez[k])
+  const double nyr = r2 * (((ur.x + dr.x) - 2 * r.x) - ((u.x + d.x) - 2 * c.x));
+  const double nyi = r2 * (((ur.y + dr.y) - 2 * r.y) - ((u.y + d.y) - 2 * c.y));
+  Hx[k] = make_double2(chx * hx.x - chxly * ((u.x - c.x) + nxr), chx * hx.y - chxly * ((u.y - c.y) + nxi));
+  Hy[k] = make_double2(chy * hy.x + chylx * ((r.x - c.x) + nyr), chy * hy.y + chylx * ((r.y - c.y) + nyi));
+}
+
+// calcE (nsFdtdTM.c:85-109), the NS scattered wave on Ezy (field.c:155-177 with gap
+// 0 and dot 1: phases w (t+1) and w t), then Ez = Ezx + Ezy (nsFdtdTM.c:68-80).
+// x: 0 C_EZX; y: 0 C_EZY
+__global__ __launch_bounds__(256) void k_ns_tm_e(GridPtrs g, Tabs t, const double *__restrict__ nsc, StepArgs s)
+{
+  FOR_CELL();
+  const size_t n = (size_t)g.NPXa * NPY;
+  double2 *Ez = g.F[0], *Ezx = g.F[1], *Ezy = g.F[2];
+  const double2 *Hx = g.F[3], *Hy = g.F[6];
+  const double cezx = t.x[0][x], cezy = t.y[0][y];
+  const double cezxlx = nsc[k], cezyly = nsc[n + k], fac = nsc[4 * n + k];
+  const double2 zx = Ezx[k], zy = Ezy[k];
+  const double2 hy = Hy[k], hyl = Hy[k - NPY], hx = Hx[k], hxb = Hx[k - 1];
+  double2 nzx, nzy;
+  nzx.x = cezx * zx.x + cezxlx * (hy.x - hyl.x);
+  nzx.y = cezx * zx.y + cezxlx * (hy.y - hyl.y);
+  nzy.x = cezy * zy.x - cezyly * (hx.x - hxb.x);
+  nzy.y = cezy * zy.y - cezyly * (hx.y - hxb.y);
+  if (fac != 0.0) {  // vacuum cells (n = 1) have a factor of exactly zero
+    const double kr = (i + 0.0) * s.ks_cos + (j + 0.0) * s.ks_sin;
+    const double amp = s.ray_coef * fac;
+    double s1, c1, s0, c0;
+    sincos(kr - s.w_s * (s.time + 1.0), &s1, &c1);
+    sincos(kr - s.w_s * s.time, &s0, &c0);
+    nzy.x += amp * (c1 - c0);
+    nzy.y += amp * (s1 - s0);
+  }
+  Ezx[k] = nzx;
+  Ezy[k] = nzy;
+  Ez[k] = make_double2(nzx.x + nzy.x, nzx.y + nzy.y);
+}
+
 // TE E half.  x: 0 C_DXJX1 1 C_DXJX0 2 C_DY, sx; y: 0 C_JX 1 C_JXHZ, sy
 template <int PLAIN>
 __global__ __launch_bounds__(256) void k_te_e(GridPtrs g, Tabs t, StepArgs s)
@@ -1048,6 +1106,8 @@ struct fdtd_grid {
   double2 *tw;
   fdtd_model_state *mstate;
   FastState *fs;  // FP32 fused path (precision FDTD_FP32); F[] then serve as FP64 export buffers
+  double *nsc;    // NS flavour: [5][ncell] per-cell coefficients (fdtd_hip_grid_set_ns_coefs)
+  double ns_r2;   // NS flavour: r/2 of the second differences
 };
 
 #define NT_GUARD NT_GUARD_DEV
@@ -1356,10 +1416,13 @@ extern "C" int fdtd_hip_grid_create(const fdtd_grid_desc *d, fdtd_grid **out)
   if (d->N_PX < 4 || d->N_PY < 4) return fdtd_fail(FDTD_EINVAL, "grid %dx%d too small", d->N_PX, d->N_PY);
   if (d->precision != FDTD_FP64 && d->precision != FDTD_FP32) return fdtd_fail(FDTD_EINVAL, "bad precision %d", d->precision);
   if (d->mode != FDTD_MODE_TM && d->mode != FDTD_MODE_TE) return fdtd_fail(FDTD_EINVAL, "bad mode %d", d->mode);
-  if (d->flavour != FDTD_FLAVOUR_MPI && d->flavour != FDTD_FLAVOUR_PLAIN && d->flavour != FDTD_FLAVOUR_SPLIT)
+  if (d->flavour != FDTD_FLAVOUR_MPI && d->flavour != FDTD_FLAVOUR_PLAIN && d->flavour != FDTD_FLAVOUR_SPLIT &&
+      d->flavour != FDTD_FLAVOUR_NS)
     return fdtd_fail(FDTD_EINVAL, "bad flavour");
   if (d->flavour == FDTD_FLAVOUR_SPLIT && (d->precision != FDTD_FP64 || d->ntff_time))
     return fdtd_fail(FDTD_EINVAL, "the split-field PML solvers run in FP64, without the per-step NTFF");
+  if (d->flavour == FDTD_FLAVOUR_NS && (d->precision != FDTD_FP64 || d->ntff_time || d->mode != FDTD_MODE_TM))
+    return fdtd_fail(FDTD_EINVAL, "the NS-FDTD solver is TM, FP64, without the per-step NTFF");
   if (d->ntff_left < 1 || d->ntff_bottom < 1 || d->ntff_right >= d->N_PX || d->ntff_top >= d->N_PY ||
       d->ntff_left >= d->ntff_right || d->ntff_bottom >= d->ntff_top)
     return fdtd_fail(FDTD_EINVAL, "NTFF contour (%d..%d x %d..%d) outside the %dx%d grid", d->ntff_left, d->ntff_right,
@@ -1385,6 +1448,10 @@ extern "C" int fdtd_hip_grid_create(const fdtd_grid_desc *d, fdtd_grid **out)
   for (int e = 0; e < 3; e++) {
     HIPCHK(dev_malloc(&g->eps[e], g->ncell * sizeof(double)));
     HIPCHK(hipMemset(g->eps[e], 0, g->ncell * sizeof(double)));
+  }
+  if (d->flavour == FDTD_FLAVOUR_NS) {
+    HIPCHK(dev_malloc(&g->nsc, 5 * g->ncell * sizeof(double)));
+    HIPCHK(hipMemset(g->nsc, 0, 5 * g->ncell * sizeof(double)));
   }
   const size_t ntab = 6 * (size_t)d->N_PX + 5 * (size_t)d->N_PY;
   HIPCHK(dev_malloc(&g->tab, ntab * sizeof(double)));
@@ -1499,6 +1566,7 @@ extern "C" int fdtd_hip_grid_destroy(fdtd_grid *g)
   for (int f = 0; f < 9; f++) (void)dev_free(g->F[f]);
   (void)dev_free(g->Fdev);
   for (int e = 0; e < 3; e++) (void)dev_free(g->eps[e]);
+  (void)dev_free(g->nsc);
   (void)dev_free(g->tab);
   for (int a = 0; a < 3; a++) (void)dev_free(g->ntb[a]);
   (void)dev_free(g->cE);
@@ -1539,6 +1607,15 @@ extern "C" int fdtd_hip_grid_set_coefs(fdtd_grid *g, const fdtd_coef_tables *t)
   HIPCHK(hipMemcpy((void *)g->tabs.sx, t->sx, nx * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy((void *)g->tabs.sy, t->sy, ny * sizeof(double), hipMemcpyHostToDevice));
   if (g->fs) return fast_state_set_coefs(g->fs, t);
+  return FDTD_OK;
+}
+
+extern "C" int fdtd_hip_grid_set_ns_coefs(fdtd_grid *g, const double *cell5, double r_half)
+{
+  if (!g || !cell5) return fdtd_fail(FDTD_EINVAL, "null argument");
+  if (g->d.flavour != FDTD_FLAVOUR_NS) return fdtd_fail(FDTD_EINVAL, "per-cell NS coefficients need the NS flavour");
+  HIPCHK(hipMemcpy(g->nsc, cell5, 5 * g->ncell * sizeof(double), hipMemcpyHostToDevice));
+  g->ns_r2 = r_half;
   return FDTD_OK;
 }
 
@@ -1634,6 +1711,12 @@ extern "C" int fdtd_hip_grid_step(fdtd_grid *g, const fdtd_step_params *p)
       hipLaunchKernelGGL(k_split_te_e, grd, blk, 0, 0, gp, g->tabs, s);
       hipLaunchKernelGGL(k_split_te_h, grd, blk, 0, 0, gp, g->tabs);
     }
+  } else if (d.flavour == FDTD_FLAVOUR_NS) {  // update (nsFdtdTM.c:68-80): calcH, calcE, NS wave, Ez
+    s.ps_on = 0;
+    s.time = p->time;
+    s.w_s = p->omega;
+    hipLaunchKernelGGL(k_ns_tm_h, grd, blk, 0, 0, gp, g->tabs, (const double *)g->nsc, g->ns_r2);
+    hipLaunchKernelGGL(k_ns_tm_e, grd, blk, 0, 0, gp, g->tabs, (const double *)g->nsc, s);
   } else if (d.mode == FDTD_MODE_TM) {
     if (plain) {
       hipLaunchKernelGGL(k_tm_h, grd, blk, 0, 0, gp, g->tabs);

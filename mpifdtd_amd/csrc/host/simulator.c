@@ -1,11 +1,11 @@
 /*
  * simulator.c -- solver dispatcher (simulator.c:19-279 of the reference).
  *
- * Selects one of the four UPML solvers or the two split-field PML solvers by
- * enum SOLVER, wires its methods into the update / init / finish / reset table
- * and drives field_nextStep after every update.  The NS-FDTD solvers are
- * outside this library; selecting them is an error, as for unknown solvers in
- * the reference.
+ * Selects one of the four UPML solvers, the two split-field PML solvers or
+ * the non-standard FDTD TM solver by enum SOLVER, wires its methods into the
+ * update / init / finish / reset table and drives field_nextStep after every
+ * update.  The NS-FDTD TE solver is outside this library; selecting it is an
+ * error, as for unknown solvers in the reference.
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -40,6 +40,13 @@ void simulator_setSolver(enum SOLVER solver)
     eps_ = fdtdTE_getEps; draw_ = fdtdTE_getEy;
     printf("TE mode \n");
     solverDir_ = "TE";
+    break;
+  case NS_TM_2D:  /* setNsTM (simulator.c:66-81) */
+    update_ = nsFdtdTM_getUpdate(); init_ = nsFdtdTM_getInit();
+    finish_ = nsFdtdTM_getFinish(); reset_ = nsFdtdTM_getReset();
+    eps_ = nsFdtdTM_getEps; draw_ = nsFdtdTM_getEz;
+    printf("NS TM mode \n");
+    solverDir_ = "NS_TM";
     break;
   case TM_UPML_2D:
     update_ = fdtdTM_upml_getUpdate(); init_ = fdtdTM_upml_getInit();

@@ -35,7 +35,7 @@ typedef struct {
   fdtd_grid *g;
   int NPXa, NPYa, off, arraySize;
   double complex *mirror[3];  /* TM: Ez Hx Hy   TE: Ex Ey Hz */
-  double complex *split_mirror[2];  /* split-field TM: Ezx Ezy   TE: Hzx Hzy */
+  double complex *split_mirror[2];  /* split-field and NS TM: Ezx Ezy   split-field TE: Hzx Hzy */
   double *eps_mirror;
   int blocked;    /* FP32 grid: steps are queued and run T per launch */
   int ntff_time;  /* the per-step time-domain NTFF runs (its outputs exist) */
@@ -58,9 +58,12 @@ static void die(const char *what)
 }
 #define CHECK(call) do { if ((call) != FDTD_OK) die(#call); } while (0)
 
-static int is_tm(int kind) { return kind == SLV_MPI_TM || kind == SLV_TM || kind == SLV_SPLIT_TM; }
+static int is_tm(int kind) { return kind == SLV_MPI_TM || kind == SLV_TM || kind == SLV_SPLIT_TM || kind == SLV_NS_TM; }
 static int is_mpi(int kind) { return kind == SLV_MPI_TM || kind == SLV_MPI_TE; }
-static int is_split(int kind) { return kind == SLV_SPLIT_TM || kind == SLV_SPLIT_TE; }
+/* the solvers with split components (FP64, no NTFF, elliptic output on reset):
+   the split-field PML TM / TE and the non-standard FDTD TM */
+static int is_split(int kind) { return kind == SLV_SPLIT_TM || kind == SLV_SPLIT_TE || kind == SLV_NS_TM; }
+static int is_ns(int kind) { return kind == SLV_NS_TM; }
 
 void fdtd_solver_setNTFFTime(int on) { ntff_time_ = on ? 1 : 0; }
 void fdtd_solver_setFileOutput(int on) { file_output_ = on ? 1 : 0; }
@@ -99,6 +102,59 @@ int fdtd_solver_getBlocking(int *T, int *R)
   return fdtd_hip_grid_get_blocking(slv_[active_].g, T, R);
 }
 
+/* the simplified beta and coef1 of nsFdtdTM.c:221-229 */
+static double ns_beta(double alpha) { return tanh(alpha) / (1 + pow(tanh(alpha), 2)); }
+static double ns_coef1(double beta) { return (1 - beta) / (1 + beta); }
+
+/* nsFdtdTM.c:226-306 and field.c:155-177: the per-cell coefficients of the NS
+   solver from the eps maps the device filled (EPS_EZ, EPS_HX, EPS_HY), each
+   formed with the reference's expression and operation order */
+static int build_ns_coefs(fdtd_grid *g)
+{
+  const int nx = N_PX, ny = N_PY;
+  const size_t n = (size_t)nx * ny;
+  double *eps = (double *)malloc(3 * n * sizeof(double)), *cell = (double *)malloc(5 * n * sizeof(double));
+  if (!eps || !cell) {
+    free(eps);
+    free(cell);
+    return FDTD_ENOMEM;
+  }
+  int rc = FDTD_OK;
+  for (int q = 0; q < 3 && rc == FDTD_OK; q++) rc = fdtd_hip_grid_get_eps(g, q, eps + q * n);
+  const double R = 1.0e-8, M = 2.0, eps0 = 1.0, C = field_getLightSpeed(), mu0 = field_getMu0();
+  const double sig_max = -(M + 1.0) * eps0 * C / N_PML * log(R);
+  const double w_s = field_getOmega(), k_s = field_getK();
+  for (int i = 0; i < nx && rc == FDTD_OK; i++) {
+    const double b_ez_x = ns_beta(sig_max * field_sigmaX(i, 0) / (2 * eps0));
+    const double b_hy_x = ns_beta(sig_max * field_sigmaX(i + 0.5, 0) / (2 * eps0));
+    for (int j = 0; j < ny; j++) {
+      const size_t k = (size_t)i * ny + j;
+      const double b_hx_y = ns_beta(sig_max * field_sigmaY(0, j + 0.5) / (2 * eps0));
+      const double e_ez = eps[k], e_hx = eps[n + k], e_hy = eps[2 * n + k];
+      const double z_ez = sqrt(mu0 / e_ez), n_ez = sqrt(e_ez / eps0);
+      const double u_ez = sin(w_s * 0.5) / sin(k_s * n_ez * 0.5);
+      cell[k] = u_ez * z_ez / (1 + b_ez_x);          /* C_EZXLX */
+      cell[n + k] = u_ez * z_ez / (1.0 + b_ez_x);    /* C_EZYLY: b_ez_x, as nsFdtdTM.c:282 has it */
+      const double z_hx = sqrt(mu0 / e_hx), n_hx = sqrt(e_hx / eps0);
+      const double u_hx = sin(w_s * 0.5) / sin(k_s * n_hx * 0.5);
+      cell[2 * n + k] = u_hx / z_hx / (1.0 + b_hx_y);  /* C_HXLY */
+      const double z_hy = sqrt(mu0 / e_hy), n_hy = sqrt(e_hy / eps0);
+      const double u_hy = sin(w_s * 0.5) / sin(k_s * n_hy * 0.5);
+      cell[3 * n + k] = u_hy / z_hy / (1.0 + b_hy_x);  /* C_HYLX */
+      /* field_nsScatteredWaveNotUPML: 1/(_n n) - 1, exactly 0 where n = 1 */
+      const double nn = sqrt(e_ez / eps0);
+      const double u0 = sin(w_s * 0.5) / sin(k_s * 0.5), u1 = sin(w_s * 0.5) / sin(nn * k_s * 0.5);
+      const double _n = u0 / u1;
+      cell[4 * n + k] = 1.0 / (_n * nn) - 1.0;
+    }
+  }
+  const double r = 1.0 / 6.0 + k_s * k_s / 180.0 - pow(k_s, 4) / 23040;  /* nsFdtdTM.c:117 */
+  if (rc == FDTD_OK) rc = fdtd_hip_grid_set_ns_coefs(g, cell, r / 2.0);
+  free(eps);
+  free(cell);
+  return rc;
+}
+
 static void build_coefs(int kind, fdtd_grid *g)
 {
   const int nx = N_PX, ny = N_PY;
@@ -110,7 +166,19 @@ static void build_coefs(int kind, fdtd_grid *g)
   for (int q = 0; q < 4; q++) Y[q] = buf + (size_t)6 * nx + (size_t)q * ny;
   SY = buf + (size_t)6 * nx + (size_t)4 * ny;
   const double sig_z = 0;
-  if (is_split(kind)) {
+  if (is_ns(kind)) {
+    /* nsFdtdTM.c:226-306: the decay factors coef1 of the simplified beta, which
+       depend on one axis only; the eps-dependent ones follow in build_ns_coefs */
+    const double sig_max = -(M + 1.0) * eps * C / N_PML * log(R);
+    for (int x = 0; x < nx; x++) {
+      X[0][x] = ns_coef1(ns_beta(sig_max * field_sigmaX(x, 0) / (2 * eps)));        /* C_EZX */
+      X[1][x] = ns_coef1(ns_beta(sig_max * field_sigmaX(x + 0.5, 0) / (2 * eps)));  /* C_HY */
+    }
+    for (int y = 0; y < ny; y++) {
+      Y[0][y] = ns_coef1(ns_beta(sig_max * field_sigmaY(0, y) / (2 * eps)));        /* C_EZY */
+      Y[1][y] = ns_coef1(ns_beta(sig_max * field_sigmaY(0, y + 0.5) / (2 * eps)));  /* C_HX */
+    }
+  } else if (is_split(kind)) {
     /* fdtdTM.c:184-243, fdtdTE.c:200-241: the conductivities (the same four tables
        for TM and TE); the coefficients are formed per cell on the device with eps
        (field_pmlCoef / field_pmlCoef_LXY of the component's eps map and MU_0_S) */
@@ -201,8 +269,9 @@ int solver_make_grid(int kind, int precision, int ntff_time, int T, int R, fdtd_
   fdtd_grid_desc d;
   memset(&d, 0, sizeof d);
   d.mode = is_tm(kind) ? FDTD_MODE_TM : FDTD_MODE_TE;
-  d.flavour = is_split(kind) ? FDTD_FLAVOUR_SPLIT : (is_mpi(kind) ? FDTD_FLAVOUR_MPI : FDTD_FLAVOUR_PLAIN);
-  if (is_split(kind)) {  /* FP64 only; the reference's fdtdTM.c / fdtdTE.c have no NTFF */
+  d.flavour = is_ns(kind) ? FDTD_FLAVOUR_NS
+                          : (is_split(kind) ? FDTD_FLAVOUR_SPLIT : (is_mpi(kind) ? FDTD_FLAVOUR_MPI : FDTD_FLAVOUR_PLAIN));
+  if (is_split(kind)) {  /* FP64 only; the reference's fdtdTM.c / fdtdTE.c / nsFdtdTM.c have no NTFF */
     precision = FDTD_FP64;
     ntff_time = 0;
   }
@@ -231,6 +300,7 @@ int solver_make_grid(int kind, int precision, int ntff_time, int T, int R, fdtd_
   build_coefs(kind, g);
   if ((rc = fdtd_hip_grid_fill_eps(g, models_getState())) == FDTD_OK && precision == FDTD_FP32)
     rc = fdtd_hip_grid_set_blocking(g, T > 0 ? T : 4, R);
+  if (rc == FDTD_OK && is_ns(kind)) rc = build_ns_coefs(g);
   if (rc != FDTD_OK) {
     fdtd_hip_grid_destroy(g);
     return rc;
@@ -474,8 +544,8 @@ static void write_elliptic(int kind)
 {
   FieldInfo fi = field_getFieldInfo();
   char name[128];
-  snprintf(name, sizeof name, "%s_%dnm.txt", kind == SLV_SPLIT_TM ? "tm" : "te", fi.h_u_nm);
-  const double complex *ez = solver_field(kind, kind == SLV_SPLIT_TM ? 0 : 1);
+  snprintf(name, sizeof name, "%s_%dnm.txt", kind == SLV_NS_TM ? "ns_tm" : (kind == SLV_SPLIT_TM ? "tm" : "te"), fi.h_u_nm);
+  const double complex *ez = solver_field(kind, is_tm(kind) ? 0 : 1);
   const double lam = field_getLambda();
   printf("output start\n");
   FILE *fp = fopen(name, "w");
@@ -508,7 +578,7 @@ static void solver_reset(int kind)
 {
   solver_t *s = &slv_[kind];
   if (!s->g) return;
-  if (is_split(kind) && file_output_) write_elliptic(kind);            /* fdtdTM.c:167-173, fdtdTE.c:269-274 */
+  if (is_split(kind) && file_output_) write_elliptic(kind);  /* fdtdTM.c:167-173, fdtdTE.c:269-274, nsFdtdTM.c:160-164 */
   if (!is_mpi(kind) && file_output_ && s->ntff_time) write_spectra();  /* fdtdTM_upml.c:87, fdtdTE_upml.c:117 */
   CHECK(fdtd_hip_grid_reset(s->g));
 }
@@ -610,7 +680,7 @@ static double complex *split_field(int kind, int q)
 {
   solver_t *s = &slv_[kind];
   if (!s->g) return NULL;
-  CHECK(fdtd_hip_grid_get_field(s->g, (kind == SLV_SPLIT_TM ? 1 : 7) + q, (double *)s->split_mirror[q]));
+  CHECK(fdtd_hip_grid_get_field(s->g, (is_tm(kind) ? 1 : 7) + q, (double *)s->split_mirror[q]));
   return s->split_mirror[q];
 }
 fdtd_method fdtdTM_getUpdate(void) { return split_tm_update; }
@@ -639,6 +709,22 @@ fdtd_dcomplex *fdtdTE_getHzx(void) { return (fdtd_dcomplex *)split_field(SLV_SPL
 fdtd_dcomplex *fdtdTE_getHzy(void) { return (fdtd_dcomplex *)split_field(SLV_SPLIT_TE, 1); }
 fdtd_dcomplex *fdtdTE_getHz(void) { return (fdtd_dcomplex *)solver_field(SLV_SPLIT_TE, 2); }
 double *fdtdTE_getEps(void) { return solver_eps(SLV_SPLIT_TE); }
+
+/* ---------------------------------------------------------------- non-standard FDTD TM (nsFdtdTM.c) */
+static void ns_tm_init(void) { solver_init(SLV_NS_TM); }
+static void ns_tm_update(void) { solver_update(SLV_NS_TM); }
+static void ns_tm_reset(void) { solver_reset(SLV_NS_TM); }
+static void ns_tm_finish(void) { solver_finish(SLV_NS_TM); }
+fdtd_method nsFdtdTM_getUpdate(void) { return ns_tm_update; }
+fdtd_method nsFdtdTM_getFinish(void) { return ns_tm_finish; }
+fdtd_method nsFdtdTM_getReset(void) { return ns_tm_reset; }
+fdtd_method nsFdtdTM_getInit(void) { return ns_tm_init; }
+fdtd_dcomplex *nsFdtdTM_getEzx(void) { return (fdtd_dcomplex *)split_field(SLV_NS_TM, 0); }
+fdtd_dcomplex *nsFdtdTM_getEzy(void) { return (fdtd_dcomplex *)split_field(SLV_NS_TM, 1); }
+fdtd_dcomplex *nsFdtdTM_getEz(void) { return (fdtd_dcomplex *)solver_field(SLV_NS_TM, 0); }
+fdtd_dcomplex *nsFdtdTM_getHy(void) { return (fdtd_dcomplex *)solver_field(SLV_NS_TM, 2); }
+fdtd_dcomplex *nsFdtdTM_getHx(void) { return (fdtd_dcomplex *)solver_field(SLV_NS_TM, 1); }
+double *nsFdtdTM_getEps(void) { return solver_eps(SLV_NS_TM); }
 
 /* ---------------------------------------------------------------- plain TE */
 static void te_init(void) { solver_init(SLV_TE); }

@@ -22,6 +22,7 @@ _GETTERS = {
     N.TE_UPML_2D: ("fdtdTE_upml_getEx", "fdtdTE_upml_getEy", "fdtdTE_upml_getHz", "fdtdTE_upml_getEps"),
     N.TM_2D: ("fdtdTM_getEz", "fdtdTM_getHx", "fdtdTM_getHy", "fdtdTM_getEps"),  # split-field PML (FP64)
     N.TE_2D: ("fdtdTE_getEx", "fdtdTE_getEy", "fdtdTE_getHz", "fdtdTE_getEps"),
+    N.NS_TM_2D: ("nsFdtdTM_getEz", "nsFdtdTM_getHx", "nsFdtdTM_getHy", "nsFdtdTM_getEps"),  # non-standard FDTD (FP64)
 }
 
 
