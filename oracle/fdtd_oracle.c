@@ -614,6 +614,56 @@ static void set_coef_ns_tm(orc_sim *s)
   }
 }
 
+/* Non-standard FDTD TE (nsFdtdTE.c:100-181): over the interior cells only (the
+   border of every map stays zero, as the reference's zeroed arrays do), with
+   alpha = sigma / (2 eps) of the component's own eps.  C[0] C_EX, C[1] C_EXLY,
+   C[2] C_EY, C[3] C_EYLX, C[4] C_HZX, C[5] C_HZXLX, C[6] C_HZY, C[7] C_HZYLY, and
+   the scattered-wave factors 1/(_n n) - 1 (field.c:155-177) of EPS_EX in C[8] and
+   of EPS_EY in C[9]; EPS[0] EPS_EX, EPS[1] EPS_EY, EPS[2] EPS_HZ */
+static double ns_wave_factor(const orc_sim *s, double eps)
+{
+  const double w_s = s->Omega_s, k_s = s->K_s;
+  double n = sqrt(eps / s->EPS0);
+  double u0 = sin(w_s * 0.5) / sin(k_s * 0.5);
+  double u1 = sin(w_s * 0.5) / sin(n * k_s * 0.5);
+  double _n = u0 / u1;
+  return 1.0 / (_n * n) - 1.0;
+}
+static void set_coef_ns_te(orc_sim *s)
+{
+  const double R = 1.0e-8, M = 2.0;
+  const double sig_max = -(M + 1.0) * s->EPS0 * s->C0 / s->N_PML * log(R);
+  const double w_s = s->Omega_s, k_s = s->K_s;
+  for (int i = 1; i < s->NPXa - 1; i++) {
+    for (int j = 1; j < s->NPYa - 1; j++) {
+      int k = i * s->NPYa + j;
+      double e_ex = s->EPS[0][k] = models_eps(s, i + 0.5, j, ORC_D_Y);
+      double e_ey = s->EPS[1][k] = models_eps(s, i, j + 0.5, ORC_D_X);
+      double e_hz = s->EPS[2][k] = models_eps(s, i + 0.5, j + 0.5, ORC_D_XY);
+      double sig_ex_y = sig_max * sigma_y(s, j), sig_ey_x = sig_max * sigma_x(s, i);
+      double sig_hz_x = sig_max * sigma_x(s, i + 0.5), sig_hz_y = sig_max * sigma_y(s, j + 0.5);
+      double b_ex_y = ns_beta(sig_ex_y / (2 * e_ex)), b_ey_x = ns_beta(sig_ey_x / (2 * e_ey));
+      double b_hz_x = ns_beta(sig_hz_x / (2 * e_hz)), b_hz_y = ns_beta(sig_hz_y / (2 * e_hz));
+      double z_hz = sqrt(s->MU0 / e_hz), n_hz = sqrt(e_hz / s->EPS0);
+      double u_hz = sin(w_s * 0.5) / sin(k_s * n_hz * 0.5);
+      s->C[4][k] = ns_coef1(0, b_hz_x);
+      s->C[5][k] = u_hz / z_hz / (1.0 + b_hz_x);
+      s->C[6][k] = ns_coef1(0, b_hz_y);
+      s->C[7][k] = u_hz / z_hz / (1.0 + b_hz_y);
+      double z_ex = sqrt(s->MU0 / e_ex), n_ex = sqrt(e_ex / s->EPS0);
+      double u_ex = sin(w_s * 0.5) / sin(k_s * n_ex * 0.5);
+      s->C[0][k] = ns_coef1(0, b_ex_y);
+      s->C[1][k] = u_ex * z_ex / (1.0 + b_ex_y);
+      double z_ey = sqrt(s->MU0 / e_ey), n_ey = sqrt(e_ey / s->EPS0);
+      double u_ey = sin(w_s * 0.5) / sin(k_s * n_ey * 0.5);
+      s->C[2][k] = ns_coef1(0, b_ey_x);
+      s->C[3][k] = u_ey * z_ey / (1.0 + b_ey_x);
+      s->C[8][k] = ns_wave_factor(s, e_ex);
+      s->C[9][k] = ns_wave_factor(s, e_ey);
+    }
+  }
+}
+
 /* TE: mpiTE_UPML.c:489-536 (MPI) and fdtdTE_upml.c:361-412 (plain); no /cos(pi/3) (Q7) */
 static void set_coef_te(orc_sim *s)
 {
@@ -662,7 +712,8 @@ orc_sim *orc_create(const orc_config *cfg)
   models_init(s);  /* models_initModel runs after field_init (simulator.c:229-231) */
   s->mpi = (cfg->solver == ORC_MPI_TM_UPML || cfg->solver == ORC_MPI_TE_UPML);
   s->tm = (cfg->solver == ORC_MPI_TM_UPML || cfg->solver == ORC_TM_UPML || cfg->solver == ORC_TM_2D);
-  s->split = (cfg->solver == ORC_TM_2D || cfg->solver == ORC_TE_2D || cfg->solver == ORC_NS_TM_2D);
+  s->split = (cfg->solver == ORC_TM_2D || cfg->solver == ORC_TE_2D || cfg->solver == ORC_NS_TM_2D ||
+              cfg->solver == ORC_NS_TE_2D);
   if (cfg->solver == ORC_NS_TM_2D) s->tm = true;
   /* MPI single rank: SUB_N_PX = N_PX + 2 with a ghost ring (mpiTM_UPML.c:737-743) */
   s->off = s->mpi ? 1 : 0;
@@ -680,6 +731,7 @@ orc_sim *orc_create(const orc_config *cfg)
   if (cfg->solver == ORC_TM_2D) set_coef_tm_split(s);
   else if (cfg->solver == ORC_TE_2D) set_coef_te_split(s);
   else if (cfg->solver == ORC_NS_TM_2D) set_coef_ns_tm(s);
+  else if (cfg->solver == ORC_NS_TE_2D) set_coef_ns_te(s);
   else if (s->tm) set_coef_tm(s);
   else set_coef_te(s);
   s->R0 = 1.0e6 * to_cell(s, 500);  /* ntffTM_init (ntffTM.c:31), ntffTE_init (ntffTE.c:16) */
@@ -1329,9 +1381,69 @@ static void ns_tm_step(orc_sim *s)
     }
 }
 
+/* Non-standard FDTD TE step (nsFdtdTE.c:242-308): calcH (Hzx, Hzy), Hz = Hzx + Hzy,
+   calcE with the non-standard second differences, then the NS scattered wave
+   (field.c:155-177) on Ex with dot cos(angle + 90) and on Ey with sin(angle + 90),
+   both at gap (0, 0.5); slots as the split-field TE solver's */
+static void ns_te_step(orc_sim *s)
+{
+  const int NPX = s->NPXa, NPY = s->NPYa, dx = NPY, dy = 1;
+  dcx *Ex = s->F[0], *Ey = s->F[3], *Hz = s->F[6], *Hzx = s->F[7], *Hzy = s->F[8];
+  double **C = s->C;
+  for (int i = 1; i < NPX - 1; i++)
+    for (int j = 1; j < NPY - 1; j++) {
+      int k = IDX(i, j);
+      Hzx[k] = C[4][k] * Hzx[k] - C[5][k] * (Ey[k + dx] - Ey[k]);
+    }
+  for (int i = 1; i < NPX - 1; i++)
+    for (int j = 1; j < NPY - 1; j++) {
+      int k = IDX(i, j);
+      Hzy[k] = C[6][k] * Hzy[k] + C[7][k] * (Ex[k + dy] - Ex[k]);
+    }
+  for (int i = 1; i < NPX - 1; i++)
+    for (int j = 1; j < NPY - 1; j++) {
+      int k = IDX(i, j);
+      Hz[k] = Hzx[k] + Hzy[k];
+    }
+  const double k_s = s->K_s;
+  const double r = 1.0 / 6.0 + k_s * k_s / 180.0 - pow(k_s, 4) / 23040;
+  const double r_2 = r / 2.0;
+  for (int i = 1; i < NPX - 1; i++)
+    for (int j = 1; j < NPY - 1; j++) {
+      int k = IDX(i, j);
+      dcx ns = r_2 * ((Hz[k + dx] + Hz[k - dx] - 2 * Hz[k]) - (Hz[k - dy + dx] + Hz[k - dy - dx] - 2 * Hz[k - dy]));
+      Ex[k] = C[0][k] * Ex[k] + C[1][k] * (Hz[k] - Hz[k - dy] + ns);
+    }
+  for (int i = 1; i < NPX - 1; i++)
+    for (int j = 1; j < NPY - 1; j++) {
+      int k = IDX(i, j);
+      dcx ns = r_2 * ((Hz[k + dy] + Hz[k - dy] - 2 * Hz[k]) - (Hz[k - dx + dy] + Hz[k - dx - dy] - 2 * Hz[k - dx]));
+      Ey[k] = C[2][k] * Ey[k] - C[3][k] * (Hz[k] - Hz[k - dx] + ns);
+    }
+  double co = cos((s->Angle_deg + 90) * M_PI / 180.0);
+  double si = sin((s->Angle_deg + 90) * M_PI / 180.0);
+  double time = s->time, w_s = s->Omega_s;
+  double rad = s->Angle_deg * M_PI / 180.0;
+  double ks_cos = cos(rad) * k_s, ks_sin = sin(rad) * k_s;
+  for (int c = 0; c < 2; c++) {
+    double dot = c == 0 ? co : si;
+    if (dot == 0.0) continue;
+    dcx *p = c == 0 ? Ex : Ey;
+    const double *fac = C[8 + c];
+    double ray_coef = s->ray_coef * dot;
+    for (int i = 1; i < NPX - 1; i++)
+      for (int j = 1; j < NPY - 1; j++) {
+        int k = IDX(i, j);
+        double kr = (i + 0) * ks_cos + (j + 0.5) * ks_sin;
+        p[k] += ray_coef * fac[k] * (cexp(I * (kr - w_s * (time + 1.0))) - cexp(I * (kr - w_s * (time))));
+      }
+  }
+}
+
 void orc_elliptic(const orc_sim *s, double *out181)
 {
-  const dcx *Ez = s->cfg.solver == ORC_TE_2D ? s->F[3] : s->F[0];  /* TE_2D: Ey (fdtdTE.c:269-274) */
+  /* TE_2D / NS_TE_2D: Ey (fdtdTE.c:269-274, nsFdtdTE.c:234-238) */
+  const dcx *Ez = (s->cfg.solver == ORC_TE_2D || s->cfg.solver == ORC_NS_TE_2D) ? s->F[3] : s->F[0];
   const int NPY = s->NPYa;
   for (int ang = 180; ang >= 0; ang--) {
     double rad = ang * M_PI / 180.0;
@@ -1361,6 +1473,9 @@ void orc_step(orc_sim *s)
     break;
   case ORC_NS_TM_2D:
     ns_tm_step(s);
+    break;
+  case ORC_NS_TE_2D:
+    ns_te_step(s);
     break;
   case ORC_MPI_TM_UPML: /* update (mpiTM_UPML.c:196-217) */
     tm_calc_jd_e(s);

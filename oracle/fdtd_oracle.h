@@ -30,7 +30,7 @@ extern "C" {
 /* enum SOLVER values (simulator.h:8-18) */
 /* enum SOLVER (simulator.h:8-18); TM_2D / TE_2D are the split-field PML solvers (fdtdTM.c, fdtdTE.c) */
 enum { ORC_TM_2D = 0, ORC_TE_2D = 1, ORC_TM_UPML = 2, ORC_TE_UPML = 3, ORC_MPI_TM_UPML = 4, ORC_MPI_TE_UPML = 5,
-       ORC_NS_TM_2D = 6 /* non-standard FDTD TM (nsFdtdTM.c) */ };
+       ORC_NS_TM_2D = 6 /* non-standard FDTD TM (nsFdtdTM.c) */, ORC_NS_TE_2D = 7 /* nsFdtdTE.c */ };
 /* enum MODEL values (models.h:5-14) */
 enum { ORC_NO_MODEL = 0, ORC_MIE_CYLINDER = 1, ORC_LAYER = 2, ORC_MORPHO_SCALE = 3, ORC_ZIGZAG = 5 };
 /* enum MODE values (models.h:16-21) */
@@ -122,7 +122,7 @@ void orc_ntff_frequency(const orc_sim *s, double *out360);
 void orc_time_spectra(const orc_sim *s, double *out);
 /* TimeTranslate only: Eth/Eph as 360*arraySize complex (first maxTime used). */
 void orc_time_translate(const orc_sim *s, double *eth, double *eph);
-/* field_outputElliptic (field.c:322-341) of the TM_2D / NS_TM_2D solvers' Ez or the TE_2D solver's Ey
+/* field_outputElliptic (field.c:322-341) of the TM_2D / NS_TM_2D solvers' Ez or the TE_2D / NS_TE_2D solvers' Ey
    (their reset() outputs): 181 values, index 180 - ang
    for ang = 180 .. 0, the squared magnitude of Ez interpolated bilinearly (myComplex.c:40-51) on
    the circle of radius 1.2 lambda around the grid centre */

@@ -13,7 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "liboracle.so")
 
 TM_2D, TE_2D = 0, 1  # split-field PML TM / TE (fdtdTM.c, fdtdTE.c)
-NS_TM_2D = 6  # non-standard FDTD TM (nsFdtdTM.c)
+NS_TM_2D, NS_TE_2D = 6, 7  # non-standard FDTD TM / TE (nsFdtdTM.c, nsFdtdTE.c)
 TM_UPML, TE_UPML, MPI_TM_UPML, MPI_TE_UPML = 2, 3, 4, 5
 NO_MODEL, MIE_CYLINDER, LAYER, MORPHO_SCALE, ZIGZAG = 0, 1, 2, 3, 5
 D_X, D_Y, D_XY = 0, 1, 2

@@ -12,7 +12,7 @@
  *   mode=run|eps   solver=<enum SOLVER>  model=<enum MODEL>  sweep=<models_isFinish calls>
  *   width=<nm> height=<nm> h=<nm> pml=<cells> lambda=<nm> angle=<deg> steps=<n>
  *   out=<directory>
- * mode=run : simulator_init, `steps` x simulator_calc, then dumps (solvers TM_2D, TE_2D, NS_TM_2D,
+ * mode=run : simulator_init, `steps` x simulator_calc, then dumps (solvers TM_2D, TE_2D, NS_TM/TE_2D,
  *            TM/TE_UPML_2D, MPI_TM/TE_UPML_2D)
  *            f0/f1/f2.bin (TM: Ez,Hx,Hy  TE: Ex,Ey,Hz; solver's own layout),
  *            eps.bin (getEps), meta.txt, ff.bin (TM: ntffTM_Frequency on the
@@ -46,6 +46,7 @@
 #include "fdtdTM.h"
 #include "fdtdTE.h"
 #include "nsFdtdTM.h"
+#include "nsFdtdTE.h"
 
 static const char *arg(int argc, char **argv, const char *key, const char *def)
 {
@@ -148,6 +149,9 @@ int main(int argc, char **argv)
   } else if (solver == TE_2D) {
     f[0] = fdtdTE_getEx(); f[1] = fdtdTE_getEy(); f[2] = fdtdTE_getHz();
     eps = fdtdTE_getEps();
+  } else if (solver == NS_TE_2D) {
+    f[0] = nsFdtdTE_getEx(); f[1] = nsFdtdTE_getEy(); f[2] = nsFdtdTE_getHz();
+    eps = nsFdtdTE_getEps();
   } else if (solver == NS_TM_2D) {
     f[0] = nsFdtdTM_getEz(); f[1] = nsFdtdTM_getHx(); f[2] = nsFdtdTM_getHy();
     eps = nsFdtdTM_getEps();
@@ -192,7 +196,7 @@ int main(int argc, char **argv)
   if (chdir(out) != 0) { perror(out); return 3; }
   if (!mpi) {
     simulator_reset(); /* plain UPML solvers write the spectra files here, TM_2D / TE_2D /
-                          NS_TM_2D tm_<h>nm.txt / te_<h>nm.txt / ns_tm_<h>nm.txt */
+                          NS_TM_2D / NS_TE_2D tm_, te_, ns_tm_, ns_te_<h>nm.txt */
   } else if (!tm) {
     mkdir("MPI_TE_UPML", 0755);
     void (*fin)(void) = mpi_fdtdTE_upml_getFinish();

@@ -36,6 +36,8 @@ RUN_CASES = [
     ("te2d_layer_a45_rect", 1, 2, 3, 1800, 2000, 20, 8, 550, 45, 150),
     ("nstm_mie_a0", 6, 1, 0, 4000, 4000, 50, 10, 500, 0, 300),        # non-standard FDTD TM (nsFdtdTM.c)
     ("nstm_zigzag_a30_rect", 6, 5, 0, 1400, 1800, 20, 8, 450, 30, 150),
+    ("nste_mie_a0", 7, 1, 0, 4000, 4000, 50, 10, 500, 0, 300),        # non-standard FDTD TE (nsFdtdTE.c)
+    ("nste_layer_a45_rect", 7, 2, 3, 1400, 1800, 20, 8, 450, 45, 150),
 ]
 
 
@@ -99,7 +101,7 @@ def main(only=()):
             if os.path.exists(spec):
                 full = np.fromfile(spec, np.float64).reshape(321, 360)
                 d["spectra"] = full[::4]           # lambda = 380, 384, ..., 700
-            for pol in ("tm", "te", "ns_tm"):  # TM_2D / TE_2D / NS_TM_2D reset(): field_outputElliptic
+            for pol in ("tm", "te", "ns_tm", "ns_te"):  # the split-field and NS solvers' reset(): field_outputElliptic
                 ell = os.path.join(out, f"{pol}_{hu}nm.txt")
                 if os.path.exists(ell):
                     d["elliptic"] = np.loadtxt(ell)[:, 1]

@@ -2,7 +2,7 @@
 
 The fixtures under tests/golden/ were written by tests/golden/make_golden.py,
 which runs the reference's C sources (mpiTM_UPML.c, mpiTE_UPML.c, fdtdTM_upml.c,
-fdtdTE_upml.c, the split-field fdtdTM.c and fdtdTE.c, nsFdtdTM.c, field.c, ntffTM.c, ntffTE.c, cfft.c, ntff.c, models.c and the
+fdtdTE_upml.c, the split-field fdtdTM.c and fdtdTE.c, nsFdtdTM.c, nsFdtdTE.c, field.c, ntffTM.c, ntffTE.c, cfft.c, ntff.c, models.c and the
 model files) as one MPI rank.  The oracle restates the same arithmetic in the
 same order, so most comparisons are bit-exact.
 """
@@ -15,7 +15,7 @@ from oracle import pyoracle as po
 RUN_CASES = ["mpitm_mie_a0", "mpitm_mie_a30_rect", "mpite_mie_a0", "mpite_mie_a60_rect",
              "tm_mie_pulse", "te_mie_pulse", "mpitm_morpho", "mpite_zigzag",
              "tm2d_mie_a0", "tm2d_zigzag_a30_rect", "te2d_mie_a0", "te2d_layer_a45_rect",
-             "nstm_mie_a0", "nstm_zigzag_a30_rect"]
+             "nstm_mie_a0", "nstm_zigzag_a30_rect", "nste_mie_a0", "nste_layer_a45_rect"]
 
 
 def oracle_for(g):
