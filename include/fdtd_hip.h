@@ -32,12 +32,15 @@ enum { FDTD_MODE_TM = 0, FDTD_MODE_TE = 1 };
  *              first, the wave on Ezx, slots Ez 0, Ezx 1, Ezy 2, Hx 3, Hy 6;
  *              TE: E half first, the wave on Ey, slots Ex 0, Ey 3, Hz 6, Hzx 7,
  *              Hzy 8.
- * ns flavour: the non-standard FDTD TM solver (nsFdtdTM.c): as the split
- *              TM flavour (N_PX x N_PY, FP64, no per-step NTFF, H half first,
- *              slots Ez 0, Ezx 1, Ezy 2, Hx 3, Hy 6), with the non-standard
- *              second differences in the H half and the NS scattered wave on
- *              Ezy; its coefficients come from fdtd_hip_grid_set_coefs (the
- *              1-D decay tables) and fdtd_hip_grid_set_ns_coefs (per cell). */
+ * ns flavour: the non-standard FDTD solvers (nsFdtdTM.c, nsFdtdTE.c): as the
+ *              split flavour (N_PX x N_PY, FP64, no per-step NTFF, H half first,
+ *              the same slots), with the non-standard second differences in
+ *              the half that reads the split field's sum.  TM: the NS scattered
+ *              wave on Ezy; coefficients from fdtd_hip_grid_set_coefs (1-D
+ *              decay tables) and fdtd_hip_grid_set_ns_coefs (per cell).  TE: the
+ *              NS wave on Ex and Ey at gap (0, 1/2); every coefficient per cell
+ *              (fdtd_hip_grid_set_ns_coefs); the eps maps sample Hz at D_XY and
+ *              keep a zero border, as nsFdtdTE.c:116-121 leaves them. */
 enum { FDTD_FLAVOUR_MPI = 0, FDTD_FLAVOUR_PLAIN = 1, FDTD_FLAVOUR_SPLIT = 2, FDTD_FLAVOUR_NS = 3 };
 enum { FDTD_FP64 = 0, FDTD_FP32 = 1 };
 
@@ -144,11 +147,14 @@ int fdtd_hip_grid_fill_eps(fdtd_grid *g, const fdtd_model_state *m);
 /* which: TM 0 EPS_EZ 1 EPS_HX 2 EPS_HY ; TE 0 EPS_EX 1 EPS_EY 2 EPS_HZ; solver layout */
 int fdtd_hip_grid_set_eps(fdtd_grid *g, int which, const double *host);
 int fdtd_hip_grid_get_eps(fdtd_grid *g, int which, double *host);
-/* NS flavour: the per-cell coefficients, five N_PX*N_PY maps in this order:
- * C_EZXLX, C_EZYLY, C_HXLY, C_HYLX (nsFdtdTM.c:276-304) and the factor
- * 1/(_n n) - 1 of the NS scattered wave (field.c:155-177) of EPS_EZ; r_half is
- * r/2 of the second differences, r = 1/6 + k^2/180 - k^4/23040 (nsFdtdTM.c:117-118) */
-int fdtd_hip_grid_set_ns_coefs(fdtd_grid *g, const double *cell5, double r_half);
+/* NS flavour: the per-cell coefficients as N_PX*N_PY maps in this order.
+ *   TM, 5 maps: C_EZXLX, C_EZYLY, C_HXLY, C_HYLX (nsFdtdTM.c:276-304) and the
+ *       factor 1/(_n n) - 1 of the NS scattered wave (field.c:155-177) of EPS_EZ;
+ *   TE, 10 maps: C_EX, C_EXLY, C_EY, C_EYLX, C_HZX, C_HZXLX, C_HZY, C_HZYLY
+ *       (nsFdtdTE.c:145-178) and the wave factors of EPS_EX and of EPS_EY.
+ * r_half is r/2 of the second differences, r = 1/6 + k^2/180 - k^4/23040
+ * (nsFdtdTM.c:117-118, nsFdtdTE.c:255-256) */
+int fdtd_hip_grid_set_ns_coefs(fdtd_grid *g, const double *cells, double r_half);
 /* one reference update(): E/H halves, source, per-step NTFF */
 int fdtd_hip_grid_step(fdtd_grid *g, const fdtd_step_params *p);
 /* which 0..8: TM Ez Jz Dz Hx Mx Bx Hy My By / TE Ex Jx Dx Ey Jy Dy Hz Mz Bz;

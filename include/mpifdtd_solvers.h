@@ -1,6 +1,6 @@
 /*
  * mpifdtd_solvers.h -- the four UPML solvers', the two split-field PML
- * solvers' and the non-standard FDTD TM solver's entry points, backed by HIP.
+ * solvers' and the two non-standard FDTD solvers' entry points, backed by HIP.
  *
  * Same symbols and signatures as the reference's solver headers:
  *   mpiTM_UPML.h:5-18   (mpiTM_UPML.c:88-136)   MPI TMz UPML, run as one rank on one GPU
@@ -15,6 +15,10 @@
  *                                               reset() writes ns_tm_<h>nm.txt;
  *                                               getEzx / getEzy, declared but not
  *                                               defined by the reference, are defined)
+ *   nsFdtdTE.h:6-17     (nsFdtdTE.c:29-39)      non-standard FDTD TEz (FP64 only; its
+ *                                               reset() writes ns_te_<h>nm.txt; getEps
+ *                                               returns EPS_EY with the reference's
+ *                                               zero border; getHzx / getHzy defined)
  * get{Update,Init,Finish,Reset} return the solver's void(void) methods; the
  * field getters return borrowed host pointers (valid from init to finish) that
  * are refreshed from device memory on every getter call; getEps returns the
@@ -110,6 +114,19 @@ fdtd_dcomplex *nsFdtdTM_getEz(void);
 fdtd_dcomplex *nsFdtdTM_getHy(void);
 fdtd_dcomplex *nsFdtdTM_getHx(void);
 double *nsFdtdTM_getEps(void);
+
+/* nsFdtdTE.h:6-17 (non-standard FDTD TEz; getEps returns EPS_EY).  nsFdtdTE_getSolver
+ * has no counterpart here, as for TM. */
+fdtd_method nsFdtdTE_getUpdate(void);
+fdtd_method nsFdtdTE_getFinish(void);
+fdtd_method nsFdtdTE_getReset(void);
+fdtd_method nsFdtdTE_getInit(void);
+fdtd_dcomplex *nsFdtdTE_getHzx(void);
+fdtd_dcomplex *nsFdtdTE_getHzy(void);
+fdtd_dcomplex *nsFdtdTE_getHz(void);
+fdtd_dcomplex *nsFdtdTE_getEy(void);
+fdtd_dcomplex *nsFdtdTE_getEx(void);
+double *nsFdtdTE_getEps(void);
 
 /* fdtdTE_upml.h:5-14 */
 fdtd_method fdtdTE_upml_getUpdate(void);

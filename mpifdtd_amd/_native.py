@@ -191,7 +191,8 @@ def lib():
               "fdtdTM_getEz", "fdtdTM_getHx", "fdtdTM_getHy", "fdtdTM_getEzx", "fdtdTM_getEzy", "fdtdTM_getEps",
               "fdtdTE_getEx", "fdtdTE_getEy", "fdtdTE_getHz", "fdtdTE_getHzx", "fdtdTE_getHzy", "fdtdTE_getEps",
               "nsFdtdTM_getEz", "nsFdtdTM_getHx", "nsFdtdTM_getHy", "nsFdtdTM_getEzx", "nsFdtdTM_getEzy",
-              "nsFdtdTM_getEps"):
+              "nsFdtdTM_getEps", "nsFdtdTE_getEx", "nsFdtdTE_getEy", "nsFdtdTE_getHz", "nsFdtdTE_getHzx",
+              "nsFdtdTE_getHzy", "nsFdtdTE_getEps"):
         sig(n, P)
     _lib = L
     return L

@@ -453,6 +453,70 @@ __global__ __launch_bounds__(256) void k_ns_tm_e(GridPtrs g, Tabs t, const doubl
   Ez[k] = make_double2(nzx.x + nzy.x, nzx.y + nzy.y);
 }
 
+// Non-standard FDTD TE (nsFdtdTE.c).  Per-cell coefficient maps nsc[q * ncell + k]:
+// 0 C_EX, 1 C_EXLY, 2 C_EY, 3 C_EYLX, 4 C_HZX, 5 C_HZXLX, 6 C_HZY, 7 C_HZYLY,
+// 8 / 9 the NS scattered-wave factors of EPS_EX / EPS_EY.
+// calcH (nsFdtdTE.c:291-308), then Hz = Hzx + Hzy (nsFdtdTE.c:243-248)
+__global__ __launch_bounds__(256) void k_ns_te_h(GridPtrs g, const double *__restrict__ nsc)
+{
+  FOR_CELL();
+  const size_t n = (size_t)g.NPXa * NPY;
+  const double2 *Ex = g.F[0], *Ey = g.F[3];
+  double2 *Hz = g.F[6], *Hzx = g.F[7], *Hzy = g.F[8];
+  const double chzx = nsc[4 * n + k], chzxlx = nsc[5 * n + k], chzy = nsc[6 * n + k], chzyly = nsc[7 * n + k];
+  const double2 ey = Ey[k], eyr = Ey[k + NPY], ex = Ex[k], ext = Ex[k + 1];
+  const double2 zx = Hzx[k], zy = Hzy[k];
+  double2 nzx, nzy;
+  nzx.x = chzx * zx.x - chzxlx * (eyr.x - ey.x);
+  nzx.y = chzx * zx.y - chzxlx * (eyr.y - ey.y);
+  nzy.x = chzy * zy.x + chzyly * (ext.x - ex.x);
+  nzy.y = chzy * zy.y + chzyly * (ext.y - ex.y);
+  Hzx[k] = nzx;
+  Hzy[k] = nzy;
+  Hz[k] = make_double2(nzx.x + nzy.x, nzx.y + nzy.y);
+}
+
+// the NS scattered-wave term (field.c:155-177) (ray_coef dot) fac (e^{i(kr - w(t+1))} - e^{i(kr - w t)})
+__device__ __forceinline__ void ns_wave_add(double2 &v, const StepArgs &s, double dot, double fac, double kr)
+{
+  const double amp = (s.ray_coef * dot) * fac;
+  double s1, c1, s0, c0;
+  sincos(kr - s.w_s * (s.time + 1.0), &s1, &c1);
+  sincos(kr - s.w_s * s.time, &s0, &c0);
+  v.x += amp * (c1 - c0);
+  v.y += amp * (s1 - s0);
+}
+
+// calcE (nsFdtdTE.c:251-289) with the non-standard second differences of Hz, then
+// the NS scattered wave on Ex (dot cos(angle + 90)) and Ey (dot sin(angle + 90)),
+// both at gap (0, 1/2) (nsFdtdTE.c:250-256)
+__global__ __launch_bounds__(256) void k_ns_te_e(GridPtrs g, const double *__restrict__ nsc, double r2, StepArgs s)
+{
+  FOR_CELL();
+  const size_t n = (size_t)g.NPXa * NPY;
+  double2 *Ex = g.F[0], *Ey = g.F[3];
+  const double2 *Hz = g.F[6];
+  // c (i,j), r (i+1,j), l (i-1,j), u (i,j+1), d (i,j-1), dr (i+1,j-1), dl (i-1,j-1), ul (i-1,j+1)
+  const double2 c = Hz[k], r = Hz[k + NPY], l = Hz[k - NPY], u = Hz[k + 1], d = Hz[k - 1];
+  const double2 dr = Hz[k + NPY - 1], dl = Hz[k - NPY - 1], ul = Hz[k - NPY + 1];
+  const double cex = nsc[k], cexly = nsc[n + k], cey = nsc[2 * n + k], ceylx = nsc[3 * n + k];
+  const double2 ex = Ex[k], ey = Ey[k];
+  // (Hz[k+dx] + Hz[k-dx] - 2 Hz[k]) - (Hz[k-dy+dx] + Hz[k-dy-dx] - 2 Hz[k-dy])
+  const double nxr = r2 * (((r.x + l.x) - 2 * c.x) - ((dr.x + dl.x) - 2 * d.x));
+  const double nxi = r2 * (((r.y + l.y) - 2 * c.y) - ((dr.y + dl.y) - 2 * d.y));
+  // (Hz[k+dy] + Hz[k-dy] - 2 Hz[k]) - (Hz[k-dx+dy] + Hz[k-dx-dy] - 2 Hz[k-dx])
+  const double nyr = r2 * (((u.x + d.x) - 2 * c.x) - ((ul.x + dl.x) - 2 * l.x));
+  const double nyi = r2 * (((u.y + d.y) - 2 * c.y) - ((ul.y + dl.y) - 2 * l.y));
+  double2 nex = make_double2(cex * ex.x + cexly * ((c.x - d.x) + nxr), cex * ex.y + cexly * ((c.y - d.y) + nxi));
+  double2 ney = make_double2(cey * ey.x - ceylx * ((c.x - l.x) + nyr), cey * ey.y - ceylx * ((c.y - l.y) + nyi));
+  const double kr = (i + 0.0) * s.ks_cos + (j + 0.5) * s.ks_sin;
+  const double fx = nsc[8 * n + k], fy = nsc[9 * n + k];
+  if (s.on[0] && fx != 0.0) ns_wave_add(nex, s, s.dot[0], fx, kr);  // vacuum cells: factor exactly zero
+  if (s.on[1] && fy != 0.0) ns_wave_add(ney, s, s.dot[1], fy, kr);
+  Ex[k] = nex;
+  Ey[k] = ney;
+}
+
 // TE E half.  x: 0 C_DXJX1 1 C_DXJX0 2 C_DY, sx; y: 0 C_JX 1 C_JXHZ, sy
 template <int PLAIN>
 __global__ __launch_bounds__(256) void k_te_e(GridPtrs g, Tabs t, StepArgs s)
@@ -1032,15 +1096,18 @@ __global__ __launch_bounds__(1024) void k_time_spectra(const double2 *U0, const 
 
 // ---------------------------------------------------------------- eps fill
 
+// variant: 0 MPI / split / NS TM, 1 plain UPML, 2 NS TE (Hz sampled at D_XY and a zero
+// border, nsFdtdTE.c:116-121)
 __global__ __launch_bounds__(256) void k_fill_eps(const fdtd_model_state *m, double *e0, double *e1, double *e2,
-                                                  int NPXa, int NPYa, int off, int tm, int plain)
+                                                  int NPXa, int NPYa, int off, int tm, int variant)
 {
   const int j = blockIdx.x * 64 + threadIdx.x;
   const int i = blockIdx.y * 4 + threadIdx.y;
   if (i >= NPXa || j >= NPYa) return;
   const size_t k = (size_t)i * NPYa + j;
-  if (off && (i == 0 || j == 0 || i == NPXa - 1 || j == NPYa - 1)) {
-    e0[k] = 0.0; e1[k] = 0.0; e2[k] = 0.0;   // ghost ring (never read by the updates)
+  const int plain = variant == 1;
+  if ((off || variant == 2) && (i == 0 || j == 0 || i == NPXa - 1 || j == NPYa - 1)) {
+    e0[k] = 0.0; e1[k] = 0.0; e2[k] = 0.0;   // ghost ring / unset border (never read by the updates)
     return;
   }
   const double x = i - off, y = j - off;
@@ -1056,7 +1123,8 @@ __global__ __launch_bounds__(256) void k_fill_eps(const fdtd_model_state *m, dou
   } else {
     e0[k] = fdtd_model_eps(m, x + 0.5, y, 0, 1);
     e1[k] = fdtd_model_eps(m, x, y + 0.5, 1, 0);
-    e2[k] = 0.5 * (fdtd_model_eps(m, x + 0.5, y + 0.5, 1, 0) + fdtd_model_eps(m, x + 0.5, y + 0.5, 0, 1));
+    e2[k] = variant == 2 ? fdtd_model_eps(m, x + 0.5, y + 0.5, 1, 1)
+                         : 0.5 * (fdtd_model_eps(m, x + 0.5, y + 0.5, 1, 0) + fdtd_model_eps(m, x + 0.5, y + 0.5, 0, 1));
   }
 }
 
@@ -1106,7 +1174,8 @@ struct fdtd_grid {
   double2 *tw;
   fdtd_model_state *mstate;
   FastState *fs;  // FP32 fused path (precision FDTD_FP32); F[] then serve as FP64 export buffers
-  double *nsc;    // NS flavour: [5][ncell] per-cell coefficients (fdtd_hip_grid_set_ns_coefs)
+  double *nsc;    // NS flavour: [nns][ncell] per-cell coefficients (fdtd_hip_grid_set_ns_coefs)
+  int nns;        // NS flavour: TM 5, TE 10 maps
   double ns_r2;   // NS flavour: r/2 of the second differences
 };
 
@@ -1421,8 +1490,8 @@ extern "C" int fdtd_hip_grid_create(const fdtd_grid_desc *d, fdtd_grid **out)
     return fdtd_fail(FDTD_EINVAL, "bad flavour");
   if (d->flavour == FDTD_FLAVOUR_SPLIT && (d->precision != FDTD_FP64 || d->ntff_time))
     return fdtd_fail(FDTD_EINVAL, "the split-field PML solvers run in FP64, without the per-step NTFF");
-  if (d->flavour == FDTD_FLAVOUR_NS && (d->precision != FDTD_FP64 || d->ntff_time || d->mode != FDTD_MODE_TM))
-    return fdtd_fail(FDTD_EINVAL, "the NS-FDTD solver is TM, FP64, without the per-step NTFF");
+  if (d->flavour == FDTD_FLAVOUR_NS && (d->precision != FDTD_FP64 || d->ntff_time))
+    return fdtd_fail(FDTD_EINVAL, "the NS-FDTD solvers run in FP64, without the per-step NTFF");
   if (d->ntff_left < 1 || d->ntff_bottom < 1 || d->ntff_right >= d->N_PX || d->ntff_top >= d->N_PY ||
       d->ntff_left >= d->ntff_right || d->ntff_bottom >= d->ntff_top)
     return fdtd_fail(FDTD_EINVAL, "NTFF contour (%d..%d x %d..%d) outside the %dx%d grid", d->ntff_left, d->ntff_right,
@@ -1450,8 +1519,9 @@ extern "C" int fdtd_hip_grid_create(const fdtd_grid_desc *d, fdtd_grid **out)
     HIPCHK(hipMemset(g->eps[e], 0, g->ncell * sizeof(double)));
   }
   if (d->flavour == FDTD_FLAVOUR_NS) {
-    HIPCHK(dev_malloc(&g->nsc, 5 * g->ncell * sizeof(double)));
-    HIPCHK(hipMemset(g->nsc, 0, 5 * g->ncell * sizeof(double)));
+    g->nns = d->mode == FDTD_MODE_TM ? 5 : 10;
+    HIPCHK(dev_malloc(&g->nsc, g->nns * g->ncell * sizeof(double)));
+    HIPCHK(hipMemset(g->nsc, 0, g->nns * g->ncell * sizeof(double)));
   }
   const size_t ntab = 6 * (size_t)d->N_PX + 5 * (size_t)d->N_PY;
   HIPCHK(dev_malloc(&g->tab, ntab * sizeof(double)));
@@ -1610,11 +1680,11 @@ extern "C" int fdtd_hip_grid_set_coefs(fdtd_grid *g, const fdtd_coef_tables *t)
   return FDTD_OK;
 }
 
-extern "C" int fdtd_hip_grid_set_ns_coefs(fdtd_grid *g, const double *cell5, double r_half)
+extern "C" int fdtd_hip_grid_set_ns_coefs(fdtd_grid *g, const double *cells, double r_half)
 {
-  if (!g || !cell5) return fdtd_fail(FDTD_EINVAL, "null argument");
+  if (!g || !cells) return fdtd_fail(FDTD_EINVAL, "null argument");
   if (g->d.flavour != FDTD_FLAVOUR_NS) return fdtd_fail(FDTD_EINVAL, "per-cell NS coefficients need the NS flavour");
-  HIPCHK(hipMemcpy(g->nsc, cell5, 5 * g->ncell * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(g->nsc, cells, g->nns * g->ncell * sizeof(double), hipMemcpyHostToDevice));
   g->ns_r2 = r_half;
   return FDTD_OK;
 }
@@ -1625,7 +1695,9 @@ extern "C" int fdtd_hip_grid_fill_eps(fdtd_grid *g, const fdtd_model_state *m)
   HIPCHK(hipMemcpy(g->mstate, m, sizeof(*m), hipMemcpyHostToDevice));
   dim3 blk(64, 4), grd((g->NPYa + 63) / 64, (g->NPXa + 3) / 4);
   hipLaunchKernelGGL(k_fill_eps, grd, blk, 0, 0, g->mstate, g->eps[0], g->eps[1], g->eps[2], g->NPXa, g->NPYa, g->off,
-                     g->d.mode == FDTD_MODE_TM ? 1 : 0, g->d.flavour == FDTD_FLAVOUR_PLAIN ? 1 : 0);
+                     g->d.mode == FDTD_MODE_TM ? 1 : 0,
+                     g->d.flavour == FDTD_FLAVOUR_PLAIN ? 1
+                     : (g->d.flavour == FDTD_FLAVOUR_NS && g->d.mode == FDTD_MODE_TE ? 2 : 0));
   HIPCHK(hipGetLastError());
   HIPCHK(hipDeviceSynchronize());
   if (g->fs) return fast_state_set_inv_eps(g->fs, g->eps, g->NPYa, g->off);
@@ -1711,12 +1783,21 @@ extern "C" int fdtd_hip_grid_step(fdtd_grid *g, const fdtd_step_params *p)
       hipLaunchKernelGGL(k_split_te_e, grd, blk, 0, 0, gp, g->tabs, s);
       hipLaunchKernelGGL(k_split_te_h, grd, blk, 0, 0, gp, g->tabs);
     }
-  } else if (d.flavour == FDTD_FLAVOUR_NS) {  // update (nsFdtdTM.c:68-80): calcH, calcE, NS wave, Ez
+  } else if (d.flavour == FDTD_FLAVOUR_NS) {
     s.ps_on = 0;
     s.time = p->time;
     s.w_s = p->omega;
-    hipLaunchKernelGGL(k_ns_tm_h, grd, blk, 0, 0, gp, g->tabs, (const double *)g->nsc, g->ns_r2);
-    hipLaunchKernelGGL(k_ns_tm_e, grd, blk, 0, 0, gp, g->tabs, (const double *)g->nsc, s);
+    if (d.mode == FDTD_MODE_TM) {  // update (nsFdtdTM.c:68-80): calcH, calcE, NS wave, Ez
+      hipLaunchKernelGGL(k_ns_tm_h, grd, blk, 0, 0, gp, g->tabs, (const double *)g->nsc, g->ns_r2);
+      hipLaunchKernelGGL(k_ns_tm_e, grd, blk, 0, 0, gp, g->tabs, (const double *)g->nsc, s);
+    } else {  // update (nsFdtdTE.c:242-257): calcH, Hz, calcE, NS waves
+      s.dot[0] = p->dot_x;
+      s.dot[1] = p->dot_y;
+      s.on[0] = p->use_x;
+      s.on[1] = p->use_y;
+      hipLaunchKernelGGL(k_ns_te_h, grd, blk, 0, 0, gp, (const double *)g->nsc);
+      hipLaunchKernelGGL(k_ns_te_e, grd, blk, 0, 0, gp, (const double *)g->nsc, g->ns_r2, s);
+    }
   } else if (d.mode == FDTD_MODE_TM) {
     if (plain) {
       hipLaunchKernelGGL(k_tm_h, grd, blk, 0, 0, gp, g->tabs);

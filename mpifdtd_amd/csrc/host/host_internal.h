@@ -9,7 +9,7 @@
 #define FDTD_HIDDEN __attribute__((visibility("hidden")))
 
 /* solver kinds of solvers.c, in the order of their entry points */
-enum { SLV_MPI_TM, SLV_MPI_TE, SLV_TM, SLV_TE, SLV_SPLIT_TM, SLV_SPLIT_TE, SLV_NS_TM, SLV_N };
+enum { SLV_MPI_TM, SLV_MPI_TE, SLV_TM, SLV_TE, SLV_SPLIT_TM, SLV_SPLIT_TE, SLV_NS_TM, SLV_NS_TE, SLV_N };
 
 /* grid of solver `kind` for the current field and model globals */
 FDTD_HIDDEN int solver_make_grid(int kind, int precision, int ntff_time, int T, int R, fdtd_grid **out);

@@ -2,10 +2,9 @@
  * simulator.c -- solver dispatcher (simulator.c:19-279 of the reference).
  *
  * Selects one of the four UPML solvers, the two split-field PML solvers or
- * the non-standard FDTD TM solver by enum SOLVER, wires its methods into the
+ * the two non-standard FDTD solvers by enum SOLVER, wires its methods into the
  * update / init / finish / reset table and drives field_nextStep after every
- * update.  The NS-FDTD TE solver is outside this library; selecting it is an
- * error, as for unknown solvers in the reference.
+ * update.
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -47,6 +46,13 @@ void simulator_setSolver(enum SOLVER solver)
     eps_ = nsFdtdTM_getEps; draw_ = nsFdtdTM_getEz;
     printf("NS TM mode \n");
     solverDir_ = "NS_TM";
+    break;
+  case NS_TE_2D:  /* setNsTE (simulator.c:83-99) */
+    update_ = nsFdtdTE_getUpdate(); init_ = nsFdtdTE_getInit();
+    finish_ = nsFdtdTE_getFinish(); reset_ = nsFdtdTE_getReset();
+    eps_ = nsFdtdTE_getEps; draw_ = nsFdtdTE_getEy;
+    printf("NS TE mode \n");
+    solverDir_ = "NS_TE";
     break;
   case TM_UPML_2D:
     update_ = fdtdTM_upml_getUpdate(); init_ = fdtdTM_upml_getInit();

@@ -35,7 +35,7 @@ typedef struct {
   fdtd_grid *g;
   int NPXa, NPYa, off, arraySize;
   double complex *mirror[3];  /* TM: Ez Hx Hy   TE: Ex Ey Hz */
-  double complex *split_mirror[2];  /* split-field and NS TM: Ezx Ezy   split-field TE: Hzx Hzy */
+  double complex *split_mirror[2];  /* split-field and NS TM: Ezx Ezy   TE: Hzx Hzy */
   double *eps_mirror;
   int blocked;    /* FP32 grid: steps are queued and run T per launch */
   int ntff_time;  /* the per-step time-domain NTFF runs (its outputs exist) */
@@ -61,9 +61,12 @@ static void die(const char *what)
 static int is_tm(int kind) { return kind == SLV_MPI_TM || kind == SLV_TM || kind == SLV_SPLIT_TM || kind == SLV_NS_TM; }
 static int is_mpi(int kind) { return kind == SLV_MPI_TM || kind == SLV_MPI_TE; }
 /* the solvers with split components (FP64, no NTFF, elliptic output on reset):
-   the split-field PML TM / TE and the non-standard FDTD TM */
-static int is_split(int kind) { return kind == SLV_SPLIT_TM || kind == SLV_SPLIT_TE || kind == SLV_NS_TM; }
-static int is_ns(int kind) { return kind == SLV_NS_TM; }
+   the split-field PML TM / TE and the non-standard FDTD TM / TE */
+static int is_split(int kind)
+{
+  return kind == SLV_SPLIT_TM || kind == SLV_SPLIT_TE || kind == SLV_NS_TM || kind == SLV_NS_TE;
+}
+static int is_ns(int kind) { return kind == SLV_NS_TM || kind == SLV_NS_TE; }
 
 void fdtd_solver_setNTFFTime(int on) { ntff_time_ = on ? 1 : 0; }
 void fdtd_solver_setFileOutput(int on) { file_output_ = on ? 1 : 0; }
@@ -106,14 +109,63 @@ int fdtd_solver_getBlocking(int *T, int *R)
 static double ns_beta(double alpha) { return tanh(alpha) / (1 + pow(tanh(alpha), 2)); }
 static double ns_coef1(double beta) { return (1 - beta) / (1 + beta); }
 
-/* nsFdtdTM.c:226-306 and field.c:155-177: the per-cell coefficients of the NS
-   solver from the eps maps the device filled (EPS_EZ, EPS_HX, EPS_HY), each
-   formed with the reference's expression and operation order */
-static int build_ns_coefs(fdtd_grid *g)
+/* the factor 1/(_n n) - 1 of field_nsScatteredWaveNotUPML (field.c:155-177),
+   exactly 0 where n = 1 */
+static double ns_wave_factor(double eps, double eps0, double w_s, double k_s)
+{
+  const double nn = sqrt(eps / eps0);
+  const double u0 = sin(w_s * 0.5) / sin(k_s * 0.5), u1 = sin(w_s * 0.5) / sin(nn * k_s * 0.5);
+  const double _n = u0 / u1;
+  return 1.0 / (_n * nn) - 1.0;
+}
+
+/* nsFdtdTE.c:100-181: all ten TE maps from EPS_EX, EPS_HZ and EPS_EY, over the
+   interior cells the reference fills (the border stays zero) */
+static void ns_te_cells(const double *eps, double *cell, size_t n)
+{
+  const int nx = N_PX, ny = N_PY;
+  const double R = 1.0e-8, M = 2.0, eps0 = 1.0, C = field_getLightSpeed(), mu0 = field_getMu0();
+  const double sig_max = -(M + 1.0) * eps0 * C / N_PML * log(R);
+  const double w_s = field_getOmega(), k_s = field_getK();
+  memset(cell, 0, 10 * n * sizeof(double));
+  for (int i = 1; i < nx - 1; i++) {
+    for (int j = 1; j < ny - 1; j++) {
+      const size_t k = (size_t)i * ny + j;
+      const double e_ex = eps[k], e_ey = eps[n + k], e_hz = eps[2 * n + k];
+      const double sig_ex_y = sig_max * field_sigmaY(i + 0.5, j), sig_ey_x = sig_max * field_sigmaX(i, j + 0.5);
+      const double sig_hz_x = sig_max * field_sigmaX(i + 0.5, j + 0.5), sig_hz_y = sig_max * field_sigmaY(i + 0.5, j + 0.5);
+      const double b_ex_y = ns_beta(sig_ex_y / (2 * e_ex)), b_ey_x = ns_beta(sig_ey_x / (2 * e_ey));
+      const double b_hz_x = ns_beta(sig_hz_x / (2 * e_hz)), b_hz_y = ns_beta(sig_hz_y / (2 * e_hz));
+      const double z_hz = sqrt(mu0 / e_hz), n_hz = sqrt(e_hz / eps0);
+      const double u_hz = sin(w_s * 0.5) / sin(k_s * n_hz * 0.5);
+      cell[4 * n + k] = ns_coef1(b_hz_x);                  /* C_HZX */
+      cell[5 * n + k] = u_hz / z_hz / (1.0 + b_hz_x);      /* C_HZXLX */
+      cell[6 * n + k] = ns_coef1(b_hz_y);                  /* C_HZY */
+      cell[7 * n + k] = u_hz / z_hz / (1.0 + b_hz_y);      /* C_HZYLY */
+      const double z_ex = sqrt(mu0 / e_ex), n_ex = sqrt(e_ex / eps0);
+      const double u_ex = sin(w_s * 0.5) / sin(k_s * n_ex * 0.5);
+      cell[k] = ns_coef1(b_ex_y);                          /* C_EX */
+      cell[n + k] = u_ex * z_ex / (1.0 + b_ex_y);          /* C_EXLY */
+      const double z_ey = sqrt(mu0 / e_ey), n_ey = sqrt(e_ey / eps0);
+      const double u_ey = sin(w_s * 0.5) / sin(k_s * n_ey * 0.5);
+      cell[2 * n + k] = ns_coef1(b_ey_x);                  /* C_EY */
+      cell[3 * n + k] = u_ey * z_ey / (1.0 + b_ey_x);      /* C_EYLX */
+      cell[8 * n + k] = ns_wave_factor(e_ex, eps0, w_s, k_s);
+      cell[9 * n + k] = ns_wave_factor(e_ey, eps0, w_s, k_s);
+    }
+  }
+}
+
+/* nsFdtdTM.c:226-306 / nsFdtdTE.c:100-181 and field.c:155-177: the per-cell
+   coefficients of the NS solvers from the eps maps the device filled (TM EPS_EZ,
+   EPS_HX, EPS_HY; TE EPS_EX, EPS_EY, EPS_HZ), each formed with the reference's
+   expression and operation order */
+static int build_ns_coefs(int kind, fdtd_grid *g)
 {
   const int nx = N_PX, ny = N_PY;
   const size_t n = (size_t)nx * ny;
-  double *eps = (double *)malloc(3 * n * sizeof(double)), *cell = (double *)malloc(5 * n * sizeof(double));
+  const int nmaps = is_tm(kind) ? 5 : 10;
+  double *eps = (double *)malloc(3 * n * sizeof(double)), *cell = (double *)malloc(nmaps * n * sizeof(double));
   if (!eps || !cell) {
     free(eps);
     free(cell);
@@ -124,7 +176,8 @@ static int build_ns_coefs(fdtd_grid *g)
   const double R = 1.0e-8, M = 2.0, eps0 = 1.0, C = field_getLightSpeed(), mu0 = field_getMu0();
   const double sig_max = -(M + 1.0) * eps0 * C / N_PML * log(R);
   const double w_s = field_getOmega(), k_s = field_getK();
-  for (int i = 0; i < nx && rc == FDTD_OK; i++) {
+  if (rc == FDTD_OK && !is_tm(kind)) ns_te_cells(eps, cell, n);
+  for (int i = 0; i < nx && rc == FDTD_OK && is_tm(kind); i++) {
     const double b_ez_x = ns_beta(sig_max * field_sigmaX(i, 0) / (2 * eps0));
     const double b_hy_x = ns_beta(sig_max * field_sigmaX(i + 0.5, 0) / (2 * eps0));
     for (int j = 0; j < ny; j++) {
@@ -141,14 +194,10 @@ static int build_ns_coefs(fdtd_grid *g)
       const double z_hy = sqrt(mu0 / e_hy), n_hy = sqrt(e_hy / eps0);
       const double u_hy = sin(w_s * 0.5) / sin(k_s * n_hy * 0.5);
       cell[3 * n + k] = u_hy / z_hy / (1.0 + b_hy_x);  /* C_HYLX */
-      /* field_nsScatteredWaveNotUPML: 1/(_n n) - 1, exactly 0 where n = 1 */
-      const double nn = sqrt(e_ez / eps0);
-      const double u0 = sin(w_s * 0.5) / sin(k_s * 0.5), u1 = sin(w_s * 0.5) / sin(nn * k_s * 0.5);
-      const double _n = u0 / u1;
-      cell[4 * n + k] = 1.0 / (_n * nn) - 1.0;
+      cell[4 * n + k] = ns_wave_factor(e_ez, eps0, w_s, k_s);
     }
   }
-  const double r = 1.0 / 6.0 + k_s * k_s / 180.0 - pow(k_s, 4) / 23040;  /* nsFdtdTM.c:117 */
+  const double r = 1.0 / 6.0 + k_s * k_s / 180.0 - pow(k_s, 4) / 23040;  /* nsFdtdTM.c:117, nsFdtdTE.c:255 */
   if (rc == FDTD_OK) rc = fdtd_hip_grid_set_ns_coefs(g, cell, r / 2.0);
   free(eps);
   free(cell);
@@ -167,14 +216,15 @@ static void build_coefs(int kind, fdtd_grid *g)
   SY = buf + (size_t)6 * nx + (size_t)4 * ny;
   const double sig_z = 0;
   if (is_ns(kind)) {
-    /* nsFdtdTM.c:226-306: the decay factors coef1 of the simplified beta, which
-       depend on one axis only; the eps-dependent ones follow in build_ns_coefs */
+    /* nsFdtdTM.c:226-306: the TM decay factors coef1 of the simplified beta, which
+       depend on one axis only; the eps-dependent ones, and every TE one (its alpha
+       divides by the cell's eps), follow in build_ns_coefs */
     const double sig_max = -(M + 1.0) * eps * C / N_PML * log(R);
-    for (int x = 0; x < nx; x++) {
+    for (int x = 0; x < nx && is_tm(kind); x++) {
       X[0][x] = ns_coef1(ns_beta(sig_max * field_sigmaX(x, 0) / (2 * eps)));        /* C_EZX */
       X[1][x] = ns_coef1(ns_beta(sig_max * field_sigmaX(x + 0.5, 0) / (2 * eps)));  /* C_HY */
     }
-    for (int y = 0; y < ny; y++) {
+    for (int y = 0; y < ny && is_tm(kind); y++) {
       Y[0][y] = ns_coef1(ns_beta(sig_max * field_sigmaY(0, y) / (2 * eps)));        /* C_EZY */
       Y[1][y] = ns_coef1(ns_beta(sig_max * field_sigmaY(0, y + 0.5) / (2 * eps)));  /* C_HX */
     }
@@ -300,7 +350,7 @@ int solver_make_grid(int kind, int precision, int ntff_time, int T, int R, fdtd_
   build_coefs(kind, g);
   if ((rc = fdtd_hip_grid_fill_eps(g, models_getState())) == FDTD_OK && precision == FDTD_FP32)
     rc = fdtd_hip_grid_set_blocking(g, T > 0 ? T : 4, R);
-  if (rc == FDTD_OK && is_ns(kind)) rc = build_ns_coefs(g);
+  if (rc == FDTD_OK && is_ns(kind)) rc = build_ns_coefs(kind, g);
   if (rc != FDTD_OK) {
     fdtd_hip_grid_destroy(g);
     return rc;
@@ -345,8 +395,9 @@ static void solver_update(int kind)
   fdtd_dcomplex ps = field_pointLight();
   p.ps_re = creal(ps);
   p.ps_im = cimag(ps);
-  if (kind == SLV_TE) {
-    /* the pulse is projected on the direction 90 deg from the wave vector (fdtdTE_upml.c:182-189) */
+  if (kind == SLV_TE || kind == SLV_NS_TE) {
+    /* the pulse / wave is projected on the direction 90 deg from the wave vector
+       (fdtdTE_upml.c:182-189, nsFdtdTE.c:250-256) */
     WaveInfo_S w = field_getWaveInfo_S();
     double co = cos((w.Angle_deg + 90) * M_PI / 180.0);
     double si = sin((w.Angle_deg + 90) * M_PI / 180.0);
@@ -372,7 +423,7 @@ static double *solver_eps(int kind)
   if (!s->g) return NULL;
   /* MPI TM: EPS_EZ, MPI TE: EPS_EY, plain TM: EPS_EZ, plain TE: EPS_EX, split TM: EPS_EZ,
      split TE: EPS_EY (fdtdTE.c:62-65) */
-  int which = (kind == SLV_MPI_TE || kind == SLV_SPLIT_TE) ? 1 : 0;
+  int which = (kind == SLV_MPI_TE || kind == SLV_SPLIT_TE || kind == SLV_NS_TE) ? 1 : 0;
   CHECK(fdtd_hip_grid_get_eps(s->g, which, s->eps_mirror));
   return s->eps_mirror;
 }
@@ -544,7 +595,8 @@ static void write_elliptic(int kind)
 {
   FieldInfo fi = field_getFieldInfo();
   char name[128];
-  snprintf(name, sizeof name, "%s_%dnm.txt", kind == SLV_NS_TM ? "ns_tm" : (kind == SLV_SPLIT_TM ? "tm" : "te"), fi.h_u_nm);
+  const char *prefix = kind == SLV_NS_TM ? "ns_tm" : kind == SLV_NS_TE ? "ns_te" : kind == SLV_SPLIT_TM ? "tm" : "te";
+  snprintf(name, sizeof name, "%s_%dnm.txt", prefix, fi.h_u_nm);
   const double complex *ez = solver_field(kind, is_tm(kind) ? 0 : 1);
   const double lam = field_getLambda();
   printf("output start\n");
@@ -725,6 +777,22 @@ fdtd_dcomplex *nsFdtdTM_getEz(void) { return (fdtd_dcomplex *)solver_field(SLV_N
 fdtd_dcomplex *nsFdtdTM_getHy(void) { return (fdtd_dcomplex *)solver_field(SLV_NS_TM, 2); }
 fdtd_dcomplex *nsFdtdTM_getHx(void) { return (fdtd_dcomplex *)solver_field(SLV_NS_TM, 1); }
 double *nsFdtdTM_getEps(void) { return solver_eps(SLV_NS_TM); }
+
+/* ---------------------------------------------------------------- non-standard FDTD TE (nsFdtdTE.c) */
+static void ns_te_init(void) { solver_init(SLV_NS_TE); }
+static void ns_te_update(void) { solver_update(SLV_NS_TE); }
+static void ns_te_reset(void) { solver_reset(SLV_NS_TE); }
+static void ns_te_finish(void) { solver_finish(SLV_NS_TE); }
+fdtd_method nsFdtdTE_getUpdate(void) { return ns_te_update; }
+fdtd_method nsFdtdTE_getFinish(void) { return ns_te_finish; }
+fdtd_method nsFdtdTE_getReset(void) { return ns_te_reset; }
+fdtd_method nsFdtdTE_getInit(void) { return ns_te_init; }
+fdtd_dcomplex *nsFdtdTE_getHzx(void) { return (fdtd_dcomplex *)split_field(SLV_NS_TE, 0); }
+fdtd_dcomplex *nsFdtdTE_getHzy(void) { return (fdtd_dcomplex *)split_field(SLV_NS_TE, 1); }
+fdtd_dcomplex *nsFdtdTE_getHz(void) { return (fdtd_dcomplex *)solver_field(SLV_NS_TE, 2); }
+fdtd_dcomplex *nsFdtdTE_getEy(void) { return (fdtd_dcomplex *)solver_field(SLV_NS_TE, 1); }
+fdtd_dcomplex *nsFdtdTE_getEx(void) { return (fdtd_dcomplex *)solver_field(SLV_NS_TE, 0); }
+double *nsFdtdTE_getEps(void) { return solver_eps(SLV_NS_TE); }
 
 /* ---------------------------------------------------------------- plain TE */
 static void te_init(void) { solver_init(SLV_TE); }

@@ -23,6 +23,7 @@ _GETTERS = {
     N.TM_2D: ("fdtdTM_getEz", "fdtdTM_getHx", "fdtdTM_getHy", "fdtdTM_getEps"),  # split-field PML (FP64)
     N.TE_2D: ("fdtdTE_getEx", "fdtdTE_getEy", "fdtdTE_getHz", "fdtdTE_getEps"),
     N.NS_TM_2D: ("nsFdtdTM_getEz", "nsFdtdTM_getHx", "nsFdtdTM_getHy", "nsFdtdTM_getEps"),  # non-standard FDTD (FP64)
+    N.NS_TE_2D: ("nsFdtdTE_getEx", "nsFdtdTE_getEy", "nsFdtdTE_getHz", "nsFdtdTE_getEps"),
 }
 
 

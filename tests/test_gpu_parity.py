@@ -188,19 +188,19 @@ def test_far_field_follows_mie_series_at_100_cells_per_wavelength(solver, pol, p
 
 
 @pytest.mark.parametrize("name", ["tm2d_mie_a0", "tm2d_zigzag_a30_rect", "te2d_mie_a0", "te2d_layer_a45_rect",
-                                  "nstm_mie_a0", "nstm_zigzag_a30_rect"])
+                                  "nstm_mie_a0", "nstm_zigzag_a30_rect", "nste_mie_a0", "nste_layer_a45_rect"])
 def test_split_field_parity(name, tmp_path, monkeypatch):
     """The split-field PML solvers (fdtdTM.c / fdtdTE.c, solvers TM_2D / TE_2D) and
-    the non-standard FDTD TM solver (nsFdtdTM.c, NS_TM_2D): the three field getters,
+    the non-standard FDTD solvers (nsFdtdTM.c / nsFdtdTE.c, NS_TM_2D / NS_TE_2D): the three field getters,
     eps and the far field against the compiled reference's golden run, every field
-    and split component against the oracle, and the tm_ / te_ / ns_tm_<h>nm.txt file
+    and split component against the oracle, and the tm_ / te_ / ns_tm_ / ns_te_<h>nm.txt file
     that finish() writes through reset() against the reference's (%lf resolution)."""
     m, po = _mods()
     g = load_golden(name)
     solver, model, sweep, w, h, hu, pml, lam, ang, steps = [int(v) for v in g["cfg"]]
     params = swept_params(model, sweep)
     tm = solver in (po.TM_2D, po.NS_TM_2D)
-    prefix = {po.TM_2D: "tm", po.TE_2D: "te", po.NS_TM_2D: "ns_tm"}[solver]
+    prefix = {po.TM_2D: "tm", po.TE_2D: "te", po.NS_TM_2D: "ns_tm", po.NS_TE_2D: "ns_te"}[solver]
     print(f"case {name}: {prefix} model {model} {w}x{h} h={hu} steps={steps}", flush=True)
     monkeypatch.chdir(tmp_path)
     sim = run_gpu(m, solver, model, w, h, hu, pml, lam, ang, steps, params, file_output=True)
