@@ -7,6 +7,9 @@ to this script.  Run only where /root/reference exists (the fixtures, not
 the reference, go wherever the tests run):
 
     python tests/golden/make_golden.py [fixture names: only these]
+
+It also runs oracle/_ref/ref_main, the reference's own main.c built from all its
+sources, in an empty directory and stores the files it writes (main_default).
 """
 import os
 import shutil
@@ -19,6 +22,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+REF_MAIN = os.path.join(ROOT, "oracle", "_ref", "ref_main")
 
 # (name, solver, model, sweep, width, height, h, pml, lambda, angle, steps)
 RUN_CASES = [
@@ -76,6 +80,29 @@ def read_text_rows(path, rows, ncols):
     return np.stack(out)
 
 
+def write_main_default(tmp):
+    """The reference's program as shipped (main.c: MIE_CYLINDER, NS_TE_2D, h = 50 nm,
+    pml 15, lambda 500 nm, 400 steps), run in an empty directory: every file it
+    writes (relative paths), the ns_te_50nm.txt rows and the image.bmp bytes."""
+    subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "_ref/ref_main"], check=True, capture_output=True)
+    cwd = os.path.join(tmp, "main_default")
+    os.makedirs(cwd)
+    r = subprocess.run([REF_MAIN], cwd=cwd, capture_output=True, text=True, timeout=600)
+    if r.returncode != 1:  # main.c ends with `return 1` (main.c:234)
+        raise RuntimeError(f"ref_main failed ({r.returncode}): {r.stderr[-2000:]}")
+    paths = sorted(os.path.relpath(os.path.join(dp, f), cwd) for dp, _, fs in os.walk(cwd) for f in fs)
+    d = dict(paths=np.array(paths), returncode=np.array(r.returncode))
+    for p in paths:
+        full = os.path.join(cwd, p)
+        if p.endswith(".txt"):
+            d["elliptic"] = np.loadtxt(full)
+        elif p.endswith(".bmp"):
+            d["bmp"] = np.frombuffer(open(full, "rb").read(), np.uint8)
+    d["stdout"] = np.array([ln for ln in r.stdout.splitlines() if ln.startswith(("field size", "NS TE mode"))])
+    np.savez_compressed(os.path.join(HERE, "ref_main_default.npz"), **d)
+    print("wrote main_default", {k: v.shape for k, v in d.items()})
+
+
 def main(only=()):
     """Writes every fixture, or only the named ones."""
     subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True, capture_output=True)
@@ -114,6 +141,8 @@ def main(only=()):
                     d[nm] = re + 1j * im
             np.savez_compressed(os.path.join(HERE, f"ref_{name}.npz"), **d)
             print("wrote", name, {k: v.shape for k, v in d.items()})
+        if not only or "main_default" in only:
+            write_main_default(tmp)
         for (name, model, sweep, w, h_nm, hu) in EPS_CASES:
             if only and name not in only:
                 continue
