@@ -32,7 +32,8 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 # launch shapes of the UPML and interior tiles (FAST_LAUNCH_* of fdtd_hip.h)
-SHAPES = ["auto", "merged", "split", "two_streams", "split_pml8", "two_streams_pml8", "split_w8", "two_streams_w8"]
+SHAPES = ["auto", "merged", "split", "two_streams", "split_pml8", "two_streams_pml8", "split_w8", "two_streams_w8",
+          "split_p16", "two_streams_p16"]
 
 # name -> (model, width_nm, height_nm, h_nm, pml, lambda_nm, angle_deg, model params, description)
 WORKLOADS = {
@@ -207,7 +208,7 @@ def run_sweep81(args, m):
         probe.set_dft(lams)
         probe.calc(40)
         for Tc, Rc in ((8, 16), (10, 16), (8, 8), (6, 8)):
-            for sh in (2, 3, 4, 5, 6, 7):
+            for sh in (2, 3, 4, 5, 6, 7, 8, 9):
                 n = 6 * Tc  # whole blocks only
                 probe.set_blocking(Tc, Rc)
                 probe.set_launch_mode(sh)
@@ -415,7 +416,10 @@ def main():
     if args.precision == "fp32":
         L.fdtd_solver_setBlocking(T, R)  # launches the queued warm-up steps
     L.fdtd_hip_synchronize()
-    L.fdtd_hip_profile_enable(1)
+    # events on every launch of a long run of short launches would cost the timed
+    # region a few microseconds per launch: then every 8th launch is timed
+    nblk = K // max(1, T or 1) if args.precision == "fp32" else 0
+    L.fdtd_hip_profile_enable(8 if nblk >= 256 else 1)
     L.fdtd_hip_synchronize()
     t0 = time.perf_counter()
     sim.calc(K)
@@ -445,6 +449,7 @@ def main():
                 "kernel": f"k_fast<{args.mode.upper()},MPI,R={R}>" + (
                     "" if shape_name == "merged" else f" interior + UPML launches ({shape_name.replace('_', ' ')})"),
                 "launch_avg_us": round(ms_k * 1e3 / nlaunch, 2), "steps_per_launch": T,
+                "launches_timed": int(nlaunch), "timed_every": 8 if nblk >= 256 else 1,
                 "bytes_per_launch": round((b_int + b_pml) / nlaunch),
                 "bytes_per_cell_update": round((b_int + b_pml) / cupd, 2)}
     else:

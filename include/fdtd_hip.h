@@ -212,7 +212,12 @@ enum {
    * wavefronts of R/2 rows: twice the wavefronts per SIMD (R = 8 or 16; with
    * R = 4 the interior tiles keep 4 wavefronts) */
   FAST_LAUNCH_SPLIT_W8 = 6,
-  FAST_LAUNCH_TWO_STREAMS_W8 = 7
+  FAST_LAUNCH_TWO_STREAMS_W8 = 7,
+  /* SPLIT_W8 / TWO_STREAMS_W8 with the UPML tiles as 16 wavefronts of 2 rows
+   * (1024 threads, four wavefronts per SIMD): for grids whose UPML band has
+   * fewer tiles than the chip has CUs.  Single grids only (not batches). */
+  FAST_LAUNCH_SPLIT_P16 = 8,
+  FAST_LAUNCH_TWO_STREAMS_P16 = 9
 };
 int fdtd_hip_grid_set_launch_mode(fdtd_grid *g, int mode);
 /* FP32 grids with contour samples: the sample packing, binning and running DFT
@@ -224,9 +229,12 @@ int fdtd_hip_grid_set_ntff_overlap(fdtd_grid *g, int on);
 int fdtd_hip_grid_get_launch_mode(const fdtd_grid *g, int *mode);
 int fdtd_hip_grid_flush(fdtd_grid *g);
 /* event timing of the FP32 fused launches since the last enable (one launch
- * advances the UPML and the interior tiles together by T steps):
- * out = {ms of the launches, 0 (reserved), launches, algorithmic bytes of the
- *        interior tiles, algorithmic bytes of the UPML tiles, cell updates} */
+ * advances the UPML and the interior tiles together by T steps).  on = k > 0
+ * brackets every k-th launch with a pair of events (the first launch after the
+ * enable is timed); 0 turns it off.  A pair costs the stream a few microseconds,
+ * so short launches are sampled rather than all timed.
+ * out = {ms of the timed launches, 0 (reserved), timed launches, algorithmic
+ *        bytes of their interior tiles, of their UPML tiles, their cell updates} */
 void fdtd_hip_profile_enable(int on);
 void fdtd_hip_profile_read(double out[6]);
 

@@ -725,6 +725,16 @@ __global__ __launch_bounds__(512) void k_fast_pml8(FastArgs a)
   fast_tiles<MODE, FLAV, 4, 1, SMP, 8>(a, a.qb[0], blockIdx.x, gridDim.x);
 }
 
+// UPML tiles (32 rows) as 16 wavefronts of 2 rows (1024 threads): a quarter of
+// the registers of the 4 x 8 form per wavefront, four wavefronts per SIMD.  On
+// grids whose UPML band has fewer tiles than the chip has CUs, the band's
+// launch is latency-bound, and more wavefronts per tile shorten it.
+template <int MODE, int FLAV, int SMP>
+__global__ __launch_bounds__(1024) void k_fast_pml16(FastArgs a)
+{
+  fast_tiles<MODE, FLAV, 2, 1, SMP, 16>(a, a.qb[0], blockIdx.x, gridDim.x);
+}
+
 // Interior tiles (4R rows) as 8 wavefronts of R/2 rows: the same tiles and
 // per-cell code as KIND 1 / 4, with half the registers per wavefront, so that
 // twice as many wavefronts share each SIMD (R = 16: four instead of two).
@@ -868,6 +878,12 @@ static void launch_pml8(const FastArgs &a, int nblocks, hipStream_t st)
   hipLaunchKernelGGL((k_fast_pml8<MODE, FLAV, SMP>), dim3(nblocks), dim3(512), 0, st, a);
 }
 
+template <int MODE, int FLAV, int SMP>
+static void launch_pml16(const FastArgs &a, int nblocks, hipStream_t st)
+{
+  hipLaunchKernelGGL((k_fast_pml16<MODE, FLAV, SMP>), dim3(nblocks), dim3(1024), 0, st, a);
+}
+
 template <int MODE, int FLAV, int RH, int SMP>
 static void launch_interior8(const FastArgs &a, int nblocks, hipStream_t st)
 {
@@ -883,6 +899,8 @@ static fast_launcher pick(int R, int kind)
   if (kind == 5) return launch_one<MODE, FLAV, 8, 5>;
   if (kind == 6) return launch_pml8<MODE, FLAV, 0>;
   if (kind == 7) return launch_pml8<MODE, FLAV, 1>;
+  if (kind == 10) return launch_pml16<MODE, FLAV, 0>;
+  if (kind == 11) return launch_pml16<MODE, FLAV, 1>;
   // interior tiles of 4R rows as 8 wavefronts of R/2 rows (R = 8, 16)
   if (kind == 8) return R == 8 ? launch_interior8<MODE, FLAV, 4, 0> : launch_interior8<MODE, FLAV, 8, 0>;
   if (kind == 9) return R == 8 ? launch_interior8<MODE, FLAV, 4, 1> : launch_interior8<MODE, FLAV, 8, 1>;
@@ -1005,7 +1023,8 @@ struct ProfRec {
   double bytes_int, bytes_pml, cells;
 };
 static struct {
-  int on = 0;
+  int on = 0;     // time every on-th launch (0: off)
+  long tick = 0;  // launches since the enable
   std::vector<ProfRec> recs;
   std::vector<hipEvent_t> pool;  // created once and reused: creating events per launch stalls the host
   double ms = 0, bytes_int = 0, bytes_pml = 0, cells = 0;
@@ -1043,7 +1062,8 @@ static void prof_resolve()
 void fast_profile_enable(int on)
 {
   prof_resolve();
-  g_prof.on = on;
+  g_prof.on = on > 0 ? on : 0;
+  g_prof.tick = 0;
   g_prof.ms = g_prof.bytes_int = g_prof.bytes_pml = g_prof.cells = 0;
   g_prof.launches = 0;
 }
@@ -1388,7 +1408,8 @@ int fast_state_flush(FastState *s)
     a.ct = s->hct;
   }
   ProfRec pr;
-  if (g_prof.on) {
+  const bool timed = g_prof.on && (g_prof.tick++ % g_prof.on) == 0;
+  if (timed) {
     if ((rc = prof_event(&pr.e0)) || (rc = prof_event(&pr.e1))) return rc;
     HIPCHK(hipEventRecord(pr.e0, 0));
   }
@@ -1409,27 +1430,30 @@ int fast_state_flush(FastState *s)
   // UPML tiles as 8 wavefronts of 4 rows (512 threads) in the *_PML8 shapes
   const bool w8 = mode == FAST_LAUNCH_SPLIT_PML8 || mode == FAST_LAUNCH_TWO_STREAMS_PML8 ||
                   mode == FAST_LAUNCH_SPLIT_W8 || mode == FAST_LAUNCH_TWO_STREAMS_W8;
-  // and in the *_W8 shapes the interior tiles as 8 wavefronts of R/2 rows (R >= 8)
-  const bool i8 = (mode == FAST_LAUNCH_SPLIT_W8 || mode == FAST_LAUNCH_TWO_STREAMS_W8) && s->R >= 8;
+  // or, in the *_P16 shapes, as 16 wavefronts of 2 rows (1024 threads)
+  const bool p16 = mode == FAST_LAUNCH_SPLIT_P16 || mode == FAST_LAUNCH_TWO_STREAMS_P16;
+  // and in the *_W8 and *_P16 shapes the interior tiles as 8 wavefronts of R/2 rows (R >= 8)
+  const bool i8 = (mode == FAST_LAUNCH_SPLIT_W8 || mode == FAST_LAUNCH_TWO_STREAMS_W8 || p16) && s->R >= 8;
   auto launch_list = [&](bool pml, hipStream_t st) {
     const int nt = pml ? s->n_pml : s->n_int, nc = pml ? s->n_pml_c : s->n_int_c;
     FastArgs b = a;
     b.nb_pml = 0;
     if (nt - nc > 0) {
       memcpy(pml ? b.qb[0] : b.qb[1], pml ? s->qpml_p : s->qint_p, sizeof b.qb[0]);
-      fast_launch(s->d.mode, s->d.flavour, s->R, pml ? (w8 ? 6 : 2) : (i8 ? 8 : 1), b, 8 * ((nt - nc + 7) / 8), st);
+      fast_launch(s->d.mode, s->d.flavour, s->R, pml ? (p16 ? 10 : w8 ? 6 : 2) : (i8 ? 8 : 1), b, 8 * ((nt - nc + 7) / 8), st);
     }
     if (nc > 0) {
       memcpy(pml ? b.qb[0] : b.qb[1], pml ? s->qpml_c : s->qint_c, sizeof b.qb[0]);
-      fast_launch(s->d.mode, s->d.flavour, s->R, pml ? (w8 ? 7 : 5) : (i8 ? 9 : 4), b, 8 * ((nc + 7) / 8), st);
+      fast_launch(s->d.mode, s->d.flavour, s->R, pml ? (p16 ? 11 : w8 ? 7 : 5) : (i8 ? 9 : 4), b, 8 * ((nc + 7) / 8), st);
     }
   };
   const int km = s->csamp ? 3 : 0;
-  if (mode == FAST_LAUNCH_SPLIT || mode == FAST_LAUNCH_SPLIT_PML8 || mode == FAST_LAUNCH_SPLIT_W8) {
+  if (mode == FAST_LAUNCH_SPLIT || mode == FAST_LAUNCH_SPLIT_PML8 || mode == FAST_LAUNCH_SPLIT_W8 ||
+      mode == FAST_LAUNCH_SPLIT_P16) {
     launch_list(false, 0);
     launch_list(true, 0);
   } else if (mode == FAST_LAUNCH_TWO_STREAMS || mode == FAST_LAUNCH_TWO_STREAMS_PML8 ||
-             mode == FAST_LAUNCH_TWO_STREAMS_W8) {
+             mode == FAST_LAUNCH_TWO_STREAMS_W8 || mode == FAST_LAUNCH_TWO_STREAMS_P16) {
     // UPML tiles on a second stream beside the interior tiles: both read the same
     // buffer and write disjoint cores, so the two launches may overlap, each at its
     // own occupancy; events order them after the previous block and before the next
@@ -1449,7 +1473,7 @@ int fast_state_flush(FastState *s)
     if (nbp + nbi > 0) fast_launch(s->d.mode, s->d.flavour, s->R, km, a, nbp + nbi, 0);
   }
   HIPCHK(hipGetLastError());
-  if (g_prof.on) {
+  if (timed) {
     HIPCHK(hipEventRecord(pr.e1, 0));
     pr.bytes_int = s->bytes_int_step * s->pending;
     pr.bytes_pml = s->bytes_pml_step * s->pending;
@@ -1471,7 +1495,7 @@ int fast_state_flush(FastState *s)
 
 int fast_state_set_launch_mode(FastState *s, int mode)
 {
-  if (mode < FAST_LAUNCH_AUTO || mode > FAST_LAUNCH_TWO_STREAMS_W8) return fdtd_fail(FDTD_EINVAL, "bad launch mode %d", mode);
+  if (mode < FAST_LAUNCH_AUTO || mode > FAST_LAUNCH_TWO_STREAMS_P16) return fdtd_fail(FDTD_EINVAL, "bad launch mode %d", mode);
   int rc = fast_state_flush(s);
   if (rc) return rc;
   s->lmode = mode;

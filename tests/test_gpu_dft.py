@@ -113,7 +113,7 @@ def test_contour_samples_independent_of_launch_shape():
     the same, and the binned accumulators agree to their atomic summation order."""
     import mpifdtd_amd as m
     D, U = [], []
-    for shape in (1, 2, 3, 4, 5, 6, 7):
+    for shape in (1, 2, 3, 4, 5, 6, 7, 8, 9):
         sim = m.Simulator(4, 1, m.field_info(6000, 6000, 20, 16, 500, 0, 300), ntff_time=True, precision="fp32",
                           T=8, R=16)
         try:

@@ -190,12 +190,12 @@ def test_fp32_large_grid_split_launch_against_fp64():
 @pytest.mark.parametrize("solver,R", [(4, 16), (3, 16), (5, 8)])
 def test_launch_shapes_are_bit_identical(solver, R):
     """Merged, split and two-stream launches of the UPML and interior tiles, with
-    the UPML tiles as 4 wavefronts of 8 rows or 8 of 4, and the interior tiles as
+    the UPML tiles as 4 wavefronts of 8 rows, 8 of 4 or 16 of 2, and the interior tiles as
     4 wavefronts of R rows or 8 of R/2, run the same per-cell code: the fields
     agree bit for bit."""
     m, N, po = _mods()
     out = []
-    for shape in (0, 1, 2, 3, 4, 5, 6, 7):
+    for shape in (0, 1, 2, 3, 4, 5, 6, 7, 8, 9):
         sim = m.Simulator(solver, 1, m.field_info(6000, 6000, 20, 16, 500, 0, 120), precision="fp32", T=4, R=R)
         try:
             sim.set_launch_mode(shape)
