@@ -50,7 +50,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=0, help="timed steps (default 8000; 400 for morpho8256)")
     ap.add_argument("--warmup", type=int, default=-1, help="untimed steps (default 400; 40 for morpho8256)")
-    ap.add_argument("--workload", default="mie1056", choices=sorted(WORKLOADS) + ["sweep64", "morpho8256_sweep81"])
+    ap.add_argument("--workload", default="mie1056",
+                    choices=sorted(WORKLOADS) + ["sweep64", "morpho8256_sweep81", "solvers_fp64"])
     ap.add_argument("--seed", type=int, default=1, help="sweep64: generator seed")
     ap.add_argument("--precision", default="fp32", choices=("fp32", "fp64"))
     ap.add_argument("--mode", default="tm", choices=("tm", "te"))
@@ -295,6 +296,49 @@ def run_sweep81(args, m):
     return line
 
 
+def run_solvers_fp64(args, m):
+    """Every solver of enum SOLVER in its FP64 reference-order path on the mie1056
+    grid (the reference's cylinder; CW source for the MPI, split-field and NS
+    solvers, Gaussian pulse for the plain UPML ones), without the per-step NTFF:
+    the step rate of each over `steps` simulator_calc() calls after 20 untimed
+    ones.  Not the headline: a measurement of the solvers beside the UPML path,
+    whose kernel durations a rocprofv3 kernel trace of this command gives."""
+    import numpy as np
+    L = m.lib()
+    model, w, h, hu, pml, lam, ang, params, _ = WORKLOADS["mie1056"]
+    steps = args.steps if args.steps_given else 400
+    solvers = [("TM_2D", m.TM_2D), ("TE_2D", m.TE_2D), ("NS_TM_2D", m.NS_TM_2D), ("NS_TE_2D", m.NS_TE_2D),
+               ("TM_UPML_2D", m.TM_UPML_2D), ("TE_UPML_2D", m.TE_UPML_2D),
+               ("MPI_TM_UPML_2D", m.MPI_TM_UPML_2D), ("MPI_TE_UPML_2D", m.MPI_TE_UPML_2D)]
+    per = {}
+    cells = 0
+    for name, solver in solvers:
+        log(f"solvers_fp64: {name}, {steps} steps")
+        sim = m.Simulator(solver, getattr(m, model), m.field_info(w, h, hu, pml, lam, ang, steps + 20),
+                          params=m.model_params(**params), ntff_time=False, precision="fp64")
+        try:
+            cells = sim.N_PX * sim.N_PY
+            sim.calc(20)
+            L.fdtd_hip_synchronize()
+            t0 = time.perf_counter()
+            sim.calc(steps)
+            L.fdtd_hip_synchronize()
+            dt = time.perf_counter() - t0
+            f0 = sim.fields()[0]
+            assert np.all(np.isfinite(f0)) and np.abs(f0).max() > 0, name
+        finally:
+            sim.finish()
+        per[name] = {"mcups": round(cells * steps / dt / 1e6, 1), "us_per_step": round(dt * 1e6 / steps, 2)}
+    return {"metric": "Mcell-updates/s, FP64 reference-order path, per solver", "value": per["NS_TE_2D"]["mcups"],
+            "unit": "Mcell-updates/s", "n_gpus": 1, "steps": steps, "warmup": 20,
+            "ms_per_step": round(per["NS_TE_2D"]["us_per_step"] / 1e3, 5), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "every solver of enum SOLVER on the 1056x1056 Mie cylinder grid (h=5 nm, "
+                                   "r=500 nm, n=1.6, lambda=550 nm), FP64, no per-step NTFF; value: NS_TE_2D, "
+                                   "the reference main.c's solver", "cells": cells, "per_solver": per},
+            "roofline": None, "cpu_baseline": None}
+
+
 def run_sweep(args, m):
     """The batched sweep: every instance to completion, timed as one job."""
     import numpy as np
@@ -385,8 +429,9 @@ def main():
     # the library's own HIP runtime does the synchronising (fdtd_hip_synchronize =
     # hipDeviceSynchronize): a GPU runtime loaded beside it would not see its queues
     assert L.fdtd_hip_device_count() >= 1, "no GPU"
-    if args.workload in ("sweep64", "morpho8256_sweep81"):
-        line = run_sweep(args, m) if args.workload == "sweep64" else run_sweep81(args, m)
+    if args.workload in ("sweep64", "morpho8256_sweep81", "solvers_fp64"):
+        line = {"sweep64": run_sweep, "morpho8256_sweep81": run_sweep81,
+                "solvers_fp64": run_solvers_fp64}[args.workload](args, m)
         libc.fflush(None)
         sys.stdout.flush()
         os.dup2(real_stdout, 1)
