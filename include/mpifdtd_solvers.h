@@ -101,9 +101,7 @@ fdtd_dcomplex *fdtdTE_getHzy(void);
 fdtd_dcomplex *fdtdTE_getHz(void);
 double *fdtdTE_getEps(void);
 
-/* nsFdtdTM.h:6-17 (non-standard FDTD TMz; getEps returns EPS_EZ).  The reference's
- * nsFdtdTM_getSolver returns a Solver whose fields it leaves unset; it has no
- * counterpart here. */
+/* nsFdtdTM.h:6-19 (non-standard FDTD TMz; getEps returns EPS_EZ). */
 fdtd_method nsFdtdTM_getUpdate(void);
 fdtd_method nsFdtdTM_getFinish(void);
 fdtd_method nsFdtdTM_getReset(void);
@@ -114,9 +112,16 @@ fdtd_dcomplex *nsFdtdTM_getEz(void);
 fdtd_dcomplex *nsFdtdTM_getHy(void);
 fdtd_dcomplex *nsFdtdTM_getHx(void);
 double *nsFdtdTM_getEps(void);
+/* solver.h:7-20's struct Solver (ten function pointers), here incomplete so
+ * that the reference's own typedef stays compatible.  As nsFdtdTM.c:45-61 and
+ * nsFdtdTE.c:42-58 do, the getters return a static Solver whose fields are all
+ * unset (null). */
+struct Solver;
+struct Solver *nsFdtdTM_getSolver(void);
+struct Solver *nsFdtdTE_getSolver(void);
 
-/* nsFdtdTE.h:6-17 (non-standard FDTD TEz; getEps returns EPS_EY).  nsFdtdTE_getSolver
- * has no counterpart here, as for TM. */
+/* nsFdtdTE.h:6-19 (non-standard FDTD TEz; getEps returns EPS_EY; nsFdtdTE_getSolver
+ * is declared with nsFdtdTM_getSolver above). */
 fdtd_method nsFdtdTE_getUpdate(void);
 fdtd_method nsFdtdTE_getFinish(void);
 fdtd_method nsFdtdTE_getReset(void);

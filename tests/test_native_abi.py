@@ -178,3 +178,18 @@ def test_reference_sweep_limits_and_errors():
                 (fi, 4, 5, 30, 0, 10, 0, None, 0),    # end before start
                 (fi, 4, 5, 0, 0, 1, 0, None, 2)):     # no output array
         assert L.fdtd_batch_sweep(*bad) < 0, bad
+
+
+def test_ns_get_solver_returns_an_unset_solver():
+    """nsFdtdTM_getSolver / nsFdtdTE_getSolver (nsFdtdTM.c:45-61, nsFdtdTE.c:42-58)
+    return a static solver.h Solver (ten function pointers) whose fields are all
+    left unset: null, the same object on every call."""
+    import ctypes
+    from mpifdtd_amd import _native as N
+    L = N.lib()
+    for name in ("nsFdtdTM_getSolver", "nsFdtdTE_getSolver"):
+        f = getattr(L, name)
+        f.restype = ctypes.POINTER(ctypes.c_void_p * 10)
+        p = f()
+        assert all(v is None for v in p.contents), name
+        assert ctypes.addressof(p.contents) == ctypes.addressof(f().contents)
