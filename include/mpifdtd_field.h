@@ -88,6 +88,38 @@ FieldInfo field_getFieldInfo(void);
 fdtd_dcomplex field_pointLight(void);                /* field.c:145-152 */
 int field_index(int i, int j);                       /* field.c:70-72 */
 int field_subIndex(int i, int j);                    /* field.c:66-68 */
+int field_getOffsetX(void);                          /* field.c:58-64 */
+int field_getOffsetY(void);
+int field_getSubNx(void);
+int field_getSubNy(void);
+int field_getSubNpx(void);
+int field_getSubNpy(void);
+int field_getSubNcell(void);
+int ind(const int i, const int j);                   /* field.c:74 */
+double field_pmlCoef(double ep_mu, double sig);      /* field.c:288-291 */
+double field_pmlCoef_LXY(double ep_mu, double sig);  /* field.c:292-295 */
+double field_ns_beta(double alpha, double alpha_aster);        /* field.c:298-303 */
+double field_ns_beta_aster(double alpha, double alpha_aster);  /* field.c:304-309 */
+/* sources added to a caller's host array over the interior cells (the solvers
+ * here form the same terms on the GPU and do not call these) */
+void field_scatteredWave(fdtd_dcomplex *p, double *eps, double gapX, double gapY);          /* field.c:202-218 */
+void field_scatteredWaveNotUPML(fdtd_dcomplex *p, double *eps, double gapX, double gapY);   /* field.c:179-199 */
+void field_nsScatteredWaveNotUPML(fdtd_dcomplex *p, double *eps, double gapX, double gapY,
+                                  double dot);                                             /* field.c:155-177 */
+void field_scatteredPulse(fdtd_dcomplex *p, double *eps, double gapX, double gapY, double dot); /* field.c:224-256 */
+/* text writers (field.c:322-389); field_outputElliptic writes 0 where its circle
+ * leaves the grid, where the reference reads outside its array */
+void field_outputElliptic(const char *fileName, fdtd_dcomplex *data);
+void field_outputAllDataComplex(const char *fileName, fdtd_dcomplex *data);
+void field_outputAllDataDouble(const char *fileName, double *data);
+
+/* myComplex.h:7-14 (myComplex.c:3-51) */
+double *newDouble(int size);
+void freeDouble(double *array);
+fdtd_dcomplex *newDComplex(int size);
+void freeDComplex(fdtd_dcomplex *array);
+double cnorm(fdtd_dcomplex c);
+fdtd_dcomplex cbilinear(fdtd_dcomplex *p, double x, double y, int width, int height);
 
 /* --- extensions (no counterpart in the reference) --- */
 /* light speed in cell units (dt = dx = 1, so this is the Courant number);

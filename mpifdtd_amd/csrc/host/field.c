@@ -86,14 +86,10 @@ void field_init(FieldInfo fi)
   waveInfo_s_.Omega_s = c0_ * waveInfo_s_.K_s;
   waveInfo_s_.Angle_deg = fi.angle_deg;
 
-  /* one rank: the whole grid plus a one-cell ghost ring */
+  /* the subdomain record stays unset, as the reference's field_init leaves it
+     (its mpiSplit call is commented out, field.c:112); the MPI solvers keep
+     their own subdomain sizes (mpi_fdtdTM_upml_getSubNx ...) */
   memset(&subFieldInfo_s_, 0, sizeof subFieldInfo_s_);
-  subFieldInfo_s_.SUB_N_X = fieldInfo_s_.N_PX;
-  subFieldInfo_s_.SUB_N_Y = fieldInfo_s_.N_PY;
-  subFieldInfo_s_.SUB_N_PX = fieldInfo_s_.N_PX + 2;
-  subFieldInfo_s_.SUB_N_PY = fieldInfo_s_.N_PY + 2;
-  subFieldInfo_s_.SUB_N_CELL = subFieldInfo_s_.SUB_N_PX * subFieldInfo_s_.SUB_N_PY;
-  subFieldInfo_s_.RtRank = subFieldInfo_s_.LtRank = subFieldInfo_s_.TpRank = subFieldInfo_s_.BmRank = -1;
 
   N_PX = fieldInfo_s_.N_PX;
   N_PY = fieldInfo_s_.N_PY;

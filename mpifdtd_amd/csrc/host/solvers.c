@@ -588,42 +588,15 @@ static void mpi_te_ntff_output(void)
   }
 }
 
-/* field_outputElliptic (field.c:322-341) of Ez (TM) or Ey (TE): |.|^2 interpolated
-   bilinearly (myComplex.c:40-51) on the circle of radius 1.2 lambda around the grid
-   centre, the split-field solvers' reset output (fdtdTM.c:167-173, fdtdTE.c:269-274) */
+/* the split-field and NS solvers' reset output (fdtdTM.c:167-173, fdtdTE.c:269-274,
+   nsFdtdTM.c:160-164, nsFdtdTE.c:231-235): field_outputElliptic of Ez (TM) or Ey (TE) */
 static void write_elliptic(int kind)
 {
   FieldInfo fi = field_getFieldInfo();
   char name[128];
   const char *prefix = kind == SLV_NS_TM ? "ns_tm" : kind == SLV_NS_TE ? "ns_te" : kind == SLV_SPLIT_TM ? "tm" : "te";
   snprintf(name, sizeof name, "%s_%dnm.txt", prefix, fi.h_u_nm);
-  const double complex *ez = solver_field(kind, is_tm(kind) ? 0 : 1);
-  const double lam = field_getLambda();
-  printf("output start\n");
-  FILE *fp = fopen(name, "w");
-  if (!fp) {
-    printf("cannot open file %s \n", name);
-    exit(2);
-  }
-  for (int ang = 180; ang >= 0; ang--) {
-    double rad = ang * M_PI / 180.0;
-    double x = 1.2 * lam * cos(rad) + N_PX / 2.0;
-    double y = 1.2 * lam * sin(rad) + N_PY / 2.0;
-    int i = (int)floor(x), j = (int)floor(y);
-    if (i < 0 || j < 0 || i + 1 >= N_PX || j + 1 >= N_PY) {
-      /* the circle leaves the grid (sides under 2.4 lambda): the reference reads
-         outside its array here; 0 is written instead */
-      fprintf(fp, "%d %lf \n", 180 - ang, 0.0);
-      continue;
-    }
-    double dx = x - i, dy = y - j;
-    size_t k = (size_t)i * N_PY + j;
-    double complex v = ez[k] * (1.0 - dx) * (1.0 - dy) + ez[k + N_PY] * dx * (1.0 - dy) + ez[k + 1] * (1.0 - dx) * dy +
-                       ez[k + N_PY + 1] * dx * dy;
-    fprintf(fp, "%d %lf \n", 180 - ang, creal(v) * creal(v) + cimag(v) * cimag(v));
-  }
-  fclose(fp);
-  printf("output to %s end\n", name);
+  field_outputElliptic(name, (fdtd_dcomplex *)solver_field(kind, is_tm(kind) ? 0 : 1));
 }
 
 static void solver_reset(int kind)

@@ -26,6 +26,13 @@
  *            simulator_calc timed with CLOCK_MONOTONIC; prints one JSON line
  *            {"N_PX", "N_PY", "steps", "seconds"} and writes nothing.  The CPU
  *            baseline of bench.py (the reference timed beside the GPU run).
+ * mode=util: field_init, `steps` x field_nextStep, then field.h / myComplex.h
+ *            helpers on fixed inputs: eps.bin, base.bin (the inputs), src.bin
+ *            (field_scatteredWave, _NotUPML, field_nsScatteredWaveNotUPML and
+ *            field_scatteredPulse each added to a copy of base), scal.bin
+ *            (pmlCoef, _LXY, ns_beta, ns_beta_aster on a 3 x 3 set), bil.bin /
+ *            nrm.bin (cbilinear, cnorm), meta.txt (subdomain getters, ind), and
+ *            ell.txt, all_c.txt, all_d.txt from the three writers.
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -86,6 +93,61 @@ int main(int argc, char **argv)
   MPI_Init(0, 0);
   models_setModel((enum MODEL)model);
   for (int k = 0; k < sweep; k++) models_isFinish();
+
+  if (strcmp(mode, "util") == 0) {
+    field_init(fi);
+    for (int t = 0; t < fi.stepNum; t++) field_nextStep();
+    FieldInfo_S s = field_getFieldInfo_S();
+    size_t n = (size_t)s.N_PX * s.N_PY;
+    double *eps = malloc(n * sizeof(double));
+    double complex *base = malloc(n * sizeof(double complex)), *src = malloc(4 * n * sizeof(double complex));
+    for (int x = 0; x < s.N_PX; x++)
+      for (int y = 0; y < s.N_PY; y++) {
+        size_t k = (size_t)x * s.N_PY + y;
+        eps[k] = ((x * 7 + y * 3) % 5 == 0) ? 1.0 : 1.0 + ((x * 13 + y * 7) % 11) / 4.0;
+        base[k] = (x - 0.5 * y) + I * (0.25 * x * y / s.N_PX);
+      }
+    for (int q = 0; q < 4; q++) memcpy(src + q * n, base, n * sizeof(double complex));
+    field_scatteredWave(src, eps, 0.5, 0);
+    field_scatteredWaveNotUPML(src + n, eps, 0, 0.5);
+    field_nsScatteredWaveNotUPML(src + 2 * n, eps, 0, 0.5, 0.6);
+    field_scatteredPulse(src + 3 * n, eps, 0.5, 0, 0.8);
+    double scal[36];
+    const double av[3] = {0, 0.3, 1.7}, bv[3] = {0.2, 1.0, 2.5};
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) {
+        double *o = scal + 4 * (3 * a + b);
+        o[0] = field_pmlCoef(1 + av[a], bv[b]);
+        o[1] = field_pmlCoef_LXY(1 + av[a], bv[b]);
+        o[2] = field_ns_beta(av[a], bv[b]);
+        o[3] = field_ns_beta_aster(av[a], bv[b]);
+      }
+    const double px[5] = {1.0, 2.5, 10.25, s.N_PX / 2.0 + 0.3, s.N_PX - 2.5};
+    const double py[5] = {1.0, 3.75, 20.5, s.N_PY / 2.0 - 0.7, s.N_PY - 2.25};
+    double complex bil[5];
+    double nrm[5];
+    for (int q = 0; q < 5; q++) {
+      bil[q] = cbilinear(base, px[q], py[q], s.N_PX, s.N_PY);
+      nrm[q] = cnorm(bil[q]);
+    }
+    dump(out, "eps.bin", eps, n * sizeof(double));
+    dump(out, "base.bin", base, n * sizeof(double complex));
+    dump(out, "src.bin", src, 4 * n * sizeof(double complex));
+    dump(out, "scal.bin", scal, sizeof scal);
+    dump(out, "bil.bin", bil, sizeof bil);
+    dump(out, "nrm.bin", nrm, sizeof nrm);
+    char meta[256];
+    int len = snprintf(meta, sizeof meta, "%d %d %d %d %d %d %d %d %d %d\n", s.N_PX, s.N_PY, field_getOffsetX(),
+                       field_getOffsetY(), field_getSubNx(), field_getSubNy(), field_getSubNpx(), field_getSubNpy(),
+                       field_getSubNcell(), ind(3, 4));
+    dump(out, "meta.txt", meta, len);
+    if (chdir(out) != 0) { perror(out); return 3; }
+    field_outputElliptic("ell.txt", src);
+    field_outputAllDataComplex("all_c.txt", src + n);
+    field_outputAllDataDouble("all_d.txt", eps);
+    MPI_Finalize();
+    return 0;
+  }
 
   if (strcmp(mode, "eps") == 0) {
     field_init(fi);

@@ -103,6 +103,28 @@ def write_main_default(tmp):
     print("wrote main_default", {k: v.shape for k, v in d.items()})
 
 
+def write_field_util(tmp):
+    """The reference's field.h / myComplex.h helpers on the harness's fixed inputs
+    (mode=util): a 90 x 100 grid (h = 20 nm), lambda 450 nm, angle 30, after 480
+    field_nextStep calls (the Gaussian pulse near its peak)."""
+    out = os.path.join(tmp, "field_util")
+    run(["mode=util", "width=1800", "height=2000", "h=20", "pml=8", "lambda=450", "angle=30", "steps=480",
+         f"out={out}"])
+    meta = [int(v) for v in open(os.path.join(out, "meta.txt")).read().split()]
+    npx, npy = meta[0], meta[1]
+    d = dict(cfg=np.array([1800, 2000, 20, 8, 450, 30, 480]), meta=np.array(meta),
+             eps=np.fromfile(os.path.join(out, "eps.bin"), np.float64).reshape(npx, npy),
+             base=np.fromfile(os.path.join(out, "base.bin"), np.complex128).reshape(npx, npy),
+             src=np.fromfile(os.path.join(out, "src.bin"), np.complex128).reshape(4, npx, npy),
+             scal=np.fromfile(os.path.join(out, "scal.bin"), np.float64).reshape(9, 4),
+             bil=np.fromfile(os.path.join(out, "bil.bin"), np.complex128),
+             nrm=np.fromfile(os.path.join(out, "nrm.bin"), np.float64))
+    for nm in ("ell", "all_c", "all_d"):
+        d[nm] = np.array(open(os.path.join(out, f"{nm}.txt")).read())
+    np.savez_compressed(os.path.join(HERE, "ref_field_util.npz"), **d)
+    print("wrote field_util", {k: v.shape for k, v in d.items()})
+
+
 def main(only=()):
     """Writes every fixture, or only the named ones."""
     subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True, capture_output=True)
@@ -143,6 +165,8 @@ def main(only=()):
             print("wrote", name, {k: v.shape for k, v in d.items()})
         if not only or "main_default" in only:
             write_main_default(tmp)
+        if not only or "field_util" in only:
+            write_field_util(tmp)
         for (name, model, sweep, w, h_nm, hu) in EPS_CASES:
             if only and name not in only:
                 continue
