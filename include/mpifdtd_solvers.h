@@ -119,6 +119,15 @@ double *nsFdtdTM_getEps(void);
 struct Solver;
 struct Solver *nsFdtdTM_getSolver(void);
 struct Solver *nsFdtdTE_getSolver(void);
+/* nsFdtdTM.c:35-37 / nsFdtdTE.c:33-35 define these without a header declaration:
+ * TM X EPS_HX, Y EPS_HY, Z EPS_EZ; TE X EPS_EX, Y EPS_EY, Z EPS_HZ (the TE maps
+ * with the reference's zero border).  Each is a host copy of its own. */
+double *nsFdtdTM_getEpsX(void);
+double *nsFdtdTM_getEpsY(void);
+double *nsFdtdTM_getEpsZ(void);
+double *nsFdtdTE_getEpsX(void);
+double *nsFdtdTE_getEpsY(void);
+double *nsFdtdTE_getEpsZ(void);
 
 /* nsFdtdTE.h:6-19 (non-standard FDTD TEz; getEps returns EPS_EY; nsFdtdTE_getSolver
  * is declared with nsFdtdTM_getSolver above). */

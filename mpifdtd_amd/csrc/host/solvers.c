@@ -37,6 +37,7 @@ typedef struct {
   double complex *mirror[3];  /* TM: Ez Hx Hy   TE: Ex Ey Hz */
   double complex *split_mirror[2];  /* split-field and NS TM: Ezx Ezy   TE: Hzx Hzy */
   double *eps_mirror;
+  double *eps_maps[3];  /* the NS solvers' getEpsX / Y / Z: one host copy per map */
   int blocked;    /* FP32 grid: steps are queued and run T per launch */
   int ntff_time;  /* the per-step time-domain NTFF runs (its outputs exist) */
   int ndft;       /* frequencies of the running DFT (fdtd_solver_setDFT) */
@@ -305,6 +306,7 @@ static void solver_free(int kind)
   for (int q = 0; q < 3; q++) free(s->mirror[q]);
   for (int q = 0; q < 2; q++) free(s->split_mirror[q]);
   free(s->eps_mirror);
+  for (int q = 0; q < 3; q++) free(s->eps_maps[q]);
   memset(s, 0, sizeof *s);
   if (active_ == kind) active_ = -1;
 }
@@ -782,6 +784,25 @@ fdtd_dcomplex *nsFdtdTE_getHz(void) { return (fdtd_dcomplex *)solver_field(SLV_N
 fdtd_dcomplex *nsFdtdTE_getEy(void) { return (fdtd_dcomplex *)solver_field(SLV_NS_TE, 1); }
 fdtd_dcomplex *nsFdtdTE_getEx(void) { return (fdtd_dcomplex *)solver_field(SLV_NS_TE, 0); }
 double *nsFdtdTE_getEps(void) { return solver_eps(SLV_NS_TE); }
+
+/* one eps map of an NS solver in a host copy of its own (TM 0 EPS_EZ, 1 EPS_HX,
+   2 EPS_HY; TE 0 EPS_EX, 1 EPS_EY, 2 EPS_HZ), refreshed on every call */
+static double *ns_eps_map(int kind, int which)
+{
+  solver_t *s = &slv_[kind];
+  if (!s->g) return NULL;
+  if (!s->eps_maps[which]) s->eps_maps[which] = (double *)calloc((size_t)s->NPXa * s->NPYa, sizeof(double));
+  CHECK(fdtd_hip_grid_get_eps(s->g, which, s->eps_maps[which]));
+  return s->eps_maps[which];
+}
+/* nsFdtdTM.c:35-37 and nsFdtdTE.c:33-35 define these without declaring them in
+   their headers */
+double *nsFdtdTM_getEpsX(void) { return ns_eps_map(SLV_NS_TM, 1); }
+double *nsFdtdTM_getEpsY(void) { return ns_eps_map(SLV_NS_TM, 2); }
+double *nsFdtdTM_getEpsZ(void) { return ns_eps_map(SLV_NS_TM, 0); }
+double *nsFdtdTE_getEpsX(void) { return ns_eps_map(SLV_NS_TE, 0); }
+double *nsFdtdTE_getEpsY(void) { return ns_eps_map(SLV_NS_TE, 1); }
+double *nsFdtdTE_getEpsZ(void) { return ns_eps_map(SLV_NS_TE, 2); }
 
 /* ---------------------------------------------------------------- plain TE */
 static void te_init(void) { solver_init(SLV_TE); }

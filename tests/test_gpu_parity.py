@@ -211,6 +211,16 @@ def test_split_field_parity(name, tmp_path, monkeypatch):
             assert f[k].shape == g[f"f{k}"].shape
             assert rel_l2(f[k], g[f"f{k}"]) <= TOL, (name, k, rel_l2(f[k], g[f"f{k}"]))
         assert np.array_equal(sim.eps(), g["eps"]), name
+        if solver in (po.NS_TM_2D, po.NS_TE_2D):
+            # getEpsX / Y / Z: TM EPS_HX, EPS_HY, EPS_EZ; TE EPS_EX, EPS_EY, EPS_HZ
+            import ctypes
+            L = m.lib()
+            pre, order = ("nsFdtdTM", (1, 2, 0)) if tm else ("nsFdtdTE", (0, 1, 2))
+            for ax, which in zip("XYZ", order):
+                f = getattr(L, f"{pre}_getEps{ax}")
+                f.restype = ctypes.c_void_p
+                got = m._native.as_array(f(), (sim.NPXa, sim.NPYa), np.float64)
+                assert np.array_equal(got, o.eps(which)), (name, ax)
         if "ff" in g:
             assert rel_l2(sim.far_field(), g["ff"]) <= TOL
         # TM: Ez, Ezx, Ezy, Hx, Hy; TE: Ex, Ey, Hz, Hzx, Hzy
