@@ -1,28 +1,39 @@
 """Benchmark of the FP32 fused UPML step through the reference's driver API.
 
-Default workload: Mie-cylinder scattering (circleModel.c, r = 500 nm, n = 1.6),
-1024 x 1024 cells + 16-cell UPML (1056 x 1056) at h = 5 nm, lambda = 550 nm,
-MPI TM solver, 8000 steps.  One step = one FDTD time step of the whole grid (E
-and H halves, UPML, scattered-wave source).  The timed region is K steps of
-simulator_calc through libmpifdtd_amd.so, whose FP32 path advances T steps per
-launch; inputs are resident in HBM.  The far field (360 angles) of the final
-fields is formed in FP64 and compared with the FP64 reference-order GPU path,
-which the test suite pins to the CPU oracle at 1e-10.
+Default workload: the Morpho-scale wavelength sweep on the large grid, the
+throughput benchmark.  The Morpho model (morphoScaleModel.c, default
+parameters) on 8192 x 4096 cells + a 32-cell UPML (8256 x 4160, 34.3 M cells,
+h = 5 nm: the integer-nm twin of 2.5 nm with every length doubled), normal
+incidence (+y), 20000 steps per run.  The 81 wavelengths 380-780 nm (every
+5 nm) come from one run of the reference's plain pulse solver (fdtdTM_upml.c /
+fdtdTE_upml.c: a Gaussian plane-wave pulse whose band covers the sweep) with
+the running DFT of the NTFF contour at the 81 wavelengths, instead of 81 CW
+runs.  One step = one FDTD time step of the whole grid (H and E halves, UPML
+auxiliaries, pulse source) plus the contour samples and their DFT update.  TM
+is the headline; TE is reported under "te".
 
---workload morpho8256 runs the large HBM-bound case instead: the Morpho-scale
-model on 8256 x 4160 cells (h = 5 nm, 32-cell UPML), 34.3 M cells, whose FP32
-state (1.6 GB) is far beyond the 256 MB Infinity Cache.
+The timed region is K steps (default 20000, a whole run) in whole blocks of T
+steps (T divides K), on inputs resident in HBM; the host copies nothing inside
+it.  roofline.achieved is the HBM traffic of one block's step launches, from
+the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of exactly this launch
+configuration under profiles/, divided by the block's event-timed duration in
+this run; the algorithmic rate (what an unblocked step would have to move) is
+reported beside it as "effective".  The DFT and far-field spectra of the FP32
+run are compared with the FP64 reference-order path on the same grid after
+`--check-steps` steps.  The CPU baseline is the reference's own plain TM UPML
+step, compiled from its sources (oracle/_ref/ref_harness), timed on one host
+core and on all host cores before the GPU is touched.
 
---workload sweep64 runs a batched sweep: 64 seeded instances (multilayer or
-zigzag, sides 512 ... 2048 cells + 16-cell UPML, TM or TE, lambda 380-780 nm,
-6000-10000 steps each) advanced together to completion; value is the aggregate
-cell-update rate and spectra_per_s the completed instances per second.
+Other workloads (--workload): mie1056 (the Mie cylinder, 1056 x 1056, MPI
+solver), morpho8256 (the same Morpho grid with the CW MPI solver), sweep64 (64
+seeded batched instances), solvers_fp64 (every solver of enum SOLVER in FP64).
 
-    python bench.py [--steps K] [--warmup W] [--workload mie1056|morpho8256] [--mode tm|te]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ...] [--mode tm|te|both]
 prints one JSON line.
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -31,6 +42,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "Mcell-updates/s + % HBM roofline; spectra/s at <=1e-4 far-field err"
 # launch shapes of the UPML and interior tiles (FAST_LAUNCH_* of fdtd_hip.h)
 SHAPES = ["auto", "merged", "split", "two_streams", "split_pml8", "two_streams_pml8", "split_w8", "two_streams_w8",
           "split_p16", "two_streams_p16"]
@@ -43,57 +55,75 @@ WORKLOADS = {
                    "Morpho scale {mode}, 8192x4096 + 32-cell UPML (8256x4160), h=5 nm, lambda=550 nm, "
                    "normal incidence (+y)"),
 }
+SWEEP_LAMBDAS = (380.0, 781.0, 5.0)  # 81 wavelengths
+RUN_STEPS = 20000  # steps of one run of the sweep
+# Launch configuration of the sweep per mode: (T, R, launch shape), the
+# fastest of `--autotune`'s candidates on this grid.  T divides the default K.
+SWEEP_CONFIG = {"tm": (10, 16, 7), "te": (10, 16, 6)}
+T_CANDIDATES = (10, 8, 5, 4, 2, 1)
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=0, help="timed steps (default 8000; 400 for morpho8256)")
-    ap.add_argument("--warmup", type=int, default=-1, help="untimed steps (default 400; 40 for morpho8256)")
-    ap.add_argument("--workload", default="mie1056",
+    ap.add_argument("--steps", type=int, default=0,
+                    help="timed steps (default: 20000 for the sweep, 8000 mie1056, 400 morpho8256)")
+    ap.add_argument("--warmup", type=int, default=-1, help="untimed steps (default 100; mie1056 400)")
+    ap.add_argument("--workload", default="morpho8256_sweep81",
                     choices=sorted(WORKLOADS) + ["sweep64", "morpho8256_sweep81", "solvers_fp64"])
     ap.add_argument("--seed", type=int, default=1, help="sweep64: generator seed")
     ap.add_argument("--precision", default="fp32", choices=("fp32", "fp64"))
-    ap.add_argument("--mode", default="tm", choices=("tm", "te"))
-    ap.add_argument("--T", type=int, default=0, help="temporal block depth (0: autotune)")
-    ap.add_argument("--R", type=int, default=0, help="rows per wavefront (0: autotune)")
+    ap.add_argument("--mode", default="", choices=("", "tm", "te", "both"),
+                    help="TM, TE or both (the sweep's default: both, TM the headline)")
+    ap.add_argument("--T", type=int, default=0, help="temporal block depth (0: the workload's default)")
+    ap.add_argument("--R", type=int, default=0, help="rows per wavefront (0: the workload's default)")
     ap.add_argument("--launch", type=int, default=-1,
                     help="launch shape of the UPML and interior tiles: 0 auto, 1 merged, 2 split, "
                          "3 split on two streams, 4 / 5: 2 / 3 with 8-wavefront UPML workgroups, "
-                         "6 / 7: 4 / 5 with 8-wavefront interior workgroups "
-                         "(-1: autotune's choice, or auto)")
+                         "6 / 7: 4 / 5 with 8-wavefront interior workgroups, 8 / 9: 6 / 7 with "
+                         "16-wavefront UPML workgroups (-1: the workload's default)")
+    ap.add_argument("--autotune", action="store_true", help="time the candidate configurations first and use the best")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--ntff-overlap", type=int, default=-1,
                     help="contour sample work beside the next block (1), after it (0), by grid size (-1)")
-    ap.add_argument("--no-check", action="store_true", help="skip the FP64 far-field check")
+    ap.add_argument("--no-check", action="store_true", help="skip the FP64 comparison")
+    ap.add_argument("--check-steps", type=int, default=8000,
+                    help="sweep: steps of the FP32 / FP64 comparison run (the scattered pulse has crossed every "
+                         "contour side by then)")
     ap.add_argument("--ntff", action="store_true",
-                    help="run the reference's per-step time-domain NTFF inside the timed steps (the MPI solvers "
-                         "do it every step; default off, the far field then comes from the final fields)")
+                    help="mie1056 / morpho8256: the reference's per-step time-domain NTFF inside the timed steps")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
-                    help="only run a plain device-to-device copy of MIB MiB (5 times): the reference kernel "
-                         "that scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters to bytes")
+                    help="only run the library's streaming device copy of MIB MiB (10 times): the HBM ceiling, "
+                         "and the kernel that scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters to bytes")
     a = ap.parse_args()
-    big = a.workload != "mie1056"
+    sweep = a.workload == "morpho8256_sweep81"
     a.steps_given = a.steps > 0
     if a.steps <= 0:
-        a.steps = 400 if big else 8000
+        a.steps = RUN_STEPS if sweep else (8000 if a.workload == "mie1056" else 400)
     if a.warmup < 0:
-        a.warmup = 40 if big else 400
+        a.warmup = 400 if a.workload == "mie1056" else (100 if sweep else 40)
+    if not a.mode:
+        a.mode = "both" if sweep else "tm"
+    if a.mode == "both" and not sweep:
+        sys.exit("--mode both is for the sweep workload")
     return a
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def copy_calibration(mib):
-    # a plain device copy of known size, for scaling the PMC byte counters (its
-    # own process: torch's runtime only, the library is not loaded)
-    import torch
-    n = mib * (1 << 20) // 4
-    a = torch.ones(n, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    torch.cuda.synchronize()
-    for _ in range(5):
-        b.copy_(a)
-    torch.cuda.synchronize()
-    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 5}), flush=True)
+    """The library's streaming copy kernel (k_stream_copy): the achievable HBM
+    rate, and the plain copy of known size whose FETCH_SIZE / WRITE_SIZE
+    counters scale the step kernels' counters to bytes."""
+    import ctypes
+    import mpifdtd_amd as m
+    out = (ctypes.c_double * 4)()
+    m.check(m.lib().fdtd_hip_copy_rate(mib << 20, 10, out), "fdtd_hip_copy_rate")
+    print(json.dumps({"copy_bytes_read": (mib << 20), "copy_bytes_written": (mib << 20), "launches": 11,
+                      "best_GBs": round(out[0], 1), "mean_GBs": round(out[1], 1), "mean_us": round(out[2], 2)}),
+          flush=True)
 
 
 def make_sim(m, precision, mode, steps, T=4, R=16, workload="mie1056", ntff=False):
@@ -111,24 +141,84 @@ def prof_read(L):
     return list(out)
 
 
-def cpu_reference(mode, workload, steps=120):
-    """The reference itself on a bounded sample of the same workload: its MPI-TM /
-    TE UPML solver compiled from its own sources (oracle/_ref/ref_harness, one MPI
-    rank, one core), `steps` simulator_calc() calls including the per-step NTFF,
-    timed inside the harness.  Runs as a child process before this process touches
-    the GPU.  None when the harness is not built or does not run."""
+# ---------------------------------------------------------------- CPU baseline
+
+def _harness(args, timeout):
     import subprocess
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    r = subprocess.run([exe] + args, capture_output=True, text=True, timeout=timeout, cwd="/tmp")
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def cpu_reference_sweep():
+    """The reference's own plain TM UPML step (fdtdTM_upml.c update: calcMB,
+    calcH, calcJD, calcE, field_scatteredPulse, ntffTM_TimeCalc), compiled from
+    its sources with gcc -O2 (oracle/_ref/ref_harness, one MPI rank per
+    process), Morpho model, h = 5 nm, 32-cell UPML, normal incidence.  Its
+    allocation, eps fill and coefficients run before the timed window.
+
+    On the 8256 x 4160 grid itself the reference's per-step NTFF indexes its
+    bins before their start (RFperC comes from the y extent only, field.c:140-142,
+    so the retarded-time shift of the far x side is about -1700 bins) and the
+    process dies with a segmentation fault; the one-core sample therefore runs a
+    square 5860 x 5860 grid of the same cell count (34.3 M).  All cores: the
+    reference's rank-per-simulation mode (main.c:114-138), one independent
+    simulation per core, each of 1/ncores of those cells (square), run
+    together.  Runs as child processes before this process touches the GPU.
+    Returns None when the harness is not built or does not run."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    if not os.access(exe, os.X_OK):
+        return None
+    import subprocess
+    try:
+        ncores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncores = os.cpu_count() or 1
+    ncores = max(1, min(16, ncores))
+    base = ["mode=time", "solver=2", "model=3", "h=5", "pml=32", "lambda=550", "angle=90", "warm=1"]
+    side1 = 5860
+    try:
+        log("cpu baseline: the reference's plain TM UPML step, 5860 x 5860, one core")
+        r1 = _harness(base + [f"width={side1 * 5}", f"height={side1 * 5}", "steps=8"], 400)
+        one = r1["N_PX"] * r1["N_PY"] * r1["steps"] / r1["seconds"] / 1e6
+        side_n = int(round(side1 / math.sqrt(ncores)))
+        log(f"cpu baseline: {ncores} ranks of {side_n} x {side_n}, together")
+        procs = [subprocess.Popen([exe] + base + [f"width={side_n * 5}", f"height={side_n * 5}", "steps=40"],
+                                  stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True, cwd="/tmp")
+                 for _ in range(ncores)]
+        outs = [p.communicate(timeout=400)[0] for p in procs]
+        recs = [json.loads(o.strip().splitlines()[-1]) for o in outs]
+        agg = sum(r["N_PX"] * r["N_PY"] * r["steps"] / r["seconds"] for r in recs) / 1e6
+        secs = max(r["seconds"] for r in recs)
+    except (OSError, ValueError, IndexError, KeyError, subprocess.SubprocessError) as e:
+        log(f"cpu baseline failed: {e}")
+        return None
+    return {"value": round(agg, 3), "unit": "Mcell-updates/s", "cores": ncores, "kind": "reference",
+            "value_1core": round(one, 3),
+            "sample": f"the reference's plain TM UPML step (with its per-step time-domain NTFF) built from its "
+                      f"sources with gcc -O2; all cores: {ncores} independent simulations of {side_n}x{side_n} "
+                      f"cells together (rank per simulation, main.c:114-138), 40 steps each ({secs:.1f} s); "
+                      f"one core: {r1['N_PX']}x{r1['N_PY']} (34.3 M cells, the 8256x4160 grid's count; the "
+                      f"reference's NTFF bins overflow on the non-square grid itself), {r1['steps']} steps "
+                      f"({r1['seconds']:.1f} s); Morpho model, h=5 nm, 32-cell UPML, eps fill outside the window"}
+
+
+def cpu_reference(mode, workload, steps=120):
+    """mie1056: the reference's MPI-TM / TE UPML solver compiled from its own
+    sources (oracle/_ref/ref_harness, one MPI rank, one core), `steps`
+    simulator_calc() calls including the per-step NTFF, timed inside the
+    harness, as a child process before this process touches the GPU."""
     exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
     if not os.access(exe, os.X_OK):
         return None
     model, w, h, hu, pml, lam, ang, params, _ = WORKLOADS[workload]
     if model != "MIE_CYLINDER" or params != {"circle_radius_nm": 500, "circle_n": 1.6}:
         return None  # the reference's cylinder is r = 500 nm, n = 1.6 (circleModel.c:6-10)
-    cmd = [exe, "mode=time", "solver=%d" % (4 if mode == "tm" else 5), "model=1", f"width={w}", f"height={h}",
-           f"h={hu}", f"pml={pml}", f"lambda={lam}", f"angle={ang}", f"steps={steps}", "warm=2"]
+    import subprocess
     try:
-        r = subprocess.run(cmd, capture_output=True, text=True, timeout=240)
-        res = json.loads(r.stdout.strip().splitlines()[-1])
+        res = _harness(["mode=time", "solver=%d" % (4 if mode == "tm" else 5), "model=1", f"width={w}",
+                        f"height={h}", f"h={hu}", f"pml={pml}", f"lambda={lam}", f"angle={ang}", f"steps={steps}",
+                        "warm=2"], 240)
     except (OSError, ValueError, IndexError, subprocess.SubprocessError):
         return None
     cells = res["N_PX"] * res["N_PY"]
@@ -139,30 +229,13 @@ def cpu_reference(mode, workload, steps=120):
                       f"sources with gcc -O2, one MPI rank ({res['seconds']:.1f} s on one host core)"}
 
 
-def cpu_baseline(mode, workload):
-    """The FP64 CPU oracle (the reference's MPI-TM/TE step restated in C, gcc -O2,
-    one core) on a bounded sample of the same workload, including the per-step
-    time-domain NTFF that every reference step performs."""
-    from oracle import pyoracle as po
-    model, w, h, hu, pml, lam, ang, params, _ = WORKLOADS[workload]
-    solver = po.MPI_TM_UPML if mode == "tm" else po.MPI_TE_UPML
-    o = po.Sim(width_nm=w, height_nm=h, h_u_nm=hu, pml=pml, lambda_nm=lam, angle_deg=ang, stepNum=8000,
-               solver=solver, model=getattr(po, model), ntff_time=1, **params)
-    o.step(2)
-    cells = o.N_PX * o.N_PY
-    n = max(4, int(130e6 // cells))  # about 5 s of one core
-    t0 = time.perf_counter()
-    o.step(n)
-    dt = time.perf_counter() - t0
-    return {"value": round(cells * n / dt / 1e6, 3), "unit": "Mcell-updates/s", "cores": 1, "kind": "port",
-            "sample": f"{n} steps of the same {o.N_PX}x{o.N_PY} {workload} {mode.upper()} grid, FP64, with the "
-                      f"per-step NTFF ({dt:.1f} s on one host core; gcc -O2 oracle)"}
+# ---------------------------------------------------------------- roofline
 
-
-def measured_traffic(mode, T, R, cells, launch):
-    """HBM bytes per launch of this configuration (launch shape included) from
-    the committed rocprofv3 counter summaries (FETCH_SIZE / WRITE_SIZE scaled by
-    a plain copy), or None."""
+def measured_traffic(key):
+    """HBM bytes per block of T steps (the step launches of one block) of the
+    launch configuration `key` from the committed rocprofv3 counter summaries
+    (FETCH_SIZE / WRITE_SIZE scaled by the same counters on a plain copy of
+    known size), and the summary's path; (None, None) when none matches."""
     import glob
     import re
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*traffic*.md"))):
@@ -171,46 +244,73 @@ def measured_traffic(mode, T, R, cells, launch):
                 rec = json.loads(block)
             except ValueError:
                 continue
-            if (rec.get("mode"), rec.get("T"), rec.get("R"), rec.get("cells"), rec.get("launch")) == \
-                    (mode, T, R, cells, launch):
+            if all(rec.get(k) == v for k, v in key.items()):
                 return rec["traffic_bytes_per_launch"], os.path.relpath(path, ROOT)
     return None, None
 
 
-def log(msg):
-    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+def roofline(prof, key, T, kernel):
+    """The roofline object of the step launches: `achieved` = counter HBM bytes
+    per block / event-timed block duration (null without a matching counter
+    summary), `effective` = the algorithmic bytes of the block's T steps (one
+    read and one write of every field per step) over the same duration."""
+    ms_k, _, nlaunch, b_int, b_pml, cupd = prof
+    if not (nlaunch > 0 and ms_k > 0):
+        return {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
+                "traffic": None}
+    block_s = ms_k * 1e-3 / nlaunch
+    alg_block = (b_int + b_pml) / nlaunch  # T steps' algorithmic bytes
+    traffic, src = measured_traffic(key)
+    achieved = traffic / block_s / 1e9 if traffic else None
+    frac = achieved / HBM_PEAK_GBS if achieved else None
+    note = None
+    if frac is not None and frac > 1.0:  # not a physical rate: do not publish it
+        note = f"counter rate {achieved:.0f} GB/s above peak: not published"
+        achieved = frac = None
+    out = {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
+           "unit": "GB/s", "frac": round(frac, 4) if frac else None, "traffic": traffic, "traffic_source": src,
+           "kernel": kernel, "launch_avg_us": round(block_s * 1e6, 2), "steps_per_launch": T,
+           "launches_timed": int(nlaunch),
+           "compulsory_bytes_per_launch": round(alg_block / T),
+           "traffic_over_compulsory": round(traffic / (alg_block / T), 3) if traffic else None,
+           "effective": {"GBs": round(alg_block / block_s / 1e9, 1),
+                         "bytes_per_launch": round(alg_block),
+                         "bytes_per_cell_update": round((b_int + b_pml) / cupd, 2) if cupd else None,
+                         "note": "algorithmic bytes of the block's T steps (52/56 B per interior cell-update, "
+                                 "148/152 B per UPML cell-update) over its duration: the rate an unblocked "
+                                 "kernel would need, not HBM traffic"}}
+    if note:
+        out["note"] = note
+    return out
 
 
-def run_sweep81(args, m):
-    """Config 4's wavelength sweep (380-780 nm every 5 nm: 81 wavelengths) on the
-    8256 x 4160 Morpho-scale grid as one run: the reference's plain pulse solver
-    (TM_UPML_2D / TE_UPML_2D, a Gaussian plane-wave pulse at normal incidence
-    whose band covers the sweep) with the running DFT of the NTFF contour at the
-    81 wavelengths.  Each wavelength's far-field spectrum (the reference's
-    |Eth|^2 / 8192 or |Eph|^2 / 8192, ntffTM.c:197-233) comes from the DFT, the
-    back-hemisphere angles 180-359 being its reflectance."""
-    import numpy as np
+# ---------------------------------------------------------------- the sweep
+
+def pick_T(T, K):
+    """T itself when it divides K, else the deepest candidate that does (the
+    timed region holds whole blocks only)."""
+    if K % T == 0:
+        return T
+    for c in T_CANDIDATES:
+        if c <= T and K % c == 0:
+            return c
+    return 1
+
+
+def tune_sweep(m, mode, lams):
+    """Time the candidate (T, R, launch shape) on the sweep grid with the DFT on."""
     L = m.lib()
     model, w, h, hu, pml, lam, ang, params, _ = WORKLOADS["morpho8256"]
-    lams = np.arange(380.0, 781.0, 5.0)
-    steps = args.steps if args.steps_given else 20000
-    solver = m.TM_UPML_2D if args.mode == "tm" else m.TE_UPML_2D
-    T, R = args.T or 8, args.R or 16
-    launch = args.launch if args.launch >= 0 else 3
-
-    def make(precision, n):
-        return m.Simulator(solver, getattr(m, model), m.field_info(w, h, hu, pml, lam, ang, n),
-                           params=m.model_params(**params), ntff_time=False, precision=precision, T=T, R=R)
-
+    solver = m.TM_UPML_2D if mode == "tm" else m.TE_UPML_2D
+    probe = m.Simulator(solver, getattr(m, model), m.field_info(w, h, hu, pml, lam, ang, 4000),
+                        params=m.model_params(**params), ntff_time=False, precision="fp32", T=10, R=16)
     tune = {}
-    if not (args.T and args.R and args.launch >= 0):
-        # a short tune of depth, rows and launch shape on the same grid with the DFT on
-        probe = make("fp32", 4000)
+    try:
         probe.set_dft(lams)
         probe.calc(40)
-        for Tc, Rc in ((8, 16), (10, 16), (8, 8), (6, 8)):
+        for Tc, Rc in ((10, 16), (8, 16), (10, 8), (8, 8)):
             for sh in (2, 3, 4, 5, 6, 7, 8, 9):
-                n = 6 * Tc  # whole blocks only
+                n = 6 * Tc
                 probe.set_blocking(Tc, Rc)
                 probe.set_launch_mode(sh)
                 probe.calc(2 * Tc)
@@ -218,83 +318,139 @@ def run_sweep81(args, m):
                 L.fdtd_hip_synchronize()
                 t0 = time.perf_counter()
                 probe.calc(n)
-                probe.set_blocking(Tc, Rc)
                 L.fdtd_hip_synchronize()
                 tune[(Tc, Rc, sh)] = (time.perf_counter() - t0) * 1e6 / n
+    finally:
         probe.finish()
+    return tune
+
+
+def run_sweep81_mode(args, m, mode):
+    import numpy as np
+    L = m.lib()
+    model, w, h, hu, pml, lam, ang, params, _ = WORKLOADS["morpho8256"]
+    lams = np.arange(*SWEEP_LAMBDAS)
+    solver = m.TM_UPML_2D if mode == "tm" else m.TE_UPML_2D
+    K, W = args.steps, args.warmup
+    T, R, launch = SWEEP_CONFIG[mode]
+    tune = {}
+    if args.autotune:
+        log(f"sweep81 {mode}: autotune")
+        tune = tune_sweep(m, mode, lams)
         T, R, launch = min(tune, key=tune.get)
-    log(f"sweep81: {len(lams)} wavelengths, {steps} steps, T={T} R={R} launch={launch}")
-    sim = make("fp32", steps)
-    cells = sim.N_PX * sim.N_PY
-    sim.set_launch_mode(launch)
-    sim.set_ntff_overlap(args.ntff_overlap)
-    sim.set_dft(lams)
-    L.fdtd_hip_synchronize()
-    L.fdtd_hip_profile_enable(1)
-    t0 = time.perf_counter()
-    sim.calc(steps)
-    L.fdtd_solver_setBlocking(T, R)  # launches the last partial block
-    L.fdtd_hip_synchronize()
-    dt = time.perf_counter() - t0
-    prof = prof_read(L)
-    L.fdtd_hip_profile_enable(0)
-    t1 = time.perf_counter()
-    spec = sim.dft_spectra()
-    t_spec = time.perf_counter() - t1
-    shape = SHAPES[sim.launch_mode()]
-    sim.finish()
-    assert np.all(np.isfinite(spec)) and spec[:, 180:360].max() > 0
-    check = {}
+    T, R = args.T or T, args.R or R
+    launch = args.launch if args.launch >= 0 else launch
+    T = pick_T(T, K)
+
+    def make(precision, n):
+        return m.Simulator(solver, getattr(m, model), m.field_info(w, h, hu, pml, lam, ang, n),
+                           params=m.model_params(**params), ntff_time=False, precision=precision, T=T, R=R)
+
+    log(f"sweep81 {mode}: {len(lams)} wavelengths, warmup {W}, timed {K} steps, T={T} R={R} "
+        f"launch={SHAPES[launch]}")
+    sim = make("fp32", W + K)
+    try:
+        cells = sim.N_PX * sim.N_PY
+        sim.set_launch_mode(launch)
+        sim.set_ntff_overlap(args.ntff_overlap)
+        sim.set_dft(lams)
+        sim.calc(W)
+        sim.set_blocking(T, R)  # launches a partial warm-up block, if any
+        L.fdtd_hip_synchronize()
+        nblk = K // T
+        every = 8 if nblk >= 256 else 1
+        L.fdtd_hip_profile_enable(every)
+        t0 = time.perf_counter()
+        sim.calc(K)  # whole blocks: the last one launches with its T-th step
+        L.fdtd_hip_synchronize()
+        dt = time.perf_counter() - t0
+        prof = prof_read(L)
+        L.fdtd_hip_profile_enable(0)
+        t1 = time.perf_counter()
+        spec = sim.dft_spectra()
+        t_spec = time.perf_counter() - t1
+        shape = SHAPES[sim.launch_mode()]
+    finally:
+        sim.finish()
+    assert np.all(np.isfinite(spec)) and spec.max() > 0, "sweep spectra"
+    ms_step = dt * 1e3 / K
+    res = {"value": round(cells * K / dt / 1e6, 1), "ms_per_step": round(ms_step, 5), "steps": K, "warmup": W,
+           "T": T, "R": R, "launch": shape, "cells": cells, "wavelengths": int(len(lams))}
+    # 81 spectra per run of RUN_STEPS steps (measured when K is a whole run)
+    res["spectra_per_s"] = round(len(lams) / (RUN_STEPS * ms_step * 1e-3 + t_spec), 3)
+    key = {"workload": "morpho8256_sweep81", "mode": mode, "T": T, "R": R, "launch": shape, "cells": cells}
+    res["roofline"] = roofline(prof, key, T, f"{mode.upper()} step launches of one block of T steps: interior "
+                                             f"k_fast R={R} + UPML tiles ({shape}), with contour samples")
+    res["roofline"]["timed_every"] = every
+    if tune:
+        res["tune_us_per_step"] = {f"T{k[0]}R{k[1]}{SHAPES[k[2]]}": round(v, 2) for k, v in tune.items()}
     if not args.no_check:
-        # FP32 against FP64 on a shorter run of the same grid and source: the 81
-        # transforms of the three accumulator rows, and the FP64 path's step rate
-        nchk = 4000  # the scattered pulse reaches the nearest contour side after about 3000 steps
-        log(f"fp32 / fp64 check over {nchk} steps")
-        res = {}
+        nchk = args.check_steps
+        log(f"sweep81 {mode}: fp32 / fp64 comparison over {nchk} steps")
+        out = {}
         for prec in ("fp32", "fp64"):
             c = make(prec, nchk)
-            if prec == "fp32":
-                c.set_launch_mode(launch)
-            c.set_dft(lams)
-            L.fdtd_hip_synchronize()
-            t2 = time.perf_counter()
-            c.calc(nchk)
-            if prec == "fp32":
-                L.fdtd_solver_setBlocking(T, R)
-            L.fdtd_hip_synchronize()
-            res[prec] = (c.dft(), time.perf_counter() - t2)
-            c.finish()
-        d32, d64 = res["fp32"][0], res["fp64"][0]
-        assert np.linalg.norm(d64) > 0, "the check run ended before the pulse reached the contour"
-        check["dft_rel_err_vs_fp64"] = float(np.linalg.norm(d32 - d64) / np.linalg.norm(d64))
-        ms64 = res["fp64"][1] * 1e3 / nchk
-        check["fp64_mcups"] = round(cells / (ms64 * 1e-3) / 1e6, 1)
-        check["fp64_spectra_per_s"] = round(len(lams) / (steps * ms64 * 1e-3), 4)
-    ms_k, _, nlaunch, b_int, b_pml, cupd = prof
-    achieved = (b_int + b_pml) / (ms_k * 1e-3) / 1e9 if nlaunch and ms_k > 0 else None
+            try:
+                if prec == "fp32":
+                    c.set_launch_mode(launch)
+                c.set_dft(lams)
+                L.fdtd_hip_synchronize()
+                t2 = time.perf_counter()
+                c.calc(nchk)
+                if prec == "fp32":
+                    c.set_blocking(T, R)
+                L.fdtd_hip_synchronize()
+                tt = time.perf_counter() - t2
+                out[prec] = (c.dft(), c.dft_spectra(), tt)
+            finally:
+                c.finish()
+        d32, s32, _ = out["fp32"]
+        d64, s64, t64 = out["fp64"]
+        assert np.linalg.norm(d64) > 0 and np.abs(s64[:, 180:]).max() > 0, \
+            "the comparison run ended before the pulse reached the contour"
+        res["check_steps"] = nchk
+        res["dft_rel_err_vs_fp64"] = float(np.linalg.norm(d32 - d64) / np.linalg.norm(d64))
+        # far-field spectra: all 360 angles, and the 180 back-hemisphere ones (reflectance)
+        res["ff_rel_err_vs_fp64"] = float(np.linalg.norm(s32 - s64) / np.linalg.norm(s64))
+        res["reflectance_rel_err_vs_fp64"] = float(np.linalg.norm(s32[:, 180:] - s64[:, 180:]) /
+                                                   np.linalg.norm(s64[:, 180:]))
+        ms64 = t64 * 1e3 / nchk
+        res["fp64_mcups"] = round(cells / (ms64 * 1e-3) / 1e6, 1)
+        res["fp64_spectra_per_s"] = round(len(lams) / (RUN_STEPS * ms64 * 1e-3), 4)
+        res["speedup_vs_fp64_spectra"] = round(res["spectra_per_s"] / res["fp64_spectra_per_s"], 2)
+    return res
+
+
+def run_sweep81(args, m, cpu):
+    modes = ["tm", "te"] if args.mode == "both" else [args.mode]
+    per = {md: run_sweep81_mode(args, m, md) for md in modes}
+    head = per[modes[0]]
     line = {
-        "metric": "Mcell-updates/s + % HBM roofline; spectra/s at <=1e-4 far-field err",
-        "value": round(cells * steps / dt / 1e6, 1), "unit": "Mcell-updates/s", "n_gpus": 1, "steps": steps,
-        "warmup": 0, "ms_per_step": round(dt * 1e3 / steps, 5), "higher_is_better": True, "scaling": "weak",
+        "metric": METRIC, "value": head["value"], "unit": "Mcell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"Morpho scale {args.mode.upper()}, 8192x4096 + 32-cell UPML (8256x4160), h=5 nm, "
-                               "plain pulse solver, normal incidence (+y), running DFT at 380-780 nm every 5 nm "
-                               "(81 wavelengths) in one run",
-                   "cells": cells, "precision": "fp32", "T": T, "R": R, "launch": shape, "wavelengths": len(lams),
-                   "tune_us_per_step": {f"T{k[0]}R{k[1]}{SHAPES[k[2]]}": round(v, 2) for k, v in tune.items()}},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-                     "traffic": None, "kernel": f"k_fast<{args.mode.upper()},PLAIN,R={R}> with contour samples",
-                     "launch_avg_us": round(ms_k * 1e3 / nlaunch, 2) if nlaunch else None,
-                     "bytes_per_cell_update": round((b_int + b_pml) / cupd, 2) if cupd else None},
-        "cpu_baseline": None,
-        "spectra_per_s": round(len(lams) / (dt + t_spec), 4),
+        "config": {"workload": f"Morpho-scale wavelength sweep {modes[0].upper()}: 8192x4096 + 32-cell UPML "
+                               "(8256x4160, 34.3 M cells), h=5 nm (the integer twin of 2.5 nm), default Morpho "
+                               "model, plain pulse solver at normal incidence (+y), running DFT of the NTFF contour "
+                               "at 380-780 nm every 5 nm (81 wavelengths) in one 20000-step run",
+                   "cells": head["cells"], "precision": "fp32", "T": head["T"], "R": head["R"],
+                   "launch": head["launch"], "wavelengths": head["wavelengths"], "run_steps": RUN_STEPS},
+        "roofline": head["roofline"],
+        "cpu_baseline": cpu,
     }
-    if check:
-        line.update(check)
-        line["speedup_vs_fp64_spectra"] = round(line["spectra_per_s"] / check["fp64_spectra_per_s"], 2)
+    for k in ("spectra_per_s", "check_steps", "dft_rel_err_vs_fp64", "ff_rel_err_vs_fp64",
+              "reflectance_rel_err_vs_fp64", "fp64_mcups", "fp64_spectra_per_s", "speedup_vs_fp64_spectra",
+              "tune_us_per_step"):
+        if k in head:
+            line[k] = head[k]
+    if cpu:
+        line["speedup_vs_cpu_all_cores"] = round(head["value"] / cpu["value"], 1)
+    for md in modes[1:]:
+        line[md] = per[md]
     return line
 
+
+# ---------------------------------------------------------------- other workloads
 
 def run_solvers_fp64(args, m):
     """Every solver of enum SOLVER in its FP64 reference-order path on the mie1056
@@ -379,9 +535,8 @@ def run_sweep(args, m):
             check[f"ff_rel_err_vs_fp64_inst{i}"] = float(np.linalg.norm(ff[i] - ff64) / np.linalg.norm(ff64))
     b.close()
     alg = sum(c * s["steps"] * (52 if s["mode"] == "tm" else 56) for c, s in zip(cells, spec))
-    achieved = alg / dt / 1e9
     line = {
-        "metric": "Mcell-updates/s + % HBM roofline; spectra/s at <=1e-4 far-field err",
+        "metric": METRIC,
         "value": round(upd / dt / 1e6, 1), "unit": "Mcell-updates/s", "n_gpus": 1, "steps": max(s["steps"] for s in spec),
         "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic",
@@ -390,9 +545,11 @@ def run_sweep(args, m):
                    "cells": sum(cells), "cell_updates": upd, "T": T, "R": R,
                    "launch": SHAPES[args.launch] if args.launch >= 2 else "two_streams_pml8",
                    "create_s": round(t_init, 2)},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "note": "algorithmic bytes (interior-cell 52/56 B) of the whole sweep over its wall time"},
+        "roofline": {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
+                     "traffic": None,
+                     "effective": {"GBs": round(alg / dt / 1e9, 1),
+                                   "note": "algorithmic bytes (interior-cell 52/56 B) of the whole sweep over its "
+                                           "wall time, not HBM traffic"}},
         "cpu_baseline": None,
         "spectra_per_s": round(len(spec) / dt, 3),
         "wall_s": round(dt, 3),
@@ -401,46 +558,14 @@ def run_sweep(args, m):
     return line
 
 
-def main():
-    args = parse()
-    if args.copy_calibration:
-        copy_calibration(args.copy_calibration)
-        return
-    # the C host prints progress messages like the reference does (printf on stdout);
-    # keep stdout for the one JSON line by sending fd 1 to stderr until the end
-    import ctypes
-    libc = ctypes.CDLL(None)
-    real_stdout = os.dup(1)
-    os.dup2(2, 1)
-    if args.gpus != 1:
-        sys.exit(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU sharding is "
-                 "described in DESIGN.md and not built")
-    # the reference's CPU path, timed first: a child process started before anything
-    # here has touched the GPU
-    cpu_ref = None
-    if not args.no_cpu and args.workload == "mie1056":
-        log("cpu baseline (the reference, one core)")
-        cpu_ref = cpu_reference(args.mode, args.workload)
+def run_single(args, m, cpu_ref):
+    """mie1056 / morpho8256: the MPI solver with its CW scattered-wave source."""
     import numpy as np
-
-    import mpifdtd_amd as m
     from mpifdtd_amd import autotune
     L = m.lib()
-    # the library's own HIP runtime does the synchronising (fdtd_hip_synchronize =
-    # hipDeviceSynchronize): a GPU runtime loaded beside it would not see its queues
-    assert L.fdtd_hip_device_count() >= 1, "no GPU"
-    if args.workload in ("sweep64", "morpho8256_sweep81", "solvers_fp64"):
-        line = {"sweep64": run_sweep, "morpho8256_sweep81": run_sweep81,
-                "solvers_fp64": run_solvers_fp64}[args.workload](args, m)
-        libc.fflush(None)
-        sys.stdout.flush()
-        os.dup2(real_stdout, 1)
-        print(json.dumps(line), flush=True)
-        return
     K, W = args.steps, args.warmup
     total = K + W
     wl = args.workload
-
     T, R = args.T, args.R
     tuned = None
     if args.precision == "fp32" and (T == 0 or R == 0):
@@ -451,6 +576,8 @@ def main():
         T = T or tuned["T"]
         R = R or tuned["R"]
     launch = args.launch if args.launch >= 0 else (tuned.get("launch", 0) if tuned else 0)
+    if args.precision == "fp32":
+        T = pick_T(T or 4, K)
     log(f"timed run: {wl} {args.precision} T={T} R={R} launch={launch} warmup={W} steps={K}")
     sim = make_sim(m, args.precision, args.mode, total, T or 4, R or 16, workload=wl, ntff=args.ntff)
     if args.precision == "fp32":
@@ -461,15 +588,11 @@ def main():
     if args.precision == "fp32":
         L.fdtd_solver_setBlocking(T, R)  # launches the queued warm-up steps
     L.fdtd_hip_synchronize()
-    # events on every launch of a long run of short launches would cost the timed
-    # region a few microseconds per launch: then every 8th launch is timed
     nblk = K // max(1, T or 1) if args.precision == "fp32" else 0
-    L.fdtd_hip_profile_enable(8 if nblk >= 256 else 1)
-    L.fdtd_hip_synchronize()
+    every = 8 if nblk >= 256 else 1
+    L.fdtd_hip_profile_enable(every)
     t0 = time.perf_counter()
-    sim.calc(K)
-    if args.precision == "fp32":
-        L.fdtd_solver_setBlocking(T, R)  # launches the last partial block inside the timed region
+    sim.calc(K)  # whole blocks
     L.fdtd_hip_synchronize()
     dt = time.perf_counter() - t0
     prof = prof_read(L)
@@ -477,35 +600,21 @@ def main():
     t1 = time.perf_counter()
     ff = sim.far_field()
     t_ff = time.perf_counter() - t1
-    f32_0 = sim.fields()[0]  # TM Ez / TE Ex (a copy)
-    shape = sim.launch_mode() if args.precision == "fp32" else 0
-    shape_name = SHAPES[shape]
+    f32_0 = sim.fields()[0]
+    shape_name = SHAPES[sim.launch_mode() if args.precision == "fp32" else 0]
     sim.finish()
     ms_step = dt * 1e3 / K
-    mcups = cells * K / dt / 1e6
-
-    # roofline of the dominant kernel: the fused launch (UPML + interior tiles, T steps)
-    ms_k, _, nlaunch, b_int, b_pml, cupd = prof
-    if args.precision == "fp32" and nlaunch > 0 and ms_k > 0:
-        achieved = (b_int + b_pml) / (ms_k * 1e-3) / 1e9
-        traffic, tsrc = measured_traffic(args.mode, T, R, cells, shape_name)
-        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": tsrc,
-                "kernel": f"k_fast<{args.mode.upper()},MPI,R={R}>" + (
-                    "" if shape_name == "merged" else f" interior + UPML launches ({shape_name.replace('_', ' ')})"),
-                "launch_avg_us": round(ms_k * 1e3 / nlaunch, 2), "steps_per_launch": T,
-                "launches_timed": int(nlaunch), "timed_every": 8 if nblk >= 256 else 1,
-                "bytes_per_launch": round((b_int + b_pml) / nlaunch),
-                "bytes_per_cell_update": round((b_int + b_pml) / cupd, 2)}
+    if args.precision == "fp32":
+        key = {"mode": args.mode, "T": T, "R": R, "cells": cells, "launch": shape_name,
+               "workload": wl + ("_ntff" if args.ntff else "")}
+        roof = roofline(prof, key, T, f"k_fast<{args.mode.upper()},MPI,R={R}> interior + UPML launches ({shape_name})")
+        roof["timed_every"] = every
     else:
         roof = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None, "traffic": None}
-
     extra = {}
-    # spectra/s: one spectrum = a full run of the workload's step count plus its far field
     spec_steps = 8000 if wl == "mie1056" else 20000
     extra["spectra_per_s"] = round(1.0 / (spec_steps * ms_step * 1e-3 + t_ff), 4)
     if not args.no_check and args.precision == "fp32":
-        # FP32 far field vs the FP64 reference-order GPU path on the same run length
         log("fp64 reference-order run for the far-field check")
         ref = make_sim(m, "fp64", args.mode, total, workload=wl, ntff=args.ntff)
         t2 = time.perf_counter()
@@ -521,25 +630,57 @@ def main():
         extra["fp64_mcups"] = round(cells * total / t64 / 1e6, 1)
         extra["fp64_spectra_per_s"] = round(1.0 / (spec_steps * t64 / total), 4)
         extra["speedup_vs_fp64_spectra"] = round(extra["spectra_per_s"] / extra["fp64_spectra_per_s"], 2)
-    # the CPU leg runs on the default workload only: the one-core oracle needs minutes
-    # just to sample the Morpho model's eps on 34 M cells
-    run_cpu = not args.no_cpu and wl == "mie1056" and cpu_ref is None
-    if run_cpu:
-        log("cpu baseline (oracle, one core)")
-    cpu = cpu_ref if cpu_ref is not None else (cpu_baseline(args.mode, wl) if run_cpu else None)
     line = {
-        "metric": "Mcell-updates/s + % HBM roofline; spectra/s at <=1e-4 far-field err",
-        "value": round(mcups, 1), "unit": "Mcell-updates/s", "n_gpus": 1, "steps": K, "warmup": W,
-        "ms_per_step": round(ms_step, 5), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32" if args.precision == "fp32" else "f64", "data": "synthetic",
+        "metric": METRIC, "value": round(cells * K / dt / 1e6, 1), "unit": "Mcell-updates/s", "n_gpus": 1,
+        "steps": K, "warmup": W, "ms_per_step": round(ms_step, 5), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32" if args.precision == "fp32" else "f64", "data": "synthetic",
         "config": {"workload": WORKLOADS[wl][8].format(mode=args.mode.upper()), "cells": cells,
                    "precision": args.precision, "T": T, "R": R,
                    "launch": shape_name if args.precision == "fp32" else None,
                    "per_step_ntff": args.ntff, "autotune": tuned},
         "roofline": roof,
-        "cpu_baseline": cpu,
+        "cpu_baseline": cpu_ref,
     }
     line.update(extra)
+    return line
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        sys.exit(f"bench.py runs on one GPU only (got --gpus {args.gpus}): the path is built and measured on one "
+                 "GPU; how it would shard across GPUs is described in the design notes, not built")
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
+    # the C host prints progress messages like the reference does (printf on stdout);
+    # keep stdout for the one JSON line by sending fd 1 to stderr until the end
+    import ctypes
+    libc = ctypes.CDLL(None)
+    real_stdout = os.dup(1)
+    os.dup2(2, 1)
+    # the reference's CPU path, timed first: child processes started before
+    # anything here has touched the GPU
+    cpu = None
+    if not args.no_cpu:
+        if args.workload == "morpho8256_sweep81":
+            cpu = cpu_reference_sweep()
+        elif args.workload == "mie1056":
+            log("cpu baseline (the reference, one core)")
+            cpu = cpu_reference(args.mode, args.workload)
+    import mpifdtd_amd as m
+    L = m.lib()
+    # the library's own HIP runtime does the synchronising (fdtd_hip_synchronize =
+    # hipDeviceSynchronize): a GPU runtime loaded beside it would not see its queues
+    assert L.fdtd_hip_device_count() >= 1, "no GPU"
+    if args.workload == "morpho8256_sweep81":
+        line = run_sweep81(args, m, cpu)
+    elif args.workload == "sweep64":
+        line = run_sweep(args, m)
+    elif args.workload == "solvers_fp64":
+        line = run_solvers_fp64(args, m)
+    else:
+        line = run_single(args, m, cpu)
     libc.fflush(None)
     sys.stdout.flush()
     os.dup2(real_stdout, 1)

@@ -183,6 +183,10 @@ int fdtd_hip_grid_set_dft(fdtd_grid *g, const double *omega, int nf);
 /* out[nf][3][360] complex: sum_m U_a[m] e^{i w m} over the rows of the three
  * time-domain accumulators as the binning would fill them */
 int fdtd_hip_grid_get_dft(fdtd_grid *g, double *out);
+/* non-finite values (NaN or Inf in either part) of the three physical fields
+ * (TM Ez Hx Hy / TE Ex Ey Hz) of the current state, after the queued steps;
+ * no counterpart in the reference, whose runs go on with NaN fields */
+int fdtd_hip_grid_count_nonfinite(fdtd_grid *g, long long out[3]);
 /* dimensions: out[0..3] = NPXa, NPYa, off, array_size */
 int fdtd_hip_grid_dims(const fdtd_grid *g, int out[4]);
 int fdtd_hip_grid_sync(fdtd_grid *g);
@@ -237,6 +241,11 @@ int fdtd_hip_grid_flush(fdtd_grid *g);
  *        bytes of their interior tiles, of their UPML tiles, their cell updates} */
 void fdtd_hip_profile_enable(int on);
 void fdtd_hip_profile_read(double out[6]);
+/* the achievable HBM rate (no counterpart in the reference): a streaming
+ * device-to-device copy of `bytes` repeated `reps` times with event timing.
+ * out = {best GB/s, mean GB/s (bytes read + written), mean us per launch,
+ *        bytes moved per launch} */
+int fdtd_hip_copy_rate(size_t bytes, int reps, double out[4]);
 
 /* Batched instances (no counterpart in the reference, whose solvers hold one
  * instance per process; its sweeps run one instance per MPI rank, main.c:114-138).

@@ -35,7 +35,9 @@ int fdtd_batch_run(fdtd_batch *b);
 /* queue `blocks` blocks of T steps (asynchronous); read nothing before fdtd_batch_sync */
 int fdtd_batch_run_blocks(fdtd_batch *b, int blocks);
 int fdtd_batch_sync(fdtd_batch *b, int *steps_done);
-/* launch shape of every block (FAST_LAUNCH_* of fdtd_hip.h; TWO_STREAMS_PML8 by default) */
+/* launch shape of every block (FAST_LAUNCH_* of fdtd_hip.h; TWO_STREAMS_PML8 by default).
+ * FDTD_EINVAL for values outside the enum and for the *_P16 shapes, whose
+ * 16-wavefront UPML tiles exist for single grids only. */
 int fdtd_batch_set_launch_mode(fdtd_batch *b, int mode);
 int fdtd_batch_size(const fdtd_batch *b);
 /* instance i: NPXa, NPYa, off (1: padded MPI layout), arraySize */

@@ -202,6 +202,19 @@ int fdtd_solver_setNTFFOverlap(int on);
 int fdtd_solver_getLaunchMode(int *mode);
 /* blocking of the active solver; FDTD_ESTATE when it does not queue steps (FP64) */
 int fdtd_solver_getBlocking(int *T, int *R);
+/* Non-finite field detection (no counterpart in the reference, whose runs carry
+ * NaN or Inf on to the end).  out[3]: values with a NaN or Inf part in each
+ * physical field of the active solver (TM Ez Hx Hy / TE Ex Ey Hz), after its
+ * queued steps; FDTD_ESTATE without an active solver. */
+int fdtd_solver_checkFinite(long long out[3]);
+/* the same check inside the step loop every `every` steps (0: off, the
+ * default): the first detection since init / reset prints the count and the
+ * step time and is kept for fdtd_solver_finiteStatus; with abort_on_detect it
+ * then exits with status 2, the reference's failure exit.  FP32 solvers launch
+ * their queued steps for each check: make `every` a multiple of the block depth. */
+int fdtd_solver_setFiniteCheck(int every, int abort_on_detect);
+/* 1 and the first detection's step time and count, or 0 (step -1, count 0) */
+int fdtd_solver_finiteStatus(int *step, long long *count);
 
 #ifdef __cplusplus
 }

@@ -59,6 +59,12 @@ class ModelParams(ctypes.Structure):
     ]
 
 
+class RunConfig(ctypes.Structure):
+    """fdtd_run_config of mpifdtd_simulator.h: the values of a config.txt (main.c:319-366)."""
+    _fields_ = [("field_info", FieldInfo), ("startAngle", ctypes.c_int), ("endAngle", ctypes.c_int),
+                ("deltaAngle", ctypes.c_int), ("model", ctypes.c_int), ("solver", ctypes.c_int)]
+
+
 class BatchItem(ctypes.Structure):
     """fdtd_batch_item of mpifdtd_batch.h: one instance of a batch."""
     _fields_ = [("field", FieldInfo), ("solver", ctypes.c_int), ("model", ctypes.c_int), ("params", ModelParams)]
@@ -130,6 +136,7 @@ def lib():
     sig("simulator_getEps", P)
     sig("simulator_elapsed", D)
     sig("fdtd_simulator_run", V, I)
+    sig("fdtd_readConfig", I, ctypes.c_char_p, ctypes.POINTER(RunConfig))
     # solver extensions
     sig("fdtd_solver_getState", I, I, dp)
     sig("fdtd_solver_setState", I, I, dp)
@@ -149,6 +156,10 @@ def lib():
     sig("fdtd_solver_setLaunchMode", I, I)
     sig("fdtd_solver_setNTFFOverlap", I, I)
     sig("fdtd_solver_getLaunchMode", I, ip)
+    llp = ctypes.POINTER(ctypes.c_longlong)
+    sig("fdtd_solver_checkFinite", I, llp)
+    sig("fdtd_solver_setFiniteCheck", I, I, I)
+    sig("fdtd_solver_finiteStatus", I, ip, llp)
     # mpifdtd_batch.h
     sig("fdtd_batch_create", I, ctypes.POINTER(BatchItem), I, I, I, ctypes.POINTER(P))
     sig("fdtd_batch_run", I, P)
@@ -182,6 +193,7 @@ def lib():
     sig("fdtd_hip_trim", I)
     sig("fdtd_hip_profile_enable", V, I)
     sig("fdtd_hip_profile_read", V, dp)
+    sig("fdtd_hip_copy_rate", I, ctypes.c_size_t, I, dp)
     sig("fdtd_hip_last_error", ctypes.c_char_p)
     sig("fdtd_hip_device_count", I)
     for n in ("mpi_fdtdTM_upml_getEz", "mpi_fdtdTM_upml_getHx", "mpi_fdtdTM_upml_getHy", "mpi_fdtdTM_upml_getEps",

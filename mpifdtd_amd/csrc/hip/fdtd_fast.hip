@@ -1593,6 +1593,17 @@ int fast_state_reset(FastState *s)
   return FDTD_OK;
 }
 
+int fast_state_fields(FastState *s, const float2 *f[3], size_t *n)
+{
+  int rc;
+  if ((rc = fast_state_flush(s))) return rc;
+  f[0] = s->h.f0[s->cur];
+  f[1] = s->h.f1[s->cur];
+  f[2] = s->h.f2[s->cur];
+  *n = (size_t)s->h.NPX * s->h.pitch;
+  return FDTD_OK;
+}
+
 int fast_state_gather(FastState *s, const Contour &ct, int off, double2 *cE, double2 *cH)
 {
   int rc;
@@ -1843,6 +1854,11 @@ int fast_batch_run(FastBatch *b, int blocks, int use_graph)
 // captured graph is rebuilt)
 int fast_batch_set_launch(FastBatch *b, int mode)
 {
+  if (mode < FAST_LAUNCH_AUTO || mode > FAST_LAUNCH_TWO_STREAMS_P16) return fdtd_fail(FDTD_EINVAL, "bad launch mode %d", mode);
+  // the 16-wavefront UPML tiles exist for single grids only (the batch kernel's
+  // tile bodies are 4 or 8 wavefronts)
+  if (mode == FAST_LAUNCH_SPLIT_P16 || mode == FAST_LAUNCH_TWO_STREAMS_P16)
+    return fdtd_fail(FDTD_EINVAL, "launch mode %d (16-wavefront UPML tiles) is not available for batches", mode);
   HIPCHK(hipStreamSynchronize(b->stream));
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   if (b->graph) (void)hipGraphDestroy(b->graph);

@@ -148,6 +148,77 @@ extern "C" int fdtd_hip_synchronize(void)
   return e == hipSuccess ? FDTD_OK : fdtd_fail(FDTD_EHIP, "hipDeviceSynchronize failed: %s", hipGetErrorString(e));
 }
 
+// Streaming copy of n 16-byte elements: each thread moves 4 consecutive
+// float4 per pass of a grid-stride loop with non-temporal loads and stores
+// (the copy is read once and written once, so it should not displace the
+// caches), one 1 KiB-per-wavefront-instruction contiguous segment per load.
+typedef float v4f __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_stream_copy(const v4f *__restrict__ src, v4f *__restrict__ dst, size_t n)
+{
+  const size_t stride = (size_t)gridDim.x * 1024;
+  for (size_t base = (size_t)blockIdx.x * 1024 + threadIdx.x; base < n; base += stride) {
+    v4f v[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const size_t k = base + 256 * u;
+      if (k < n) v[u] = __builtin_nontemporal_load(src + k);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const size_t k = base + 256 * u;
+      if (k < n) __builtin_nontemporal_store(v[u], dst + k);
+    }
+  }
+}
+
+// The achievable HBM rate: a device-to-device copy of `bytes` (rounded down to
+// 16 B) repeated `reps` times, each launch timed with events.  out[0] best and
+// out[1] mean rate in GB/s of bytes read plus bytes written; out[2] mean
+// launch time in microseconds; out[3] bytes moved per launch.
+extern "C" int fdtd_hip_copy_rate(size_t bytes, int reps, double *out)
+{
+  if (!out || reps < 1 || bytes < 16) return fdtd_fail(FDTD_EINVAL, "fdtd_hip_copy_rate: bad argument");
+  const size_t n = bytes / 16;
+  void *a = nullptr, *b = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = FDTD_OK;
+  double best = 0, sum_ms = 0;
+  int dev = 0, cus = 256;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  const int nblk = (int)std::min<size_t>((size_t)cus * 16, (n + 1023) / 1024);
+  if (hipMalloc(&a, n * 16) != hipSuccess || hipMalloc(&b, n * 16) != hipSuccess) {
+    rc = fdtd_fail(FDTD_ENOMEM, "fdtd_hip_copy_rate: hipMalloc of 2 x %zu bytes failed", n * 16);
+  } else if (hipMemset(a, 0, n * 16) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+             hipEventCreate(&e1) != hipSuccess) {
+    rc = fdtd_fail(FDTD_EHIP, "fdtd_hip_copy_rate: set-up failed");
+  } else {
+    hipLaunchKernelGGL(k_stream_copy, dim3(nblk), dim3(256), 0, 0, (const v4f *)a, (v4f *)b, n);  // warm
+    for (int r = 0; r < reps && rc == FDTD_OK; r++) {
+      float ms = 0;
+      if (hipEventRecord(e0, 0) != hipSuccess) rc = fdtd_fail(FDTD_EHIP, "event record failed");
+      hipLaunchKernelGGL(k_stream_copy, dim3(nblk), dim3(256), 0, 0, (const v4f *)a, (v4f *)b, n);
+      if (rc == FDTD_OK && (hipEventRecord(e1, 0) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                            hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+        rc = fdtd_fail(FDTD_EHIP, "copy timing failed: %s", hipGetErrorString(hipGetLastError()));
+      sum_ms += ms;
+      if (ms > 0) best = std::max(best, 2.0 * n * 16 / (ms * 1e-3) / 1e9);
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  if (rc == FDTD_OK) {
+    const double mean_ms = sum_ms / reps;
+    out[0] = best;
+    out[1] = mean_ms > 0 ? 2.0 * n * 16 / (mean_ms * 1e-3) / 1e9 : 0.0;
+    out[2] = mean_ms * 1e3;
+    out[3] = 2.0 * n * 16;
+  }
+  return rc;
+}
+
 // ---------------------------------------------------------------- kernels
 
 struct GridPtrs {
@@ -1170,6 +1241,7 @@ struct fdtd_grid {
   double *cosA, *sinA, *cosB, *sinB;
   double coef;
   double2 *ffout;
+  unsigned long long *nfcnt;  // [3] non-finite values per physical field (fdtd_hip_grid_count_nonfinite)
   double *spec;
   double2 *tw;
   fdtd_model_state *mstate;
@@ -1562,6 +1634,7 @@ extern "C" int fdtd_hip_grid_create(const fdtd_grid_desc *d, fdtd_grid **out)
   // 1/(4 pi c R), R = 1e6, applied per step by the MPI solvers (mpiTM_UPML.c:852-854)
   g->coef = 1.0 / (4 * M_PI * d->c0 * 1.0e6);
   HIPCHK(dev_malloc(&g->ffout, 360 * sizeof(double2)));
+  HIPCHK(dev_malloc(&g->nfcnt, 3 * sizeof(unsigned long long)));
   HIPCHK(dev_malloc(&g->spec, 321 * 360 * sizeof(double)));
   HIPCHK(dev_malloc(&g->tw, 4096 * sizeof(double2)));
   HIPCHK(dev_malloc(&g->mstate, sizeof(fdtd_model_state)));
@@ -1653,6 +1726,7 @@ extern "C" int fdtd_hip_grid_destroy(fdtd_grid *g)
   (void)dev_free(g->cosB);
   (void)dev_free(g->sinB);
   (void)dev_free(g->ffout);
+  (void)dev_free(g->nfcnt);
   (void)dev_free(g->spec);
   (void)dev_free(g->tw);
   (void)dev_free(g->mstate);
@@ -1911,6 +1985,47 @@ extern "C" int fdtd_hip_grid_far_field(fdtd_grid *g, double k, double coef_re, d
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpy(out360, g->ffout, 360 * sizeof(double2), hipMemcpyDeviceToHost));
+  return FDTD_OK;
+}
+
+// Non-finite values (NaN or +-Inf in the real or the imaginary part) of a field
+// array: each thread counts over a grid-stride loop and adds its count to *cnt
+// only when it found one, so a healthy field costs one read of the array.
+template <class V>
+__global__ __launch_bounds__(256) void k_count_nonfinite(const V *a, size_t n, unsigned long long *cnt)
+{
+  unsigned long long c = 0;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+    const V v = a[i];
+    c += (isfinite(v.x) && isfinite(v.y)) ? 0 : 1;
+  }
+  if (c) atomicAdd(cnt, c);
+}
+
+extern "C" int fdtd_hip_grid_count_nonfinite(fdtd_grid *g, long long out[3])
+{
+  if (!g || !out) return fdtd_fail(FDTD_EINVAL, "null argument");
+  HIPCHK(hipMemset(g->nfcnt, 0, 3 * sizeof(unsigned long long)));
+  int dev = 0, cus = 256;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  auto blocks = [&](size_t n) { return (unsigned)std::max<size_t>(1, std::min<size_t>((size_t)cus * 8, (n + 255) / 256)); };
+  if (g->fs) {
+    const float2 *f[3];
+    size_t n = 0;
+    int rc = fast_state_fields(g->fs, f, &n);
+    if (rc) return rc;
+    for (int q = 0; q < 3; q++)
+      hipLaunchKernelGGL(k_count_nonfinite<float2>, dim3(blocks(n)), dim3(256), 0, 0, f[q], n, g->nfcnt + q);
+  } else {
+    for (int q = 0; q < 3; q++)
+      hipLaunchKernelGGL(k_count_nonfinite<double2>, dim3(blocks(g->ncell)), dim3(256), 0, 0, g->F[3 * q], g->ncell,
+                         g->nfcnt + q);
+  }
+  HIPCHK(hipGetLastError());
+  unsigned long long h[3];
+  HIPCHK(hipMemcpy(h, g->nfcnt, sizeof h, hipMemcpyDeviceToHost));
+  for (int q = 0; q < 3; q++) out[q] = (long long)h[q];
   return FDTD_OK;
 }
 

@@ -132,6 +132,9 @@ int fast_state_export(FastState *s, int slot, double2 *dst_dev, int NPYa, int of
 int fast_state_import(FastState *s, int slot, const double2 *src_dev, int NPYa, int off);
 int fast_state_reset(FastState *s);
 int fast_state_gather(FastState *s, const Contour &ct, int off, double2 *cE, double2 *cH);
+// the three physical fields of the current buffer (after launching the queued
+// steps): TM Ez Hx Hy / TE Ex Ey Hz, each NPX x pitch float2; *n = NPX * pitch
+int fast_state_fields(FastState *s, const float2 *f[3], size_t *n);
 // Per-step contour sampling inside the fused kernel: after every launch of n
 // steps, post(ctx, samples, n, times) receives the device buffer
 // [n][3][total] float2 and the time of each step.

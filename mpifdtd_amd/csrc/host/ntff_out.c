@@ -45,16 +45,14 @@ void ntff_outputEnormBin(double **e_norm, const char *file_name)
   fclose(fp);
 }
 
-/* next non-comment line with the part after '#' cut (parser.c:3-20) */
+/* next value line (parser.c:3-20): lines whose first character is '#', NUL or
+ * '\n' are skipped, any other line is a value line (a line of blanks, or the
+ * "\r\n" of a CRLF file, reads as 0, as in the reference); the value stops at a
+ * trailing "# note" because the conversion stops at the first non-digit */
 static int next_line(FILE *fp, char *buf, size_t n)
 {
   while (fgets(buf, (int)n, fp)) {
-    if (buf[0] == '#' || buf[0] == '\0' || buf[0] == '\n' || buf[0] == '\r') continue;
-    char *p = strchr(buf, '#');
-    if (p) *p = '\0';
-    char *q = buf;
-    while (*q && isspace((unsigned char)*q)) q++;
-    if (!*q) continue;
+    if (buf[0] == '#' || buf[0] == '\0' || buf[0] == '\n') continue;
     return 1;
   }
   return 0;
@@ -76,7 +74,8 @@ int fdtd_readConfig(const char *path, fdtd_run_config *out)
       printf("parse error in %s: expected 11 values, got %d\n", path, i);
       return -2;
     }
-    v[i] = (int)strtod(buf, NULL);
+    /* lambda with strtod into the int field, the others with atoi (main.c:330-358) */
+    v[i] = i == 4 ? (int)strtod(buf, NULL) : atoi(buf);
   }
   fclose(fp);
   memset(out, 0, sizeof *out);

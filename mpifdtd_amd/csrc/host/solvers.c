@@ -49,6 +49,11 @@ static int ntff_time_ = 1;
 static int file_output_ = 1;
 static int precision_ = FDTD_FP64;
 static int block_T_ = 0, block_R_ = 16;  /* 0: default depth */
+/* non-finite field check (fdtd_solver_setFiniteCheck): every finite_every_ steps,
+   0 off; the first detection since init / reset (step time and count) */
+static int finite_every_ = 0, finite_abort_ = 0, finite_since_ = 0;
+static int finite_step_ = -1;
+static long long finite_count_ = 0;
 
 static const int slot_of_[3] = {0, 3, 6};
 
@@ -104,6 +109,29 @@ int fdtd_solver_getBlocking(int *T, int *R)
 {
   if (active_ < 0 || !slv_[active_].blocked) return FDTD_ESTATE;
   return fdtd_hip_grid_get_blocking(slv_[active_].g, T, R);
+}
+
+int fdtd_solver_checkFinite(long long out[3])
+{
+  if (!out) return FDTD_EINVAL;
+  if (active_ < 0 || !slv_[active_].g) return FDTD_ESTATE;
+  return fdtd_hip_grid_count_nonfinite(slv_[active_].g, out);
+}
+
+int fdtd_solver_setFiniteCheck(int every, int abort_on_detect)
+{
+  if (every < 0) return FDTD_EINVAL;
+  finite_every_ = every;
+  finite_abort_ = abort_on_detect ? 1 : 0;
+  finite_since_ = 0;
+  return FDTD_OK;
+}
+
+int fdtd_solver_finiteStatus(int *step, long long *count)
+{
+  if (step) *step = finite_step_;
+  if (count) *count = finite_count_;
+  return finite_step_ >= 0 ? 1 : 0;
 }
 
 /* the simplified beta and coef1 of nsFdtdTM.c:221-229 */
@@ -381,6 +409,9 @@ static void solver_init(int kind)
   for (int q = 0; q < 2 && split; q++) s->split_mirror[q] = (double complex *)calloc(n, sizeof(double complex));
   s->eps_mirror = (double *)calloc(n, sizeof(double));
   active_ = kind;
+  finite_since_ = 0;
+  finite_step_ = -1;
+  finite_count_ = 0;
 }
 
 static void solver_update(int kind)
@@ -409,6 +440,19 @@ static void solver_update(int kind)
     p.use_y = si != 0.0;
   }
   CHECK(fdtd_hip_grid_step(s->g, &p));
+  if (finite_every_ > 0 && ++finite_since_ >= finite_every_) {
+    finite_since_ = 0;
+    long long c[3];
+    CHECK(fdtd_hip_grid_count_nonfinite(s->g, c));
+    const long long n = c[0] + c[1] + c[2];
+    if (n > 0 && finite_step_ < 0) {
+      finite_step_ = (int)p.time;
+      finite_count_ = n;
+      printf("mpifdtd_amd: %lld non-finite field values after the step of time %d\n", n, finite_step_);
+      fflush(stdout);
+      if (finite_abort_) exit(2);
+    }
+  }
 }
 
 static double complex *solver_field(int kind, int q)
@@ -605,6 +649,9 @@ static void solver_reset(int kind)
 {
   solver_t *s = &slv_[kind];
   if (!s->g) return;
+  finite_since_ = 0;
+  finite_step_ = -1;
+  finite_count_ = 0;
   if (is_split(kind) && file_output_) write_elliptic(kind);  /* fdtdTM.c:167-173, fdtdTE.c:269-274, nsFdtdTM.c:160-164 */
   if (!is_mpi(kind) && file_output_ && s->ntff_time) write_spectra();  /* fdtdTM_upml.c:87, fdtdTE_upml.c:117 */
   CHECK(fdtd_hip_grid_reset(s->g));

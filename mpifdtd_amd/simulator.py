@@ -140,6 +140,23 @@ class Simulator:
         check(N.lib().fdtd_solver_getNTFFTime(which, N.dptr(out)), "fdtd_solver_getNTFFTime")
         return out
 
+    def check_finite(self):
+        """Values with a NaN or Inf part in each physical field (TM Ez Hx Hy, TE Ex Ey Hz)."""
+        out = (ctypes.c_longlong * 3)()
+        check(N.lib().fdtd_solver_checkFinite(out), "fdtd_solver_checkFinite")
+        return list(out)
+
+    def set_finite_check(self, every, abort_on_detect=False):
+        """Check the fields every `every` steps inside the step loop (0: off)."""
+        check(N.lib().fdtd_solver_setFiniteCheck(int(every), 1 if abort_on_detect else 0),
+              "fdtd_solver_setFiniteCheck")
+
+    def finite_status(self):
+        """(step time, count) of the first non-finite detection since init / reset, or None."""
+        st, cnt = ctypes.c_int(), ctypes.c_longlong()
+        hit = N.lib().fdtd_solver_finiteStatus(ctypes.byref(st), ctypes.byref(cnt))
+        return (st.value, cnt.value) if hit else None
+
     def far_field(self):
         out = np.empty(360, np.complex128)
         check(N.lib().fdtd_solver_farField(N.dptr(out)), "fdtd_solver_farField")

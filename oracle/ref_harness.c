@@ -33,6 +33,11 @@
  *            (pmlCoef, _LXY, ns_beta, ns_beta_aster on a 3 x 3 set), bil.bin /
  *            nrm.bin (cbilinear, cnorm), meta.txt (subdomain getters, ind), and
  *            ell.txt, all_c.txt, all_d.txt from the three writers.
+ * mode=config: path=<file>: the 11 values main.c's readConfig (main.c:319-366,
+ *            commented out in the reference) takes from a config.txt, through
+ *            the reference's own parser_nextLine (parser.c:3-20) and the same
+ *            atoi / strtod conversions, printed as one JSON list; [] when the
+ *            file holds fewer than 11 value lines.
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -54,6 +59,7 @@
 #include "fdtdTE.h"
 #include "nsFdtdTM.h"
 #include "nsFdtdTE.h"
+#include "parser.h"
 
 static const char *arg(int argc, char **argv, const char *key, const char *def)
 {
@@ -89,6 +95,21 @@ int main(int argc, char **argv)
   fi.stepNum = atoi(arg(argc, argv, "steps", "100"));
   const char *out = arg(argc, argv, "out", ".");
   mkdir(out, 0755);
+
+  if (strcmp(mode, "config") == 0) {
+    FILE *fp = fopen(arg(argc, argv, "path", "config.txt"), "r");
+    if (!fp) { perror("config"); return 3; }
+    char buf[1024];
+    int v[11];
+    for (int i = 0; i < 11; i++) {
+      if (!parser_nextLine(fp, buf)) { printf("[]\n"); fclose(fp); return 0; }
+      v[i] = i == 4 ? (int)strtod(buf, NULL) : atoi(buf); /* lambda: strtod, main.c:340 */
+    }
+    fclose(fp);
+    printf("[");
+    for (int i = 0; i < 11; i++) printf("%d%s", v[i], i < 10 ? ", " : "]\n");
+    return 0;
+  }
 
   MPI_Init(0, 0);
   models_setModel((enum MODEL)model);

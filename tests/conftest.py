@@ -14,11 +14,21 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs through the HIP C-ABI)")
 
 
+class ZeroReference(AssertionError):
+    """The reference of a relative comparison is identically zero: the check
+    would compare nothing (a run too short for the field to reach what it
+    samples, say)."""
+
+
 def rel_l2(a, b):
+    """Relative L2 distance of `a` from the reference `b`; raises ZeroReference
+    when `b` is all zero, so that no comparison passes on two zero arrays."""
     a = np.asarray(a)
     b = np.asarray(b)
     den = np.linalg.norm(b.ravel())
-    return float(np.linalg.norm((a - b).ravel()) / (den if den > 0 else 1.0))
+    if not den > 0:
+        raise ZeroReference(f"reference of shape {b.shape} is identically zero (or not finite)")
+    return float(np.linalg.norm((a - b).ravel()) / den)
 
 
 # Model parameters after `sweep` calls of the reference's models_isFinish():

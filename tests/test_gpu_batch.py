@@ -91,7 +91,21 @@ def test_batch_chunks_and_graph_replays_agree():
         assert np.array_equal(a, c)
 
 
-@pytest.mark.parametrize("shape", [2, 3, 4, 6, 7])
+@pytest.mark.parametrize("shape", [-1, 8, 9, 10])
+def test_batch_rejects_unavailable_launch_shapes(shape):
+    """Values outside FAST_LAUNCH_* and the 16-wavefront UPML shapes (single
+    grids only) are refused with FDTD_EINVAL, and the batch keeps running in its
+    previous shape."""
+    import mpifdtd_amd as m
+    from mpifdtd_amd import batch as B
+    with B.Batch(_items(m, B), T=T, R=R) as b:
+        with pytest.raises(RuntimeError, match="launch mode"):
+            b.set_launch_mode(shape)
+        b.run()
+        assert b.sync() == [c[8] for c in CASES]
+
+
+@pytest.mark.parametrize("shape", [2, 3, 4, 5, 6, 7])
 def test_batch_launch_shapes_equal_default(shape):
     """The launches in a row or side by side (a fork and join captured into the
     replayed graph), with either tile kind as 4 or 8 wavefronts, give the same

@@ -193,3 +193,19 @@ def test_ns_get_solver_returns_an_unset_solver():
         p = f()
         assert all(v is None for v in p.contents), name
         assert ctypes.addressof(p.contents) == ctypes.addressof(f().contents)
+
+
+def test_finite_check_host_logic():
+    """The non-finite check's host side without a GPU call: no active solver
+    (FDTD_ESTATE), argument checks, and the empty status."""
+    import ctypes
+    from mpifdtd_amd import _native as N
+    L = N.lib()
+    out = (ctypes.c_longlong * 3)()
+    assert L.fdtd_solver_checkFinite(out) == -4  # FDTD_ESTATE: no solver
+    assert L.fdtd_solver_checkFinite(None) == -1  # FDTD_EINVAL
+    assert L.fdtd_solver_setFiniteCheck(-1, 0) == -1
+    assert L.fdtd_solver_setFiniteCheck(0, 0) == 0
+    st, cnt = ctypes.c_int(7), ctypes.c_longlong(7)
+    assert L.fdtd_solver_finiteStatus(ctypes.byref(st), ctypes.byref(cnt)) == 0
+    assert (st.value, cnt.value) == (-1, 0)
