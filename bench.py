@@ -44,7 +44,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 METRIC = "Mcell-updates/s + % HBM roofline; spectra/s at <=1e-4 far-field err"
 # launch shapes of the UPML and interior tiles (FAST_LAUNCH_* of fdtd_hip.h)
-SHAPES = ["auto", "merged", "split", "two_streams", "split_pml8", "two_streams_pml8", "split_w8", "two_streams_w8",
+SHAPES = ["auto", "merged", "split", "two_streams", "split_upml48", "two_streams_upml48", "split_w8", "two_streams_w8",
           "split_p16", "two_streams_p16"]
 
 # name -> (model, width_nm, height_nm, h_nm, pml, lambda_nm, angle_deg, model params, description)
@@ -79,7 +79,7 @@ def parse():
     ap.add_argument("--R", type=int, default=0, help="rows per wavefront (0: the workload's default)")
     ap.add_argument("--launch", type=int, default=-1,
                     help="launch shape of the UPML and interior tiles: 0 auto, 1 merged, 2 split, "
-                         "3 split on two streams, 4 / 5: 2 / 3 with 8-wavefront UPML workgroups, "
+                         "3 split on two streams, 4 / 5: 2 / 3 with the 48-row UPML tiles, "
                          "6 / 7: 4 / 5 with 8-wavefront interior workgroups, 8 / 9: 6 / 7 with "
                          "16-wavefront UPML workgroups (-1: the workload's default)")
     ap.add_argument("--autotune", action="store_true", help="time the candidate configurations first and use the best")
@@ -119,8 +119,9 @@ def copy_calibration(mib):
     counters scale the step kernels' counters to bytes."""
     import ctypes
     import mpifdtd_amd as m
+    from mpifdtd_amd._native import check
     out = (ctypes.c_double * 4)()
-    m.check(m.lib().fdtd_hip_copy_rate(mib << 20, 10, out), "fdtd_hip_copy_rate")
+    check(m.lib().fdtd_hip_copy_rate(mib << 20, 10, out), "fdtd_hip_copy_rate")
     print(json.dumps({"copy_bytes_read": (mib << 20), "copy_bytes_written": (mib << 20), "launches": 11,
                       "best_GBs": round(out[0], 1), "mean_GBs": round(out[1], 1), "mean_us": round(out[2], 2)}),
           flush=True)
@@ -372,7 +373,9 @@ def run_sweep81_mode(args, m, mode):
         shape = SHAPES[sim.launch_mode()]
     finally:
         sim.finish()
-    assert np.all(np.isfinite(spec)) and spec.max() > 0, "sweep spectra"
+    # a short timed run ends before the pulse reaches the contour (spectra all 0);
+    # the comparison run below checks the spectra once they are formed
+    assert np.all(np.isfinite(spec)), "sweep spectra"
     ms_step = dt * 1e3 / K
     res = {"value": round(cells * K / dt / 1e6, 1), "ms_per_step": round(ms_step, 5), "steps": K, "warmup": W,
            "T": T, "R": R, "launch": shape, "cells": cells, "wavelengths": int(len(lams))}

@@ -208,8 +208,10 @@ enum {
   FAST_LAUNCH_MERGED = 1,
   FAST_LAUNCH_SPLIT = 2,
   FAST_LAUNCH_TWO_STREAMS = 3,
-  /* SPLIT / TWO_STREAMS with the UPML tiles as 8 wavefronts of 4 rows (512
-   * threads): half the registers per wavefront, two wavefronts per SIMD */
+  /* SPLIT / TWO_STREAMS with the wide UPML tiles: blocks of 48 rows (27 core
+   * rows at T = 10, against 11 for the 32-row blocks of the other shapes), as
+   * 12 wavefronts of 4 rows (TM) or 8 of 6 (TE).  (Named for the first form of
+   * this body, 8 wavefronts of 4 rows of a 32-row block.) */
   FAST_LAUNCH_SPLIT_PML8 = 4,
   FAST_LAUNCH_TWO_STREAMS_PML8 = 5,
   /* SPLIT_PML8 / TWO_STREAMS_PML8 with the interior tiles (4R rows) as 8
@@ -217,8 +219,8 @@ enum {
    * R = 4 the interior tiles keep 4 wavefronts) */
   FAST_LAUNCH_SPLIT_W8 = 6,
   FAST_LAUNCH_TWO_STREAMS_W8 = 7,
-  /* SPLIT_W8 / TWO_STREAMS_W8 with the UPML tiles as 16 wavefronts of 2 rows
-   * (1024 threads, four wavefronts per SIMD): for grids whose UPML band has
+  /* SPLIT_W8 / TWO_STREAMS_W8 with the wide UPML tiles as 16 wavefronts of 3
+   * rows (1024 threads, four wavefronts per SIMD): for grids whose UPML band has
    * fewer tiles than the chip has CUs.  Single grids only (not batches). */
   FAST_LAUNCH_SPLIT_P16 = 8,
   FAST_LAUNCH_TWO_STREAMS_P16 = 9

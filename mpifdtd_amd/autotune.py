@@ -20,7 +20,7 @@ _cache = {}
 
 
 def _key(sim):
-    return f"{sim.solver}:{sim.N_PX // 256}x{sim.N_PY // 256}:{'ntff' if sim.per_step_ntff else 'plain'}:v9"
+    return f"{sim.solver}:{sim.N_PX // 256}x{sim.N_PY // 256}:{'ntff' if sim.per_step_ntff else 'plain'}:v10"
 
 
 def _load():
@@ -69,7 +69,7 @@ def tune(m, make_sim, probe_steps=240, candidates=CANDIDATES):
 
     try:
         sim.calc(40)
-        shapes_tried = ((1, "merged"), (2, "split"), (3, "2streams"), (4, "split8"), (5, "2streams8"),
+        shapes_tried = ((1, "merged"), (2, "split"), (3, "2streams"), (4, "splitu48"), (5, "2streamsu48"),
                         (6, "splitw8"), (7, "2streamsw8"), (8, "splitp16"), (9, "2streamsp16"))
         for T, R in candidates:
             for shape, tag in shapes_tried:

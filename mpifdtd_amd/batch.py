@@ -56,10 +56,16 @@ def reference_sweep(solver, model, h_u_nm, pml, lambda_nm, steps, start_deg=0, e
 
 
 class Batch:
-    def __init__(self, items, T=8, R=16):
+    def __init__(self, items, T=8, R=16, c0=0.7071, ntff_inset=5):
+        """c0 and ntff_inset: the field settings every instance is created with
+        (field_setLightSpeed, field_setNTFFInset; the reference's values by
+        default), set here because the C host keeps them as process globals."""
         L = N.lib()
         if L.fdtd_hip_device_count() < 1:
             raise RuntimeError("no HIP device visible: the solvers run on the GPU only")
+        L.field_setLightSpeed(c0)
+        L.field_setNTFFInset(ntff_inset)
+        L.field_setPointSource(0)
         arr = (BatchItem * len(items))(*items)
         h = ctypes.c_void_p()
         check(L.fdtd_batch_create(arr, len(items), int(T), int(R), ctypes.byref(h)), "fdtd_batch_create")

@@ -166,7 +166,6 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
       cur = uni_i(a.binst[t.inst].cur);
     }
     const int nxt = cur ^ 1;
-    const int H0 = a.T + 1;  // halo of the tiling depth; T <= a.T
     __syncthreads();  // the previous tile's last reads of the shared arrays are done
     if (BATCH && T == 0) return;  // instance finished (uniform)
     if (w == 0) {
@@ -207,8 +206,8 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
       gpx = uni_p(I.pmlx);
       gpy = uni_p(I.pmly);
     }
-    const int jj = t.j0 - H0 + lane;
-    const int ib0 = t.i0 - H0;  // first row of the block
+    const int jj = t.bj + lane;
+    const int ib0 = t.bi;  // first row of the block
     const int ib = ib0 + w * R;
     const bool jin = jj >= 0 && jj < NPY;
     const int jc = min(max(jj, 0), NPY - 1);  // clamped column: every load below stays in bounds
@@ -258,6 +257,11 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
     unsigned inb = 0, pml = 0, scat0 = 0, scat1 = 0;
     float2 A0[PML ? R : 1], A1[PML ? R : 1], A2[PML ? R : 1], A3[PML ? R : 1], A4[PML ? R : 1], A5[PML ? R : 1];
     unsigned char pmlx[PML ? R : 1];
+    // UPML tiles: auxiliaries are loaded only outside the box that holds no UPML
+    // cell (half the lanes of a side-band tile, most rows of an end-band tile's
+    // inner part); inside it they are never read
+    const bool colbox = PML && jj >= uni_i(I.aylo) && jj < uni_i(I.ayhi);
+    const int axlo = PML ? uni_i(I.axlo) : 0, axhi = PML ? uni_i(I.axhi) : 0;
 #pragma unroll
     for (int r = 0; r < R; r++) {
       const int i = ib + r;
@@ -270,12 +274,15 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
       ie1[r] = MODE == FDTD_MODE_TE ? ldo(gie1, 4u * k) : 1.0f;
       if (PML) {
         pmlx[r] = gpx[ic];
-        A0[r] = ldo(ga[0], 8u * k);
-        A1[r] = ldo(ga[1], 8u * k);
-        A2[r] = ldo(ga[2], 8u * k);
-        A3[r] = ldo(ga[3], 8u * k);
-        A4[r] = ldo(ga[4], 8u * k);
-        A5[r] = ldo(ga[5], 8u * k);
+        A0[r] = A1[r] = A2[r] = A3[r] = A4[r] = A5[r] = make_float2(0.f, 0.f);
+        if (!(colbox && i >= axlo && i < axhi)) {
+          A0[r] = ldo(ga[0], 8u * k);
+          A1[r] = ldo(ga[1], 8u * k);
+          A2[r] = ldo(ga[2], 8u * k);
+          A3[r] = ldo(ga[3], 8u * k);
+          A4[r] = ldo(ga[4], 8u * k);
+          A5[r] = ldo(ga[5], 8u * k);
+        }
       }
       if (jin && i >= 0 && i < NPX) inb |= 1u << r;
     }
@@ -717,22 +724,27 @@ __attribute__((amdgpu_waves_per_eu((KIND == 1 || KIND == 4) ? fast_interior_wave
   }
 }
 
-// UPML tiles (32 rows) as 8 wavefronts of 4 rows: half the registers of the
-// 4 x 8 form per wavefront, so that two wavefronts fit each SIMD
+// The wide UPML tiles (FAST_UPML_ROWS_WIDE = 48 rows).  TM: 12 wavefronts of 4
+// rows (768 threads, 156-162 VGPRs: three wavefronts per SIMD, one workgroup
+// per compute unit).  TE, whose body holds a second eps and source and needs
+// 168+ VGPRs at 4 rows: 8 wavefronts of 6 rows (512 threads, two per SIMD).  A
+// 48-row tile keeps 48 - 2T - 1 core rows (27 at T = 10) where a 32-row tile
+// keeps 11: the halo rows the UPML tiles compute again fall by 2.5x.
+__host__ __device__ constexpr int pml48_waves(int mode) { return mode == FDTD_MODE_TM ? 12 : 8; }
 template <int MODE, int FLAV, int SMP>
-__global__ __launch_bounds__(512) void k_fast_pml8(FastArgs a)
+__global__ __launch_bounds__(64 * pml48_waves(MODE)) void k_fast_pml48(FastArgs a)
 {
-  fast_tiles<MODE, FLAV, 4, 1, SMP, 8>(a, a.qb[0], blockIdx.x, gridDim.x);
+  fast_tiles<MODE, FLAV, FAST_UPML_ROWS_WIDE / pml48_waves(MODE), 1, SMP, pml48_waves(MODE)>(a, a.qb[0], blockIdx.x,
+                                                                                            gridDim.x);
 }
 
-// UPML tiles (32 rows) as 16 wavefronts of 2 rows (1024 threads): a quarter of
-// the registers of the 4 x 8 form per wavefront, four wavefronts per SIMD.  On
-// grids whose UPML band has fewer tiles than the chip has CUs, the band's
-// launch is latency-bound, and more wavefronts per tile shorten it.
+// The wide UPML tiles as 16 wavefronts of 3 rows (1024 threads): fewer
+// registers per wavefront, four wavefronts per SIMD, for grids whose UPML band
+// has fewer tiles than the chip has CUs (the band's launch is then latency-bound).
 template <int MODE, int FLAV, int SMP>
 __global__ __launch_bounds__(1024) void k_fast_pml16(FastArgs a)
 {
-  fast_tiles<MODE, FLAV, 2, 1, SMP, 16>(a, a.qb[0], blockIdx.x, gridDim.x);
+  fast_tiles<MODE, FLAV, 3, 1, SMP, 16>(a, a.qb[0], blockIdx.x, gridDim.x);
 }
 
 // Interior tiles (4R rows) as 8 wavefronts of R/2 rows: the same tiles and
@@ -873,9 +885,9 @@ static void launch_one(const FastArgs &a, int nblocks, hipStream_t st)
 }
 
 template <int MODE, int FLAV, int SMP>
-static void launch_pml8(const FastArgs &a, int nblocks, hipStream_t st)
+static void launch_pml48(const FastArgs &a, int nblocks, hipStream_t st)
 {
-  hipLaunchKernelGGL((k_fast_pml8<MODE, FLAV, SMP>), dim3(nblocks), dim3(512), 0, st, a);
+  hipLaunchKernelGGL((k_fast_pml48<MODE, FLAV, SMP>), dim3(nblocks), dim3(64 * pml48_waves(MODE)), 0, st, a);
 }
 
 template <int MODE, int FLAV, int SMP>
@@ -897,8 +909,8 @@ static fast_launcher pick(int R, int kind)
 {
   if (kind == 2) return launch_one<MODE, FLAV, 8, 2>;
   if (kind == 5) return launch_one<MODE, FLAV, 8, 5>;
-  if (kind == 6) return launch_pml8<MODE, FLAV, 0>;
-  if (kind == 7) return launch_pml8<MODE, FLAV, 1>;
+  if (kind == 6) return launch_pml48<MODE, FLAV, 0>;
+  if (kind == 7) return launch_pml48<MODE, FLAV, 1>;
   if (kind == 10) return launch_pml16<MODE, FLAV, 0>;
   if (kind == 11) return launch_pml16<MODE, FLAV, 1>;
   // interior tiles of 4R rows as 8 wavefronts of R/2 rows (R = 8, 16)
@@ -926,7 +938,8 @@ static fast_launcher pick(int R, int kind)
 
 // kind 0 (3 with NTFF samples): nblocks = a.nb_pml UPML workgroups + the interior workgroups;
 // kind 1 (4 with samples): interior workgroups only; kind 2 (5 with samples): UPML workgroups only;
-// kind 6 (7 with samples): UPML workgroups of 8 wavefronts x 4 rows (512 threads)
+// kind 6 (7 with samples): wide UPML tiles (TM 12 wavefronts x 4 rows, TE 8 x 6);
+// kind 10 (11 with samples): wide UPML tiles as 16 wavefronts x 3 rows (1024 threads)
 int fast_launch(int mode, int flavour, int R, int kind, const FastArgs &a, int nblocks, hipStream_t st)
 {
   fast_launcher f;
@@ -987,6 +1000,10 @@ struct FastState {
   int qint_p[9], qint_c[9], qpml_p[9], qpml_c[9];
   FastStep hsteps[FAST_MAX_T];
   int T = 4, R = 16, tilesT = -1, tilesR = -1;
+  // rows of the UPML tiles (FAST_UPML_ROWS or FAST_UPML_ROWS_WIDE, by launch
+  // shape: each UPML body covers one height) and the height the lists hold
+  int upmlRows = FAST_UPML_ROWS, tilesU = -1;
+  bool batch = false;  // an instance of a batch: its UPML bodies are the 32-row ones
   int cur = 0, pending = 0;
   int src_on = 0;
   double prev_c_re = 0, prev_c_im = 0;
@@ -1204,16 +1221,42 @@ int fast_state_set_inv_eps(FastState *s, double *const *eps_dev, int NPYa, int o
   return FDTD_OK;
 }
 
-static void tile_band(std::vector<FastTile> &v, int i_lo, int i_hi, int j_lo, int j_hi, int CH, int CW)
+// Cover [lo, hi) of an axis of n cells with tile cores for blocks of B cells and
+// T steps: a core needs T + 1 cells of halo below and T above, except at the
+// grid's edges, whose far side is the zero ghost ring (or a fixed edge) and
+// needs none: a core starting at 0 loads its block from 0 and keeps B - T
+// cells; a block reaching n keeps every core cell up to n.  {core start,
+// core length, block start} per segment.
+struct Seg1 {
+  int c0, n, b0;
+};
+static std::vector<Seg1> cover(int lo, int hi, int n, int B, int T)
 {
-  for (int i0 = i_lo; i0 < i_hi; i0 += CH)
-    for (int j0 = j_lo; j0 < j_hi; j0 += CW) {
+  std::vector<Seg1> v;
+  for (int c = lo; c < hi;) {
+    const int b0 = c == 0 ? 0 : c - (T + 1);
+    const int end = b0 + B >= n ? n : b0 + B - T;
+    const int e = std::min(hi, end);
+    v.push_back({c, e - c, b0});
+    c = e;
+  }
+  return v;
+}
+
+static void tile_band(std::vector<FastTile> &v, int i_lo, int i_hi, int j_lo, int j_hi, int NPX, int NPY, int Brow,
+                      int T)
+{
+  const std::vector<Seg1> rows = cover(i_lo, i_hi, NPX, Brow, T), cols = cover(j_lo, j_hi, NPY, 64, T);
+  for (const Seg1 &r : rows)
+    for (const Seg1 &c : cols) {
       FastTile t;
       t.inst = 0;
-      t.i0 = i0;
-      t.j0 = j0;
-      t.ni = std::min(CH, i_hi - i0);
-      t.nj = std::min(CW, j_hi - j0);
+      t.i0 = r.c0;
+      t.ni = r.n;
+      t.bi = r.b0;
+      t.j0 = c.c0;
+      t.nj = c.n;
+      t.bj = c.b0;
       v.push_back(t);
     }
 }
@@ -1227,7 +1270,8 @@ static int build_tiles(FastState *s)
 {
   const int T = s->T, R = s->R, H0 = T + 1;
   const int NPX = s->h.NPX, NPY = s->h.NPY;
-  const int CW = 64 - 2 * T - 1, CHi = 4 * R - 2 * T - 1, CHp = 32 - 2 * T - 1;
+  const int UR = s->upmlRows;  // rows of a UPML block
+  const int CW = 64 - 2 * T - 1, CHi = 4 * R - 2 * T - 1, CHp = UR - 2 * T - 1;
   if (CW < 1 || CHi < 1 || CHp < 1) return fdtd_fail(FDTD_EINVAL, "temporal block depth %d too deep for R=%d", T, R);
   // extent of the cells without UPML recurrence
   int xlo = 0, xhi = NPX, ylo = 0, yhi = NPY;
@@ -1239,6 +1283,12 @@ static int build_tiles(FastState *s)
     if (s->pmlx[x]) xhi = xlo;  // UPML cells inside: treat everything as UPML
   for (int y = ylo; y < yhi; y++)
     if (s->pmly[y]) yhi = ylo;
+  // the box without UPML cells, where the UPML tiles load no auxiliary
+  s->h.axlo = xlo;
+  s->h.axhi = xhi;
+  s->h.aylo = ylo;
+  s->h.ayhi = yhi;
+  HIPCHK(hipMemcpy(s->dinst, &s->h, sizeof s->h, hipMemcpyHostToDevice));
   if (s->d.flavour == FDTD_FLAVOUR_PLAIN) {
     // the plain solvers never update their edge cells (fdtdTM_upml.c:158-159): keep
     // them out of the interior tiles' dependency cones, which the interior kernel
@@ -1254,13 +1304,13 @@ static int build_tiles(FastState *s)
   const int I_lo = xlo + H0, I_hi = xhi - T, J_lo = ylo + H0, J_hi = yhi - T;
   std::vector<FastTile> ti, tp;
   if (I_lo < I_hi && J_lo < J_hi) {
-    tile_band(ti, I_lo, I_hi, J_lo, J_hi, CHi, CW);
-    tile_band(tp, 0, I_lo, 0, NPY, CHp, CW);
-    tile_band(tp, I_hi, NPX, 0, NPY, CHp, CW);
-    tile_band(tp, I_lo, I_hi, 0, J_lo, CHp, CW);
-    tile_band(tp, I_lo, I_hi, J_hi, NPY, CHp, CW);
+    tile_band(ti, I_lo, I_hi, J_lo, J_hi, NPX, NPY, 4 * R, T);
+    tile_band(tp, 0, I_lo, 0, NPY, NPX, NPY, UR, T);
+    tile_band(tp, I_hi, NPX, 0, NPY, NPX, NPY, UR, T);
+    tile_band(tp, I_lo, I_hi, 0, J_lo, NPX, NPY, UR, T);
+    tile_band(tp, I_lo, I_hi, J_hi, NPY, NPX, NPY, UR, T);
   } else {
-    tile_band(tp, 0, NPX, 0, NPY, CHp, CW);
+    tile_band(tp, 0, NPX, 0, NPY, NPX, NPY, UR, T);
   }
   auto byrow = [](const FastTile &a, const FastTile &b) { return a.i0 != b.i0 ? a.i0 < b.i0 : a.j0 < b.j0; };
   std::sort(ti.begin(), ti.end(), byrow);
@@ -1324,6 +1374,7 @@ static int build_tiles(FastState *s)
   s->cells_per_step = ncell;
   s->tilesT = T;
   s->tilesR = R;
+  s->tilesU = UR;
   return FDTD_OK;
 }
 
@@ -1389,9 +1440,17 @@ int fast_state_flush(FastState *s)
   if (s->pending == 0) return FDTD_OK;
   int rc;
   const int Tsave = s->T;
+  // the UPML bodies of the *_PML8, *_W8 and *_P16 shapes cover the wide tiles
+  {
+    const int lm = s->lmode;
+    const bool wide = !s->batch && (lm == FAST_LAUNCH_SPLIT_PML8 || lm == FAST_LAUNCH_TWO_STREAMS_PML8 ||
+                                    lm == FAST_LAUNCH_SPLIT_W8 || lm == FAST_LAUNCH_TWO_STREAMS_W8 ||
+                                    lm == FAST_LAUNCH_SPLIT_P16 || lm == FAST_LAUNCH_TWO_STREAMS_P16);
+    s->upmlRows = wide ? FAST_UPML_ROWS_WIDE : FAST_UPML_ROWS;
+  }
   // a partial block runs with T = pending; the tiles of the full depth stay valid
   // for any smaller depth, so only the block depth passed to the kernel changes
-  if (s->tilesT != s->T || s->tilesR != s->R)
+  if (s->tilesT != s->T || s->tilesR != s->R || s->tilesU != s->upmlRows)
     if ((rc = build_tiles(s))) return rc;
   FastArgs a;
   memset(&a, 0, sizeof a);
@@ -1766,7 +1825,9 @@ int fast_batch_create(FastState *const *st, const fdtd_step_params *p0, const in
       fast_batch_destroy(b);
       return rc;
     }
-    if (s->tilesT != s->T || s->tilesR != s->R) {
+    s->batch = true;
+    s->upmlRows = FAST_UPML_ROWS;
+    if (s->tilesT != s->T || s->tilesR != s->R || s->tilesU != s->upmlRows) {
       if ((rc = build_tiles(s))) {
         fast_batch_destroy(b);
         return rc;

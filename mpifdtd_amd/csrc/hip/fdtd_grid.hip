@@ -2162,8 +2162,8 @@ extern "C" int fdtd_hip_batch_run(fdtd_hip_batch *b, int blocks)
 
 extern "C" int fdtd_hip_batch_set_launch_mode(fdtd_hip_batch *b, int mode)
 {
-  if (!b || mode < FAST_LAUNCH_AUTO || mode > FAST_LAUNCH_TWO_STREAMS_W8) return fdtd_fail(FDTD_EINVAL, "bad argument");
-  return fast_batch_set_launch(b->fb, mode);
+  if (!b) return fdtd_fail(FDTD_EINVAL, "null batch");
+  return fast_batch_set_launch(b->fb, mode);  // checks the mode
 }
 
 extern "C" int fdtd_hip_batch_sync(fdtd_hip_batch *b, int *steps_done)

@@ -37,9 +37,12 @@ static inline hipError_t dev_malloc(T **p, size_t bytes)
 }
 
 // ---------------------------------------------------------------- FP32 fast path
-// One core tile of the fused kernel: cells [i0, i0+ni) x [j0, j0+nj) of instance `inst`.
+// A tile of instance `inst`: core cells [i0, i0 + ni) x [j0, j0 + nj) written after the block's
+// steps, computed from the block of cells loaded from row bi and column bj
+// (bi = i0 - (T + 1) and bj = j0 - (T + 1) inside the grid; a core that
+// starts at the grid's first row or column needs no halo on that side).
 struct FastTile {
-  int inst, i0, j0, ni, nj;
+  int inst, i0, j0, ni, nj, bi, bj;
 };
 
 // Per inner step scalars of the sources (host-computed in FP64, rounded to FP32).
@@ -53,6 +56,11 @@ struct FastStep {
 };
 
 #define FAST_MAX_T 16
+// rows of a UPML tile's block: the 4-wavefront UPML body (4 x 8 rows, the
+// merged, split and two-stream shapes, and batches) and the wide bodies of the
+// *_PML8, *_W8 and *_P16 shapes (12 x 4 or 16 x 3 rows)
+#define FAST_UPML_ROWS 32
+#define FAST_UPML_ROWS_WIDE 48
 
 // Device view of one FP32 instance (arrays in the physical N_PX x N_PY layout, row pitch `pitch`).
 struct FastInst {
@@ -67,6 +75,9 @@ struct FastInst {
   // 1-D coefficient tables (FP32), same slot meaning as fdtd_coef_tables
   const float *tx[5], *tsx, *ty[4], *tsy;
   const unsigned char *pmlx, *pmly;  // 1 where the cell uses the UPML recurrence
+  // the box [axlo, axhi) x [aylo, ayhi) holds no UPML cell: the UPML tiles load
+  // no auxiliary there (empty when UPML rows or columns lie inside the grid)
+  int axlo, axhi, aylo, ayhi;
   const float2 *ex, *ey;    // e^{i kx x}, e^{i ky y} (MPI scattered wave)
   float cpc, spc, w;        // plain pulse: cos/c0, sin/c0, omega
   float gx[2], gy[2], dot[2];

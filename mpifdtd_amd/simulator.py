@@ -86,9 +86,9 @@ class Simulator:
         check(N.lib().fdtd_solver_setBlocking(int(T), int(R)), "fdtd_solver_setBlocking")
 
     def set_launch_mode(self, mode):
-        """FP32: 0 auto, 1 merged, 2 split, 3 split on two streams, 4 / 5 as 2 / 3 with the UPML
-        tiles as 8 wavefronts of 4 rows, 6 / 7 as 4 / 5 with the interior tiles as 8 wavefronts
-        of R/2 rows, 8 / 9 as 6 / 7 with the UPML tiles as 16 wavefronts of 2 rows
+        """FP32: 0 auto, 1 merged, 2 split, 3 split on two streams, 4 / 5 as 2 / 3 with the 48-row
+        UPML tiles (TM 12 wavefronts of 4 rows, TE 8 of 6), 6 / 7 as 4 / 5 with the interior tiles
+        as 8 wavefronts of R/2 rows, 8 / 9 as 6 / 7 with the UPML tiles as 16 wavefronts of 3 rows
         (FAST_LAUNCH_* of fdtd_hip.h)."""
         check(N.lib().fdtd_solver_setLaunchMode(int(mode)), "fdtd_solver_setLaunchMode")
 

@@ -1,8 +1,11 @@
-"""Reflectance surrogate: an ensemble of MLPs trained on batched-FDTD spectra
+"""Reflectance surrogate: an ensemble of MLPs trained on FDTD spectra
 (mpifdtd_surrogate.h).
 
-    spec = morpho_spec(seed=1, n=2000)                  # seeded geometries + wavelengths
-    X, Y = generate(spec)                               # batched FDTD: features, reflectance
+    geoms = morpho_geometries(seed=1, n=247)            # seeded morpho-scale geometries
+    X, Y = generate_sweep(geoms)                        # 81 wavelengths per geometry: one pulse
+                                                        # run on the 8256 x 4160 grid each
+    spec = morpho_spec(seed=1, n=2000)                  # or: seeded (geometry, wavelength) pairs
+    X, Y = generate(spec)                               # batched CW FDTD on a reduced grid
     ens = Ensemble([13, 512, 512, 512, 180], k=4)
     ens.fit(X, Y, epochs=200)                           # BF16 matrix-core training
     R, used_fdtd = predict_with_fallback(ens, spec_q)   # FDTD where the members disagree
@@ -42,9 +45,15 @@ GEOMETRY = (
 LAMBDA_RANGE = (380, 780)
 N_FEATURES = len(GEOMETRY) + 1
 N_ANGLES = 180
-# the grid every instance runs on: 4.4 um square at h = 10 nm with a 16-cell
-# UPML, normal incidence along +y (angle 90); far-field angles 180..359 face back
+# generate(): the reduced grid every batched CW instance runs on: 4.4 um square
+# at h = 10 nm with a 16-cell UPML, normal incidence along +y (angle 90);
+# far-field angles 180..359 face back
 GRID = dict(side_nm=4400, h_nm=10, pml=16, angle_deg=90, steps=2500)
+# generate_sweep(): the large Morpho grid, 8192 x 4096 cells + a 32-cell UPML
+# (8256 x 4160) at h = 5 nm, the plain pulse solver at normal incidence (+y),
+# 20000 steps, the running DFT of the NTFF contour at 380..780 nm every 5 nm
+SWEEP_GRID = dict(width_nm=41280, height_nm=20800, h_nm=5, pml=32, angle_deg=90, steps=20000)
+SWEEP_LAMBDAS = tuple(range(380, 781, 5))
 
 
 def _lib():
@@ -215,6 +224,15 @@ def morpho_spec(seed=1, n=64):
     return out
 
 
+def morpho_geometries(seed=1, n=64):
+    """Seeded morpho-scale geometries (the 12 parameters only): plain dicts."""
+    g = SplitMix64(seed)
+    out = []
+    for _ in range(n):
+        out.append({name: (g.integer(lo, hi) if kind is int else g.uniform(lo, hi)) for name, lo, hi, kind in GEOMETRY})
+    return out
+
+
 def features(spec):
     """[n, 13] in [0, 1]: the 12 parameters and the wavelength over their ranges."""
     X = np.empty((len(spec), N_FEATURES), np.float32)
@@ -253,6 +271,38 @@ def generate(spec, T=8, R=16, chunk=128, log=None, grid=None):
         if log:
             log(f"generated {min(s0 + chunk, len(spec))}/{len(spec)}")
     return features(spec), Y
+
+
+def generate_sweep(geoms, T=10, R=16, launch=7, log=None, grid=None):
+    """The FDTD spectra of each geometry at the 81 wavelengths SWEEP_LAMBDAS
+    from one run on the large grid (default SWEEP_GRID): the reference's plain
+    TM pulse solver in FP32 with the running DFT of the NTFF contour, the
+    far-field spectra |E|^2 / 8192 (ntffTM.c:197-233) at the exact wavelengths,
+    back-hemisphere angles 180..359.  Returns the features [81 n, 13] and the
+    spectra [81 n, 180], geometry-major."""
+    from .simulator import Simulator, field_info, model_params
+    G = grid or SWEEP_GRID
+    lams = np.asarray(SWEEP_LAMBDAS, np.float64)
+    spec, rows = [], []
+    for k, d in enumerate(geoms):
+        params = {name: d[name] for name, *_ in GEOMETRY}
+        sim = Simulator(N.TM_UPML_2D, N.MORPHO_SCALE,
+                        field_info(G["width_nm"], G["height_nm"], G["h_nm"], G["pml"], 550, G["angle_deg"], G["steps"]),
+                        params=model_params(mo_randomness=0, mo_seed=0, **params), ntff_time=False, precision="fp32",
+                        T=T, R=R)
+        try:
+            sim.set_launch_mode(launch)
+            sim.set_dft(lams)
+            sim.calc(G["steps"])
+            S = sim.dft_spectra()
+        finally:
+            sim.finish()
+        for f, lam in enumerate(lams):
+            spec.append(dict(d, lambda_nm=int(lam)))
+            rows.append(S[f, 180:360])
+        if log and (k % 10 == 0 or k == len(geoms) - 1):
+            log(f"swept {k + 1}/{len(geoms)} geometries")
+    return features(spec), np.asarray(rows, np.float64)
 
 
 def predict_with_fallback(ens, spec, scale, threshold=0.02, fdtd=None):
@@ -310,12 +360,16 @@ class Targets:
 
 
 def experiment(n_train=20000, n_test=2000, epochs=200, k=4, seed=1, threshold=0.02, log=print, save=None,
-               load=None, lr=1e-3, lr_end=None, target="linear"):
+               load=None, lr=1e-3, lr_end=None, target="linear", source="sweep"):
     """Generate, train, evaluate: the numbers of the surrogate's acceptance
     (held-out error, fallback rate, error after fallback, throughput against
-    the batched FDTD), as a dict.  `save` writes the generated spectra to an
-    .npz file, `load` reads them from one instead of running the FDTD (its
-    generation rate is then the one recorded in the file)."""
+    the FDTD that made the data), as a dict.  source "sweep": the spectra of
+    ceil(n / 81) seeded geometries at the 81 wavelengths each, from one pulse
+    run per geometry on the large grid (generate_sweep; held-out geometries
+    from another seed); "cw": n seeded (geometry, wavelength) pairs from
+    batched CW runs on the reduced grid (generate).  `save` writes the spectra
+    to an .npz file, `load` reads them from one instead of running the FDTD
+    (its generation rate is then the one recorded in the file)."""
     import time
     L = _lib()
     if load:
@@ -324,11 +378,15 @@ def experiment(n_train=20000, n_test=2000, epochs=200, k=4, seed=1, threshold=0.
         t_gen = float(z["t_gen"])
         n_train, n_test = len(Xtr), len(Xte)
     else:
-        spec_tr = morpho_spec(seed=seed, n=n_train)
-        spec_te = morpho_spec(seed=seed + 1, n=n_test)
         t0 = time.perf_counter()
-        Xtr, Ytr = generate(spec_tr, log=log)
-        Xte, Yte = generate(spec_te, log=log)
+        if source == "sweep":
+            nl = len(SWEEP_LAMBDAS)
+            Xtr, Ytr = generate_sweep(morpho_geometries(seed, -(-n_train // nl)), log=log)
+            Xte, Yte = generate_sweep(morpho_geometries(seed + 1, -(-n_test // nl)), log=log)
+            n_train, n_test = len(Xtr), len(Xte)
+        else:
+            Xtr, Ytr = generate(morpho_spec(seed=seed, n=n_train), log=log)
+            Xte, Yte = generate(morpho_spec(seed=seed + 1, n=n_test), log=log)
         L.fdtd_hip_synchronize()
         t_gen = time.perf_counter() - t0
         if save:
@@ -363,8 +421,8 @@ def experiment(n_train=20000, n_test=2000, epochs=200, k=4, seed=1, threshold=0.
     sur_rate = reps * 65536 / (time.perf_counter() - t0)
     return {
         "n_train": n_train, "n_test": n_test, "epochs": epochs, "ensemble": k, "lr": lr, "lr_end": lr_end,
-        "target": target,
-        "grid": GRID, "fdtd_spectra_per_s": fdtd_rate, "fdtd_seconds": t_gen,
+        "target": target, "source": source,
+        "grid": SWEEP_GRID if source == "sweep" else GRID, "fdtd_spectra_per_s": fdtd_rate, "fdtd_seconds": t_gen,
         "train_seconds": t_train, "final_train_mse": hist[-1],
         "heldout_rel_err_mean": float(err.mean()), "heldout_rel_err_median": float(np.median(err)),
         "heldout_rel_err_p95": float(np.quantile(err, 0.95)),
@@ -415,6 +473,9 @@ if __name__ == "__main__":
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--lr-end", type=float, default=None, help="cosine-anneal the rate to this value")
     ap.add_argument("--target", default="linear", choices=("linear", "log"))
+    ap.add_argument("--source", default="sweep", choices=("sweep", "cw"),
+                    help="sweep: 81 wavelengths per geometry on the 8256 x 4160 grid; cw: batched CW runs on the "
+                         "reduced grid")
     ap.add_argument("--save", default=None, help="write the generated spectra to this .npz")
     ap.add_argument("--load", default=None, help="read the spectra from this .npz instead of running the FDTD")
     ap.add_argument("--subsets", default=None,
@@ -433,5 +494,5 @@ if __name__ == "__main__":
         print(json.dumps({"data_scaling": rows, "median_exponent": slope}), flush=True)
         raise SystemExit(0)
     res = experiment(a.train, a.test, a.epochs, a.k, a.seed, log=lambda s: print(s, flush=True), save=a.save,
-                     load=a.load, lr=a.lr, lr_end=a.lr_end, target=a.target)
+                     load=a.load, lr=a.lr, lr_end=a.lr_end, target=a.target, source=a.source)
     print(json.dumps(res), flush=True)

@@ -8,6 +8,23 @@
  * (double complex, real x complex products, no FMA) so that the restatement
  * can be checked bit-for-bit against the compiled reference (oracle/_ref).
  * Each function cites the reference file:line it restates.
+ *
+ * How each part is pinned (tests/):
+ *   steps of all eight solvers, eps maps of the five models, ntffTM_Frequency,
+ *   the plain solvers' spectra, the MPI-TE Eth/Eph output, the field.h
+ *   helpers: bit-exact (spectra 1e-12, text files to their printed
+ *   resolution) against golden runs of the compiled reference
+ *   (test_oracle_golden.py, test_field_util.py);
+ *   the TE far field (no working reference routine): the Mie series of the
+ *   dielectric cylinder (test_oracle_mie.py);
+ *   mpi_tm_ntff (accumulators the reference never writes out): the CW
+ *   steady-state identity with the golden-pinned frequency-domain NTFF, 7.5e-5
+ *   (test_oracle_ntff_pin.py);
+ *   the point source and the Courant setting (never used by the reference):
+ *   closed forms of the first steps, mirror symmetry, the front speed
+ *   (test_oracle_ntff_pin.py);
+ *   the whole file: clean under ASan / UBSan (san_driver.c,
+ *   test_oracle_sanitized.py).
  */
 #include <complex.h>
 #include <math.h>

@@ -16,11 +16,13 @@ pytestmark = pytest.mark.gpu
 
 # solver, model, width_nm, height_nm, h, pml, lambda, angle, steps, params
 CASES = [
-    (4, 1, 6000, 6000, 20, 16, 500, 0, 150, {}),                                # MPI TM Mie, 300 x 300
-    (5, 1, 8400, 7200, 20, 16, 560, 30, 231, {}),                               # MPI TE Mie, 420 x 360, oblique
-    (4, 2, 5120, 5120, 20, 12, 610, 0, 97, {"ml_layer_num": 3}),                # MPI TM multilayer, stops early
-    (5, 5, 6400, 6400, 20, 16, 450, 0, 180, {"zz_layer_num": 3}),               # MPI TE zigzag
+    (4, 1, 6000, 6000, 20, 16, 500, 0, 400, {}),                                # MPI TM Mie, 300 x 300
+    (5, 1, 8400, 7200, 20, 16, 560, 30, 450, {}),                               # MPI TE Mie, 420 x 360, oblique
+    (4, 2, 5120, 5120, 20, 12, 610, 0, 300, {"ml_layer_num": 3}),               # MPI TM multilayer, stops early
+    (5, 5, 6400, 6400, 20, 16, 450, 0, 420, {"zz_layer_num": 3}),               # MPI TE zigzag
 ]
+# every instance runs long enough for its scattered wave to cross the NTFF
+# contour: far fields of 1.5e-2 to 5e-2 (oracle), none of them zero
 T, R = 4, 16
 
 

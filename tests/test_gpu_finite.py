@@ -1,6 +1,8 @@
 """Non-finite field detection on the GPU (fdtd_solver_checkFinite and the
 periodic check of fdtd_solver_setFiniteCheck), with the condition forced from
-the host: a NaN written into one Ez cell through fdtd_solver_setState."""
+the host: a NaN written into one cell of Ez and of Dz through
+fdtd_solver_setState (the FP64 MPI step rebuilds Ez from Dz every step, the
+FP32 interior update carries Ez itself)."""
 import numpy as np
 import pytest
 
@@ -17,10 +19,11 @@ def test_host_forced_nan_is_counted_and_recorded(precision):
         sim.calc(20)
         assert sim.check_finite() == [0, 0, 0]
         assert sim.finite_status() is None
-        ez = sim.state(0)
         c = sim.NPXa // 2
-        ez[c, c] = np.nan
-        sim.set_state(0, ez)
+        for slot in (0, 2):  # Ez, Dz
+            a = sim.state(slot)
+            a[c, c] = np.nan
+            sim.set_state(slot, a)
         assert sim.check_finite() == [1, 0, 0]
         sim.set_finite_check(4)
         sim.calc(8)

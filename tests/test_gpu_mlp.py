@@ -154,3 +154,41 @@ def test_train_epoch_matches_steps_over_the_same_batches():
         assert np.array_equal(wa, wb) and np.array_equal(ba, bb)
     a.close()
     b.close()
+
+
+def test_configured_shape_one_training_step():
+    """The surrogate's configured network, 13 -> 512 -> 512 -> 512 -> 180 at a
+    batch of 1024: forward outputs and every layer's gradients against the FP64
+    oracle (3e-2, the BF16 activations' share), then one Adam step equal to the
+    oracle's Adam on the step's own gradients to FP32 rounding."""
+    N, S, om = _mods()
+    dims, B = [13, 512, 512, 512, 180], 1024
+    rng = np.random.default_rng(11)
+    net = S.MLP(dims, batch_max=B, seed=11)
+    params = [((rng.standard_normal((dims[i + 1], dims[i])) / np.sqrt(dims[i])).astype(np.float32),
+               (0.1 * rng.standard_normal(dims[i + 1])).astype(np.float32)) for i in range(len(dims) - 1)]
+    for l, (w, b) in enumerate(params):
+        net.set_params(l, w, b)
+    x = rng.random((B, dims[0])).astype(np.float32)
+    t = rng.standard_normal((B, dims[-1])).astype(np.float32)
+    y = net.forward(x)
+    y_ref, _ = om.forward(params, x)
+    assert rel_l2(y, y_ref) <= 3e-2, rel_l2(y, y_ref)
+    g_ref, loss_ref = om.grads(params, x, t)
+    lr = 1e-3
+    loss = net.train_step(x, t, lr)
+    assert abs(loss - loss_ref) <= 3e-2 * loss_ref
+    gr = []
+    for l in range(len(params)):
+        dw, db = net.layer_grads(l)
+        assert rel_l2(dw, g_ref[l][0]) <= 3e-2, (l, rel_l2(dw, g_ref[l][0]))
+        assert rel_l2(db, g_ref[l][1]) <= 3e-2, (l, rel_l2(db, g_ref[l][1]))
+        gr.append((np.float64(dw), np.float64(db)))
+    st = [[np.zeros_like(w, np.float64), np.zeros_like(w, np.float64), np.zeros_like(b, np.float64),
+           np.zeros_like(b, np.float64)] for w, b in params]
+    ref = om.adam_step(params, gr, st, 1, lr)
+    for l in range(len(params)):
+        w, b = net.params(l)
+        assert np.max(np.abs(w - ref[l][0])) <= 5e-7
+        assert np.max(np.abs(b - ref[l][1])) <= 5e-7
+    net.close()
