@@ -163,6 +163,17 @@ int fdtd_solver_getNTFFTime(int which, double *out);
  * TM ntffTM_Frequency (ntffTM.c:72-158); TE single-rank ntffTE_FrequencySplit
  * (ntffTE.c:341-470) with R0 of ntffTE_init */
 int fdtd_solver_farField(double *out360);
+/* fdtd_solver_farField without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops).  The fields are read in place (FP32
+ * solvers in FP32, without the FP64 export buffers fdtd_solver_farField
+ * allocates on its first call) and the partial sums live in a file-scope
+ * device array of 1.11 MB per GPU, shared by every call on that GPU: calls that
+ * overlap on one GPU race.  FDTD_EINVAL for an NTFF contour of more than 65536
+ * cells (sides up to ~16000 cells), FDTD_ESTATE while FP32 steps are queued
+ * (any getter launches them).  Same values as fdtd_solver_farField up to the
+ * order of the FP64 sums. */
+int fdtd_solver_farFieldNoAlloc(double *out360);
 /* time-domain far-field spectra 380..700 nm x 360 angles, as the plain
  * solvers' ntff*_TimeOutput computes them (ntffTM.c:197-233, ntffTE.c:160-196) */
 int fdtd_solver_timeSpectra(double *out321x360);

@@ -142,6 +142,7 @@ def lib():
     sig("fdtd_solver_setState", I, I, dp)
     sig("fdtd_solver_getNTFFTime", I, I, dp)
     sig("fdtd_solver_farField", I, dp)
+    sig("fdtd_solver_farFieldNoAlloc", I, dp)
     sig("fdtd_solver_timeSpectra", I, dp)
     sig("fdtd_solver_setDFT", I, dp, I)
     sig("fdtd_solver_getDFT", I, dp)

@@ -1663,6 +1663,14 @@ int fast_state_fields(FastState *s, const float2 *f[3], size_t *n)
   return FDTD_OK;
 }
 
+int fast_state_pending(const FastState *s) { return s->pending; }
+
+int fast_state_pitch(const FastState *s, int *pitch)
+{
+  *pitch = s->h.pitch;
+  return FDTD_OK;
+}
+
 int fast_state_gather(FastState *s, const Contour &ct, int off, double2 *cE, double2 *cH)
 {
   int rc;

@@ -988,109 +988,220 @@ struct FFArgs {
   const double *cosA, *sinA;
 };
 
-// ntffTM_Frequency (ntffTM.c:72-158) on the physical cells of either layout
-__global__ __launch_bounds__(256) void k_far_tm(const double2 *Ez, const double2 *Hx, const double2 *Hy, FFArgs a, double2 *out)
+// A field of either precision read in FP64 at physical cell (x, y): stored at
+// (x + off, y + off) of an nx x ny array with row pitch `pitch` (FP64 grids: the
+// solver layout, off 1 for the padded MPI arrays; FP32 grids: the physical
+// layout, off 0).  at() reads 0 outside the stored extent (the MPI ghost ring).
+template <class V>
+struct FieldView {
+  const V *p;
+  int nx, ny, off, pitch;
+  __device__ __forceinline__ double2 atu(int x, int y) const
+  {
+    const V v = p[(size_t)(x + off) * pitch + (y + off)];
+    return make_double2((double)v.x, (double)v.y);
+  }
+  __device__ __forceinline__ double2 at(int x, int y) const
+  {
+    const int ax = x + off, ay = y + off;
+    if (ax < 0 || ay < 0 || ax >= nx || ay >= ny) return make_double2(0.0, 0.0);
+    return atu(x, y);
+  }
+};
+
+// contour cells of ntffTM_Frequency (ntffTM.c:72-158): bottom, right, top, left
+__host__ __device__ __forceinline__ int far_tm_cells(const FFArgs &a)
+{
+  return 2 * (a.right - a.left) + 2 * (a.top - a.bottom);
+}
+
+// contour cell c of ntffTM_Frequency: its terms added to Nz, Lx, Ly
+template <class V>
+__device__ __forceinline__ void far_tm_cell(int c, const FieldView<V> &Ez, const FieldView<V> &Hx, const FieldView<V> &Hy,
+                                            const FFArgs &a, double rx, double ry, double2 &Nz, double2 &Lx, double2 &Ly)
+{
+  const int nx = a.right - a.left, ny = a.top - a.bottom;
+  int side, px, py;
+  if (c < nx) { side = 0; px = a.left + c; py = a.bottom; }
+  else if (c < nx + ny) { side = 1; px = a.right; py = a.bottom + (c - nx); }
+  else if (c < 2 * nx + ny) { side = 2; px = a.left + (c - nx - ny); py = a.top; }
+  else { side = 3; px = a.left; py = a.bottom + (c - 2 * nx - ny); }
+  const double r2x = px - a.cx, r2y = py - a.cy;
+  const double ip = rx * r2x + ry * r2y;
+  double sn, cs;
+  sincos(a.k * ip, &sn, &cs);
+  const double2 ez = Ez.atu(px, py);
+  double2 hv;
+  if (side == 0 || side == 2) {
+    const double2 h0 = Hx.atu(px, py), h1 = Hx.atu(px, py - 1);
+    hv.x = 0.5 * (h0.x + h1.x);
+    hv.y = 0.5 * (h0.y + h1.y);
+  } else {
+    const double2 h0 = Hy.atu(px, py), h1 = Hy.atu(px - 1, py);
+    hv.x = 0.5 * (h0.x + h1.x);
+    hv.y = 0.5 * (h0.y + h1.y);
+  }
+  const double sg = (side < 2) ? 1.0 : -1.0;
+  cmul_acc(Nz, hv, cs, sn, sg);
+  if (side == 0 || side == 2) cmul_acc(Lx, ez, cs, sn, sg);
+  else cmul_acc(Ly, ez, cs, sn, sg);
+}
+
+// v = {Nz, Lx, Ly}: Lphi = -Lx sin + Ly cos; E = coef (Z0 Nz + Lphi) sqrt(h)
+__device__ __forceinline__ double2 far_tm_result(const double v[6], const FFArgs &a, double rx, double ry)
+{
+  const double sn = ry, cs = rx;
+  const double lpr = -v[2] * sn + v[4] * cs, lpi = -v[3] * sn + v[5] * cs;
+  const double tr = a.z0 * v[0] + lpr, ti = a.z0 * v[1] + lpi;
+  double2 r;
+  r.x = (a.coef_re * tr - a.coef_im * ti) * a.scale;
+  r.y = (a.coef_re * ti + a.coef_im * tr) * a.scale;
+  return r;
+}
+
+// side extents of single-rank ntffTE_FrequencySplit (ntffTE.c:341-470) in padded coordinates
+struct TeSides {
+  int L, B, nb, nr, nt, nl;
+};
+__host__ __device__ __forceinline__ TeSides far_te_sides(const FFArgs &a)
+{
+  const int SPX = a.N_PX + 2, SPY = a.N_PY + 2;
+  const int L = a.left > 1 ? a.left : 1, Rx = a.right < SPX - 1 ? a.right : SPX - 1;
+  const int B = a.bottom > 1 ? a.bottom : 1, T = a.top < SPY - 1 ? a.top : SPY - 1;
+  const int w = Rx - L > 0 ? Rx - L : 0, h = T - B > 0 ? T - B : 0;
+  TeSides t;
+  t.L = L;
+  t.B = B;
+  t.nb = (0 < a.bottom && a.bottom < SPY - 1) ? w : 0;
+  t.nr = (0 < a.right && a.right < SPX - 1) ? h : 0;
+  t.nt = (0 < a.top && a.top < SPY - 1) ? w : 0;
+  t.nl = (0 < a.left && a.left < SPX - 1) ? h : 0;
+  return t;
+}
+
+// contour cell c of the TE far field: its terms added to Nx, Ny, Lz
+template <class V>
+__device__ __forceinline__ void far_te_cell(int c, const TeSides &t, const FieldView<V> &Ex, const FieldView<V> &Ey,
+                                            const FieldView<V> &Hz, const FFArgs &a, double rx, double ry, double2 &Nx,
+                                            double2 &Ny, double2 &Lz)
+{
+  int side, pi, pj;
+  double r2x, r2y;
+  if (c < t.nb) { side = 0; pi = t.L + c; pj = a.bottom; r2x = pi - a.cx + 0.5; r2y = pj - a.cy; }
+  else if (c < t.nb + t.nr) { side = 1; pi = a.right; pj = t.B + (c - t.nb); r2x = pi - a.cx; r2y = pj - a.cy + 0.5; }
+  else if (c < t.nb + t.nr + t.nt) { side = 2; pi = t.L + (c - t.nb - t.nr); pj = a.top; r2x = pi - a.cx + 0.5; r2y = pj - a.cy; }
+  else { side = 3; pi = a.left; pj = t.B + (c - t.nb - t.nr - t.nt); r2x = pi - a.cx; r2y = pj - a.cy + 0.5; }
+  const double ip = rx * r2x + ry * r2y;
+  double sn, cs;
+  sincos(a.k * ip, &sn, &cs);
+  // padded (pi, pj) is physical (pi - 1, pj - 1); the ghost ring reads 0
+  const double2 h0 = Hz.at(pi - 1, pj - 1);
+  const double2 h1 = (side == 0 || side == 2) ? Hz.at(pi - 1, pj - 2) : Hz.at(pi - 2, pj - 1);
+  double2 hv;
+  hv.x = 0.5 * (h0.x + h1.x);
+  hv.y = 0.5 * (h0.y + h1.y);
+  const double2 ev = (side == 0 || side == 2) ? Ex.at(pi - 1, pj - 1) : Ey.at(pi - 1, pj - 1);
+  const double sg = (side < 2) ? -1.0 : 1.0;
+  if (side == 0 || side == 2) cmul_acc(Nx, hv, cs, sn, sg);
+  else cmul_acc(Ny, hv, cs, sn, sg);
+  cmul_acc(Lz, ev, cs, sn, sg);
+}
+
+// v = {Nx, Ny, Lz}: N_phi = -Nx sin + Ny cos; E = coef (Z0 N_phi - Lz) h
+__device__ __forceinline__ double2 far_te_result(const double v[6], const FFArgs &a, double rx, double ry)
+{
+  const double sn = ry, cs = rx;
+  const double npr = -v[0] * sn + v[2] * cs, npi = -v[1] * sn + v[3] * cs;
+  const double tr = a.z0 * npr - v[4], ti = a.z0 * npi - v[5];
+  double2 r;
+  r.x = (a.coef_re * tr - a.coef_im * ti) * a.scale;
+  r.y = (a.coef_re * ti + a.coef_im * tr) * a.scale;
+  return r;
+}
+
+__host__ __device__ __forceinline__ int far_cells(int mode, const FFArgs &a)
+{
+  if (mode == FDTD_MODE_TM) return far_tm_cells(a);
+  const TeSides t = far_te_sides(a);
+  return t.nb + t.nr + t.nt + t.nl;
+}
+
+// sums over contour cells [c0, c1) of angle `ang`: {Nz, Lx, Ly} (TM) or {Nx, Ny, Lz} (TE), block-reduced
+template <int MODE, class V>
+__device__ __forceinline__ void far_sums(const FieldView<V> &f0, const FieldView<V> &f1, const FieldView<V> &f2,
+                                         const FFArgs &a, int ang, int c0, int c1, double v[6], double *red)
+{
+  const double rx = a.cosA[ang], ry = a.sinA[ang];
+  double2 s0 = {0, 0}, s1 = {0, 0}, s2 = {0, 0};
+  const TeSides t = far_te_sides(a);
+  for (int c = c0 + (int)threadIdx.x; c < c1; c += blockDim.x) {
+    if (MODE == FDTD_MODE_TM) far_tm_cell(c, f0, f1, f2, a, rx, ry, s0, s1, s2);
+    else far_te_cell(c, t, f0, f1, f2, a, rx, ry, s0, s1, s2);
+  }
+  v[0] = s0.x; v[1] = s0.y; v[2] = s1.x; v[3] = s1.y; v[4] = s2.x; v[5] = s2.y;
+  for (int q = 0; q < 6; q++) v[q] = block_sum(v[q], red);
+}
+
+// ntffTM_Frequency (ntffTM.c:72-158) / single-rank ntffTE_FrequencySplit
+// (ntffTE.c:341-470) on the physical cells of either layout: one workgroup per angle
+template <int MODE>
+__global__ __launch_bounds__(256) void k_far(const double2 *F0, const double2 *F1, const double2 *F2, FFArgs a, double2 *out)
 {
   __shared__ double red[8];
   const int ang = blockIdx.x;
-  const double rx = a.cosA[ang], ry = a.sinA[ang];
-  const int NPY = a.NPYa;
-  auto idx = [&](int px, int py) { return (size_t)(px + a.off) * NPY + (py + a.off); };
-  double2 Nz = {0, 0}, Lx = {0, 0}, Ly = {0, 0};
-  const int nx = a.right - a.left, ny = a.top - a.bottom;
-  for (int c = threadIdx.x; c < 2 * nx + 2 * ny; c += blockDim.x) {
-    int side, px, py;
-    if (c < nx) { side = 0; px = a.left + c; py = a.bottom; }
-    else if (c < nx + ny) { side = 1; px = a.right; py = a.bottom + (c - nx); }
-    else if (c < 2 * nx + ny) { side = 2; px = a.left + (c - nx - ny); py = a.top; }
-    else { side = 3; px = a.left; py = a.bottom + (c - 2 * nx - ny); }
-    const double r2x = px - a.cx, r2y = py - a.cy;
-    const double ip = rx * r2x + ry * r2y;
-    double sn, cs;
-    sincos(a.k * ip, &sn, &cs);
-    const size_t kk = idx(px, py);
-    const double2 ez = Ez[kk];
-    double2 hv;
-    if (side == 0 || side == 2) {
-      const double2 h0 = Hx[kk], h1 = Hx[kk - 1];
-      hv.x = 0.5 * (h0.x + h1.x);
-      hv.y = 0.5 * (h0.y + h1.y);
-    } else {
-      const double2 h0 = Hy[kk], h1 = Hy[kk - NPY];
-      hv.x = 0.5 * (h0.x + h1.x);
-      hv.y = 0.5 * (h0.y + h1.y);
-    }
-    const double sg = (side < 2) ? 1.0 : -1.0;
-    cmul_acc(Nz, hv, cs, sn, sg);
-    if (side == 0 || side == 2) cmul_acc(Lx, ez, cs, sn, sg);
-    else cmul_acc(Ly, ez, cs, sn, sg);
-  }
-  double v[6] = {Nz.x, Nz.y, Lx.x, Lx.y, Ly.x, Ly.y};
-  for (int q = 0; q < 6; q++) v[q] = block_sum(v[q], red);
+  const FieldView<double2> f0 = {F0, a.NPXa, a.NPYa, a.off, a.NPYa}, f1 = {F1, a.NPXa, a.NPYa, a.off, a.NPYa},
+                           f2 = {F2, a.NPXa, a.NPYa, a.off, a.NPYa};
+  double v[6];
+  far_sums<MODE>(f0, f1, f2, a, ang, 0, far_cells(MODE, a), v, red);
+  if (threadIdx.x == 0) out[ang] = MODE == FDTD_MODE_TM ? far_tm_result(v, a, a.cosA[ang], a.sinA[ang])
+                                                        : far_te_result(v, a, a.cosA[ang], a.sinA[ang]);
+}
+
+// ---------------------------------------------------------------- allocation-free far field
+//
+// The same far field without any device allocation: the fields are read in
+// place (FP32 grids in FP32, converted per value, instead of exported to FP64
+// arrays first), the contour of each angle is cut into parts of FF_NA_CHUNK
+// cells, one workgroup per (part, angle), and the parts' sums and the 360
+// results live in the file-scope array `scratch` below.  Every call on a GPU
+// shares that GPU's copy: calls that overlap on one GPU race.
+#define FF_NA_CHUNK 1024
+#define FF_NA_PARTS 64  // at most 65536 contour cells
+#define FF_NA_SCRATCH (360 * FF_NA_PARTS * 3 + 360)
+static __device__ __align__(256) double2 scratch[FF_NA_SCRATCH];  // 1.11 MB per GPU
+
+template <int MODE, class V>
+__global__ __launch_bounds__(256) void k_far_part(FieldView<V> f0, FieldView<V> f1, FieldView<V> f2, FFArgs a, int n,
+                                                  double2 *part)
+{
+  __shared__ double red[8];
+  const int p = blockIdx.x, ang = blockIdx.y;
+  double v[6];
+  far_sums<MODE>(f0, f1, f2, a, ang, p * FF_NA_CHUNK, min(n, (p + 1) * FF_NA_CHUNK), v, red);
   if (threadIdx.x == 0) {
-    const double sn = ry, cs = rx;
-    // Lphi = -Lx sin + Ly cos ; E = coef (Z0 Nz + Lphi) sqrt(h)
-    const double lpr = -v[2] * sn + v[4] * cs, lpi = -v[3] * sn + v[5] * cs;
-    const double tr = a.z0 * v[0] + lpr, ti = a.z0 * v[1] + lpi;
-    double2 r;
-    r.x = (a.coef_re * tr - a.coef_im * ti) * a.scale;
-    r.y = (a.coef_re * ti + a.coef_im * tr) * a.scale;
-    out[ang] = r;
+    double2 *o = part + ((size_t)ang * FF_NA_PARTS + p) * 3;
+    o[0] = make_double2(v[0], v[1]);
+    o[1] = make_double2(v[2], v[3]);
+    o[2] = make_double2(v[4], v[5]);
   }
 }
 
-// single-rank ntffTE_FrequencySplit (ntffTE.c:341-470) in padded coordinates
-__global__ __launch_bounds__(256) void k_far_te(const double2 *Ex, const double2 *Ey, const double2 *Hz, FFArgs a, double2 *out)
+// the parts of each angle summed in order, then the angle's far-field value
+template <int MODE>
+__global__ __launch_bounds__(64) void k_far_join(const double2 *part, int nparts, FFArgs a, double2 *out)
 {
-  __shared__ double red[8];
-  const int ang = blockIdx.x;
-  const double rx = a.cosA[ang], ry = a.sinA[ang];
-  const int SPX = a.N_PX + 2, SPY = a.N_PY + 2;
-  // padded (pi, pj) -> value in the solver layout; the ghost ring reads 0
-  auto val = [&](const double2 *f, int pi, int pj) -> double2 {
-    const int ai = pi - 1 + a.off, aj = pj - 1 + a.off;
-    if (ai < 0 || aj < 0 || ai >= a.NPXa || aj >= a.NPYa) return make_double2(0.0, 0.0);
-    return f[(size_t)ai * a.NPYa + aj];
-  };
-  const int L = max(1, a.left), Rx = min(SPX - 1, a.right);
-  const int B = max(1, a.bottom), T = min(SPY - 1, a.top);
-  const int nb = (0 < a.bottom && a.bottom < SPY - 1) ? max(0, Rx - L) : 0;
-  const int nr = (0 < a.right && a.right < SPX - 1) ? max(0, T - B) : 0;
-  const int nt = (0 < a.top && a.top < SPY - 1) ? max(0, Rx - L) : 0;
-  const int nl = (0 < a.left && a.left < SPX - 1) ? max(0, T - B) : 0;
-  double2 Nx = {0, 0}, Ny = {0, 0}, Lz = {0, 0};
-  for (int c = threadIdx.x; c < nb + nr + nt + nl; c += blockDim.x) {
-    int side, pi, pj;
-    double r2x, r2y;
-    if (c < nb) { side = 0; pi = L + c; pj = a.bottom; r2x = pi - a.cx + 0.5; r2y = pj - a.cy; }
-    else if (c < nb + nr) { side = 1; pi = a.right; pj = B + (c - nb); r2x = pi - a.cx; r2y = pj - a.cy + 0.5; }
-    else if (c < nb + nr + nt) { side = 2; pi = L + (c - nb - nr); pj = a.top; r2x = pi - a.cx + 0.5; r2y = pj - a.cy; }
-    else { side = 3; pi = a.left; pj = B + (c - nb - nr - nt); r2x = pi - a.cx; r2y = pj - a.cy + 0.5; }
-    const double ip = rx * r2x + ry * r2y;
-    double sn, cs;
-    sincos(a.k * ip, &sn, &cs);
-    const double2 h0 = val(Hz, pi, pj);
-    const double2 h1 = (side == 0 || side == 2) ? val(Hz, pi, pj - 1) : val(Hz, pi - 1, pj);
-    double2 hv;
-    hv.x = 0.5 * (h0.x + h1.x);
-    hv.y = 0.5 * (h0.y + h1.y);
-    const double2 ev = (side == 0 || side == 2) ? val(Ex, pi, pj) : val(Ey, pi, pj);
-    const double sg = (side < 2) ? -1.0 : 1.0;
-    if (side == 0 || side == 2) cmul_acc(Nx, hv, cs, sn, sg);
-    else cmul_acc(Ny, hv, cs, sn, sg);
-    cmul_acc(Lz, ev, cs, sn, sg);
+  const int ang = blockIdx.x * 64 + threadIdx.x;
+  if (ang >= 360) return;
+  double v[6] = {0, 0, 0, 0, 0, 0};
+  for (int p = 0; p < nparts; p++) {
+    const double2 *o = part + ((size_t)ang * FF_NA_PARTS + p) * 3;
+    for (int q = 0; q < 3; q++) {
+      v[2 * q] += o[q].x;
+      v[2 * q + 1] += o[q].y;
+    }
   }
-  double v[6] = {Nx.x, Nx.y, Ny.x, Ny.y, Lz.x, Lz.y};
-  for (int q = 0; q < 6; q++) v[q] = block_sum(v[q], red);
-  if (threadIdx.x == 0) {
-    const double sn = ry, cs = rx;
-    const double npr = -v[0] * sn + v[2] * cs, npi = -v[1] * sn + v[3] * cs;
-    const double tr = a.z0 * npr - v[4], ti = a.z0 * npi - v[5];
-    double2 r;
-    r.x = (a.coef_re * tr - a.coef_im * ti) * a.scale;
-    r.y = (a.coef_re * ti + a.coef_im * tr) * a.scale;
-    out[ang] = r;
-  }
+  out[ang] = MODE == FDTD_MODE_TM ? far_tm_result(v, a, a.cosA[ang], a.sinA[ang])
+                                  : far_te_result(v, a, a.cosA[ang], a.sinA[ang]);
 }
 
 // ---------------------------------------------------------------- time-domain far field + spectra
@@ -1978,13 +2089,84 @@ extern "C" int fdtd_hip_grid_far_field(fdtd_grid *g, double k, double coef_re, d
   a.coef_re = coef_re; a.coef_im = coef_im; a.scale = scale;
   a.cosA = g->cosA; a.sinA = g->sinA;
   if (d.mode == FDTD_MODE_TM)
-    hipLaunchKernelGGL(k_far_tm, dim3(360), dim3(256), 0, 0, g->F[0], g->F[3], g->F[6], a, g->ffout);
+    hipLaunchKernelGGL(k_far<FDTD_MODE_TM>, dim3(360), dim3(256), 0, 0, g->F[0], g->F[3], g->F[6], a, g->ffout);
   else {
     a.cx = d.N_PX / 2; a.cy = d.N_PY / 2;
-    hipLaunchKernelGGL(k_far_te, dim3(360), dim3(256), 0, 0, g->F[0], g->F[3], g->F[6], a, g->ffout);
+    hipLaunchKernelGGL(k_far<FDTD_MODE_TE>, dim3(360), dim3(256), 0, 0, g->F[0], g->F[3], g->F[6], a, g->ffout);
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpy(out360, g->ffout, 360 * sizeof(double2), hipMemcpyDeviceToHost));
+  return FDTD_OK;
+}
+
+// Device address of `scratch` on each GPU, looked up on the first call there
+// (the array's name is a host-side stand-in that kernels cannot use).
+static std::mutex g_scratch_mu;
+static std::map<int, double2 *> g_scratch_ptr;
+
+static int scratch_ptr(double2 **out)
+{
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_ptr.find(dev);
+  if (it == g_scratch_ptr.end()) {
+    void *p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    it = g_scratch_ptr.emplace(dev, (double2 *)p).first;
+  }
+  *out = it->second;
+  return FDTD_OK;
+}
+
+// fdtd_hip_grid_far_field without any allocation (see the scratch array above):
+// FDTD_ESTATE while FP32 steps are queued (launching them may build tiles),
+// FDTD_EINVAL for a contour longer than FF_NA_PARTS * FF_NA_CHUNK cells.
+extern "C" __attribute__((visibility("hidden"))) int fdtd_hip_grid_far_field_noalloc(fdtd_grid *g, double k, double coef_re, double coef_im,
+                                                         double scale, double *out360)
+{
+  if (!g || !out360) return fdtd_fail(FDTD_EINVAL, "null argument");
+  const fdtd_grid_desc &d = g->d;
+  FFArgs a;
+  a.N_PX = d.N_PX; a.N_PY = d.N_PY; a.NPXa = g->NPXa; a.NPYa = g->NPYa; a.off = g->off;
+  a.top = d.ntff_top; a.bottom = d.ntff_bottom; a.left = d.ntff_left; a.right = d.ntff_right;
+  a.cx = d.ntff_cx; a.cy = d.ntff_cy; a.k = k; a.z0 = d.z0;
+  a.coef_re = coef_re; a.coef_im = coef_im; a.scale = scale;
+  a.cosA = g->cosA; a.sinA = g->sinA;
+  if (d.mode == FDTD_MODE_TE) { a.cx = d.N_PX / 2; a.cy = d.N_PY / 2; }
+  const int n = far_cells(d.mode, a);
+  const int nparts = (n + FF_NA_CHUNK - 1) / FF_NA_CHUNK;
+  if (nparts > FF_NA_PARTS)
+    return fdtd_fail(FDTD_EINVAL, "allocation-free far field: contour of %d cells exceeds the limit of %d", n,
+                     FF_NA_PARTS * FF_NA_CHUNK);
+  double2 *scr;
+  int rc;
+  if ((rc = scratch_ptr(&scr))) return rc;
+  double2 *part = scr, *res = scr + (size_t)360 * FF_NA_PARTS * 3;
+  const dim3 grid(std::max(1, nparts), 360);
+  if (g->fs) {
+    if (fast_state_pending(g->fs))
+      return fdtd_fail(FDTD_ESTATE, "allocation-free far field: FP32 steps are queued; launch them first "
+                                    "(fdtd_hip_grid_flush)");
+    const float2 *f[3];
+    size_t nc = 0;
+    int pitch = 0;
+    if ((rc = fast_state_fields(g->fs, f, &nc)) || (rc = fast_state_pitch(g->fs, &pitch))) return rc;
+    const FieldView<float2> v0 = {f[0], d.N_PX, d.N_PY, 0, pitch}, v1 = {f[1], d.N_PX, d.N_PY, 0, pitch},
+                            v2 = {f[2], d.N_PX, d.N_PY, 0, pitch};
+    if (d.mode == FDTD_MODE_TM) hipLaunchKernelGGL((k_far_part<FDTD_MODE_TM, float2>), grid, dim3(256), 0, 0, v0, v1, v2, a, n, part);
+    else hipLaunchKernelGGL((k_far_part<FDTD_MODE_TE, float2>), grid, dim3(256), 0, 0, v0, v1, v2, a, n, part);
+  } else {
+    const FieldView<double2> v0 = {g->F[0], g->NPXa, g->NPYa, g->off, g->NPYa},
+                             v1 = {g->F[3], g->NPXa, g->NPYa, g->off, g->NPYa},
+                             v2 = {g->F[6], g->NPXa, g->NPYa, g->off, g->NPYa};
+    if (d.mode == FDTD_MODE_TM) hipLaunchKernelGGL((k_far_part<FDTD_MODE_TM, double2>), grid, dim3(256), 0, 0, v0, v1, v2, a, n, part);
+    else hipLaunchKernelGGL((k_far_part<FDTD_MODE_TE, double2>), grid, dim3(256), 0, 0, v0, v1, v2, a, n, part);
+  }
+  if (d.mode == FDTD_MODE_TM) hipLaunchKernelGGL(k_far_join<FDTD_MODE_TM>, dim3(6), dim3(64), 0, 0, part, nparts, a, res);
+  else hipLaunchKernelGGL(k_far_join<FDTD_MODE_TE>, dim3(6), dim3(64), 0, 0, part, nparts, a, res);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpy(out360, res, 360 * sizeof(double2), hipMemcpyDeviceToHost));
   return FDTD_OK;
 }
 

@@ -146,6 +146,9 @@ int fast_state_gather(FastState *s, const Contour &ct, int off, double2 *cE, dou
 // the three physical fields of the current buffer (after launching the queued
 // steps): TM Ez Hx Hy / TE Ex Ey Hz, each NPX x pitch float2; *n = NPX * pitch
 int fast_state_fields(FastState *s, const float2 *f[3], size_t *n);
+// steps queued and not yet launched; the row pitch of the physical layout (cells)
+int fast_state_pending(const FastState *s);
+int fast_state_pitch(const FastState *s, int *pitch);
 // Per-step contour sampling inside the fused kernel: after every launch of n
 // steps, post(ctx, samples, n, times) receives the device buffer
 // [n][3][total] float2 and the time of each step.

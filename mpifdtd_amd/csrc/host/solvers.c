@@ -492,6 +492,15 @@ int fdtd_solver_farField(double *out360)
   return fdtd_hip_grid_far_field(slv_[active_].g, field_getK(), creal(coef), cimag(coef), sqrt(fi.h_u_nm), out360);
 }
 
+int fdtd_solver_farFieldNoAlloc(double *out360)
+{
+  if (active_ < 0) return FDTD_ESTATE;
+  double complex coef = solver_ff_coef();
+  FieldInfo fi = field_getFieldInfo();
+  return fdtd_hip_grid_far_field_noalloc(slv_[active_].g, field_getK(), creal(coef), cimag(coef), sqrt(fi.h_u_nm),
+                                         out360);
+}
+
 static void twiddles(double *tw)
 {
   /* e^{+i pi k/4096} with the argument formed as fft1x does (cfft.c:138-141) */

@@ -162,6 +162,12 @@ class Simulator:
         check(N.lib().fdtd_solver_farField(N.dptr(out)), "fdtd_solver_farField")
         return out
 
+    def far_field_noalloc(self):
+        """far_field() through the allocation-free entry point (fdtd_solver_farFieldNoAlloc)."""
+        out = np.empty(360, np.complex128)
+        check(N.lib().fdtd_solver_farFieldNoAlloc(N.dptr(out)), "fdtd_solver_farFieldNoAlloc")
+        return out
+
     def time_spectra(self):
         out = np.empty((321, 360), np.float64)
         check(N.lib().fdtd_solver_timeSpectra(N.dptr(out)), "fdtd_solver_timeSpectra")
