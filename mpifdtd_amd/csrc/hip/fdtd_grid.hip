@@ -845,51 +845,80 @@ __global__ __launch_bounds__(NTFF_GRP * NS) void k_ntff_bin(const double2 *cE, c
 // ---------------------------------------------------------------- running DFT of the contour
 
 // V_f[c] += sum_s val_s[c] e^{i w_f tb_s}, tb_s = tb0 + s the binning's base time
-// of step s: n steps of the [cell][n] sample buffers per launch.  A workgroup
-// covers DFT_CC cells x DFT_FB frequencies: lane (c, j) loads cell c's n samples
-// (one cache line) and adds them to frequencies j, j + 8, ... of the group,
-// whose n phase factors each (FP64 sincos) sit in LDS.  The accumulators are
-// [f][cell], so the 32 cells of a frequency update together.
-#define DFT_CC 32
-#define DFT_FB 16
-template <int NS>  // NS >= n
-__global__ __launch_bounds__(256) void k_ctr_dft(const double2 *cE, const double2 *cH, int n, int P, const double *omega,
-                                                 int nf, long tb0, double2 *VE, double2 *VH)
+// of step s, for n steps of samples val_s[c] at src[s * ss + c * sp] (the
+// staging buffers [step][cell], or the binning's [cell][n]).  A workgroup covers
+// DFT_CELLS cells and DFT_LANES * NFL frequencies: lane (c, l) keeps the sums of
+// cell c at frequencies l, l + DFT_LANES, ... in registers over all n steps and
+// adds them to the FP64 accumulators ([f][cell]) once.  The samples of
+// DFT_CHUNK steps and their phase factors (ph[s][f] = e^{i w_f tb_s}, from
+// k_dft_phase) pass through LDS.
+#define DFT_CELLS 32
+#define DFT_LANES 8
+#define DFT_CHUNK 16
+#define DFT_STAGE_STEPS 128  // steps staged before a DFT launch
+template <int NFL>
+__global__ __launch_bounds__(256) void k_ctr_dft(const double2 *cE, const double2 *cH, long ss, long sp, int n, int P,
+                                                 const double2 *ph, int nf, double2 *VE, double2 *VH)
 {
-  __shared__ double2 ph[DFT_FB][FAST_MAX_T];
-  const int f0 = blockIdx.y * DFT_FB, nfl = min(DFT_FB, nf - f0);
-  for (int q = threadIdx.x; q < nfl * n; q += 256) {
-    const int f = q / n, s = q % n;
-    double sn, cs;
-    sincos(omega[f0 + f] * (double)(tb0 + s), &sn, &cs);
-    ph[f][s] = make_double2(cs, sn);
-  }
-  __syncthreads();
-  const int c = threadIdx.x % DFT_CC, fl = threadIdx.x / DFT_CC;
-  const int p = blockIdx.x * DFT_CC + c;
-  if (p >= P) return;
-  double2 e[NS], h[NS];
+  constexpr int FW = DFT_LANES * NFL;
+  __shared__ double2 w[DFT_CHUNK][FW];
+  __shared__ double2 se[DFT_CHUNK][DFT_CELLS], sh[DFT_CHUNK][DFT_CELLS];
+  const int c = threadIdx.x % DFT_CELLS, l = threadIdx.x / DFT_CELLS;
+  const int p = blockIdx.x * DFT_CELLS + c, f0 = blockIdx.y * FW;
+  double ar[NFL], ai[NFL], br[NFL], bi[NFL];
 #pragma unroll
-  for (int s = 0; s < NS; s++) {
-    e[s] = s < n ? cE[(size_t)p * n + s] : make_double2(0.0, 0.0);
-    h[s] = s < n ? cH[(size_t)p * n + s] : make_double2(0.0, 0.0);
-  }
-  for (int f = fl; f < nfl; f += 256 / DFT_CC) {
-    const size_t k = (size_t)(f0 + f) * P + p;
-    double2 ae = VE[k], ah = VH[k];
-    // every s < NS unrolled (the samples stay in registers); steps past n hold
-    // zeros and add nothing
-#pragma unroll
-    for (int s = 0; s < NS; s++) {
-      const double2 w = s < n ? ph[f][s] : make_double2(0.0, 0.0);
-      ae.x += e[s].x * w.x - e[s].y * w.y;
-      ae.y += e[s].x * w.y + e[s].y * w.x;
-      ah.x += h[s].x * w.x - h[s].y * w.y;
-      ah.y += h[s].x * w.y + h[s].y * w.x;
+  for (int k = 0; k < NFL; k++) ar[k] = ai[k] = br[k] = bi[k] = 0.0;
+  for (int s0 = 0; s0 < n; s0 += DFT_CHUNK) {
+    const int ns = min(DFT_CHUNK, n - s0);
+    __syncthreads();  // the previous chunk is consumed
+    for (int q = threadIdx.x; q < ns * FW; q += 256) {
+      const int s = q / FW, f = q % FW;
+      w[s][f] = f0 + f < nf ? ph[(size_t)(s0 + s) * nf + f0 + f] : make_double2(0.0, 0.0);
     }
-    VE[k] = ae;
-    VH[k] = ah;
+    for (int q = threadIdx.x; q < ns * DFT_CELLS; q += 256) {
+      const int s = q / DFT_CELLS, cc = q % DFT_CELLS, pp = blockIdx.x * DFT_CELLS + cc;
+      const size_t k = (size_t)(s0 + s) * ss + (size_t)pp * sp;
+      se[s][cc] = pp < P ? cE[k] : make_double2(0.0, 0.0);
+      sh[s][cc] = pp < P ? cH[k] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    for (int s = 0; s < ns; s++) {
+      const double2 e = se[s][c], h = sh[s][c];
+#pragma unroll
+      for (int k = 0; k < NFL; k++) {
+        const double2 x = w[s][l + DFT_LANES * k];
+        ar[k] += e.x * x.x - e.y * x.y;
+        ai[k] += e.x * x.y + e.y * x.x;
+        br[k] += h.x * x.x - h.y * x.y;
+        bi[k] += h.x * x.y + h.y * x.x;
+      }
+    }
   }
+  if (p >= P) return;
+#pragma unroll
+  for (int k = 0; k < NFL; k++) {
+    const int f = f0 + l + DFT_LANES * k;
+    if (f >= nf) continue;
+    const size_t q = (size_t)f * P + p;
+    double2 a = VE[q], b = VH[q];
+    a.x += ar[k];
+    a.y += ai[k];
+    b.x += br[k];
+    b.y += bi[k];
+    VE[q] = a;
+    VH[q] = b;
+  }
+}
+
+// ph[s][f] = e^{i w_f (tb0 + s)} (FP64 sincos), s < n
+__global__ __launch_bounds__(256) void k_dft_phase(const double *omega, int nf, long tb0, int n, double2 *ph)
+{
+  const int q = blockIdx.x * 256 + threadIdx.x;
+  if (q >= n * nf) return;
+  const int s = q / nf, f = q % nf;
+  double sn, cs;
+  sincos(omega[f] * (double)(tb0 + s), &sn, &cs);
+  ph[q] = make_double2(cs, sn);
 }
 
 // The DFT at w_f of the three time-domain accumulator rows of every angle,
@@ -1343,6 +1372,12 @@ struct fdtd_grid {
   double2 *dftE, *dftH;  // [nf][total] FP64 accumulators
   double *dftTs;         // [360][total] the binning's time shifts
   double2 *dftOut;       // [nf][3][360]
+  // samples waiting for the DFT: columns [0, sFill) of the [sCap][total]
+  // buffers hold consecutive steps from base time sTb0 (running DFT without the
+  // time-domain binning); dftPh [sCap][nf] the phase factors of a launch
+  double2 *sE, *sH, *dftPh;
+  int sCap, sFill;
+  long sTb0;
   int ctrSet;            // FP32: the fused kernel samples the contour
   // FP32: stream of the sample packing, binning and running DFT, beside the next
   // block's step kernel; ev_k orders it after the block that took the samples
@@ -1585,29 +1620,58 @@ static void launch_bin(fdtd_grid *g, int n, double time0, hipStream_t st)
 #undef NTFF_BIN
 }
 
-// n steps of samples in cE / cH ([cell][n]) into the running DFT
-static void launch_dft(fdtd_grid *g, int n, double time0, hipStream_t st)
+// n steps of samples at E[s * ss + c * sp], H[...] taken at base times
+// tb0 + s into the running DFT
+static void dft_add(fdtd_grid *g, const double2 *E, const double2 *H, long ss, long sp, int n, long tb0, hipStream_t st)
 {
-  const long tb0 = (long)llround(time0) - 1;
-  const dim3 grid((g->ct.total + DFT_CC - 1) / DFT_CC, (g->nf + DFT_FB - 1) / DFT_FB);
-#define CTR_DFT(NS)                                                                                                 \
-  hipLaunchKernelGGL(k_ctr_dft<NS>, grid, dim3(256), 0, st, g->cE, g->cH, n, g->ct.total, g->dftOmega, g->nf, tb0, \
-                     g->dftE, g->dftH)
-  if (n <= 1)
+  const int nf = g->nf, P = g->ct.total;
+  if (n <= 0 || P <= 0) return;
+  hipLaunchKernelGGL(k_dft_phase, dim3((n * nf + 255) / 256), dim3(256), 0, st, g->dftOmega, nf, tb0, n, g->dftPh);
+  const int per = (nf + DFT_LANES - 1) / DFT_LANES;  // frequencies per lane
+#define CTR_DFT(NFL)                                                                                             \
+  hipLaunchKernelGGL(k_ctr_dft<NFL>, dim3((P + DFT_CELLS - 1) / DFT_CELLS, (nf + DFT_LANES * NFL - 1) / (DFT_LANES * NFL)), \
+                     dim3(256), 0, st, E, H, ss, sp, n, P, g->dftPh, nf, g->dftE, g->dftH)
+  if (per <= 1)
     CTR_DFT(1);
-  else if (n <= 4)
+  else if (per <= 2)
+    CTR_DFT(2);
+  else if (per <= 4)
     CTR_DFT(4);
-  else if (n <= 8)
+  else if (per <= 8)
     CTR_DFT(8);
+  else if (per <= 12)
+    CTR_DFT(12);
   else
     CTR_DFT(16);
 #undef CTR_DFT
 }
 
+// the staged samples into the running DFT
+static void dft_flush(fdtd_grid *g, hipStream_t st)
+{
+  if (g->sFill) dft_add(g, g->sE, g->sH, g->ct.total, 1, g->sFill, g->sTb0, st);
+  g->sFill = 0;
+}
+
+// the staging column of n steps from time0 (the first at base time
+// llround(time0) - 1): the staged steps go to the DFT first when these do not
+// follow them or do not fit
+static int dft_stage(fdtd_grid *g, int n, double time0, hipStream_t st)
+{
+  const long tb0 = (long)llround(time0) - 1;
+  if (g->sFill && (tb0 != g->sTb0 + g->sFill || g->sFill + n > g->sCap)) dft_flush(g, st);
+  if (!g->sFill) g->sTb0 = tb0;
+  const int col = g->sFill;
+  g->sFill += n;
+  return col;
+}
+
 // contour samples of n inner steps of the FP32 kernel ([n][3][total] float2:
 // signed E, H on the side, H one cell back) -> the packing of k_ntff_gather,
-// step-minor ([cell][n])
-__global__ __launch_bounds__(256) void k_ctr_combine(const float2 *smp, Contour ct, int n, double2 *cE, double2 *cH)
+// step s of cell p at cE[s * ss + p * sp] (the binning's [cell][n], or the
+// DFT's staging [step][cell])
+__global__ __launch_bounds__(256) void k_ctr_combine(const float2 *smp, Contour ct, long ss, long sp, double2 *cE,
+                                                     double2 *cH)
 {
   const int p = blockIdx.x * 256 + threadIdx.x, s = blockIdx.y;
   if (p >= ct.total) return;
@@ -1616,8 +1680,8 @@ __global__ __launch_bounds__(256) void k_ctr_combine(const float2 *smp, Contour 
   const double hs = ct.s[q].hs;
   const float2 *sm = smp + (size_t)s * 3 * ct.total;
   const float2 e = sm[p], h0 = sm[ct.total + p], h1 = sm[2 * ct.total + p];
-  cE[(size_t)p * n + s] = make_double2((double)e.x, (double)e.y);
-  cH[(size_t)p * n + s] = make_double2(hs * ((double)h0.x + (double)h1.x), hs * ((double)h0.y + (double)h1.y));
+  cE[(size_t)s * ss + (size_t)p * sp] = make_double2((double)e.x, (double)e.y);
+  cH[(size_t)s * ss + (size_t)p * sp] = make_double2(hs * ((double)h0.x + (double)h1.x), hs * ((double)h0.y + (double)h1.y));
 }
 
 #define NTFF_OVERLAP_CELLS (8L << 20)
@@ -1645,11 +1709,22 @@ static int grid_bin_samples(void *ctx, const float2 *samples, int n, const doubl
     HIPCHK(hipStreamWaitEvent(g->ns, g->ev_k, 0));
     st = g->ns;
   }
-  hipLaunchKernelGGL(k_ctr_combine, dim3((g->ct.total + 255) / 256, n), dim3(256), 0, st, samples, g->ct, n, g->cE,
-                     g->cH);
-  HIPCHK(hipEventRecord(consumed, st));
-  if (g->ntn) launch_bin(g, n, times[0], st);
-  if (g->nf) launch_dft(g, n, times[0], st);
+  const dim3 cgrid((g->ct.total + 255) / 256, n);
+  if (g->ntn) {
+    // binned step by step: [cell][n]; the DFT reads the same buffers
+    hipLaunchKernelGGL(k_ctr_combine, cgrid, dim3(256), 0, st, samples, g->ct, 1L, (long)n, g->cE, g->cH);
+    HIPCHK(hipEventRecord(consumed, st));
+    launch_bin(g, n, times[0], st);
+    if (g->nf) dft_add(g, g->cE, g->cH, 1, n, n, (long)llround(times[0]) - 1, st);
+  } else if (g->nf) {
+    // DFT only: staged, the DFT runs over up to sCap steps at once
+    const size_t col = (size_t)dft_stage(g, n, times[0], st) * g->ct.total;
+    hipLaunchKernelGGL(k_ctr_combine, cgrid, dim3(256), 0, st, samples, g->ct, (long)g->ct.total, 1L, g->sE + col,
+                       g->sH + col);
+    HIPCHK(hipEventRecord(consumed, st));
+  } else {
+    HIPCHK(hipEventRecord(consumed, st));
+  }
   HIPCHK(hipGetLastError());
   return FDTD_OK;
 }
@@ -1832,6 +1907,9 @@ extern "C" int fdtd_hip_grid_destroy(fdtd_grid *g)
   (void)dev_free(g->dftH);
   (void)dev_free(g->dftTs);
   (void)dev_free(g->dftOut);
+  (void)dev_free(g->sE);
+  (void)dev_free(g->sH);
+  (void)dev_free(g->dftPh);
   (void)dev_free(g->cosA);
   (void)dev_free(g->sinA);
   (void)dev_free(g->cosB);
@@ -2001,11 +2079,16 @@ extern "C" int fdtd_hip_grid_step(fdtd_grid *g, const fdtd_step_params *p)
     }
   }
   HIPCHK(hipGetLastError());
-  if ((g->ntn || g->nf) && g->ct.total > 0) {
+  if (g->ntn && g->ct.total > 0) {
     hipLaunchKernelGGL(k_ntff_gather, dim3((g->ct.total + 255) / 256), dim3(256), 0, 0, (const double2 *const *)g->Fdev,
                        g->ct, g->cE, g->cH);
-    if (g->ntn) launch_bin(g, 1, p->time, 0);
-    if (g->nf) launch_dft(g, 1, p->time, 0);
+    launch_bin(g, 1, p->time, 0);
+    if (g->nf) dft_add(g, g->cE, g->cH, 1, 1, 1, (long)llround(p->time) - 1, 0);
+    HIPCHK(hipGetLastError());
+  } else if (g->nf && g->ct.total > 0) {
+    const size_t col = (size_t)dft_stage(g, 1, p->time, 0) * g->ct.total;
+    hipLaunchKernelGGL(k_ntff_gather, dim3((g->ct.total + 255) / 256), dim3(256), 0, 0, (const double2 *const *)g->Fdev,
+                       g->ct, g->sE + col, g->sH + col);
     HIPCHK(hipGetLastError());
   }
   return FDTD_OK;
@@ -2067,6 +2150,7 @@ extern "C" int fdtd_hip_grid_reset(fdtd_grid *g)
     HIPCHK(hipMemset(g->dftE, 0, (size_t)g->nf * g->ct.total * sizeof(double2)));
     HIPCHK(hipMemset(g->dftH, 0, (size_t)g->nf * g->ct.total * sizeof(double2)));
   }
+  g->sFill = 0;
   HIPCHK(hipDeviceSynchronize());
   if (g->fs) return fast_state_reset(g->fs);
   return FDTD_OK;
@@ -2244,12 +2328,22 @@ extern "C" int fdtd_hip_grid_set_dft(fdtd_grid *g, const double *omega, int nf)
   (void)dev_free(g->dftH);
   (void)dev_free(g->dftTs);
   (void)dev_free(g->dftOut);
+  (void)dev_free(g->sE);
+  (void)dev_free(g->sH);
+  (void)dev_free(g->dftPh);
   g->dftOmega = nullptr;
   g->dftE = g->dftH = g->dftOut = nullptr;
+  g->sE = g->sH = g->dftPh = nullptr;
   g->dftTs = nullptr;
   g->nf = 0;
+  g->sFill = 0;
   if (nf == 0) return FDTD_OK;
   const int P = g->ct.total;
+  // staging for DFT_STAGE_STEPS steps (at least one block of the FP32 path)
+  g->sCap = std::max(DFT_STAGE_STEPS, g->nbuf);
+  HIPCHK(dev_malloc(&g->sE, (size_t)g->sCap * P * sizeof(double2)));
+  HIPCHK(dev_malloc(&g->sH, (size_t)g->sCap * P * sizeof(double2)));
+  HIPCHK(dev_malloc(&g->dftPh, (size_t)g->sCap * nf * sizeof(double2)));
   HIPCHK(dev_malloc(&g->dftOmega, nf * sizeof(double)));
   HIPCHK(hipMemcpy(g->dftOmega, omega, nf * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(dev_malloc(&g->dftE, (size_t)nf * P * sizeof(double2)));
@@ -2277,6 +2371,7 @@ extern "C" int fdtd_hip_grid_get_dft(fdtd_grid *g, double *out)
   int rc;
   if (g->fs && (rc = fast_state_flush(g->fs))) return rc;
   if ((rc = ntff_sync(g))) return rc;
+  dft_flush(g, 0);
   const double cf = g->d.flavour == FDTD_FLAVOUR_PLAIN ? 1.0 : g->coef;
   hipLaunchKernelGGL(k_dft_far, dim3(360, g->nf), dim3(256), 0, 0, g->dftE, g->dftH, g->ct, g->dftTs, g->dftOmega, cf,
                      g->dftOut);

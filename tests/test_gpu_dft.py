@@ -128,3 +128,56 @@ def test_contour_samples_independent_of_launch_shape():
         assert np.array_equal(D[k], D[0]), k
         for a in range(3):
             assert rel_l2(U[k][a], U[0][a]) <= 1e-12, (k, a)
+
+
+def _dft_run(m, solver, precision, steps, lams, ntff_time, T=5, stops=(), reset_at=None):
+    kw = dict(T=T, R=16) if precision == "fp32" else {}
+    sim = m.Simulator(solver, 1, m.field_info(4000, 4000, 50, 10, 500, 0, steps), ntff_time=ntff_time,
+                      precision=precision, **kw)
+    try:
+        sim.set_dft(lams)
+        done = 0
+        for s in stops:  # read the DFT part way (the staged steps go in first)
+            sim.calc(s - done)
+            done = s
+            sim.dft()
+        if reset_at is not None:
+            sim.calc(reset_at - done)
+            sim.reset()
+            done = 0
+        sim.calc(steps - done)
+        return sim.dft()
+    finally:
+        sim.finish()
+
+
+@pytest.mark.parametrize("nlam", [1, 3, 130])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("solver", [2, 5])
+def test_running_dft_staged_equals_per_block(solver, precision, nlam):
+    """Without the time-domain binning the samples of up to 128 steps are staged
+    and transformed in one launch; with it every block (FP32) or step (FP64) is
+    transformed as it is binned.  Same samples, same sums in another order:
+    FP64 rounding apart.  130 wavelengths take two frequency blocks of the
+    transform, 1 and 3 the narrow forms."""
+    import mpifdtd_amd as m
+    lams = list(np.linspace(420.0, 700.0, nlam)) if nlam > 1 else [500.0]
+    steps = 400
+    a = _dft_run(m, solver, precision, steps, lams, ntff_time=False)
+    b = _dft_run(m, solver, precision, steps, lams, ntff_time=True)
+    assert np.linalg.norm(b) > 0
+    assert rel_l2(a, b) <= 1e-12, rel_l2(a, b)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_running_dft_read_part_way_and_reset(precision):
+    """Reading the DFT part way (at steps that end inside a staged run) leaves the
+    result unchanged; after a reset the DFT starts again from zero."""
+    import mpifdtd_amd as m
+    lams = [450.0, 500.0, 610.0]
+    base = _dft_run(m, 4, precision, 400, lams, ntff_time=False)
+    mid = _dft_run(m, 4, precision, 400, lams, ntff_time=False, stops=(55, 200, 330))
+    assert np.linalg.norm(base) > 0
+    assert rel_l2(mid, base) <= 1e-12, rel_l2(mid, base)
+    again = _dft_run(m, 4, precision, 400, lams, ntff_time=False, stops=(60,), reset_at=150)
+    assert rel_l2(again, base) <= 1e-12, rel_l2(again, base)
