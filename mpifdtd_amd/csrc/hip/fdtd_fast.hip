@@ -36,7 +36,16 @@
 __device__ __forceinline__ float2 c_add(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 c_sub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ float2 c_scale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
-__device__ __forceinline__ float2 c_mul(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+// complex product with its roundings spelled out (one product of each part fused
+// into its sum, the other rounded on its own), so that every kernel that inlines
+// it rounds alike whatever the compiler's contraction choices elsewhere
+__device__ __forceinline__ float2 c_mul(float2 a, float2 b)
+{
+  return make_float2(fmaf(a.x, b.x, -(a.y * b.y)), fmaf(a.x, b.y, a.y * b.x));
+}
+// b + a s and b - a s per part, each a single rounding (fused)
+__device__ __forceinline__ float2 c_fma(float2 a, float s, float2 b) { return make_float2(fmaf(s, a.x, b.x), fmaf(s, a.y, b.y)); }
+__device__ __forceinline__ float2 c_fnma(float2 a, float s, float2 b) { return make_float2(fmaf(-s, a.x, b.x), fmaf(-s, a.y, b.y)); }
 __device__ __forceinline__ float2 sel(bool c, float2 a, float2 b) { return make_float2(c ? a.x : b.x, c ? a.y : b.y); }
 // Neighbours along j (the lane axis) through DPP wave shifts, a VALU operand
 // modifier with no LDS round trip (the compiler folds the move into the
@@ -307,6 +316,10 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
         if (!(pml & (1u << r))) A0[r] = A1[r] = A2[r] = A3[r] = A4[r] = A5[r] = make_float2(0.f, 0.f);
       }
     }
+    // UPML tiles: a wavefront none of whose cells uses the UPML recurrence (the
+    // margin rows of the end bands' tiles) runs the regular update under the
+    // update mask only; its auxiliaries stay zero and are never stored
+    const bool wpm = PML && __ballot(pml != 0) != 0;
     // TM: the plain Gaussian pulse and the MPI scattered wave act on Ez (component 0)
     // TE: MPI scattered wave on Ey only (mpiTE_UPML.c:296); plain pulse on Ex and Ey
     if (MODE == FDTD_MODE_TE && FLAV == FDTD_FLAVOUR_MPI) scat0 = 0;
@@ -314,6 +327,41 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
       if (!uni_i(I.on[0])) scat0 = 0;
       if (!uni_i(I.on[1])) scat1 = 0;
     }
+
+    // Interior tiles of the TE and MPI bodies: the 1/eps of each cell are kept in
+    // LDS rather than registers (each wavefront writes and reads its own rows
+    // only), which keeps these bodies within the register budget of two
+    // workgroups per compute unit (the plain TM body fits with them in registers)
+    constexpr bool IEL = !PML && (MODE == FDTD_MODE_TE || FLAV == FDTD_FLAVOUR_MPI);
+    __shared__ float sie[IEL ? 2 : 1][IEL ? NR : 1][64];
+    if constexpr (IEL) {
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        sie[0][w * R + r][lane] = ie0[r];
+        sie[1][w * R + r][lane] = ie1[r];
+      }
+    }
+    // b + a s, b - a s of the regular updates: the bodies with 1/eps in LDS spell
+    // the fused rounding out (their merged and split instantiations otherwise
+    // contract it differently, and the launch shapes must agree bit for bit);
+    // the other bodies leave the contraction to the compiler (spelled out, the
+    // plain TM interior body is scheduled 4 % slower)
+    auto fma2 = [&](float2 a, float sc, float2 b) -> float2 {
+      if constexpr (IEL) return c_fma(a, sc, b);
+      else return c_add(b, c_scale(a, sc));
+    };
+    auto fnma2 = [&](float2 a, float sc, float2 b) -> float2 {
+      if constexpr (IEL) return c_fnma(a, sc, b);
+      else return c_sub(b, c_scale(a, sc));
+    };
+    auto IE0 = [&](int r) -> float {
+      if constexpr (IEL) return sie[0][w * R + r][lane];
+      else return ie0[r];
+    };
+    auto IE1 = [&](int r) -> float {
+      if constexpr (IEL) return sie[1][w * R + r][lane];
+      else return ie1[r];
+    };
 
     // source term of component c in row r of a scatterer cell: (value now, change since the previous step)
     auto source = [&](int c, int r, float ie, const FastStep &st, float2 &now, float2 &delta) {
@@ -371,10 +419,11 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
         source(c, r, ie, st, sn, sd);
       }
     };
-    // UPML tiles run both recurrences on every cell and keep the right one with
-    // per-cell selects (no divergent branches); an auxiliary is written back
-    // only where the cell is updated, so the registers of the other cells may
-    // hold anything.
+    // UPML tiles run both recurrences on every cell of a wavefront that holds a
+    // UPML cell and keep the right one with per-cell selects (no divergent
+    // branches), the regular one alone in the other wavefronts (a uniform
+    // branch); an auxiliary is written back only where the cell is updated, so
+    // the registers of the other cells may hold anything.
     const float s2p = 2.0f + sy, s2m = 2.0f - sy;  // C_BYMY1/0, C_DYJY1/0 numerators of this column
 
     auto e_rows = [&](const FastStep &st, auto SRCT) {
@@ -388,25 +437,30 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
           const float2 hxm = shfl_up1(F1[r]);
           const float2 curl = c_sub(c_sub(F2[r], hym), c_sub(F1[r], hxm));
           if (!PML) {
-            float2 e = c_add(F0[r], c_scale(curl, ie0[r]));
+            float2 e = fma2(curl, IE0(r), F0[r]);
             if (FLAV == FDTD_FLAVOUR_MPI) {
-              if (SRC) e = add_src(e, r, ie0[r], st);
+              if (SRC) e = add_src(e, r, IE0(r), st);
             } else if (scat0 & bit) {
               float2 sn, sd;
-              source(0, r, ie0[r], st, sn, sd);
+              source(0, r, IE0(r), st, sn, sd);
               e = c_add(e, sd);
             }
             F0[r] = e;
           } else {
             float2 sn = make_float2(0.f, 0.f), sd = sn;
-            src_any(0, r, ie0[r], st, SRCT, sn, sd);
-            const float cjz = srow[0][rr], cjzh = srow[1][rr];
-            const float2 nj = c_add(c_scale(A0[r], cjz), c_scale(curl, cjzh));
-            const float2 nd = c_add(c_scale(A1[r], cy0), c_scale(c_sub(nj, A0[r]), cy1));
-            A0[r] = nj;
-            A1[r] = nd;
-            const float2 e = sel(pml & bit, c_add(c_scale(nd, ie0[r]), sn), c_add(c_add(F0[r], c_scale(curl, ie0[r])), sd));
-            F0[r] = sel(upd & bit, e, F0[r]);
+            src_any(0, r, IE0(r), st, SRCT, sn, sd);
+            const float2 reg = c_add(fma2(curl, IE0(r), F0[r]), sd);
+            if (wpm) {
+              const float cjz = srow[0][rr], cjzh = srow[1][rr];
+              const float2 nj = c_add(c_scale(A0[r], cjz), c_scale(curl, cjzh));
+              const float2 nd = c_add(c_scale(A1[r], cy0), c_scale(c_sub(nj, A0[r]), cy1));
+              A0[r] = nj;
+              A1[r] = nd;
+              const float2 e = sel(pml & bit, c_add(c_scale(nd, IE0(r)), sn), reg);
+              F0[r] = sel(upd & bit, e, F0[r]);
+            } else {
+              F0[r] = sel(upd & bit, reg, F0[r]);
+            }
           }
         } else {
           const float2 hzj = shfl_up1(F2[r]);
@@ -414,19 +468,19 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
           const float2 cx = c_sub(F2[r], hzj);   // Hz - Hz(j-1)
           const float2 cyv = c_sub(hzi, F2[r]);  // Hz(i-1) - Hz
           if (!PML) {
-            float2 ex = c_add(F0[r], c_scale(cx, ie0[r]));
-            float2 ey = c_add(F1[r], c_scale(cyv, ie1[r]));
+            float2 ex = fma2(cx, IE0(r), F0[r]);
+            float2 ey = fma2(cyv, IE1(r), F1[r]);
             if (FLAV == FDTD_FLAVOUR_MPI) {
-              if (SRC) ey = add_src(ey, r, ie1[r], st);
+              if (SRC) ey = add_src(ey, r, IE1(r), st);
             } else {
               if (scat0 & bit) {
                 float2 sn, sd;
-                source(0, r, ie0[r], st, sn, sd);
+                source(0, r, IE0(r), st, sn, sd);
                 ex = c_add(ex, sd);
               }
               if (scat1 & bit) {
                 float2 sn, sd;
-                source(1, r, ie1[r], st, sn, sd);
+                source(1, r, IE1(r), st, sn, sd);
                 ey = c_add(ey, sd);
               }
             }
@@ -435,9 +489,14 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
           } else {
             float2 sn0 = make_float2(0.f, 0.f), sd0 = sn0, sn1 = sn0, sd1 = sn0;
             // TE MPI: the scattered wave acts on Ey only
-            if (FLAV == FDTD_FLAVOUR_PLAIN) src_any(0, r, ie0[r], st, SRCT, sn0, sd0);
-            src_any(1, r, ie1[r], st, SRCT, sn1, sd1);
-            {
+            if (FLAV == FDTD_FLAVOUR_PLAIN) src_any(0, r, IE0(r), st, SRCT, sn0, sd0);
+            src_any(1, r, IE1(r), st, SRCT, sn1, sd1);
+            const float2 rex = c_add(fma2(cx, IE0(r), F0[r]), sd0);
+            const float2 rey = c_add(fma2(cyv, IE1(r), F1[r]), sd1);
+            if (!wpm) {
+              F0[r] = sel(upd & bit, rex, F0[r]);
+              F1[r] = sel(upd & bit, rey, F1[r]);
+            } else {
               // Jx/Dx: x tables 0 C_DXJX1 1 C_DXJX0; y: 0 C_JX 1 C_JXHZ
               const float2 njx = c_add(c_scale(A0[r], cy0), c_scale(cx, cy1));
               const float2 ndx = c_sub(c_add(A1[r], c_scale(njx, srow[0][rr])), c_scale(A0[r], srow[1][rr]));
@@ -448,8 +507,8 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
               const float2 ndy = c_sub(c_add(c_scale(A3[r], srow[2][rr]), c_scale(njy, c1)), c_scale(A2[r], c0));
               A0[r] = njx; A1[r] = ndx; A2[r] = njy; A3[r] = ndy;
               const bool p = pml & bit, u = upd & bit;
-              const float2 ex = sel(p, c_add(c_scale(ndx, ie0[r]), sn0), c_add(c_add(F0[r], c_scale(cx, ie0[r])), sd0));
-              const float2 ey = sel(p, c_add(c_scale(ndy, ie1[r]), sn1), c_add(c_add(F1[r], c_scale(cyv, ie1[r])), sd1));
+              const float2 ex = sel(p, c_add(c_scale(ndx, IE0(r)), sn0), rex);
+              const float2 ey = sel(p, c_add(c_scale(ndy, IE1(r)), sn1), rey);
               F0[r] = sel(u, ex, F0[r]);
               F1[r] = sel(u, ey, F1[r]);
             }
@@ -471,10 +530,14 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
           const float2 dEy = c_sub(ejp, F0[r]);  // Ez(j+1) - Ez
           const float2 dEx = c_sub(eip, F0[r]);  // Ez(i+1) - Ez
           if (!PML) {
-            F1[r] = c_sub(F1[r], c_scale(dEy, c2));
-            F2[r] = c_add(F2[r], c_scale(dEx, c2));
+            F1[r] = fnma2(dEy, c2, F1[r]);
+            F2[r] = fma2(dEx, c2, F2[r]);
           } else {
-            {
+            const float2 rhx = fnma2(dEy, c2, F1[r]), rhy = fma2(dEx, c2, F2[r]);
+            if (!wpm) {
+              F1[r] = sel(upd & bit, rhx, F1[r]);
+              F2[r] = sel(upd & bit, rhy, F2[r]);
+            } else {
               // Mx/Bx: y 2 C_MX 3 C_MXEZ, x 2 C_BXMX1 3 C_BXMX0 ; My/By: x 4 C_BY, C_BYMY1/0
               const float2 nmx = c_sub(c_scale(A2[r], cy2), c_scale(dEy, cy3));
               const float2 nbx = c_sub(c_add(A3[r], c_scale(nmx, srow[2][rr])), c_scale(A2[r], srow[3][rr]));
@@ -484,8 +547,8 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
               const float2 nby = c_sub(c_add(c_scale(A5[r], srow[4][rr]), c_scale(nmy, c1)), c_scale(A4[r], c0));
               A2[r] = nmx; A3[r] = nbx; A4[r] = nmy; A5[r] = nby;
               const bool p = pml & bit, u = upd & bit;
-              const float2 hx = sel(p, c_scale(nbx, c2), c_sub(F1[r], c_scale(dEy, c2)));
-              const float2 hy = sel(p, c_scale(nby, c2), c_add(F2[r], c_scale(dEx, c2)));
+              const float2 hx = sel(p, c_scale(nbx, c2), rhx);
+              const float2 hy = sel(p, c_scale(nby, c2), rhy);
               F1[r] = sel(u, hx, F1[r]);
               F2[r] = sel(u, hy, F2[r]);
             }
@@ -496,14 +559,19 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
           // ((Ey(i+1) - Ey) - Ex(j+1)) + Ex
           const float2 ce = c_add(c_sub(c_sub(eyi, F1[r]), exj), F0[r]);
           if (!PML) {
-            F2[r] = c_sub(F2[r], c_scale(ce, c2));
+            F2[r] = fnma2(ce, c2, F2[r]);
           } else {
-            // Mz/Bz: x 3 C_MZ 4 C_MZEXEY ; y 2 C_BZ 3 C_BZMZ
-            const float2 nmz = c_sub(c_scale(A4[r], srow[3][rr]), c_scale(ce, srow[4][rr]));
-            const float2 nbz = c_sub(c_add(c_scale(A5[r], cy2), c_scale(nmz, cy3)), c_scale(A4[r], cy3));
-            A4[r] = nmz; A5[r] = nbz;
-            const float2 hz = sel(pml & bit, c_scale(nbz, c2), c_sub(F2[r], c_scale(ce, c2)));
-            F2[r] = sel(upd & bit, hz, F2[r]);
+            const float2 rhz = fnma2(ce, c2, F2[r]);
+            if (!wpm) {
+              F2[r] = sel(upd & bit, rhz, F2[r]);
+            } else {
+              // Mz/Bz: x 3 C_MZ 4 C_MZEXEY ; y 2 C_BZ 3 C_BZMZ
+              const float2 nmz = c_sub(c_scale(A4[r], srow[3][rr]), c_scale(ce, srow[4][rr]));
+              const float2 nbz = c_sub(c_add(c_scale(A5[r], cy2), c_scale(nmz, cy3)), c_scale(A4[r], cy3));
+              A4[r] = nmz; A5[r] = nbz;
+              const float2 hz = sel(pml & bit, c_scale(nbz, c2), rhz);
+              F2[r] = sel(upd & bit, hz, F2[r]);
+            }
           }
         }
       }
@@ -608,7 +676,7 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
           constexpr bool SRC = decltype(SRCT)::value;
           if (SRC) {
 #pragma unroll
-            for (int r = 0; r < R; r++) sg[w * R + r][lane] = src_g(r, MODE == FDTD_MODE_TM ? ie0[r] : ie1[r]);
+            for (int r = 0; r < R; r++) sg[w * R + r][lane] = src_g(r, MODE == FDTD_MODE_TM ? IE0(r) : IE1(r));
           }
           for (int s = 0; s < T; s++) {
             e_rows(sst[s], SRCT);
