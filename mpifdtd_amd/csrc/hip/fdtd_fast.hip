@@ -1381,8 +1381,26 @@ static int build_tiles(FastState *s)
     tile_band(tp, 0, NPX, 0, NPY, NPX, NPY, UR, T);
   }
   auto byrow = [](const FastTile &a, const FastTile &b) { return a.i0 != b.i0 ? a.i0 < b.i0 : a.j0 < b.j0; };
-  std::sort(ti.begin(), ti.end(), byrow);
   std::sort(tp.begin(), tp.end(), byrow);
+  // interior tiles: 8 vertical stripes of tile columns, each walked row by row,
+  // so that the eighth of the list an XCD takes is one stripe, and the tiles it
+  // runs at a time are neighbours in both directions (their halos overlap in
+  // its L2)
+  {
+    std::vector<int> cols;
+    for (const FastTile &t : ti) cols.push_back(t.j0);
+    std::sort(cols.begin(), cols.end());
+    cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+    const int nc = (int)cols.size();
+    auto stripe = [&](const FastTile &t) {
+      const int c = (int)(std::lower_bound(cols.begin(), cols.end(), t.j0) - cols.begin());
+      return (int)((long)c * 8 / std::max(1, nc));
+    };
+    std::sort(ti.begin(), ti.end(), [&](const FastTile &a, const FastTile &b) {
+      const int sa = stripe(a), sb = stripe(b);
+      return sa != sb ? sa < sb : byrow(a, b);
+    });
+  }
   // tiles whose core holds a contour cell the kernel samples (a side's line fi or
   // its neighbour line fi - 1, within the side's range) go last in their list
   auto on_contour = [&](const FastTile &t) {
