@@ -1334,6 +1334,8 @@ static void split_queues(int *q, int base, int n)
   for (int k = 0; k <= 8; k++) q[k] = base + (int)((long)n * k / 8);
 }
 
+// vertical stripes of the interior tile list (see build_tiles)
+#define FAST_STRIPES 4
 static int build_tiles(FastState *s)
 {
   const int T = s->T, R = s->R, H0 = T + 1;
@@ -1382,10 +1384,12 @@ static int build_tiles(FastState *s)
   }
   auto byrow = [](const FastTile &a, const FastTile &b) { return a.i0 != b.i0 ? a.i0 < b.i0 : a.j0 < b.j0; };
   std::sort(tp.begin(), tp.end(), byrow);
-  // interior tiles: 8 vertical stripes of tile columns, each walked row by row,
-  // so that the eighth of the list an XCD takes is one stripe, and the tiles it
-  // runs at a time are neighbours in both directions (their halos overlap in
-  // its L2)
+  // interior tiles: FAST_STRIPES vertical stripes of tile columns, each walked
+  // row by row, so that the eighth of the list an XCD takes is (half) a stripe
+  // and the tiles it runs at a time are neighbours in both directions, their
+  // halos overlapping in its L2 (8256 x 4160 grid: 4 stripes of 48 tile
+  // columns 1 % faster than 8 of 24 and than 2 of 96, 2 % faster than rows
+  // across the whole grid, 4 % faster than 24 stripes of 8)
   {
     std::vector<int> cols;
     for (const FastTile &t : ti) cols.push_back(t.j0);
@@ -1394,7 +1398,7 @@ static int build_tiles(FastState *s)
     const int nc = (int)cols.size();
     auto stripe = [&](const FastTile &t) {
       const int c = (int)(std::lower_bound(cols.begin(), cols.end(), t.j0) - cols.begin());
-      return (int)((long)c * 8 / std::max(1, nc));
+      return (int)((long)c * FAST_STRIPES / std::max(1, nc));
     };
     std::sort(ti.begin(), ti.end(), [&](const FastTile &a, const FastTile &b) {
       const int sa = stripe(a), sb = stripe(b);
