@@ -273,7 +273,7 @@ def generate(spec, T=8, R=16, chunk=128, log=None, grid=None):
     return features(spec), Y
 
 
-def generate_sweep(geoms, T=10, R=16, launch=7, log=None, grid=None):
+def generate_sweep(geoms, T=10, R=16, launch=8, log=None, grid=None):
     """The FDTD spectra of each geometry at the 81 wavelengths SWEEP_LAMBDAS
     from one run on the large grid (default SWEEP_GRID): the reference's plain
     TM pulse solver in FP32 with the running DFT of the NTFF contour, the
