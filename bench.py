@@ -59,9 +59,9 @@ SWEEP_LAMBDAS = (380.0, 781.0, 5.0)  # 81 wavelengths
 RUN_STEPS = 20000  # steps of one run of the sweep
 # Launch configuration of the sweep per mode: (T, R, launch shape), the
 # fastest of `--autotune`'s candidates on this grid, in the split form of the
-# shapes (the two-stream forms time within 0.5 %, and the split form times each
+# shapes (the two-stream forms time within 1 %, and the split form times each
 # launch on its own, as the counter summaries need).  T divides the default K.
-SWEEP_CONFIG = {"tm": (10, 16, 8), "te": (10, 16, 6)}
+SWEEP_CONFIG = {"tm": (10, 16, 8), "te": (10, 16, 8)}
 T_CANDIDATES = (10, 8, 5, 4, 2, 1)
 
 
@@ -312,7 +312,7 @@ def tune_sweep(m, mode, lams):
         probe.set_dft(lams)
         probe.calc(40)
         for Tc, Rc in ((10, 16), (8, 16), (10, 8), (8, 8)):
-            for sh in (2, 3, 4, 5, 6, 7, 8, 9):
+            for sh in (1, 2, 3, 4, 5, 6, 7, 8, 9):
                 n = 6 * Tc
                 probe.set_blocking(Tc, Rc)
                 probe.set_launch_mode(sh)
