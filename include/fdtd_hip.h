@@ -178,7 +178,10 @@ int fdtd_hip_grid_time_spectra(fdtd_grid *g, double coef_re, double coef_im, con
 /* running DFT of the NTFF contour at nf angular frequencies (rad per step):
  * every later step adds its contour samples times e^{i w tb} (tb = time - 1,
  * the binning's base time) into FP64 per-cell accumulators; nf = 0 turns it off.
- * No counterpart in the reference (ntffTM.c:279-371 bins in the time domain). */
+ * Without the per-step time-domain NTFF the samples of up to 128 consecutive
+ * steps are staged on the device and added in one launch (get_dft and reset
+ * take the staged steps into account).  No counterpart in the reference
+ * (ntffTM.c:279-371 bins in the time domain). */
 int fdtd_hip_grid_set_dft(fdtd_grid *g, const double *omega, int nf);
 /* out[nf][3][360] complex: sum_m U_a[m] e^{i w m} over the rows of the three
  * time-domain accumulators as the binning would fill them */
