@@ -1587,6 +1587,16 @@ int fast_state_flush(FastState *s)
     const int nt = pml ? s->n_pml : s->n_int, nc = pml ? s->n_pml_c : s->n_int_c;
     FastArgs b = a;
     b.nb_pml = 0;
+    if (pml && nc > 0) {
+      // UPML tiles with contour samples: all of them in one launch of the sampling
+      // kind (a tile without contour cells samples nothing).  The UPML tiles are
+      // few and long, one workgroup per compute unit, so two launches would each
+      // end on a part-filled dispatch round; the interior tiles, most of which
+      // hold no contour cell, keep the plain kind for those
+      memcpy(b.qb[0], s->qpml, sizeof b.qb[0]);
+      fast_launch(s->d.mode, s->d.flavour, s->R, p16 ? 11 : w8 ? 7 : 5, b, 8 * ((nt + 7) / 8), st);
+      return;
+    }
     if (nt - nc > 0) {
       memcpy(pml ? b.qb[0] : b.qb[1], pml ? s->qpml_p : s->qint_p, sizeof b.qb[0]);
       fast_launch(s->d.mode, s->d.flavour, s->R, pml ? (p16 ? 10 : w8 ? 6 : 2) : (i8 ? 8 : 1), b, 8 * ((nt - nc + 7) / 8), st);
