@@ -1387,9 +1387,9 @@ static int build_tiles(FastState *s)
   // interior tiles: FAST_STRIPES vertical stripes of tile columns, each walked
   // row by row, so that the eighth of the list an XCD takes is (half) a stripe
   // and the tiles it runs at a time are neighbours in both directions, their
-  // halos overlapping in its L2 (8256 x 4160 grid: 4 stripes of 48 tile
-  // columns 1 % faster than 8 of 24 and than 2 of 96, 2 % faster than rows
-  // across the whole grid, 4 % faster than 24 stripes of 8)
+  // halos overlapping in its L2 (8256 x 4160 grid, 95 tile columns: 4 stripes
+  // of 24 columns 1 % faster than 8 of 12 and than 2 of 48, 2 % faster than
+  // rows across the whole grid, 4 % faster than 24 stripes of 4)
   {
     std::vector<int> cols;
     for (const FastTile &t : ti) cols.push_back(t.j0);
