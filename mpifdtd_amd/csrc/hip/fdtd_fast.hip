@@ -826,8 +826,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(fast_interi
 
 // Batched instances: every workgroup pulls tiles from the launch's two device
 // queues, first the UPML tiles (the longest per tile), then the interior tiles,
-// each queue ordered by decreasing instance size, until both are empty; every
-// workgroup then exits.  One launch advances every unfinished instance by its
+// each queue ordered by decreasing instance size and cut into 8 parts, one per
+// XCD (a workgroup takes its own XCD's part first, then the others'), until both
+// are empty; every workgroup then exits.  One launch advances every unfinished instance by its
 // own depth; k_batch_advance between launches moves the progress records.
 // KQ: the queues pulled, 0 UPML tiles, 1 interior tiles, 2 both (UPML first).
 // A one-queue instantiation carries one tile body only, so it runs at that
@@ -841,12 +842,24 @@ __attribute__((amdgpu_waves_per_eu(KQ == 1 ? fast_interior_waves(W8 ? R / 2 : R)
   static_assert(!W8 || (KQ != 2 && R >= 8), "8-wavefront bodies: one queue, R >= 8");
   __shared__ int s_tid;
   for (int k = (KQ == 1 ? 1 : 0); k < (KQ == 0 ? 1 : 2); k++) {
-    for (;;) {
+    // the queue in 8 contiguous parts, one per XCD (workgroup b runs on XCD b % 8):
+    // a workgroup pulls from its XCD's part, so neighbouring tiles of an instance
+    // meet in one L2, then from the other parts once its own is empty
+    const int xcd = blockIdx.x & 7;
+    for (int pp = 0; pp < 8;) {
       __syncthreads();  // every thread has read s_tid of the previous pull
-      if (threadIdx.x == 0) s_tid = atomicAdd(&a.qctr[k], 1);
+      if (threadIdx.x == 0) {
+        const int q = (xcd + pp) & 7;
+        const int lo = (int)((long)a.qn[k] * q / 8), hi = (int)((long)a.qn[k] * (q + 1) / 8);
+        const int t = lo + atomicAdd(&a.qctr[8 * k + q], 1);
+        s_tid = t < hi ? t : -1;
+      }
       __syncthreads();
       const int tid = s_tid;
-      if (tid >= a.qn[k]) break;
+      if (tid < 0) {
+        pp++;  // this part is empty (uniform: s_tid is shared)
+        continue;
+      }
       if constexpr (W8) {
         if (KQ == 0) fast_tile<MODE, FDTD_FLAVOUR_MPI, 4, 1, 1, 0, 8>(a, a.qoff[0] + tid);
         else fast_tile<MODE, FDTD_FLAVOUR_MPI, R / 2, 0, 1, 0, 8>(a, a.qoff[1] + tid);
@@ -1799,7 +1812,7 @@ struct FastBatch {
   FastInst *dinst = nullptr;
   FastBatchInst *dbinst = nullptr;
   FastTile *dtiles = nullptr;
-  int *dqctr = nullptr;             // [2 modes][2 queues]
+  int *dqctr = nullptr;             // [2 modes][2 queues][8 XCD parts]
   int nt[2][2] = {{0, 0}, {0, 0}};  // tiles per mode and queue
   int off[2][2] = {{0, 0}, {0, 0}};
   int nblk[2][2] = {{0, 0}, {0, 0}};  // workgroups per mode and queue
@@ -1845,7 +1858,7 @@ static FastArgs batch_args(const FastBatch *b, int m)
   a.T = b->T;
   a.src = 1;
   a.binst = b->dbinst;
-  a.qctr = b->dqctr + 2 * m;
+  a.qctr = b->dqctr + 16 * m;
   a.qn[0] = b->nt[m][0];
   a.qn[1] = b->nt[m][1];
   a.qoff[0] = b->off[m][0];
@@ -1873,7 +1886,7 @@ static int batch_level(FastBatch *b)
     HIPCHK(hipStreamWaitEvent(b->stream, b->ev_join, 0));
   }
   hipLaunchKernelGGL(k_batch_advance, dim3((std::max(b->n, 4) + 255) / 256), dim3(256), 0, b->stream, b->dbinst, b->n,
-                     b->T, b->dqctr, 4, 0);
+                     b->T, b->dqctr, 32, 0);
   HIPCHK(hipGetLastError());
   return FDTD_OK;
 }
@@ -1978,14 +1991,14 @@ int fast_batch_create(FastState *const *st, const fdtd_step_params *p0, const in
   HIPCHK(dev_malloc(&b->dinst, n * sizeof(FastInst)));
   HIPCHK(dev_malloc(&b->dbinst, n * sizeof(FastBatchInst)));
   HIPCHK(dev_malloc(&b->dtiles, std::max<size_t>(1, all.size()) * sizeof(FastTile)));
-  HIPCHK(dev_malloc(&b->dqctr, 4 * sizeof(int)));
+  HIPCHK(dev_malloc(&b->dqctr, 32 * sizeof(int)));
   HIPCHK(hipMemcpy(b->dinst, hinst.data(), n * sizeof(FastInst), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(b->dbinst, hb.data(), n * sizeof(FastBatchInst), hipMemcpyHostToDevice));
   if (!all.empty()) HIPCHK(hipMemcpy(b->dtiles, all.data(), all.size() * sizeof(FastTile), hipMemcpyHostToDevice));
   HIPCHK(hipDeviceSynchronize());
   // depths of the first block, queue counters cleared
   hipLaunchKernelGGL(k_batch_advance, dim3((std::max(n, 4) + 255) / 256), dim3(256), 0, b->stream, b->dbinst, n, T,
-                     b->dqctr, 4, 1);
+                     b->dqctr, 32, 1);
   HIPCHK(hipGetLastError());
   *out = b;
   return FDTD_OK;
