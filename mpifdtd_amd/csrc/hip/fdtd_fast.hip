@@ -768,12 +768,15 @@ __device__ __forceinline__ void fast_tiles(const FastArgs &a, const int *qbeg, i
 // Interior tile bodies are held to the occupancy their registers allow (R = 16:
 // two wavefronts per SIMD, R = 8: four, R = 4: six): the step loop is
 // straight-line code, and unconstrained scheduling spends registers on overlap
-// past the point where a further wavefront would no longer fit.
+// past the point where a further wavefront would no longer fit.  The TM kinds
+// that run UPML tiles are held to two wavefronts per SIMD (256 registers): the
+// sampling ones needed 262-264 and ran at one, for a spill of 6-8 registers;
+// the TE bodies would spill far more and keep their one.
 __host__ __device__ constexpr int fast_interior_waves(int R) { return R >= 16 ? 2 : (R >= 8 ? 4 : 6); }
 
 template <int MODE, int FLAV, int R, int KIND>
 __global__ __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu((KIND == 1 || KIND == 4) ? fast_interior_waves(R) : 1))) void k_fast(FastArgs a)
+__attribute__((amdgpu_waves_per_eu((KIND == 1 || KIND == 4) ? fast_interior_waves(R) : (MODE == FDTD_MODE_TM ? 2 : 1)))) void k_fast(FastArgs a)
 {
   if (KIND == 3) {
     if ((int)blockIdx.x < a.nb_pml) fast_tiles<MODE, FLAV, 8, 1, 1>(a, a.qb[0], blockIdx.x, a.nb_pml);
