@@ -221,7 +221,7 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
     const bool jin = jj >= 0 && jj < NPY;
     const int jc = min(max(jj, 0), NPY - 1);  // clamped column: every load below stays in bounds
     const float c2 = uni_f(I.c2);
-    if ((int)threadIdx.x < T) sst[threadIdx.x] = BATCH ? batch_step(a.binst[t.inst], threadIdx.x) : a.st[threadIdx.x];
+    if ((int)threadIdx.x < T) sst[threadIdx.x] = BATCH ? a.bsteps[t.inst * FAST_MAX_T + threadIdx.x] : a.st[threadIdx.x];
     for (int r = threadIdx.x; r < NR; r += 64 * W) {
       const int i = ib0 + r;
       const bool ok = i >= 0 && i < NPX;
@@ -877,7 +877,8 @@ __attribute__((amdgpu_waves_per_eu(KQ == 1 ? fast_interior_waves(W8 ? R / 2 : R)
 // After a batched launch: count its steps into every instance, flip its
 // ping-pong index, set the depth of the next launch (0 once finished) and clear
 // the queue counters.  With n0 = 1 it only sets the depths (first launch).
-__global__ __launch_bounds__(256) void k_batch_advance(FastBatchInst *b, int n, int T, int *qctr, int nq, int first)
+__global__ __launch_bounds__(256) void k_batch_advance(FastBatchInst *b, int n, int T, int *qctr, int nq, int first,
+                                                       FastStep *steps)
 {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i < nq) qctr[i] = 0;
@@ -890,6 +891,9 @@ __global__ __launch_bounds__(256) void k_batch_advance(FastBatchInst *b, int n, 
     const int left = x.steps - x.n;
     x.Ti = left <= 0 ? 0 : (left < T ? left : T);
     b[i] = x;
+    // the next launch's step scalars, formed here once per instance (in FP64)
+    // rather than by every tile, whose registers the FP64 code would weigh down
+    for (int s = 0; s < x.Ti; s++) steps[i * FAST_MAX_T + s] = batch_step(x, s);
   }
 }
 
@@ -1814,6 +1818,7 @@ struct FastBatch {
   std::vector<int> steps;
   FastInst *dinst = nullptr;
   FastBatchInst *dbinst = nullptr;
+  FastStep *dsteps = nullptr;       // [instance][FAST_MAX_T]
   FastTile *dtiles = nullptr;
   int *dqctr = nullptr;             // [2 modes][2 queues][8 XCD parts]
   int nt[2][2] = {{0, 0}, {0, 0}};  // tiles per mode and queue
@@ -1861,6 +1866,7 @@ static FastArgs batch_args(const FastBatch *b, int m)
   a.T = b->T;
   a.src = 1;
   a.binst = b->dbinst;
+  a.bsteps = b->dsteps;
   a.qctr = b->dqctr + 16 * m;
   a.qn[0] = b->nt[m][0];
   a.qn[1] = b->nt[m][1];
@@ -1889,7 +1895,7 @@ static int batch_level(FastBatch *b)
     HIPCHK(hipStreamWaitEvent(b->stream, b->ev_join, 0));
   }
   hipLaunchKernelGGL(k_batch_advance, dim3((std::max(b->n, 4) + 255) / 256), dim3(256), 0, b->stream, b->dbinst, b->n,
-                     b->T, b->dqctr, 32, 0);
+                     b->T, b->dqctr, 32, 0, b->dsteps);
   HIPCHK(hipGetLastError());
   return FDTD_OK;
 }
@@ -1902,6 +1908,7 @@ void fast_batch_destroy(FastBatch *b)
   if (b->graph) (void)hipGraphDestroy(b->graph);
   if (b->dinst) (void)dev_free(b->dinst);
   if (b->dbinst) (void)dev_free(b->dbinst);
+  if (b->dsteps) (void)dev_free(b->dsteps);
   if (b->dtiles) (void)dev_free(b->dtiles);
   if (b->dqctr) (void)dev_free(b->dqctr);
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -1993,6 +2000,7 @@ int fast_batch_create(FastState *const *st, const fdtd_step_params *p0, const in
   HIPCHK(hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming));
   HIPCHK(dev_malloc(&b->dinst, n * sizeof(FastInst)));
   HIPCHK(dev_malloc(&b->dbinst, n * sizeof(FastBatchInst)));
+  HIPCHK(dev_malloc(&b->dsteps, (size_t)n * FAST_MAX_T * sizeof(FastStep)));
   HIPCHK(dev_malloc(&b->dtiles, std::max<size_t>(1, all.size()) * sizeof(FastTile)));
   HIPCHK(dev_malloc(&b->dqctr, 32 * sizeof(int)));
   HIPCHK(hipMemcpy(b->dinst, hinst.data(), n * sizeof(FastInst), hipMemcpyHostToDevice));
@@ -2001,7 +2009,7 @@ int fast_batch_create(FastState *const *st, const fdtd_step_params *p0, const in
   HIPCHK(hipDeviceSynchronize());
   // depths of the first block, queue counters cleared
   hipLaunchKernelGGL(k_batch_advance, dim3((std::max(n, 4) + 255) / 256), dim3(256), 0, b->stream, b->dbinst, n, T,
-                     b->dqctr, 32, 1);
+                     b->dqctr, 32, 1, b->dsteps);
   HIPCHK(hipGetLastError());
   *out = b;
   return FDTD_OK;

@@ -122,6 +122,7 @@ struct FastArgs {
   // batched runs (k_fast_batch): per-instance progress and the launch's two tile
   // queues, UPML (0) and interior (1), at tiles[qoff[k], qoff[k] + qn[k])
   FastBatchInst *binst;
+  const FastStep *bsteps;   // [instance][FAST_MAX_T] the step scalars of the launch's steps
   int *qctr;
   int qn[2], qoff[2];
   // per-step NTFF samples: after every inner step s, each contour cell of a tile's
