@@ -697,6 +697,7 @@ __global__ __launch_bounds__(256) void k_ntff_gather(const double2 *const *F, Co
 #define NTFF_WIN 1024  // bin offsets per LDS window of k_ntff_bin
 #define NTFF_GRP 32    // sample groups per workgroup of k_ntff_bin
 #define NTFF_RUN 16    // samples per group and part
+#define NTFF_AHEAD 4   // records of a group's run loaded together
 
 // Binning of n consecutive steps per launch.
 //
@@ -751,62 +752,71 @@ __global__ __launch_bounds__(NTFF_GRP * NS) void k_ntff_bin(const double2 *cE, c
   double xr0 = 0.0, xr1 = 0.0, xr2 = 0.0, xi0 = 0.0, xi1 = 0.0, xi2 = 0.0;
   int rc = -1;  // offset (window-relative) of the run's current sample; xr0 is bin rc + s
   const BinRec *rr = rec + W.rec0 + grp;
-  const BinRec pad = {-1, 0, 0.0};
   auto value = [&](int p) {
     // padding (p < 0) and lanes past n read nothing; the address stays in cell 0
     const int pc = p < 0 ? 0 : p;
     const double2 *src = ((pc >> 30) ? cH : cE) + (size_t)(pc & 0x3fffffff) * n;
     return (p >= 0 && act) ? src[s] : make_double2(0.0, 0.0);
   };
-  BinRec R = W.per > 0 ? rr[0] : pad;
-  double2 v = value(R.p);
-  for (int i = 0; i < W.per && R.p >= 0; i++) {
-    // the next record and its value in flight while this one is added
-    const BinRec Rn = i + 1 < W.per ? rr[(size_t)(i + 1) * NTFF_GRP] : pad;
-    const double2 vn = value(Rn.p);
-    if (R.r != rc) {
-      if (rc >= 0) {
-        const int delta = R.r - rc;
-        if (act) {
-          if (delta >= 3) {
-            atomicAdd(&lds[rc + s][0], xr0);
-            atomicAdd(&lds[rc + s][1], xi0);
-            atomicAdd(&lds[rc + s + 1][0], xr1);
-            atomicAdd(&lds[rc + s + 1][1], xi1);
-            atomicAdd(&lds[rc + s + 2][0], xr2);
-            atomicAdd(&lds[rc + s + 2][1], xi2);
-            xr0 = xr1 = xr2 = xi0 = xi1 = xi2 = 0.0;
-          } else if (delta == 2) {
-            atomicAdd(&lds[rc + s][0], xr0);
-            atomicAdd(&lds[rc + s][1], xi0);
-            atomicAdd(&lds[rc + s + 1][0], xr1);
-            atomicAdd(&lds[rc + s + 1][1], xi1);
-            xr0 = xr2;
-            xi0 = xi2;
-            xr1 = xr2 = xi1 = xi2 = 0.0;
-          } else {
-            atomicAdd(&lds[rc + s][0], xr0);
-            atomicAdd(&lds[rc + s][1], xi0);
-            xr0 = xr1;
-            xi0 = xi1;
-            xr1 = xr2;
-            xi1 = xi2;
-            xr2 = xi2 = 0.0;
+  // the run is taken NTFF_AHEAD records at a time: all their record loads, then
+  // all their value loads, are in flight together (a record's value load
+  // depends on the record); padding records (p < 0) sit only at the run's end
+  for (int i0 = 0; i0 < W.per; i0 += NTFF_AHEAD) {
+    BinRec R[NTFF_AHEAD];
+    double2 v[NTFF_AHEAD];
+#pragma unroll
+    for (int k = 0; k < NTFF_AHEAD; k++) {
+      R[k] = rr[(size_t)min(i0 + k, W.per - 1) * NTFF_GRP];
+      if (i0 + k >= W.per) R[k].p = -1;
+    }
+#pragma unroll
+    for (int k = 0; k < NTFF_AHEAD; k++) v[k] = value(R[k].p);
+#pragma unroll
+    for (int k = 0; k < NTFF_AHEAD; k++) {
+      if (R[k].p < 0) break;
+      if (R[k].r != rc) {
+        if (rc >= 0) {
+          const int delta = R[k].r - rc;
+          if (act) {
+            if (delta >= 3) {
+              atomicAdd(&lds[rc + s][0], xr0);
+              atomicAdd(&lds[rc + s][1], xi0);
+              atomicAdd(&lds[rc + s + 1][0], xr1);
+              atomicAdd(&lds[rc + s + 1][1], xi1);
+              atomicAdd(&lds[rc + s + 2][0], xr2);
+              atomicAdd(&lds[rc + s + 2][1], xi2);
+              xr0 = xr1 = xr2 = xi0 = xi1 = xi2 = 0.0;
+            } else if (delta == 2) {
+              atomicAdd(&lds[rc + s][0], xr0);
+              atomicAdd(&lds[rc + s][1], xi0);
+              atomicAdd(&lds[rc + s + 1][0], xr1);
+              atomicAdd(&lds[rc + s + 1][1], xi1);
+              xr0 = xr2;
+              xi0 = xi2;
+              xr1 = xr2 = xi1 = xi2 = 0.0;
+            } else {
+              atomicAdd(&lds[rc + s][0], xr0);
+              atomicAdd(&lds[rc + s][1], xi0);
+              xr0 = xr1;
+              xi0 = xi1;
+              xr1 = xr2;
+              xi1 = xi2;
+              xr2 = xi2 = 0.0;
+            }
           }
         }
+        rc = R[k].r;
       }
-      rc = R.r;
+      const double a = R[k].a, b = 1.0 - a;
+      const double vr = v[k].x * cf, vi = v[k].y * cf;
+      xr0 += vr * b;
+      xi0 += vi * b;
+      xr1 += vr * (a - b);
+      xi1 += vi * (a - b);
+      xr2 -= vr * a;
+      xi2 -= vi * a;
     }
-    const double a = R.a, b = 1.0 - a;
-    const double vr = v.x * cf, vi = v.y * cf;
-    xr0 += vr * b;
-    xi0 += vi * b;
-    xr1 += vr * (a - b);
-    xi1 += vi * (a - b);
-    xr2 -= vr * a;
-    xi2 -= vi * a;
-    R = Rn;
-    v = vn;
+    if (R[NTFF_AHEAD - 1].p < 0) break;
   }
   if (rc >= 0 && act) {
     atomicAdd(&lds[rc + s][0], xr0);
