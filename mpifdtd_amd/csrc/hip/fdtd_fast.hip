@@ -98,6 +98,20 @@ __device__ __forceinline__ GPTR(P) uni_p(P *p)
   return (GPTR(P))(((unsigned long long)hi << 32) | lo);
 }
 
+// Cell index of (i, j) in the UPML auxiliaries, which store only the cells
+// outside the box [axlo, axhi) x [aylo, ayhi) that holds no UPML cell: first the
+// rows below and above the box (whole rows of `pitch` cells), then each row of
+// the box's row range as its acw cells left and right of the box.  (i, j) must
+// lie outside the box.  On the 8256 x 4160 grid with its 32-cell UPML this is
+// 2.3 % of the full layout.
+__host__ __device__ __forceinline__ unsigned aux_cell(int i, int j, int NPX, int pitch, int axlo, int axhi, int aylo,
+                                                      int ayhi, int acw)
+{
+  if (i < axlo) return (unsigned)(i * pitch + j);
+  if (i >= axhi) return (unsigned)((i - axhi + axlo) * pitch + j);
+  return (unsigned)((axlo + NPX - axhi) * pitch + (i - axlo) * acw + (j < aylo ? j : j - ayhi + aylo));
+}
+
 // Constants of the plain flavour's Gaussian pulse, held in registers for a tile.
 struct PulseC {
   float cpc, spc, w;
@@ -269,8 +283,9 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
     // UPML tiles: auxiliaries are loaded only outside the box that holds no UPML
     // cell (half the lanes of a side-band tile, most rows of an end-band tile's
     // inner part); inside it they are never read
-    const bool colbox = PML && jj >= uni_i(I.aylo) && jj < uni_i(I.ayhi);
     const int axlo = PML ? uni_i(I.axlo) : 0, axhi = PML ? uni_i(I.axhi) : 0;
+    const int aylo = PML ? uni_i(I.aylo) : 0, ayhi = PML ? uni_i(I.ayhi) : 0, acw = PML ? uni_i(I.acw) : 0;
+    const bool colbox = PML && jc >= aylo && jc < ayhi;
 #pragma unroll
     for (int r = 0; r < R; r++) {
       const int i = ib + r;
@@ -284,13 +299,15 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
       if (PML) {
         pmlx[r] = gpx[ic];
         A0[r] = A1[r] = A2[r] = A3[r] = A4[r] = A5[r] = make_float2(0.f, 0.f);
-        if (!(colbox && i >= axlo && i < axhi)) {
-          A0[r] = ldo(ga[0], 8u * k);
-          A1[r] = ldo(ga[1], 8u * k);
-          A2[r] = ldo(ga[2], 8u * k);
-          A3[r] = ldo(ga[3], 8u * k);
-          A4[r] = ldo(ga[4], 8u * k);
-          A5[r] = ldo(ga[5], 8u * k);
+        // (ic, jc) lies outside the box, so its auxiliary cell exists
+        if (!(colbox && ic >= axlo && ic < axhi)) {
+          const unsigned ka = 8u * aux_cell(ic, jc, NPX, P, axlo, axhi, aylo, ayhi, acw);
+          A0[r] = ldo(ga[0], ka);
+          A1[r] = ldo(ga[1], ka);
+          A2[r] = ldo(ga[2], ka);
+          A3[r] = ldo(ga[3], ka);
+          A4[r] = ldo(ga[4], ka);
+          A5[r] = ldo(ga[5], ka);
         }
       }
       if (jin && i >= 0 && i < NPX) inb |= 1u << r;
@@ -731,12 +748,14 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
         sto(o1, bo, F1[r]);
         sto(o2, bo, F2[r]);
         if (PML && (pml & upd & (1u << r))) {
-          sto(oa[0], bo, A0[r]);
-          sto(oa[1], bo, A1[r]);
-          sto(oa[2], bo, A2[r]);
-          sto(oa[3], bo, A3[r]);
-          sto(oa[4], bo, A4[r]);
-          sto(oa[5], bo, A5[r]);
+          // a UPML cell lies outside the box
+          const unsigned ba = 8u * aux_cell(i, jj, NPX, P, axlo, axhi, aylo, ayhi, acw);
+          sto(oa[0], ba, A0[r]);
+          sto(oa[1], ba, A1[r]);
+          sto(oa[2], ba, A2[r]);
+          sto(oa[3], ba, A3[r]);
+          sto(oa[4], ba, A4[r]);
+          sto(oa[5], ba, A5[r]);
         }
       }
     }
@@ -918,6 +937,30 @@ __global__ __launch_bounds__(256) void k_fast_import(const double2 *src, int NPY
   dst[(size_t)i * pitch + j] = make_float2((float)v.x, (float)v.y);
 }
 
+// The same for a UPML auxiliary in its compact layout (aux_cell): the cells of
+// the box, which has no storage, export as zero (no update ever writes them)
+// and import nothing.  src == nullptr (auxiliaries not allocated yet): zeros.
+__global__ __launch_bounds__(256) void k_fast_aux_export(const float2 *src, FastInst h, double2 *dst, int NPYa, int off)
+{
+  const int j = blockIdx.x * 64 + threadIdx.x;
+  const int i = blockIdx.y * 4 + threadIdx.y;
+  if (i >= h.NPX || j >= h.NPY) return;
+  const bool box = i >= h.axlo && i < h.axhi && j >= h.aylo && j < h.ayhi;
+  float2 v = make_float2(0.f, 0.f);
+  if (src && !box) v = src[aux_cell(i, j, h.NPX, h.pitch, h.axlo, h.axhi, h.aylo, h.ayhi, h.acw)];
+  dst[(size_t)(i + off) * NPYa + (j + off)] = make_double2((double)v.x, (double)v.y);
+}
+
+__global__ __launch_bounds__(256) void k_fast_aux_import(const double2 *src, int NPYa, int off, FastInst h, float2 *dst)
+{
+  const int j = blockIdx.x * 64 + threadIdx.x;
+  const int i = blockIdx.y * 4 + threadIdx.y;
+  if (i >= h.NPX || j >= h.NPY) return;
+  if (i >= h.axlo && i < h.axhi && j >= h.aylo && j < h.ayhi) return;
+  const double2 v = src[(size_t)(i + off) * NPYa + (j + off)];
+  dst[aux_cell(i, j, h.NPX, h.pitch, h.axlo, h.axhi, h.aylo, h.ayhi, h.acw)] = make_float2((float)v.x, (float)v.y);
+}
+
 // 1/eps in FP32 from the FP64 eps map (solver layout)
 __global__ __launch_bounds__(256) void k_fast_inv_eps(const double *eps, int NPYa, int off, int NPX, int NPY, float *ie, int pitch)
 {
@@ -1075,6 +1118,10 @@ struct FastState {
   FastInst *dinst = nullptr;
   std::vector<void *> allocs;
   float *ftab = nullptr;
+  // the 12 UPML auxiliary arrays (6 x ping-pong) in one block of 12 x auxn
+  // cells, allocated when the coefficients fix the box without UPML cells
+  float2 *auxbuf = nullptr;
+  size_t auxn = 0;
   unsigned char *pmlflags = nullptr;
   std::vector<unsigned char> pmlx, pmly;
   float2 *exy = nullptr;
@@ -1196,6 +1243,36 @@ void fast_profile_read(double *ms, long *launches, double *bytes, double *cells)
   if (cells) *cells = o[5];
 }
 
+// The box without UPML cells from the row and column flags: [axlo, axhi) the
+// rows between the leading and trailing UPML rows, [aylo, ayhi) the same for the
+// columns, empty in a direction whose UPML rows (columns) do not all lie at the
+// grid's ends; acw the auxiliary cells of a row of the box's row range.
+static void upml_box(FastState *s)
+{
+  const int NPX = s->h.NPX, NPY = s->h.NPY;
+  int xlo = 0, xhi = NPX, ylo = 0, yhi = NPY;
+  while (xlo < NPX && s->pmlx[xlo]) xlo++;
+  while (xhi > xlo && s->pmlx[xhi - 1]) xhi--;
+  while (ylo < NPY && s->pmly[ylo]) ylo++;
+  while (yhi > ylo && s->pmly[yhi - 1]) yhi--;
+  for (int x = xlo; x < xhi; x++)
+    if (s->pmlx[x]) xhi = xlo;  // UPML cells inside: treat everything as UPML
+  for (int y = ylo; y < yhi; y++)
+    if (s->pmly[y]) yhi = ylo;
+  s->h.axlo = xlo;
+  s->h.axhi = xhi;
+  s->h.aylo = ylo;
+  s->h.ayhi = yhi;
+  s->h.acw = ylo + NPY - yhi;
+}
+
+// cells of one auxiliary array (aux_cell's layout), rounded up to whole 64-cell lines
+static size_t aux_cells(const FastInst &h)
+{
+  const size_t n = (size_t)(h.axlo + h.NPX - h.axhi) * h.pitch + (size_t)(h.axhi - h.axlo) * h.acw;
+  return std::max<size_t>(64, (n + 63) / 64 * 64);
+}
+
 template <class T>
 static int dev_alloc(FastState *s, T **p, size_t n)
 {
@@ -1225,8 +1302,6 @@ int fast_state_create(const fdtd_grid_desc &d, FastState **out)
     if ((rc = dev_alloc(s, &h.f0[b], n))) return rc;
     if ((rc = dev_alloc(s, &h.f1[b], n))) return rc;
     if ((rc = dev_alloc(s, &h.f2[b], n))) return rc;
-    for (int q = 0; q < 6; q++)
-      if ((rc = dev_alloc(s, &h.aux[q][b], n))) return rc;
   }
   float *ie0, *ie1;
   if ((rc = dev_alloc(s, &ie0, n))) return rc;
@@ -1247,6 +1322,7 @@ int fast_state_create(const fdtd_grid_desc &d, FastState **out)
   if ((rc = dev_alloc(s, &s->dinst, 1))) return rc;
   s->pmlx.assign(h.NPX, 1);
   s->pmly.assign(h.NPY, 1);
+  upml_box(s);
   HIPCHK(hipMemcpy(s->dinst, &h, sizeof h, hipMemcpyHostToDevice));
   return FDTD_OK;
 }
@@ -1261,6 +1337,7 @@ void fast_state_destroy(FastState *s)
     (void)hipEventDestroy(s->ev_out);
   }
   for (void *p : s->allocs) (void)dev_free(p);
+  if (s->auxbuf) (void)dev_free(s->auxbuf);
   if (s->dtiles) (void)dev_free(s->dtiles);
   for (int b = 0; b < 2; b++) {
     if (s->csampb[b]) (void)dev_free(s->csampb[b]);
@@ -1296,6 +1373,21 @@ int fast_state_set_coefs(FastState *s, const fdtd_coef_tables *t)
   std::copy(s->pmlx.begin(), s->pmlx.end(), fl.begin());
   std::copy(s->pmly.begin(), s->pmly.end(), fl.begin() + nx);
   HIPCHK(hipMemcpy(s->pmlflags, fl.data(), fl.size(), hipMemcpyHostToDevice));
+  // the auxiliaries' compact layout follows the box: a new box gets new, zeroed
+  // arrays (the coefficients are set before the first step)
+  const int box0[5] = {s->h.axlo, s->h.axhi, s->h.aylo, s->h.ayhi, s->h.acw};
+  upml_box(s);
+  const int box1[5] = {s->h.axlo, s->h.axhi, s->h.aylo, s->h.ayhi, s->h.acw};
+  if (!s->auxbuf || memcmp(box0, box1, sizeof box0) != 0) {
+    if (s->auxbuf) HIPCHK(dev_free(s->auxbuf));
+    s->auxbuf = nullptr;
+    s->auxn = aux_cells(s->h);
+    HIPCHK(dev_malloc(&s->auxbuf, 12 * s->auxn * sizeof(float2)));
+    HIPCHK(hipMemset(s->auxbuf, 0, 12 * s->auxn * sizeof(float2)));
+    for (int b = 0; b < 2; b++)
+      for (int q = 0; q < 6; q++) s->h.aux[q][b] = s->auxbuf + (size_t)(6 * b + q) * s->auxn;
+  }
+  HIPCHK(hipMemcpy(s->dinst, &s->h, sizeof s->h, hipMemcpyHostToDevice));
   s->tilesT = -1;  // rebuild tiles
   return FDTD_OK;
 }
@@ -1363,21 +1455,9 @@ static int build_tiles(FastState *s)
   const int UR = s->upmlRows;  // rows of a UPML block
   const int CW = 64 - 2 * T - 1, CHi = 4 * R - 2 * T - 1, CHp = UR - 2 * T - 1;
   if (CW < 1 || CHi < 1 || CHp < 1) return fdtd_fail(FDTD_EINVAL, "temporal block depth %d too deep for R=%d", T, R);
-  // extent of the cells without UPML recurrence
-  int xlo = 0, xhi = NPX, ylo = 0, yhi = NPY;
-  while (xlo < NPX && s->pmlx[xlo]) xlo++;
-  while (xhi > xlo && s->pmlx[xhi - 1]) xhi--;
-  while (ylo < NPY && s->pmly[ylo]) ylo++;
-  while (yhi > ylo && s->pmly[yhi - 1]) yhi--;
-  for (int x = xlo; x < xhi; x++)
-    if (s->pmlx[x]) xhi = xlo;  // UPML cells inside: treat everything as UPML
-  for (int y = ylo; y < yhi; y++)
-    if (s->pmly[y]) yhi = ylo;
-  // the box without UPML cells, where the UPML tiles load no auxiliary
-  s->h.axlo = xlo;
-  s->h.axhi = xhi;
-  s->h.aylo = ylo;
-  s->h.ayhi = yhi;
+  if (!s->auxbuf) return fdtd_fail(FDTD_ESTATE, "the FP32 solver's UPML coefficients are not set");
+  // extent of the cells without UPML recurrence: the box of the auxiliaries' layout
+  int xlo = s->h.axlo, xhi = s->h.axhi, ylo = s->h.aylo, yhi = s->h.ayhi;
   HIPCHK(hipMemcpy(s->dinst, &s->h, sizeof s->h, hipMemcpyHostToDevice));
   if (s->d.flavour == FDTD_FLAVOUR_PLAIN) {
     // the plain solvers never update their edge cells (fdtdTM_upml.c:158-159): keep
@@ -1732,14 +1812,20 @@ static float2 *slot_ptr(FastState *s, int slot, int buf)
   default: return nullptr;
   }
 }
+static bool aux_slot(int slot) { return slot == 1 || slot == 2 || slot == 4 || slot == 5 || slot == 7 || slot == 8; }
 
 int fast_state_export(FastState *s, int slot, double2 *dst, int NPYa, int off)
 {
   int rc;
   if ((rc = fast_state_flush(s))) return rc;
-  float2 *src = slot_ptr(s, slot, s->cur);
-  if (!src) return fdtd_fail(FDTD_EINVAL, "bad slot %d", slot);
-  fast_export(src, s->h.pitch, s->h.NPX, s->h.NPY, dst, NPYa, off, 0);
+  if (slot < 0 || slot > 8) return fdtd_fail(FDTD_EINVAL, "bad slot %d", slot);
+  if (aux_slot(slot)) {
+    dim3 blk(64, 4), grd((s->h.NPY + 63) / 64, (s->h.NPX + 3) / 4);
+    hipLaunchKernelGGL(k_fast_aux_export, grd, blk, 0, 0, s->auxbuf ? slot_ptr(s, slot, s->cur) : nullptr, s->h, dst,
+                       NPYa, off);
+  } else {
+    fast_export(slot_ptr(s, slot, s->cur), s->h.pitch, s->h.NPX, s->h.NPY, dst, NPYa, off, 0);
+  }
   HIPCHK(hipGetLastError());
   return FDTD_OK;
 }
@@ -1748,9 +1834,14 @@ int fast_state_import(FastState *s, int slot, const double2 *src, int NPYa, int 
 {
   int rc;
   if ((rc = fast_state_flush(s))) return rc;
-  float2 *dst = slot_ptr(s, slot, s->cur);
-  if (!dst) return fdtd_fail(FDTD_EINVAL, "bad slot %d", slot);
-  fast_import(src, NPYa, off, s->h.NPX, s->h.NPY, dst, s->h.pitch, 0);
+  if (slot < 0 || slot > 8) return fdtd_fail(FDTD_EINVAL, "bad slot %d", slot);
+  if (aux_slot(slot)) {
+    if (!s->auxbuf) return fdtd_fail(FDTD_ESTATE, "the FP32 solver's UPML coefficients are not set");
+    dim3 blk(64, 4), grd((s->h.NPY + 63) / 64, (s->h.NPX + 3) / 4);
+    hipLaunchKernelGGL(k_fast_aux_import, grd, blk, 0, 0, src, NPYa, off, s->h, slot_ptr(s, slot, s->cur));
+  } else {
+    fast_import(src, NPYa, off, s->h.NPX, s->h.NPY, slot_ptr(s, slot, s->cur), s->h.pitch, 0);
+  }
   HIPCHK(hipGetLastError());
   return FDTD_OK;
 }
@@ -1763,8 +1854,8 @@ int fast_state_reset(FastState *s)
     HIPCHK(hipMemset(s->h.f0[b], 0, n));
     HIPCHK(hipMemset(s->h.f1[b], 0, n));
     HIPCHK(hipMemset(s->h.f2[b], 0, n));
-    for (int q = 0; q < 6; q++) HIPCHK(hipMemset(s->h.aux[q][b], 0, n));
   }
+  if (s->auxbuf) HIPCHK(hipMemset(s->auxbuf, 0, 12 * s->auxn * sizeof(float2)));
   s->cur = 0;
   s->prev_c_re = s->prev_c_im = 0;
   s->steps_since_reset = 0;

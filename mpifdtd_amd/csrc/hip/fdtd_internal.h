@@ -68,16 +68,19 @@ struct FastInst {
   float c2;                 // 1/mu0 = c0^2
   // fields, ping-pong [2]: TM: f0 = Ez, f1 = Hx, f2 = Hy ; TE: f0 = Ex, f1 = Ey, f2 = Hz
   float2 *f0[2], *f1[2], *f2[2];
-  // UPML auxiliaries (touched only in PML cells), ping-pong
+  // UPML auxiliaries (touched only in PML cells), ping-pong, in the compact
+  // layout of aux_cell (the cells outside the box [axlo, axhi) x [aylo, ayhi))
   //   TM: a0 Jz a1 Dz a2 Mx a3 Bx a4 My a5 By ; TE: a0 Jx a1 Dx a2 Jy a3 Dy a4 Mz a5 Bz
   float2 *aux[6][2];
   const float *ie0, *ie1;   // 1/eps of the E components (TM: Ez ; TE: Ex, Ey)
   // 1-D coefficient tables (FP32), same slot meaning as fdtd_coef_tables
   const float *tx[5], *tsx, *ty[4], *tsy;
   const unsigned char *pmlx, *pmly;  // 1 where the cell uses the UPML recurrence
-  // the box [axlo, axhi) x [aylo, ayhi) holds no UPML cell: the UPML tiles load
-  // no auxiliary there (empty when UPML rows or columns lie inside the grid)
-  int axlo, axhi, aylo, ayhi;
+  // the box [axlo, axhi) x [aylo, ayhi) holds no UPML cell: the auxiliaries have
+  // no storage there and the UPML tiles load none (an empty row or column range
+  // when UPML rows or columns lie inside the grid); acw = aylo + NPY - ayhi, the
+  // auxiliary cells of one row of the box's row range
+  int axlo, axhi, aylo, ayhi, acw;
   const float2 *ex, *ey;    // e^{i kx x}, e^{i ky y} (MPI scattered wave)
   float cpc, spc, w;        // plain pulse: cos/c0, sin/c0, omega
   float gx[2], gy[2], dot[2];
