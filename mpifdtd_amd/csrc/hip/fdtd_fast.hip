@@ -457,12 +457,8 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
             float2 e = fma2(curl, IE0(r), F0[r]);
             if (FLAV == FDTD_FLAVOUR_MPI) {
               if (SRC) e = add_src(e, r, IE0(r), st);
-            } else if (scat0 & bit) {
-              float2 sn, sd;
-              source(0, r, IE0(r), st, sn, sd);
-              e = c_add(e, sd);
             }
-            F0[r] = e;
+            F0[r] = e;  // plain: the pulse is added below, after every row's stencil
           } else {
             float2 sn = make_float2(0.f, 0.f), sd = sn;
             src_any(0, r, IE0(r), st, SRCT, sn, sd);
@@ -489,19 +485,8 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
             float2 ey = fma2(cyv, IE1(r), F1[r]);
             if (FLAV == FDTD_FLAVOUR_MPI) {
               if (SRC) ey = add_src(ey, r, IE1(r), st);
-            } else {
-              if (scat0 & bit) {
-                float2 sn, sd;
-                source(0, r, IE0(r), st, sn, sd);
-                ex = c_add(ex, sd);
-              }
-              if (scat1 & bit) {
-                float2 sn, sd;
-                source(1, r, IE1(r), st, sn, sd);
-                ey = c_add(ey, sd);
-              }
             }
-            F0[r] = ex;
+            F0[r] = ex;  // plain: the pulse is added below, after every row's stencil
             F1[r] = ey;
           } else {
             float2 sn0 = make_float2(0.f, 0.f), sd0 = sn0, sn1 = sn0, sd1 = sn0;
@@ -529,6 +514,27 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
               F0[r] = sel(u, ex, F0[r]);
               F1[r] = sel(u, ey, F1[r]);
             }
+          }
+        }
+      }
+      // Plain interior tiles: the pulse's change in the scatterer cells, added to
+      // every row after all rows' stencils (the same two roundings as adding it
+      // right after each stencil: it depends on the cell and step only), so the
+      // stencils and the next step's H rows form one block without branches,
+      // whose lane shifts fold into the subtractions that use them
+      if (FLAV == FDTD_FLAVOUR_PLAIN && !PML && wsrc) {
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const unsigned bit = 1u << r;
+          if (scat0 & bit) {
+            float2 sn, sd;
+            source(0, r, IE0(r), st, sn, sd);
+            F0[r] = c_add(F0[r], sd);
+          }
+          if (MODE == FDTD_MODE_TE && (scat1 & bit)) {
+            float2 sn, sd;
+            source(1, r, IE1(r), st, sn, sd);
+            F1[r] = c_add(F1[r], sd);
           }
         }
       }
