@@ -178,7 +178,7 @@ def cpu_reference_sweep():
     except AttributeError:
         ncores = os.cpu_count() or 1
     ncores = max(1, min(16, ncores))
-    base = ["mode=time", "solver=2", "model=3", "h=5", "pml=32", "lambda=550", "angle=90", "warm=1"]
+    base = ["mode=time", "solver=2", "model=3", "h=5", "pml=32", "lambda=550", "angle=90", "warm=1", "ntff=1"]
     side1 = 5860
     try:
         log("cpu baseline: the reference's plain TM UPML step, 5860 x 5860, one core")
@@ -193,17 +193,30 @@ def cpu_reference_sweep():
         recs = [json.loads(o.strip().splitlines()[-1]) for o in outs]
         agg = sum(r["N_PX"] * r["N_PY"] * r["steps"] / r["seconds"] for r in recs) / 1e6
         secs = max(r["seconds"] for r in recs)
+
+        # the step without its per-step NTFF: the NTFF timed alone for the same
+        # calls on the same fields (ntff=1), subtracted
+        def no_ntff(r):
+            t = r["seconds"] - r.get("ntff_seconds", -1.0)
+            return r["N_PX"] * r["N_PY"] * r["steps"] / t if 0 <= r.get("ntff_seconds", -1.0) and t > 0 else None
+        rates = [no_ntff(r) for r in recs]
+        agg_no = sum(rates) / 1e6 if all(x is not None for x in rates) else None
+        one_no = no_ntff(r1)
     except (OSError, ValueError, IndexError, KeyError, subprocess.SubprocessError) as e:
         log(f"cpu baseline failed: {e}")
         return None
     return {"value": round(agg, 3), "unit": "Mcell-updates/s", "cores": ncores, "kind": "reference",
             "value_1core": round(one, 3),
+            "value_no_ntff": round(agg_no, 3) if agg_no is not None else None,
+            "value_1core_no_ntff": round(one_no / 1e6, 3) if one_no is not None else None,
             "sample": f"the reference's plain TM UPML step (with its per-step time-domain NTFF) built from its "
                       f"sources with gcc -O2; all cores: {ncores} independent simulations of {side_n}x{side_n} "
                       f"cells together (rank per simulation, main.c:114-138), 40 steps each ({secs:.1f} s); "
                       f"one core: {r1['N_PX']}x{r1['N_PY']} (34.3 M cells, the 8256x4160 grid's count; the "
                       f"reference's NTFF bins overflow on the non-square grid itself), {r1['steps']} steps "
-                      f"({r1['seconds']:.1f} s); Morpho model, h=5 nm, 32-cell UPML, eps fill outside the window"}
+                      f"({r1['seconds']:.1f} s); Morpho model, h=5 nm, 32-cell UPML, eps fill outside the window; "
+                      f"*_no_ntff: the same steps less their per-step NTFF (ntffTM_TimeCalc, fdtdTM_upml.c:65), "
+                      f"timed alone for the same calls on the same fields and subtracted"}
 
 
 def cpu_reference(mode, workload, steps=120):
@@ -450,6 +463,8 @@ def run_sweep81(args, m, cpu):
             line[k] = head[k]
     if cpu:
         line["speedup_vs_cpu_all_cores"] = round(head["value"] / cpu["value"], 1)
+        if cpu.get("value_no_ntff"):
+            line["speedup_vs_cpu_all_cores_no_ntff"] = round(head["value"] / cpu["value_no_ntff"], 1)
     for md in modes[1:]:
         line[md] = per[md]
     return line

@@ -24,8 +24,10 @@
  *            N_PX x N_PY cells.
  * mode=time: simulator_init, `warm` x simulator_calc, then `steps` x
  *            simulator_calc timed with CLOCK_MONOTONIC; prints one JSON line
- *            {"N_PX", "N_PY", "steps", "seconds"} and writes nothing.  The CPU
- *            baseline of bench.py (the reference timed beside the GPU run).
+ *            {"N_PX", "N_PY", "steps", "seconds", "ntff_seconds"} and writes
+ *            nothing.  ntff=1 (plain TM): then `steps` x ntffTM_TimeCalc alone on
+ *            the same fields (ntff_seconds; -1 otherwise).  The CPU baseline of
+ *            bench.py (the reference timed beside the GPU run).
  * mode=util: field_init, `steps` x field_nextStep, then field.h / myComplex.h
  *            helpers on fixed inputs: eps.bin, base.bin (the inputs), src.bin
  *            (field_scatteredWave, _NotUPML, field_nsScatteredWaveNotUPML and
@@ -203,8 +205,24 @@ int main(int argc, char **argv)
     for (int n = 0; n < fi.stepNum; n++) simulator_calc();
     clock_gettime(CLOCK_MONOTONIC, &t1);
     FieldInfo_S st = field_getFieldInfo_S();
-    printf("{\"N_PX\": %d, \"N_PY\": %d, \"steps\": %d, \"seconds\": %.6f}\n", st.N_PX, st.N_PY, fi.stepNum,
-           (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec));
+    /* ntff=1 (plain TM): the per-step time-domain NTFF the plain TM update calls
+     * (fdtdTM_upml.c:65), timed alone for the same number of calls on the same
+     * fields into accumulators of the reference's size, so that the step's rate
+     * without it follows by difference (the update's other parts are file-static) */
+    double ntff_s = -1.0;
+    if (atoi(arg(argc, argv, "ntff", "0")) && solver == TM_UPML_2D) {
+      const size_t na = (size_t)360 * field_getNTFFInfo().arraySize;
+      double complex *U0 = calloc(na, sizeof *U0), *U1 = calloc(na, sizeof *U1), *U2 = calloc(na, sizeof *U2);
+      if (!U0 || !U1 || !U2) { fprintf(stderr, "out of memory\n"); _exit(1); }
+      struct timespec n0, n1;
+      clock_gettime(CLOCK_MONOTONIC, &n0);
+      for (int n = 0; n < fi.stepNum; n++)
+        ntffTM_TimeCalc(fdtdTM_upml_getHx(), fdtdTM_upml_getHy(), fdtdTM_upml_getEz(), U0, U1, U2);
+      clock_gettime(CLOCK_MONOTONIC, &n1);
+      ntff_s = (double)(n1.tv_sec - n0.tv_sec) + 1e-9 * (double)(n1.tv_nsec - n0.tv_nsec);
+    }
+    printf("{\"N_PX\": %d, \"N_PY\": %d, \"steps\": %d, \"seconds\": %.6f, \"ntff_seconds\": %.6f}\n", st.N_PX,
+           st.N_PY, fi.stepNum, (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec), ntff_s);
     fflush(stdout);
     _exit(0);  /* no finish(): it would write the reference's output files */
   }
