@@ -714,15 +714,16 @@ __global__ __launch_bounds__(256) void k_ntff_gather(const double2 *const *F, Co
 // NS >= n lanes; group g takes a contiguous run of the part's sorted samples
 // (records stored [iteration][group], so that their loads coalesce) and lane s
 // of the group handles step s: the n steps of a sample are contiguous in the
-// step-minor sample buffers ([cell][n]), so one load instruction reads whole
-// cache lines.  Lane s keeps the three bins d + s - 1 + k of its step in
+// step-minor sample buffer ([cell][nbuf], E then H), so one load instruction
+// reads whole cache lines; a record holds the element of its sample's first
+// step, so a value's address is one add.  Lane s keeps the three bins d + s - 1 + k of its step in
 // registers; as the run's offsets grow the window shifts, and only the bins it
 // leaves go to the LDS window with an atomic.  The LDS window goes to the
 // accumulators once per launch: its first and last n + 2 bins with atomics (the
 // neighbouring parts of the same row reach them), the others with a plain
 // read-modify-write.
 struct BinRec {
-  int p;     // cell | H << 30, or -1 (padding at the end of a group's run)
+  int p;     // element of the sample's first step in the sample buffer, or -1 (padding at the end of a group's run)
   int r;     // d minus the part's smallest offset
   double a;  // tap fraction
 };
@@ -733,7 +734,7 @@ struct BinPart {
 };
 
 template <int NS>
-__global__ __launch_bounds__(NTFF_GRP * NS) void k_ntff_bin(const double2 *cE, const double2 *cH, int n,
+__global__ __launch_bounds__(NTFF_GRP * NS) void k_ntff_bin(const double2 *cS, int n,
                                                              const BinPart *parts, const BinRec *rec, double2 *U0,
                                                              double2 *U1, double2 *U2, int arraySize, long tb0,
                                                              double coef, int useCoef)
@@ -753,10 +754,9 @@ __global__ __launch_bounds__(NTFF_GRP * NS) void k_ntff_bin(const double2 *cE, c
   int rc = -1;  // offset (window-relative) of the run's current sample; xr0 is bin rc + s
   const BinRec *rr = rec + W.rec0 + grp;
   auto value = [&](int p) {
-    // padding (p < 0) and lanes past n read nothing; the address stays in cell 0
-    const int pc = p < 0 ? 0 : p;
-    const double2 *src = ((pc >> 30) ? cH : cE) + (size_t)(pc & 0x3fffffff) * n;
-    return (p >= 0 && act) ? src[s] : make_double2(0.0, 0.0);
+    // p: the element of the sample's first step; padding (p < 0) and lanes past
+    // n read nothing
+    return (p >= 0 && act) ? cS[p + s] : make_double2(0.0, 0.0);
   };
   // the run is taken NTFF_AHEAD records at a time: all their record loads, then
   // all their value loads, are in flight together (a record's value load
@@ -808,7 +808,7 @@ __global__ __launch_bounds__(NTFF_GRP * NS) void k_ntff_bin(const double2 *cE, c
         rc = R[k].r;
       }
       const double a = R[k].a, b = 1.0 - a;
-      const double vr = v[k].x * cf, vi = v[k].y * cf;
+      const double vr = v[k].x, vi = v[k].y;  // the coefficient is applied to the window's bins
       xr0 += vr * b;
       xi0 += vi * b;
       xr1 += vr * (a - b);
@@ -829,7 +829,7 @@ __global__ __launch_bounds__(NTFF_GRP * NS) void k_ntff_bin(const double2 *cE, c
   __syncthreads();
   // window bins [tb0 + dlo - 1, + nb) into the accumulators
   for (int q = threadIdx.x; q < nb; q += NT) {
-    const double re = lds[q][0], im = lds[q][1];
+    const double re = lds[q][0] * cf, im = lds[q][1] * cf;
     if (re == 0.0 && im == 0.0) continue;
     const long m = tb0 + W.dlo - 1 + q;
     // taps outside the [360][arraySize] allocation (+ guard) are dropped; the
@@ -1553,6 +1553,10 @@ static int build_bin_groups(fdtd_grid *g, const std::vector<double> &tab)
 {
   const Contour &ct = g->ct;
   const int P = ct.total;
+  // a record addresses its sample's first step directly: cell p of E at p * nb,
+  // of H at (P + p) * nb (the sample buffer holds E then H, nb steps per cell)
+  const int nb = g->nbuf;
+  if ((size_t)2 * P * nb > (size_t)INT_MAX) return fdtd_fail(FDTD_ENOMEM, "NTFF contour of %d cells too long", P);
   std::vector<BinPart> parts;
   std::vector<BinRec> rec;
   std::vector<int> side(P);
@@ -1569,8 +1573,8 @@ static int build_bin_groups(fdtd_grid *g, const std::vector<double> &tab)
     for (int p = 0; p < P; p++) {
       const Seg &sg = ct.s[side[p]];
       const int dE = (int)floor(ts[p] + 0.5), dH = (int)floor(ts[p]) + 1;
-      by[sg.ua].push_back({dE, p, (0.5 + ts[p]) - (double)dE});
-      by[sg.wa].push_back({dH, p | (1 << 30), ts[p] - (double)(dH - 1)});
+      by[sg.ua].push_back({dE, p * nb, (0.5 + ts[p]) - (double)dE});
+      by[sg.wa].push_back({dH, (P + p) * nb, ts[p] - (double)(dH - 1)});
     }
     for (int arr = 0; arr < 3; arr++) {
       std::vector<Smp> &v = by[arr];
@@ -1609,7 +1613,7 @@ static int build_bin_groups(fdtd_grid *g, const std::vector<double> &tab)
   return FDTD_OK;
 }
 
-// n steps of samples in cE / cH ([cell][n]) into the accumulators
+// the first n steps of the samples in cE / cH ([cell][nbuf]) into the accumulators
 static void launch_bin(fdtd_grid *g, int n, double time0, hipStream_t st)
 {
   const fdtd_grid_desc &d = g->d;
@@ -1617,7 +1621,7 @@ static void launch_bin(fdtd_grid *g, int n, double time0, hipStream_t st)
   const int useCoef = d.flavour == FDTD_FLAVOUR_PLAIN ? 0 : 1;
   if (g->nparts <= 0) return;
 #define NTFF_BIN(NS)                                                                                              \
-  hipLaunchKernelGGL(k_ntff_bin<NS>, dim3(g->nparts), dim3(NTFF_GRP * NS), 0, st, g->cE, g->cH, n, g->binPart, g->binRec, \
+  hipLaunchKernelGGL(k_ntff_bin<NS>, dim3(g->nparts), dim3(NTFF_GRP * NS), 0, st, g->cE, n, g->binPart, g->binRec,        \
                      g->nt[0], g->nt[1], g->nt[2], d.array_size, tb0, g->coef, useCoef)
   if (n <= 2)
     NTFF_BIN(2);
@@ -1721,11 +1725,11 @@ static int grid_bin_samples(void *ctx, const float2 *samples, int n, const doubl
   }
   const dim3 cgrid((g->ct.total + 255) / 256, n);
   if (g->ntn) {
-    // binned step by step: [cell][n]; the DFT reads the same buffers
-    hipLaunchKernelGGL(k_ctr_combine, cgrid, dim3(256), 0, st, samples, g->ct, 1L, (long)n, g->cE, g->cH);
+    // binned step by step: [cell][nbuf], the first n steps of each cell; the DFT reads the same buffers
+    hipLaunchKernelGGL(k_ctr_combine, cgrid, dim3(256), 0, st, samples, g->ct, 1L, (long)g->nbuf, g->cE, g->cH);
     HIPCHK(hipEventRecord(consumed, st));
     launch_bin(g, n, times[0], st);
-    if (g->nf) dft_add(g, g->cE, g->cH, 1, n, n, (long)llround(times[0]) - 1, st);
+    if (g->nf) dft_add(g, g->cE, g->cH, 1, g->nbuf, n, (long)llround(times[0]) - 1, st);
   } else if (g->nf) {
     // DFT only: staged, the DFT runs over up to sCap steps at once
     const size_t col = (size_t)dft_stage(g, n, times[0], st) * g->ct.total;
@@ -1813,8 +1817,10 @@ extern "C" int fdtd_hip_grid_create(const fdtd_grid_desc *d, fdtd_grid **out)
   const int P = std::max(1, g->ct.total);
   // FP32 grids bin a whole block of steps at once: sample buffers for FAST_MAX_T steps
   g->nbuf = d->precision == FDTD_FP32 ? FAST_MAX_T : 1;
-  HIPCHK(dev_malloc(&g->cE, (size_t)g->nbuf * P * sizeof(double2)));
-  HIPCHK(dev_malloc(&g->cH, (size_t)g->nbuf * P * sizeof(double2)));
+  // the sample buffer: E samples of every cell, then H, nbuf steps per cell
+  // (the binning's records address both from cE)
+  HIPCHK(dev_malloc(&g->cE, (size_t)2 * g->nbuf * P * sizeof(double2)));
+  g->cH = g->cE + (size_t)g->nbuf * P;
   if (d->ntff_time && g->ct.total > 0) {
     int rc = build_bin_groups(g, tab);
     if (rc) return rc;
@@ -1908,8 +1914,7 @@ extern "C" int fdtd_hip_grid_destroy(fdtd_grid *g)
   (void)dev_free(g->nsc);
   (void)dev_free(g->tab);
   for (int a = 0; a < 3; a++) (void)dev_free(g->ntb[a]);
-  (void)dev_free(g->cE);
-  (void)dev_free(g->cH);
+  (void)dev_free(g->cE);  // cH lies in the same block
   (void)dev_free(g->binPart);
   (void)dev_free(g->binRec);
   (void)dev_free(g->dftOmega);
