@@ -154,7 +154,9 @@ double *fdtdTE_upml_getEps(void);
 
 /* --- extensions on the active solver (no counterpart in the reference) --- */
 /* which 0..8 in the solver's slot order (TM Ez Jz Dz Hx Mx Bx Hy My By,
- * TE Ex Jx Dx Ey Jy Dy Hz Mz Bz); NPXa*NPYa complex doubles */
+ * TE Ex Jx Dx Ey Jy Dy Hz Mz Bz); NPXa*NPYa complex doubles.  FP32 solvers
+ * store the UPML auxiliaries (slots 1 2 4 5 7 8) only outside the box that
+ * holds no UPML cell: there getState reads zero and setState drops the values */
 int fdtd_solver_getState(int which, double *out);
 int fdtd_solver_setState(int which, const double *in);
 /* time-domain NTFF accumulators (which 0..2: Ux Uy Wz / Wx Wy Uz), 360*arraySize complex */
