@@ -96,6 +96,47 @@ def test_rows_per_wave_variants_agree():
 
 
 @pytest.mark.parametrize("solver", [2, 3, 4, 5])
+def test_rows_per_wave_20_in_the_eight_wavefront_shapes(solver):
+    """R = 20: interior tiles of 80 rows as 8 wavefronts of 10 rows, in the four
+    shapes that have them (split / two streams, with 8- or 16-wavefront UPML
+    tiles): bit-identical among themselves, within 1e-6 of R = 16 and within the
+    FP32 tolerance of the FP64 oracle; another shape refuses R = 20."""
+    m, N, po = _mods()
+    args = (solver, 1, 6000, 6000, 20, 16, 500, 0, 120)
+    print(f"R=20, solver {solver}", flush=True)
+    out = []
+    for R, shape in ((20, 6), (20, 7), (20, 8), (20, 9), (16, 8)):
+        sim = m.Simulator(solver, 1, m.field_info(*args[2:]), precision="fp32", T=4, R=R)
+        try:
+            sim.set_launch_mode(shape)
+            sim.calc(args[-1])
+            out.append([np.array(f) for f in sim.fields()])
+        finally:
+            sim.finish()
+    for o in out[1:4]:
+        for x, y in zip(o, out[0]):
+            assert np.array_equal(x, y)
+    for x, y in zip(out[4], out[0]):
+        assert rel_l2(x, y) <= 1e-6
+    o = po.Sim(width_nm=6000, height_nm=6000, h_u_nm=20, pml=16, lambda_nm=500, angle_deg=0, stepNum=args[-1],
+               solver=solver, model=1, ntff_time=0)
+    o.step(args[-1])
+    for k, which in enumerate((0, 3, 6)):
+        assert rel_l2(out[0][k], o.field(which)) <= FIELD_TOL, which
+    # split with 4-wavefront interior tiles has no R = 20 body: the launch fails
+    # when the queued steps go out, here at the state read (4 steps queued of a
+    # block of 8, so the step calls themselves launch nothing)
+    sim = m.Simulator(solver, 1, m.field_info(*args[2:]), precision="fp32", T=8, R=20)
+    try:
+        sim.set_launch_mode(2)
+        sim.calc(4)
+        with pytest.raises(RuntimeError):
+            sim.state(0)
+    finally:
+        sim.finish()
+
+
+@pytest.mark.parametrize("solver", [2, 3, 4, 5])
 def test_fp32_time_domain_ntff_against_oracle(solver):
     """FP32 fields with the per-step time-domain NTFF binned in FP64: the fused
     kernel samples the contour after every inner step, so temporal blocking
