@@ -58,11 +58,12 @@ WORKLOADS = {
 SWEEP_LAMBDAS = (380.0, 781.0, 5.0)  # 81 wavelengths
 RUN_STEPS = 20000  # steps of one run of the sweep
 # Launch configuration of the sweep per mode: (T, R, launch shape), the
-# fastest of `--autotune`'s candidates on this grid (80-row interior tiles as 8
-# wavefronts of 10 rows, 16-wavefront UPML tiles), in the split form of the
+# fastest of `--autotune`'s candidates on this grid (interior tiles as 8
+# wavefronts of 10 rows, TM, or 9 rows, TE, whose 10-row body spills; UPML tiles
+# as 16 wavefronts of 3 rows), in the split form of the
 # shapes (the two-stream forms time within 2 %, and the split form times each
 # launch on its own, as the counter summaries need).  T divides the default K.
-SWEEP_CONFIG = {"tm": (10, 20, 8), "te": (10, 20, 8)}
+SWEEP_CONFIG = {"tm": (10, 20, 8), "te": (10, 18, 8)}
 T_CANDIDATES = (10, 8, 5, 4, 2, 1)
 
 
@@ -109,8 +110,8 @@ def parse():
         a.mode = "both" if sweep else "tm"
     if a.mode == "both" and not sweep:
         sys.exit("--mode both is for the sweep workload")
-    if a.R == 20 and a.launch not in (6, 7, 8, 9) and not (sweep and a.launch == -1):
-        sys.exit("--R 20 (80-row interior tiles as 8 wavefronts of 10 rows) needs --launch 6, 7, 8 or 9")
+    if a.R in (18, 20) and a.launch not in (6, 7, 8, 9) and not (sweep and a.launch == -1):
+        sys.exit(f"--R {a.R} ({4 * a.R}-row interior tiles as 8 wavefronts) needs --launch 6, 7, 8 or 9")
     return a
 
 
@@ -327,10 +328,10 @@ def tune_sweep(m, mode, lams):
     try:
         probe.set_dft(lams)
         probe.calc(40)
-        for Tc, Rc in ((10, 20), (10, 16), (8, 16), (10, 8), (8, 8)):
+        for Tc, Rc in ((10, 20), (10, 18), (10, 16), (8, 16), (10, 8), (8, 8)):
             for sh in (1, 2, 3, 4, 5, 6, 7, 8, 9):
-                if Rc == 20 and sh not in (6, 7, 8, 9):
-                    continue  # 80-row interior tiles exist as 8 wavefronts of 10 rows only
+                if Rc in (18, 20) and sh not in (6, 7, 8, 9):
+                    continue  # 72- and 80-row interior tiles exist as 8 wavefronts only
                 n = 6 * Tc
                 probe.set_blocking(Tc, Rc)
                 probe.set_launch_mode(sh)

@@ -195,9 +195,9 @@ int fdtd_hip_grid_dims(const fdtd_grid *g, int out[4]);
 int fdtd_hip_grid_sync(fdtd_grid *g);
 
 /* FP32 grids: temporal block depth T (steps per fused launch, 1..15) and rows
- * per wavefront R (4, 8, 16, 20) of the interior tiles, which are 4R rows high
- * (R = 20 only with the 8-wavefront interior shapes, *_W8 and *_P16; a launch
- * in another shape fails with FDTD_EINVAL).  Steps are queued by
+ * per wavefront R (4, 8, 16, 18, 20) of the interior tiles, which are 4R rows
+ * high (R = 18 and 20 only with the 8-wavefront interior shapes, *_W8 and *_P16;
+ * a launch in another shape fails with FDTD_EINVAL).  Steps are queued by
  * fdtd_hip_grid_step and launched T at a time; every read of the state
  * (get_field, far_field, ...) first launches the queued steps. */
 int fdtd_hip_grid_set_blocking(fdtd_grid *g, int T, int R);

@@ -13,8 +13,8 @@ import json
 import os
 import time
 
-CANDIDATES = [(1, 16), (2, 16), (3, 16), (4, 16), (5, 16), (6, 16), (8, 16), (10, 16), (8, 20), (10, 20), (4, 8), (6, 8),
-              (8, 8), (10, 8), (4, 4)]
+CANDIDATES = [(1, 16), (2, 16), (3, 16), (4, 16), (5, 16), (6, 16), (8, 16), (10, 16), (10, 18), (8, 20), (10, 20), (4, 8),
+              (6, 8), (8, 8), (10, 8), (4, 4)]
 
 _cache = {}
 
@@ -75,8 +75,8 @@ def tune(m, make_sim, probe_steps=240, candidates=CANDIDATES):
             for shape, tag in shapes_tried:
                 if shape in (6, 7) and R < 8:
                     continue  # R = 4 interior tiles keep 4 wavefronts: the same as shapes 4 / 5
-                if R == 20 and shape not in (6, 7, 8, 9):
-                    continue  # 80-row interior tiles exist as 8 wavefronts of 10 rows only
+                if R in (18, 20) and shape not in (6, 7, 8, 9):
+                    continue  # 72- and 80-row interior tiles exist as 8 wavefronts only
                 probe(f"T{T}R{R}{tag}", T, R, shape)
         sim.set_launch_mode(0)
     finally:

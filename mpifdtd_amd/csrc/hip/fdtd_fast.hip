@@ -1050,10 +1050,15 @@ static fast_launcher pick(int R, int kind)
   if (kind == 7) return launch_pml48<MODE, FLAV, 1>;
   if (kind == 10) return launch_pml16<MODE, FLAV, 0>;
   if (kind == 11) return launch_pml16<MODE, FLAV, 1>;
-  // interior tiles of 4R rows as 8 wavefronts of R/2 rows (R = 8, 16, 20; R = 20
-  // has only these bodies: the launch refuses it for the other shapes)
-  if (kind == 8) return R == 8 ? launch_interior8<MODE, FLAV, 4, 0> : R == 20 ? launch_interior8<MODE, FLAV, 10, 0> : launch_interior8<MODE, FLAV, 8, 0>;
-  if (kind == 9) return R == 8 ? launch_interior8<MODE, FLAV, 4, 1> : R == 20 ? launch_interior8<MODE, FLAV, 10, 1> : launch_interior8<MODE, FLAV, 8, 1>;
+  // interior tiles of 4R rows as 8 wavefronts of R/2 rows (R = 8, 16, 18, 20;
+  // R = 18 and 20 have only these bodies: the launch refuses them for the other shapes)
+  if (kind == 8 || kind == 9) {
+    const bool smp = kind == 9;
+    if (R == 8) return smp ? launch_interior8<MODE, FLAV, 4, 1> : launch_interior8<MODE, FLAV, 4, 0>;
+    if (R == 18) return smp ? launch_interior8<MODE, FLAV, 9, 1> : launch_interior8<MODE, FLAV, 9, 0>;
+    if (R == 20) return smp ? launch_interior8<MODE, FLAV, 10, 1> : launch_interior8<MODE, FLAV, 10, 0>;
+    return smp ? launch_interior8<MODE, FLAV, 8, 1> : launch_interior8<MODE, FLAV, 8, 0>;
+  }
   if (kind == 4) {
     if (R == 4) return launch_one<MODE, FLAV, 4, 4>;
     if (R == 8) return launch_one<MODE, FLAV, 8, 4>;
@@ -1578,7 +1583,7 @@ static int build_tiles(FastState *s)
 int fast_state_set_blocking(FastState *s, int T, int R)
 {
   if (T < 1 || T >= FAST_MAX_T) return fdtd_fail(FDTD_EINVAL, "T=%d out of range", T);
-  if (R != 4 && R != 8 && R != 16 && R != 20) return fdtd_fail(FDTD_EINVAL, "R=%d not in {4,8,16,20}", R);
+  if (R != 4 && R != 8 && R != 16 && R != 18 && R != 20) return fdtd_fail(FDTD_EINVAL, "R=%d not in {4,8,16,18,20}", R);
   int rc;
   if ((rc = fast_state_flush(s))) return rc;
   s->T = T;
@@ -1644,11 +1649,12 @@ int fast_state_flush(FastState *s)
                                     lm == FAST_LAUNCH_SPLIT_W8 || lm == FAST_LAUNCH_TWO_STREAMS_W8 ||
                                     lm == FAST_LAUNCH_SPLIT_P16 || lm == FAST_LAUNCH_TWO_STREAMS_P16);
     s->upmlRows = wide ? FAST_UPML_ROWS_WIDE : FAST_UPML_ROWS;
-    // R = 20 (80-row interior tiles) exists only as 8 wavefronts of 10 rows
+    // R = 18 and 20 (72- and 80-row interior tiles) exist only as 8 wavefronts of 9 and 10 rows
     const bool i8 = lm == FAST_LAUNCH_SPLIT_W8 || lm == FAST_LAUNCH_TWO_STREAMS_W8 || lm == FAST_LAUNCH_SPLIT_P16 ||
                     lm == FAST_LAUNCH_TWO_STREAMS_P16;
-    if (s->R == 20 && !i8)
-      return fdtd_fail(FDTD_EINVAL, "R=20 needs a launch shape with 8-wavefront interior tiles (*_W8, *_P16), not %d", lm);
+    if ((s->R == 18 || s->R == 20) && !i8)
+      return fdtd_fail(FDTD_EINVAL, "R=%d needs a launch shape with 8-wavefront interior tiles (*_W8, *_P16), not %d",
+                       s->R, lm);
   }
   // a partial block runs with T = pending; the tiles of the full depth stay valid
   // for any smaller depth, so only the block depth passed to the kernel changes

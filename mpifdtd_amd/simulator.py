@@ -83,7 +83,7 @@ class Simulator:
 
     def set_blocking(self, T, R=16):
         """FP32: steps per fused launch and rows per wavefront of the interior tiles
-        (4, 8, 16, or 20 with the 8-wavefront interior launch shapes 6 to 9)."""
+        (4, 8, 16, or 18 and 20 with the 8-wavefront interior launch shapes 6 to 9)."""
         check(N.lib().fdtd_solver_setBlocking(int(T), int(R)), "fdtd_solver_setBlocking")
 
     def set_launch_mode(self, mode):
