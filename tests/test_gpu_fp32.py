@@ -95,7 +95,7 @@ def test_rows_per_wave_variants_agree():
             assert rel_l2(a, b) <= 1e-6
 
 
-@pytest.mark.parametrize("solver,RR", [(2, 20), (3, 20), (4, 20), (5, 20), (2, 18), (3, 18)])
+@pytest.mark.parametrize("solver,RR", [(2, 20), (3, 20), (4, 20), (5, 20), (2, 18), (3, 18), (4, 18), (5, 18)])
 def test_rows_per_wave_18_20_in_the_eight_wavefront_shapes(solver, RR):
     """R = 18 / 20: interior tiles of 72 / 80 rows as 8 wavefronts of 9 / 10 rows,
     in the four shapes that have them (split / two streams, with 8- or
