@@ -137,6 +137,32 @@ def test_rows_per_wave_18_20_in_the_eight_wavefront_shapes(solver, RR):
         sim.finish()
 
 
+@pytest.mark.parametrize("solver,RR", [(2, 18), (2, 20), (4, 18), (4, 20), (3, 20), (5, 18)])
+def test_rows_per_wave_18_20_on_an_odd_grid(solver, RR):
+    """72- and 80-row interior tiles on a 573 x 335 grid (no side a multiple of a
+    tile; oblique incidence): within 1e-6 of R = 16 and within the FP32
+    tolerance of the FP64 oracle."""
+    m, N, po = _mods()
+    w, h, hu, pml, lam, ang, steps = 5730, 3350, 10, 12, 500, 30, 160
+    print(f"odd grid R={RR}, solver {solver}", flush=True)
+    out = []
+    for R in (RR, 16):
+        sim = m.Simulator(solver, 1, m.field_info(w, h, hu, pml, lam, ang, steps), precision="fp32", T=8, R=R)
+        try:
+            sim.set_launch_mode(8)
+            sim.calc(steps)
+            out.append([np.array(f) for f in sim.fields()])
+        finally:
+            sim.finish()
+    for x, y in zip(out[0], out[1]):
+        assert rel_l2(x, y) <= 1e-6
+    o = po.Sim(width_nm=w, height_nm=h, h_u_nm=hu, pml=pml, lambda_nm=lam, angle_deg=ang, stepNum=steps,
+               solver=solver, model=1, ntff_time=0)
+    o.step(steps)
+    for k, which in enumerate((0, 3, 6)):
+        assert rel_l2(out[0][k], o.field(which)) <= FIELD_TOL, which
+
+
 @pytest.mark.parametrize("solver", [2, 3, 4, 5])
 def test_fp32_time_domain_ntff_against_oracle(solver):
     """FP32 fields with the per-step time-domain NTFF binned in FP64: the fused
