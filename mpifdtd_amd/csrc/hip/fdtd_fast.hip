@@ -410,8 +410,14 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
     // per tile in wavefronts holding a scatterer cell (TM: on Ez with 1/eps_z;
     // TE: on Ey with 1/eps_y) and kept in LDS (each wavefront writes and reads
     // only its own rows; registers would cost the second wavefront per SIMD);
-    // other tiles form it at each step.  The change g * dc is four FMAs.
-    constexpr bool GROWS = FLAV == FDTD_FLAVOUR_MPI && !PML && !SMP;
+    // the other tiles, and the 8-wavefront TM interior bodies, form it at each
+    // step (the same roundings).  Those bodies fit their 128 registers that way:
+    // the two step loops of the LDS form (with and without the source) cost them
+    // 8 spilled registers at 8 rows per wavefront, and the 9-row body fits
+    // without spilling; the 4-wavefront bodies have registers to spare and keep
+    // the LDS form, which saves them the per-step work.  The change g * dc is
+    // four FMAs.
+    constexpr bool GROWS = FLAV == FDTD_FLAVOUR_MPI && !PML && !SMP && !(MODE == FDTD_MODE_TM && W == 8);
     __shared__ float2 sg[GROWS ? NR : 1][64];
     auto add_src = [&](float2 e, int r, float ie, const FastStep &st) -> float2 {
       float2 g;
