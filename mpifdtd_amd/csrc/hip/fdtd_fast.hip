@@ -1489,7 +1489,28 @@ static int build_tiles(FastState *s)
   // Interior tile cores [I_lo, I_hi) x [J_lo, J_hi): with the block loaded from
   // T+1 rows below to T rows above the core, every cell whose update a core
   // cell depends on lies in [xlo+1, xhi-1] x [ylo+1, yhi-1].
-  const int I_lo = xlo + H0, I_hi = xhi - T, J_lo = ylo + H0, J_hi = yhi - T;
+  int I_lo = xlo + H0, I_hi = xhi - T, J_lo = ylo + H0, J_hi = yhi - T;
+  // An interior extent that is not a whole number of tile cores ends in a
+  // partial row (column) of tiles, which costs a full one.  The frame's tiles
+  // often have room to spare (a 43-row end band in two 48-row UPML blocks keeps
+  // up to 65 rows), so the remainder moves into the frame wherever its tile
+  // count stays the same (8256 x 4160, T = 10: 138 rows of 80-row interior
+  // tiles instead of 139, the last of which kept 29 of its 59 core rows).
+  auto trim = [&](int &lo, int &hi, int n, int core, int Bf) {
+    const int span = hi - lo, rem = span % core;
+    if (span <= core || rem == 0) return;
+    const size_t f0 = cover(0, lo, n, Bf, T).size(), f1 = cover(hi, n, n, Bf, T).size();
+    for (int a = 0; a <= rem; a++)
+      if (cover(0, lo + a, n, Bf, T).size() == f0 && cover(hi - (rem - a), n, n, Bf, T).size() == f1) {
+        lo += a;
+        hi -= rem - a;
+        return;
+      }
+  };
+  if (I_lo < I_hi && J_lo < J_hi) {
+    trim(I_lo, I_hi, NPX, CHi, UR);
+    trim(J_lo, J_hi, NPY, CW, 64);
+  }
   std::vector<FastTile> ti, tp;
   if (I_lo < I_hi && J_lo < J_hi) {
     tile_band(ti, I_lo, I_hi, J_lo, J_hi, NPX, NPY, 4 * R, T);
