@@ -59,11 +59,11 @@ SWEEP_LAMBDAS = (380.0, 781.0, 5.0)  # 81 wavelengths
 RUN_STEPS = 20000  # steps of one run of the sweep
 # Launch configuration of the sweep per mode: (T, R, launch shape), the
 # fastest of `--autotune`'s candidates on this grid (interior tiles as 8
-# wavefronts of 10 rows, TM, or 9 rows, TE, whose 10-row body spills; UPML tiles
-# as 16 wavefronts of 3 rows), in the split form of the
-# shapes (the two-stream forms time within 2 %, and the split form times each
-# launch on its own, as the counter summaries need).  T divides the default K.
-SWEEP_CONFIG = {"tm": (10, 20, 8), "te": (10, 18, 8)}
+# wavefronts of 10 rows; UPML tiles as 16 wavefronts of 3 rows), in the split
+# form of the shapes (the two-stream forms time within 2 %, and the split form
+# times each launch on its own, as the counter summaries need).  T divides the
+# default K.
+SWEEP_CONFIG = {"tm": (10, 20, 8), "te": (10, 20, 8)}
 T_CANDIDATES = (10, 8, 5, 4, 2, 1)
 
 

@@ -494,6 +494,11 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
             }
             F0[r] = ex;  // plain: the pulse is added below, after every row's stencil
             F1[r] = ey;
+            // 10 rows per wavefront: a scheduling fence after each row keeps the
+            // rows' 1/eps reads from LDS next to their use; hoisted ahead of the
+            // whole E phase, their 20 live registers took the body past the 128
+            // of four wavefronts per SIMD (6 spilled, 124 with the fence)
+            if constexpr (IEL && R == 10) __builtin_amdgcn_sched_barrier(0);
           } else {
             float2 sn0 = make_float2(0.f, 0.f), sd0 = sn0, sn1 = sn0, sd1 = sn0;
             // TE MPI: the scattered wave acts on Ey only
