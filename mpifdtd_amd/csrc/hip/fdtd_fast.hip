@@ -438,7 +438,7 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
           sn = c_mul(g, st.c);
           sd = c_mul(g, st.dc);
         }
-      } else if ((c == 0 ? scat0 : scat1) & (1u << r)) {
+      } else if (SRC && ((c == 0 ? scat0 : scat1) & (1u << r))) {
         source(c, r, ie, st, sn, sd);
       }
     };
@@ -466,7 +466,9 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
             }
             F0[r] = e;  // plain: the pulse is added below, after every row's stencil
           } else {
-            float2 sn = make_float2(0.f, 0.f), sd = sn;
+            // -0 is the additive identity (x + -0 == x for every x, -0 included), so
+            // the source additions fold away where no source is formed
+            float2 sn = make_float2(-0.f, -0.f), sd = sn;
             src_any(0, r, IE0(r), st, SRCT, sn, sd);
             const float2 reg = c_add(fma2(curl, IE0(r), F0[r]), sd);
             if (wpm) {
@@ -500,7 +502,7 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
             // of four wavefronts per SIMD (6 spilled, 124 with the fence)
             if constexpr (IEL && R == 10) __builtin_amdgcn_sched_barrier(0);
           } else {
-            float2 sn0 = make_float2(0.f, 0.f), sd0 = sn0, sn1 = sn0, sd1 = sn0;
+            float2 sn0 = make_float2(-0.f, -0.f), sd0 = sn0, sn1 = sn0, sd1 = sn0;  // -0: as in TM
             // TE MPI: the scattered wave acts on Ey only
             if (FLAV == FDTD_FLAVOUR_PLAIN) src_any(0, r, IE0(r), st, SRCT, sn0, sd0);
             src_any(1, r, IE1(r), st, SRCT, sn1, sd1);
@@ -740,7 +742,19 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
       for (int s = 0; s < T; s++) {
         h_phase();
         __syncthreads();
-        e_rows(sst[s], std::false_type());
+        // TM UPML tiles of 8 or more wavefronts: the E rows without the pulse in
+        // wavefronts that hold no scatterer cell (a second copy of them costs
+        // the TE bodies, and the 4-wavefront kernels that also run interior
+        // tiles, spilled registers); the interior tiles add the pulse after
+        // their stencils
+        if constexpr (PML && MODE == FDTD_MODE_TM && W >= 8) {
+          if (wsrc) e_rows(sst[s], std::true_type());
+          else e_rows(sst[s], std::false_type());
+        } else if constexpr (PML) {
+          e_rows(sst[s], std::true_type());
+        } else {
+          e_rows(sst[s], std::false_type());
+        }
         __syncthreads();
         if (SMP) sample(s);
       }
