@@ -575,12 +575,16 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
               F2[r] = sel(upd & bit, rhy, F2[r]);
             } else {
               // Mx/Bx: y 2 C_MX 3 C_MXEZ, x 2 C_BXMX1 3 C_BXMX0 ; My/By: x 4 C_BY, C_BYMY1/0
-              const float2 nmx = c_sub(c_scale(A2[r], cy2), c_scale(dEy, cy3));
-              const float2 nbx = c_sub(c_add(A3[r], c_scale(nmx, srow[2][rr])), c_scale(A2[r], srow[3][rr]));
-              const float2 nmy = c_add(A4[r], dEx);
+              // the old Mx / My terms are taken first, so that the new values can
+              // take the old ones' registers (no copies at the end of the step)
               const float rsx = srow[5][rr];
               const float c1 = s2p * rsx, c0 = s2m * rsx;
-              const float2 nby = c_sub(c_add(c_scale(A5[r], srow[4][rr]), c_scale(nmy, c1)), c_scale(A4[r], c0));
+              const float2 bx0 = c_sub(A3[r], c_scale(A2[r], srow[3][rr]));
+              const float2 by0 = c_sub(c_scale(A5[r], srow[4][rr]), c_scale(A4[r], c0));
+              const float2 nmx = c_sub(c_scale(A2[r], cy2), c_scale(dEy, cy3));
+              const float2 nmy = c_add(A4[r], dEx);
+              const float2 nbx = c_add(bx0, c_scale(nmx, srow[2][rr]));
+              const float2 nby = c_add(by0, c_scale(nmy, c1));
               A2[r] = nmx; A3[r] = nbx; A4[r] = nmy; A5[r] = nby;
               const bool p = pml & bit, u = upd & bit;
               const float2 hx = sel(p, c_scale(nbx, c2), rhx);
