@@ -576,7 +576,7 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, int tid)
             } else {
               // Mx/Bx: y 2 C_MX 3 C_MXEZ, x 2 C_BXMX1 3 C_BXMX0 ; My/By: x 4 C_BY, C_BYMY1/0
               // the old Mx / My terms are taken first, so that the new values can
-              // take the old ones' registers (no copies at the end of the step)
+              // take the old ones' registers (4 fewer register copies per row and step)
               const float rsx = srow[5][rr];
               const float c1 = s2p * rsx, c0 = s2m * rsx;
               const float2 bx0 = c_sub(A3[r], c_scale(A2[r], srow[3][rr]));
