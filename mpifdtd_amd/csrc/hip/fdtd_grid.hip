@@ -857,57 +857,48 @@ __global__ __launch_bounds__(NTFF_GRP * NS) void k_ntff_bin(const double2 *cS, i
 // V_f[c] += sum_s val_s[c] e^{i w_f tb_s}, tb_s = tb0 + s the binning's base time
 // of step s, for n steps of samples val_s[c] at src[s * ss + c * sp] (the
 // staging buffers [step][cell], or the binning's [cell][n]).  A workgroup covers
-// DFT_CELLS cells and DFT_LANES * NFL frequencies: lane (c, l) keeps the sums of
-// cell c at frequencies l, l + DFT_LANES, ... in registers over all n steps and
-// adds them to the FP64 accumulators ([f][cell]) once.  The samples of
-// DFT_CHUNK steps and their phase factors (ph[s][f] = e^{i w_f tb_s}, from
-// k_dft_phase) pass through LDS.
-#define DFT_CELLS 32
-#define DFT_LANES 8
-#define DFT_CHUNK 16
+// 64 cells, one per lane, and DFT_WAVES * NFL frequencies: wavefront w keeps the
+// sums of its lane's cell at frequencies w, w + DFT_WAVES, ... in registers over
+// all n steps and adds them to the FP64 accumulators ([f][cell]) once.  A
+// frequency is the same for the whole wavefront, so its phase factor
+// (ph[s][f] = e^{i w_f tb_s}, from k_dft_phase) is a scalar load, and each
+// complex product-sum is four fused multiply-adds per accumulator pair.
+#define DFT_WAVES 4
+#define DFT_NFL_MAX 8
 #define DFT_STAGE_STEPS 128  // steps staged before a DFT launch
 template <int NFL>
-__global__ __launch_bounds__(256) void k_ctr_dft(const double2 *cE, const double2 *cH, long ss, long sp, int n, int P,
-                                                 const double2 *ph, int nf, double2 *VE, double2 *VH)
+__global__ __launch_bounds__(64 * DFT_WAVES) void k_ctr_dft(const double2 *cE, const double2 *cH, long ss, long sp,
+                                                           int n, int P, const double2 *ph, int nf, double2 *VE,
+                                                           double2 *VH)
 {
-  constexpr int FW = DFT_LANES * NFL;
-  __shared__ double2 w[DFT_CHUNK][FW];
-  __shared__ double2 se[DFT_CHUNK][DFT_CELLS], sh[DFT_CHUNK][DFT_CELLS];
-  const int c = threadIdx.x % DFT_CELLS, l = threadIdx.x / DFT_CELLS;
-  const int p = blockIdx.x * DFT_CELLS + c, f0 = blockIdx.y * FW;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int p = blockIdx.x * 64 + lane;
+  const int fb = blockIdx.y * (DFT_WAVES * NFL) + wv;  // this wavefront's frequencies: fb + DFT_WAVES k
+  const size_t pc = (size_t)min(p, P - 1) * (size_t)sp;  // clamped cell: every load stays in bounds
   double ar[NFL], ai[NFL], br[NFL], bi[NFL];
 #pragma unroll
   for (int k = 0; k < NFL; k++) ar[k] = ai[k] = br[k] = bi[k] = 0.0;
-  for (int s0 = 0; s0 < n; s0 += DFT_CHUNK) {
-    const int ns = min(DFT_CHUNK, n - s0);
-    __syncthreads();  // the previous chunk is consumed
-    for (int q = threadIdx.x; q < ns * FW; q += 256) {
-      const int s = q / FW, f = q % FW;
-      w[s][f] = f0 + f < nf ? ph[(size_t)(s0 + s) * nf + f0 + f] : make_double2(0.0, 0.0);
-    }
-    for (int q = threadIdx.x; q < ns * DFT_CELLS; q += 256) {
-      const int s = q / DFT_CELLS, cc = q % DFT_CELLS, pp = blockIdx.x * DFT_CELLS + cc;
-      const size_t k = (size_t)(s0 + s) * ss + (size_t)pp * sp;
-      se[s][cc] = pp < P ? cE[k] : make_double2(0.0, 0.0);
-      sh[s][cc] = pp < P ? cH[k] : make_double2(0.0, 0.0);
-    }
-    __syncthreads();
-    for (int s = 0; s < ns; s++) {
-      const double2 e = se[s][c], h = sh[s][c];
+  for (int s = 0; s < n; s++) {
+    const double2 e = cE[(size_t)s * ss + pc], h = cH[(size_t)s * ss + pc];
+    const double2 *row = ph + (size_t)s * nf;
 #pragma unroll
-      for (int k = 0; k < NFL; k++) {
-        const double2 x = w[s][l + DFT_LANES * k];
-        ar[k] += e.x * x.x - e.y * x.y;
-        ai[k] += e.x * x.y + e.y * x.x;
-        br[k] += h.x * x.x - h.y * x.y;
-        bi[k] += h.x * x.y + h.y * x.x;
-      }
+    for (int k = 0; k < NFL; k++) {
+      const double2 x = row[min(fb + DFT_WAVES * k, nf - 1)];  // clamped: sums past nf are never stored
+      ar[k] = fma(e.x, x.x, ar[k]);
+      ar[k] = fma(-e.y, x.y, ar[k]);
+      ai[k] = fma(e.x, x.y, ai[k]);
+      ai[k] = fma(e.y, x.x, ai[k]);
+      br[k] = fma(h.x, x.x, br[k]);
+      br[k] = fma(-h.y, x.y, br[k]);
+      bi[k] = fma(h.x, x.y, bi[k]);
+      bi[k] = fma(h.y, x.x, bi[k]);
     }
   }
   if (p >= P) return;
 #pragma unroll
   for (int k = 0; k < NFL; k++) {
-    const int f = f0 + l + DFT_LANES * k;
+    const int f = fb + DFT_WAVES * k;
     if (f >= nf) continue;
     const size_t q = (size_t)f * P + p;
     double2 a = VE[q], b = VH[q];
@@ -1639,24 +1630,28 @@ static void launch_bin(fdtd_grid *g, int n, double time0, hipStream_t st)
 static void dft_add(fdtd_grid *g, const double2 *E, const double2 *H, long ss, long sp, int n, long tb0, hipStream_t st)
 {
   const int nf = g->nf, P = g->ct.total;
-  if (n <= 0 || P <= 0) return;
+  if (n <= 0 || P <= 0 || nf <= 0) return;
   hipLaunchKernelGGL(k_dft_phase, dim3((n * nf + 255) / 256), dim3(256), 0, st, g->dftOmega, nf, tb0, n, g->dftPh);
-  const int per = (nf + DFT_LANES - 1) / DFT_LANES;  // frequencies per lane
-#define CTR_DFT(NFL)                                                                                             \
-  hipLaunchKernelGGL(k_ctr_dft<NFL>, dim3((P + DFT_CELLS - 1) / DFT_CELLS, (nf + DFT_LANES * NFL - 1) / (DFT_LANES * NFL)), \
-                     dim3(256), 0, st, E, H, ss, sp, n, P, g->dftPh, nf, g->dftE, g->dftH)
-  if (per <= 1)
-    CTR_DFT(1);
-  else if (per <= 2)
-    CTR_DFT(2);
-  else if (per <= 4)
-    CTR_DFT(4);
-  else if (per <= 8)
-    CTR_DFT(8);
-  else if (per <= 12)
-    CTR_DFT(12);
-  else
-    CTR_DFT(16);
+  // frequencies per wavefront: the NFL (<= DFT_NFL_MAX) that leaves the fewest
+  // unused frequency slots in the launch's blocks of DFT_WAVES * NFL
+  int nfl = 1, best = INT_MAX;
+  for (int c = DFT_NFL_MAX; c >= 1; c--) {
+    const int slots = (nf + DFT_WAVES * c - 1) / (DFT_WAVES * c) * DFT_WAVES * c;
+    if (slots < best) best = slots, nfl = c;
+  }
+#define CTR_DFT(NFL)                                                                                              \
+  hipLaunchKernelGGL(k_ctr_dft<NFL>, dim3((P + 63) / 64, (nf + DFT_WAVES * NFL - 1) / (DFT_WAVES * NFL)),          \
+                     dim3(64 * DFT_WAVES), 0, st, E, H, ss, sp, n, P, g->dftPh, nf, g->dftE, g->dftH)
+  switch (nfl) {
+    case 1: CTR_DFT(1); break;
+    case 2: CTR_DFT(2); break;
+    case 3: CTR_DFT(3); break;
+    case 4: CTR_DFT(4); break;
+    case 5: CTR_DFT(5); break;
+    case 6: CTR_DFT(6); break;
+    case 7: CTR_DFT(7); break;
+    default: CTR_DFT(8); break;
+  }
 #undef CTR_DFT
 }
 
